@@ -1,0 +1,153 @@
+/*
+ * searay_uppe.h -- C ABI of the MI355X UPPE stepper (libsearay_uppe.so).
+ *
+ * This is the native boundary under the Python drop-in for SeaRay's `modules/uppe_kernel.py`.
+ * The Python `track`/`propagator` (searay_amd/uppe_kernel.py) keep the reference signatures; underneath,
+ * they call these entry points in place of the pyopencl calls listed against each function.
+ *
+ * Conventions
+ *  - Every function returns an int status: 0 = OK, otherwise an UPPE_ERR_* code; uppe_last_error(h) has
+ *    the text (a static string for a NULL handle).
+ *  - Host arrays use the reference layout: spectral arrays are complex128 interleaved (re, im) with shape
+ *    (Nw, N1, N2) in C order, time-domain arrays are float64 (Nt, N1, N2), Nt = 2 (Nw - 1).
+ *    Host pointers are owned by the caller and only read or written during the call.
+ *  - The handle owns all device memory. A handle is not thread-safe; separate handles are independent.
+ *  - No C++ types or exceptions cross this boundary.
+ */
+#ifndef SEARAY_UPPE_H
+#define SEARAY_UPPE_H
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+enum {
+  UPPE_OK = 0,
+  UPPE_ERR_ARG = 1,         /* bad argument: sizes, NULL pointers, grid not 2^n(+1) (uppe_kernel.py:282-285,354-360) */
+  UPPE_ERR_DEVICE = 2,      /* HIP runtime failure or no device */
+  UPPE_ERR_STATE = 3,       /* call out of order (e.g. advance before set_field) */
+  UPPE_ERR_UNSUPPORTED = 4  /* valid request this build does not implement */
+};
+
+enum { UPPE_RADIAL = 0, UPPE_CARTESIAN = 1 };
+
+enum {
+  UPPE_PREC_FP64_PLAIN = 0, /* one kernel per reference operation, reference layout, FP64, host-driven dz */
+  UPPE_PREC_FP64 = 1,       /* fused MI355X kernels in FP64 */
+  UPPE_PREC_FP32 = 2,       /* fused kernels in FP32 (FP64 linear phase, FP64 plasma scan) */
+  UPPE_PREC_BF16X3 = 3      /* as FP32, Hankel contraction as three BF16 MFMA products, FP32 accumulation */
+};
+
+enum { UPPE_ION_NONE = 0, UPPE_ION_EXPLICIT = 1, UPPE_ION_SERIES = 2 };
+
+/* Grid description. Replaces the shapes pyopencl arrays carry (uppe_kernel.py:95-119) and the transverse
+ * tool choice (uppe_kernel.py:374-377). nw = 2^n + 1; n1, n2 powers of 2; nk <= n1 radial modes (radial only). */
+typedef struct uppe_desc {
+  int geometry;   /* UPPE_RADIAL or UPPE_CARTESIAN */
+  int nw;         /* frequency nodes */
+  int n1;         /* radial cells or x cells */
+  int n2;         /* azimuthal nodes or y cells */
+  int nk;         /* radial modes kept; ignored for Cartesian */
+  int precision;  /* UPPE_PREC_* */
+  int device;     /* HIP device ordinal */
+  int reserved;
+} uppe_desc;
+
+/* Rate-kernel parameters, exactly the scalars Ionizer.RateCL passes (ionization.py:131-150). */
+typedef struct uppe_ionization {
+  int kind;       /* UPPE_ION_* */
+  int reserved;
+  double e_conv, rate_conv, e_cutoff;
+  double c_pre, c_pow, c_exp;   /* kind == EXPLICIT: rate = rate_conv c_pre E^c_pow exp(c_exp/E) */
+  double c[6];                  /* kind == SERIES:   rate = rate_conv exp(sum_n c_n log(E)^n) */
+} uppe_ionization;
+
+/* Medium and filters. Replaces Material.UpdateMaterial's device uploads (uppe_kernel.py:67-90) and the
+ * Source filter arrays (uppe_kernel.py:96,120-129). */
+typedef struct uppe_medium {
+  const double* w;            /* [nw] frequency nodes, w[0] must be 0 */
+  const double* kz;           /* complex128 [nw][n1][n2] */
+  const double* kg;           /* [nw] Galilean wavenumber -w/vg */
+  const double* ngas;         /* [n1][n2] gas density in plasma units */
+  const double* nl_filter;    /* [nw] nonlinear band filter */
+  const double* cond_filter;  /* [nw] field conditioner (zero below bin 4) */
+  double redchi3;             /* chi3 / n_ref */
+  double dt;                  /* pi / max(w) */
+  uppe_ionization ion;
+} uppe_medium;
+
+/* One call runs subcycles sub0 .. sub0+n_sub-1 of length `length`. Subcycle s starts at z0 = zi + s*length
+ * and its steps use the phase coordinate (z0 - zi) + z, formed in that order as the reference does.
+ * Replaces the subcycle/step loop of propagator (uppe_kernel.py:299-325). */
+typedef struct uppe_step_control {
+  double zi;            /* start of the diagnostic interval */
+  double length;        /* subcycle length L = (zf - zi)/subcycles */
+  int sub0;             /* index of the first subcycle run by this call */
+  int max_steps;        /* stop after this many steps, leaving the subcycle unfinished (0 = no limit) */
+  double dzmin;         /* 'minimum step' */
+  double dphi;          /* 0.4 in the reference (uppe_kernel.py:105) */
+  double rel_amin;      /* 1e-2 in the reference (uppe_kernel.py:106) */
+  int n_sub;            /* subcycles to run (>= 1) */
+  int n_schedule;       /* entries in dz_schedule; 0 = adaptive step control */
+  const double* dz_schedule; /* replayed steps (nullable) */
+  double* dz_record;    /* receives the steps taken (nullable) */
+  int record_cap;       /* capacity of dz_record */
+  int reserved;
+} uppe_step_control;
+
+typedef struct uppe_info {
+  long long steps;      /* RK4 steps taken since set_field */
+  long long evals;      /* right-hand-side evaluations (4 per step) */
+  int nan_steps;        /* steps whose field modulus sum was NaN (printed as '!' by the reference) */
+  int reserved;
+  double z_last;        /* local z reached in the last subcycle */
+} uppe_info;
+
+typedef struct uppe_handle uppe_handle;
+
+/* Allocate device buffers for one grid. Replaces Source.__init__ device arrays (uppe_kernel.py:108-119)
+ * and HankelTransformTool.AllocateDeviceMemory (grid_tools.py:299-303). */
+int uppe_create(const uppe_desc* desc, uppe_handle** out);
+
+/* Radial basis: H[nk][nr][nm] eigenvectors and Lambda[nr] root volumes (grid_tools.py:294,263,300-301). */
+int uppe_set_hankel(uppe_handle* h, const double* H, const double* lambda_);
+
+/* Medium tables; may be called again between advance calls (uppe_kernel.py:303). */
+int uppe_set_medium(uppe_handle* h, const uppe_medium* m);
+
+/* Upload the r-space field a[nw][n1][n2] (complex128) and transform it to k-space
+ * (uppe_kernel.py:290-291: Source(...) then T.kspacex(src.qw_dev)). */
+int uppe_set_field(uppe_handle* h, const double* a_r);
+
+/* RK4 interaction-picture stepping with step control (uppe_kernel.py:299-325, load_source :188-232).
+ * steps_out (nullable) receives the steps taken by this call. */
+int uppe_advance(uppe_handle* h, const uppe_step_control* c, int* steps_out);
+
+/* finish() semantics (uppe_kernel.py:234-258): propagate to z_total = zf - zi, transform to r-space and
+ * evaluate the diagnostics. Outputs (each nullable): a_r, j_r complex128 [nw][n1][n2]; chi_t, ne_t float64
+ * [nt][n1][n2] (the caller forms rfft along axis 0 as the reference does on the host). */
+int uppe_finish(uppe_handle* h, double z_total, double* a_r, double* j_r, double* chi_t, double* ne_t);
+
+/* Read the k-space state q[nw][n1][n2] (complex128). No reference counterpart: used for checks. */
+int uppe_get_state(uppe_handle* h, double* q_k);
+
+/* Counters since the last uppe_set_field. */
+int uppe_get_info(uppe_handle* h, uppe_info* info);
+
+/* Per-kernel-class timing with HIP events on the handle's stream (for measurement). enable != 0 starts
+ * recording and clears totals; uppe_profile_read returns total milliseconds and launch count of a class. */
+enum { UPPE_KC_HANKEL = 0, UPPE_KC_SOURCE = 1, UPPE_KC_POINT = 2, UPPE_KC_REDUCE = 3, UPPE_KC_TRANSVERSE_FFT = 4,
+       UPPE_KC_COUNT = 5 };
+int uppe_profile_enable(uppe_handle* h, int enable);
+int uppe_profile_read(uppe_handle* h, int kernel_class, double* total_ms, long long* launches);
+
+/* Block until all work on the handle's stream is done. */
+int uppe_synchronize(uppe_handle* h);
+
+int uppe_destroy(uppe_handle* h);
+const char* uppe_last_error(const uppe_handle* h);
+
+#ifdef __cplusplus
+}
+#endif
+#endif /* SEARAY_UPPE_H */

@@ -1,0 +1,218 @@
+'''Synthetic UPPE workloads: input dictionaries in the reference's format.
+
+Each builder returns a ``vol_dict`` for ``uppe_kernel.track``. The seeded inputs are:
+
+* ``radial_kerr_air``: radial Kerr-only propagation in air, 800 nm, 50 fs FWHM Gaussian, w0 = 1 mm, 0.5 P_cr,
+  1 % random spectral phase, Nw = 513 over a 1 ps window, Nr = 128 Hankel modes, 100 fixed steps over 1 m.
+* ``radial_o2_ionization``: radial Kerr + O2 photoionisation (Ip = 12.07 eV, Z = 0.53, 5.4e18 cm^-3) with plasma
+  current, 800 nm, 40 fs, w0 = 100 um, 3 P_cr, Nw = 2049 over band (0, 8), Nr = 256, 400 adaptive
+  subcycles over 20 cm; rate model 'explicit' (PPT tunnelling limit) or 'series' (PPT quintic fit).
+* ``cartesian_filamentation``: 3-D Cartesian, super-Gaussian beam w0 = 1.2 mm with 10 % amplitude noise at
+  20 P_cr, Kerr + O2, Nw = 513, 256 x 256 over 4 mm x 4 mm, 200 steps over 10 cm.
+* ``gaussian_vacuum``: the linear vacuum Gaussian beam of the reference's UPPE test (test/gaussian_test.py:
+  110-131), Cartesian or radial.
+
+``scale`` shrinks a workload for tests: (Nw-1) and the transverse grid are divided by it (powers of 2), keeping
+the physical window, and ``steps`` overrides the number of subcycles.
+'''
+import numpy as np
+from scipy import constants as C
+
+from . import dispersion
+from . import input_tools
+from . import ionization
+from . import volume
+
+MKS_LENGTH = 0.8e-6 / (2 * np.pi)
+N2_AIR = 1e-23        # m^2/W (examples/uppe/air_fil.py:28)
+GAS_DENSITY = '5.4e18 cm-3'
+
+
+def critical_power(n2=N2_AIR, wavelength=0.8e-6):
+    '''P_cr = 3.77 lambda^2 / (8 pi n2) in W.'''
+    return 3.77 * wavelength ** 2 / (8 * np.pi * n2)
+
+
+def _box(vol):
+    obj = volume.AnalyticBox('air')
+    obj.Initialize(vol)
+    return obj
+
+
+def _gaussian_spectrum(w, t0):
+    sigma = 2.0 / t0
+    return np.exp(-(w - 1.0) ** 2 / sigma ** 2)
+
+
+def _normalise(spec_w, a0):
+    '''Scale so that the real time-domain field has peak a0.'''
+    peak = np.max(np.abs(np.fft.irfft(spec_w)))
+    return a0 / peak
+
+
+def _base(helper, band, size, grid, nl_band, subcycles, dzmin, coords, modes=None):
+    vol = {
+        'frequency band': band,
+        'nonlinear band': nl_band,
+        'size': size,
+        'wave grid': grid,
+        'wave coordinates': coords,
+        'subcycles': subcycles,
+        'minimum step': dzmin,
+        'density reference': helper.dnum(GAS_DENSITY),
+        'density function': '1.0',
+        'density lambda': lambda x, y, z, r2: np.ones(r2.shape),
+        'propagator': 'uppe',
+    }
+    if modes is not None:
+        vol['radial modes'] = modes
+    vol['object'] = _box(vol)
+    return vol
+
+
+def radial_kerr_air(scale=1, steps=None, seed=1000, precision=None):
+    helper = input_tools.InputHelper(MKS_LENGTH)
+    nw = (512 // scale) + 1
+    nr = max(8, 128 // scale)
+    T = helper.dnum('1 ps')
+    dw = 2 * np.pi / T
+    band = (0.0, 513 * dw)   # smaller grids keep the band and shorten the time window
+    L = helper.dnum('1 m')
+    subcycles = 100 if steps is None else steps
+    air = dispersion.SimpleAir(MKS_LENGTH)
+    vol = _base(helper, band, (helper.dnum('8 mm'), helper.dnum('8 mm'), L), (nw, nr, 1, 2), band, subcycles,
+                L / subcycles, 'cylindrical', nr)
+    vol['dispersion inside'] = air
+    vol['dispersion outside'] = dispersion.Vacuum()
+    vol['window speed'] = air.GroupVelocityMagnitude(1.0)
+    vol['chi3'] = helper.chi3(1.0, '%g m2/W' % N2_AIR)
+    t0 = helper.dnum('50 fs') / np.sqrt(2 * np.log(2))
+    w0 = helper.dnum('1 mm')
+    P0 = 0.5 * critical_power()
+    energy = P0 * (t0 * helper.t1) / np.sqrt(2 / np.pi) / helper.u1
+    a0 = helper.a0(energy, t0, w0, 1.0)
+    def incoming(w, r, phi):
+        rng = np.random.default_rng(seed)
+        phase = 2 * np.pi * 0.01 * (rng.random(w.shape) - 0.5)
+        spec = _gaussian_spectrum(w, t0)
+        amp = _normalise(spec, a0)
+        return amp * np.einsum('i,j,k', spec * np.exp(1j * phase), np.exp(-r ** 2 / w0 ** 2), np.ones(phi.shape))
+    vol['incoming wave'] = incoming
+    if precision:
+        vol['uppe precision'] = precision
+    return vol
+
+
+def o2_ionizer(kind='explicit'):
+    helper = input_tools.InputHelper(MKS_LENGTH)
+    Uion = helper.dnum('12.07 eV')
+    if kind == 'explicit':
+        return ionization.PPT_Tunneling(MKS_LENGTH, False, Uion, 0.53, lstar=0, l=0, m=0, w0=1.0, terms=80)
+    return ionization.PPT(MKS_LENGTH, False, Uion, 0.53, lstar=0, l=0, m=0, w0=1.0, terms=80)
+
+
+def radial_o2_ionization(scale=1, steps=None, seed=1001, rate='explicit', precision=None, power=3.0,
+                         length='20 cm'):
+    helper = input_tools.InputHelper(MKS_LENGTH)
+    nw = (2048 // scale) + 1
+    nr = max(8, 256 // scale)
+    band = (0.0, 8.0 / scale)
+    L = helper.dnum(length)
+    subcycles = 400 if steps is None else steps
+    air = dispersion.SimpleAir(MKS_LENGTH)
+    air.add_opacity_region(helper.dnum('0.1 cm'), helper.dnum('0.01 um'), helper.dnum('0.3 um'))
+    vol = _base(helper, band, (helper.dnum('2 mm'), helper.dnum('2 mm'), L), (nw, nr, 1, 2),
+                (0.0, min(4.0, band[1])), subcycles, 1.0, 'cylindrical', nr)
+    vol['dispersion inside'] = air
+    vol['dispersion outside'] = dispersion.Vacuum()
+    vol['window speed'] = air.GroupVelocityMagnitude(1.0)
+    vol['chi3'] = helper.chi3(1.0, '%g m2/W' % N2_AIR)
+    vol['ionizer'] = o2_ionizer(rate)
+    t0 = helper.dnum('40 fs') / np.sqrt(2 * np.log(2))
+    w0 = helper.dnum('100 um')
+    P0 = power * critical_power()
+    energy = P0 * (t0 * helper.t1) / np.sqrt(2 / np.pi) / helper.u1
+    a0 = helper.a0(energy, t0, w0, 1.0)
+    def incoming(w, r, phi):
+        rng = np.random.default_rng(seed)
+        phase = 2 * np.pi * 0.01 * (rng.random(w.shape) - 0.5)
+        spec = _gaussian_spectrum(w, t0)
+        amp = _normalise(spec, a0)
+        return amp * np.einsum('i,j,k', spec * np.exp(1j * phase), np.exp(-r ** 2 / w0 ** 2), np.ones(phi.shape))
+    vol['incoming wave'] = incoming
+    if precision:
+        vol['uppe precision'] = precision
+    return vol
+
+
+def cartesian_filamentation(scale=1, steps=None, seed=1002, precision=None, length='10 cm'):
+    helper = input_tools.InputHelper(MKS_LENGTH)
+    nw = (512 // scale) + 1
+    nx = max(8, 256 // scale)
+    T = helper.dnum('1 ps')
+    dw = 2 * np.pi / T
+    band = (0.0, 513 * dw)
+    L = helper.dnum(length)
+    subcycles = 200 if steps is None else steps
+    air = dispersion.SimpleAir(MKS_LENGTH)
+    vol = _base(helper, band, (helper.dnum('4 mm'), helper.dnum('4 mm'), L), (nw, nx, nx, 2), band, subcycles,
+                L / subcycles, 'cartesian')
+    vol['dispersion inside'] = air
+    vol['dispersion outside'] = dispersion.Vacuum()
+    vol['window speed'] = air.GroupVelocityMagnitude(1.0)
+    vol['chi3'] = helper.chi3(1.0, '%g m2/W' % N2_AIR)
+    vol['ionizer'] = o2_ionizer('explicit')
+    t0 = helper.dnum('50 fs') / np.sqrt(2 * np.log(2))
+    w0 = helper.dnum('1.2 mm')
+    P0 = 20 * critical_power()
+    energy = P0 * (t0 * helper.t1) / np.sqrt(2 / np.pi) / helper.u1
+    a0 = helper.a0(energy, t0, w0, 1.0)
+    def incoming(w, x, y):
+        rng = np.random.default_rng(seed)
+        spec = _gaussian_spectrum(w, t0)
+        amp = _normalise(spec, a0)
+        r2 = x[:, None] ** 2 + y[None, :] ** 2
+        noise = 1.0 + 0.1 * (2 * rng.random(r2.shape) - 1.0)
+        return amp * np.einsum('i,jk', spec, np.exp(-(r2 / w0 ** 2) ** 4) * noise)
+    vol['incoming wave'] = incoming
+    if precision:
+        vol['uppe precision'] = precision
+    return vol
+
+
+def gaussian_vacuum(coords='cartesian', n=128, nw=65, precision=None):
+    '''The reference's linear UPPE test setup: vacuum, a00 = 1, w00 = 1, r00 = 0.01 cm, band (0, 2),
+    size 36 r00 x 36 r00 x 4 zR, three planes (test/gaussian_test.py:18-21, 60-63, 110-131).'''
+    cm = 100 * MKS_LENGTH
+    a00, w00, r00 = 1.0, 1.0, 0.01 / cm
+    zR = 0.5 * w00 * r00 ** 2
+    vol = {
+        'density reference': 1.0,
+        'density function': '1.0',
+        'density lambda': lambda x, y, z, r2: np.ones(x.shape),
+        'frequency band': (0.0, 2.0),
+        'nonlinear band': (0.0, 2.0),
+        'wave grid': (nw, n, n if coords == 'cartesian' else 1, 3),
+        'wave coordinates': coords,
+        'dispersion inside': dispersion.Vacuum(),
+        'dispersion outside': dispersion.Vacuum(),
+        'size': (36 * r00, 36 * r00, 4 * zR),
+        'propagator': 'uppe',
+        'subcycles': 1,
+        'minimum step': 1,
+    }
+    if coords != 'cartesian':
+        vol['radial modes'] = n
+
+    def f(x, x0, dx):
+        return np.exp(-(x - x0) ** 2 / dx ** 2)
+
+    def incoming(w, x, y):
+        if coords == 'cartesian':
+            return a00 * np.einsum('i,j,k', f(w, w00, 0.1), f(x, 0, r00), f(y, 0, r00))
+        return a00 * np.einsum('i,j,k', f(w, w00, 0.1), f(x, 0, r00), np.ones(y.shape))
+    vol['incoming wave'] = incoming
+    vol['object'] = _box(vol)
+    if precision:
+        vol['uppe precision'] = precision
+    return vol, dict(a00=a00, w00=w00, r00=r00, zR=zR)

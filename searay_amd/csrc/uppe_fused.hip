@@ -1,0 +1,858 @@
+// uppe_fused.hip -- the fused MI355X UPPE stepper (FP64 / FP32 / BF16x3 Hankel).
+//
+// Layout: every spectral array is strip-major, [P][NWP] complex with P = N1*N2 transverse points and the
+// frequency axis contiguous (NWP = Nw padded to a multiple of 32). In this layout
+//   - the radial transforms are plain row-major GEMMs:  E_r = Hb . E_k,  J_k = Hf . J_r  (complex columns
+//     interleaved), Hb[i][k] = H[k][i] L_k/L_i, Hf[k][j] = H[k][j] L_j/L_k (grid_tools.py:308-325);
+//   - the time-domain source is one contiguous strip per transverse point.
+// One RHS evaluation (uppe_kernel.py:188-216) is three kernels:
+//   1. k2r GEMM; its B-operand load forms the RK stage input, propagates it linearly and converts it to the
+//      electric field (uppe.cl:22-108, uppe_kernel.py:210-213);
+//   2. source: one workgroup per strip does irfft -> rate -> plasma scan -> soft window -> Kerr ->
+//      P, J_p -> one complex FFT for both rffts -> spectral current and filter (uppe_kernel.py:139-186);
+//   3. r2k GEMM; its epilogue forms the ODE right-hand side (uppe.cl:73-91) and the RK bookkeeping, the
+//      per-point step estimate (paraxial.cl:40-58) and the field conditioner (uppe_kernel.py:131-137).
+// Step control (uppe_kernel.py:218-227, 299-325) runs on the device: two single-thread control kernels per
+// step, so a whole subcycle runs without host synchronisation.
+#include "uppe_internal.h"
+
+#include <algorithm>
+#include <cstring>
+
+namespace uppe {
+
+// ---------------------------------------------------------------------------------------------------
+// device step state
+// ---------------------------------------------------------------------------------------------------
+struct StepState {
+  double zi, L, dzmin, dphi, rel_amin;
+  double zoff;  // (z0 - zi) of the current subcycle
+  double z;     // local z within the subcycle
+  double dz;    // current step
+  double z_last;
+  unsigned long long dzraw;  // bits of the running minimum of the per-point estimate
+  int sub, sub_end, active, fixed;
+  int nanflag, nan_steps, steps, sched_n, sched_pos, rec_cap, error, max_steps;
+};
+
+struct FusedState {
+  int prec = 0;
+  int nwp = 0;
+  size_t el = 0;  // sizeof(T)
+  // tables
+  double* phs = nullptr;   // [P][nwp] Re kz + kg (FP64)
+  void* kzc = nullptr;     // [P][nwp] complex T
+  void* w = nullptr;       // [nwp] T
+  void* nlf = nullptr;     // [nwp] T
+  void* cond = nullptr;    // [nwp] T
+  double* ng = nullptr;    // [P]
+  void* Hb = nullptr;      // [Nr][Nk] T
+  void* Hf = nullptr;      // [Nk][Nr] T
+  void* tw = nullptr;      // [Nt/2] complex T
+  double redchi3 = 0, dt = 0;
+  uppe_ionization ion{};
+  // state and work buffers: complex T [P][nwp]
+  void *q = nullptr, *k1 = nullptr, *k2 = nullptr, *k3 = nullptr, *er = nullptr, *jr = nullptr;
+  void *chi_t = nullptr, *ne_t = nullptr;  // [P][Nt] T
+  double* stage = nullptr;                 // host-layout FP64 staging (max of 2*npt, ntt)
+  unsigned long long *qmax = nullptr, *qmax_next = nullptr;
+  StepState* st = nullptr;
+  StepState* st_host = nullptr;
+  double *sched = nullptr, *rec = nullptr;
+  double* rec_active = nullptr;  // rec while the current advance records steps, else null
+  int sched_cap = 0, rec_cap = 0;
+};
+
+constexpr int BM = 64, BN = 64, BK = 16;
+
+enum { PRO_NONE = 0, PRO_STAGE = 1, PRO_FINISH = 2 };
+enum { EPI_NONE = 0, EPI_STAGE = 1 };
+
+__device__ inline unsigned long long dbits(double x) { return (unsigned long long)__double_as_longlong(x); }
+__device__ inline double bitsd(unsigned long long b) { return __longlong_as_double((long long)b); }
+
+// exp(i phs z) exp(-ki z) with the phase formed in FP64 (long-z cancellation) and, for FP32, range-reduced
+// before the FP32 sincos.
+template <typename T>
+__device__ inline cplx<T> linear_factor(double phs, T ki, double z, int sgn);
+
+template <>
+__device__ inline cplx<double> linear_factor<double>(double phs, double ki, double z, int sgn) {
+  // the reference forms ar = -Ki z, ai = (Kr + kg) z and exp(ar)(cos ai, sin ai) (uppe.cl:66-70, 86-90)
+  double ar = sgn > 0 ? -ki * z : ki * z;
+  double ai = sgn > 0 ? phs * z : -(phs * z);
+  double ex = exp(ar);
+  double s, c;
+  sincos(ai, &s, &c);
+  return cmk(c * ex, s * ex);
+}
+
+template <>
+__device__ inline cplx<float> linear_factor<float>(double phs, float ki, double z, int sgn) {
+  double ai = phs * z;
+  double red = ai - 6.283185307179586 * rint(ai * 0.15915494309189535);
+  float ar = (float)(-(double)ki * z);
+  if (sgn < 0) {
+    red = -red;
+    ar = -ar;
+  }
+  float ex = expf(ar);
+  float s, c;
+  sincosf((float)red, &s, &c);
+  return cmk(c * ex, s * ex);
+}
+
+// ---------------------------------------------------------------------------------------------------
+// GEMM with fused prologue / epilogue:  out[M][2 nwp] = A[M][K] . B[K][2 nwp]
+// ---------------------------------------------------------------------------------------------------
+template <typename T, int PRO, int EPI>
+__global__ __launch_bounds__(256) void f_gemm(const T* __restrict__ A, const cplx<T>* __restrict__ B,
+                                              cplx<T>* __restrict__ out, int M, int K, int nwp,
+                                              const double* __restrict__ phs, const cplx<T>* __restrict__ kzc,
+                                              const T* __restrict__ cond, cplx<T>* __restrict__ q,
+                                              const cplx<T>* __restrict__ kprev, cplx<T>* __restrict__ k1,
+                                              cplx<T>* __restrict__ k2, cplx<T>* __restrict__ k3,
+                                              StepState* __restrict__ st, unsigned long long* __restrict__ qmax,
+                                              unsigned long long* __restrict__ qmax_next, int stage, double zfix) {
+  if (st && !st->active) return;
+  __shared__ T As[BK][BM + 1];
+  __shared__ T Bs[BK][BN];
+  __shared__ double red[8];
+  const int tid = threadIdx.x, tx = tid & 15, ty = tid >> 4;
+  const int row0 = blockIdx.y * BM, col0 = blockIdx.x * BN;
+  double zph = zfix, cu = 0.0, dz = 0.0;
+  if (st) {
+    dz = st->dz;
+    double cs = stage == 1 ? 0.0 : (stage == 4 ? 1.0 : 0.5);
+    zph = st->zoff + (st->z + cs * dz);
+    cu = stage == 1 ? 0.0 : (stage == 4 ? dz : dz / 2);
+  }
+  T acc[4][4];
+#pragma unroll
+  for (int i = 0; i < 4; i++)
+#pragma unroll
+    for (int j = 0; j < 4; j++) acc[i][j] = T(0);
+
+  for (int k0 = 0; k0 < K; k0 += BK) {
+#pragma unroll
+    for (int i = 0; i < 4; i++) {
+      int e = tid + 256 * i, r = e >> 4, kk = e & 15, gr = row0 + r, gk = k0 + kk;
+      As[kk][r] = (gr < M && gk < K) ? A[(long)gr * K + gk] : T(0);
+    }
+#pragma unroll
+    for (int i = 0; i < 2; i++) {
+      int e = tid + 256 * i, kk = e >> 5, cf = e & 31, gk = k0 + kk, f = (col0 >> 1) + cf;
+      cplx<T> v = cmk(T(0), T(0));
+      if (gk < K) {
+        long idx = (long)gk * nwp + f;
+        v = B[idx];
+        if (PRO == PRO_STAGE) {
+          if (stage > 1) {
+            cplx<T> kp = kprev[idx];
+            T c = (T)cu;
+            v = cscale(cmk(v.re + c * kp.re, v.im + c * kp.im), cond[f]);
+          }
+          cplx<T> kz = kzc[idx];
+          cplx<T> a = cmul(linear_factor<T>(phs[idx], kz.im, zph, +1), v);
+          v = cmul(cmk(-kz.im, kz.re), a);  // E = i kz A
+        } else if (PRO == PRO_FINISH) {
+          cplx<T> kz = kzc[idx];
+          v = cmul(linear_factor<T>(phs[idx], kz.im, zph, +1), v);
+        }
+      }
+      Bs[kk][2 * cf] = v.re;
+      Bs[kk][2 * cf + 1] = v.im;
+    }
+    __syncthreads();
+#pragma unroll
+    for (int kk = 0; kk < BK; kk++) {
+      T a[4], b[4];
+#pragma unroll
+      for (int i = 0; i < 4; i++) a[i] = As[kk][ty * 4 + i];
+#pragma unroll
+      for (int j = 0; j < 4; j++) b[j] = Bs[kk][tx * 4 + j];
+#pragma unroll
+      for (int i = 0; i < 4; i++)
+#pragma unroll
+        for (int j = 0; j < 4; j++) acc[i][j] += a[i] * b[j];
+    }
+    __syncthreads();
+  }
+
+  double loc = (EPI == EPI_STAGE && stage == 4) ? 0.0 : INFINITY;
+  bool nan = false;
+#pragma unroll
+  for (int i = 0; i < 4; i++) {
+    const int r = row0 + ty * 4 + i;
+    if (r >= M) continue;
+#pragma unroll
+    for (int jj = 0; jj < 2; jj++) {
+      const int f = (col0 >> 1) + tx * 2 + jj;
+      const long idx = (long)r * nwp + f;
+      cplx<T> J = cmk(acc[i][2 * jj], acc[i][2 * jj + 1]);
+      if (EPI == EPI_NONE) {
+        out[idx] = J;
+        continue;
+      }
+      // S = 0.5 i exp(-i(kz+kg) z) J / Re kz (uppe.cl:73-91)
+      cplx<T> kz = kzc[idx];
+      cplx<T> t = cmul(linear_factor<T>(phs[idx], kz.im, zph, -1), J);
+      cplx<T> S = cmk((T(-0.5) * t.im) / kz.re, (T(0.5) * t.re) / kz.re);
+      if (stage == 1) {
+        k1[idx] = S;
+        cplx<T> Q = q[idx];
+        double qr = (double)Q.re, qi = (double)Q.im;
+        double mod = sqrt(qr * qr + qi * qi);
+        if (mod != mod) nan = true;
+        if (!st->fixed) {
+          // LoadStepSize (paraxial.cl:40-58) with amin = rel_amin * max|q| (uppe_kernel.py:221)
+          double amin = st->rel_amin * bitsd(*qmax);
+          double sr = (double)S.re, si = (double)S.im;
+          double den = qr * qr + qi * qi + amin * amin;
+          double alpha = (qr * sr + qi * si) / den;
+          double keff = (qr * si - qi * sr) / den;
+          double Kr = 0.5 * st->dphi / st->L + fabs(keff) - alpha * (double)(alpha < 0.0);
+          double d = st->dphi / Kr;
+          loc = fmin(loc, d);
+        }
+      } else if (stage == 2) {
+        k2[idx] = S;
+      } else if (stage == 3) {
+        k3[idx] = S;
+      } else {
+        // qf accumulation of RKStage x3 + RKFinish, then the conditioner (uppe.cl:22-52)
+        T c6 = (T)(dz / 6), c3 = (T)(dz / 3);
+        cplx<T> Q = q[idx], a1 = k1[idx], a2 = k2[idx], a3 = k3[idx];
+        T re = Q.re + c6 * a1.re;
+        T im = Q.im + c6 * a1.im;
+        re = re + c3 * a2.re;
+        im = im + c3 * a2.im;
+        re = re + c3 * a3.re;
+        im = im + c3 * a3.im;
+        re = re + c6 * S.re;
+        im = im + c6 * S.im;
+        T cf = cond[f];
+        cplx<T> qn = cmk(re * cf, im * cf);
+        q[idx] = qn;
+        double m = sqrt((double)qn.re * qn.re + (double)qn.im * qn.im);
+        loc = fmax(loc, m);
+      }
+    }
+  }
+  if (EPI == EPI_NONE) return;
+  if (stage == 1 && nan) atomicOr(&st->nanflag, 1);
+  if (stage == 2 || stage == 3) return;
+  if (stage == 1 && st->fixed) return;
+  // block reduction: min of the dz estimate (stage 1) or max |q_new| (stage 4)
+  const bool is_min = stage == 1;
+  for (int o = 32; o > 0; o >>= 1) {
+    double v = __shfl_xor(loc, o);
+    loc = is_min ? fmin(loc, v) : fmax(loc, v);
+  }
+  if ((tid & 63) == 0) red[tid >> 6] = loc;
+  __syncthreads();
+  if (tid == 0) {
+    double v = red[0];
+    for (int i = 1; i < 4; i++) v = is_min ? fmin(v, red[i]) : fmax(v, red[i]);
+    if (is_min)
+      atomicMin(&st->dzraw, dbits(v));
+    else
+      atomicMax(qmax_next, dbits(v));
+  }
+}
+
+// ---------------------------------------------------------------------------------------------------
+// source kernel: one workgroup per strip
+// ---------------------------------------------------------------------------------------------------
+template <typename T, int SPT, bool DIAG>
+__global__ __launch_bounds__(256) void f_source(const cplx<T>* __restrict__ er, cplx<T>* __restrict__ jr,
+                                                T* __restrict__ chi_out, T* __restrict__ ne_out,
+                                                const cplx<T>* __restrict__ tw, const T* __restrict__ w,
+                                                const T* __restrict__ nlf, const double* __restrict__ ng, int nt,
+                                                int lgn, int nwp, double redchi3, double dt, uppe_ionization ion,
+                                                const StepState* __restrict__ st) {
+  if (st && !st->active) return;
+  extern __shared__ char smem[];
+  cplx<T>* Y = reinterpret_cast<cplx<T>*>(smem);
+  __shared__ double carry[256];
+  const int N = nt, M = nt / 2, lgm = lgn - 1;
+  const long p = blockIdx.x;
+  const int tid = threadIdx.x, nth = blockDim.x;
+  const cplx<T>* X = er + p * nwp;
+
+  // 1. irfft (numpy semantics; imaginary parts of the DC and Nyquist bins are ignored, fft.cl:271-278)
+  for (int k = tid; k < M; k += nth) {
+    cplx<T> xk = X[k], xm = X[M - k];
+    if (k == 0) {
+      xk.im = T(0);
+      xm.im = T(0);
+    }
+    cplx<T> ze = cscale(cadd(xk, cconj(xm)), T(0.5));
+    cplx<T> zo = cmul(cscale(csub(xk, cconj(xm)), T(0.5)), cconj(tw[k]));
+    Y[k] = cmk(ze.re - zo.im, ze.im + zo.re);
+  }
+  __syncthreads();
+  lds_fft_radix2(Y, M, lgm, tw, N, +1);
+  const T invm = T(1) / T(M);
+  const T* ys = reinterpret_cast<const T*>(Y);
+  const int a0 = tid * SPT;
+  T e[SPT + 1];
+#pragma unroll
+  for (int j = 0; j <= SPT; j++) {
+    int n = a0 + j;
+    e[j] = n < N ? ys[n] * invm : T(0);
+  }
+  __syncthreads();
+
+  // 2. rate, backward trapezoid with depletion (ionization.cl:127-155), soft window (fft.cl:331-355)
+  const double ngp = ng[p];
+  double ne[SPT];
+  if (ion.kind != UPPE_ION_NONE) {
+    double r[SPT + 1];
+#pragma unroll
+    for (int j = 0; j <= SPT; j++) {
+      int n = a0 + j;
+      double ea = fabs((double)e[j]);
+      double v;
+      if (ion.kind == UPPE_ION_EXPLICIT) {
+        double emag = ion.e_conv * ea + ion.e_cutoff;
+        v = ion.rate_conv * ion.c_pre * pow(emag, ion.c_pow) * exp(ion.c_exp / emag);
+      } else {
+        double lg = log(ion.e_conv * ea + ion.e_cutoff);
+        v = ion.rate_conv * exp(ion.c[0] + ion.c[1] * lg + ion.c[2] * lg * lg + ion.c[3] * (lg * lg * lg) +
+                                ion.c[4] * (lg * lg * lg * lg) + ion.c[5] * (lg * lg * lg * lg * lg));
+      }
+      r[j] = (n >= N - 1) ? 0.0 : v;  // last sample zeroed (ionization.cl:147); r[N] is the initial 0
+    }
+    double s = 0.0;
+#pragma unroll
+    for (int j = SPT - 1; j >= 0; j--) {
+      s += 0.5 * (r[j] + r[j + 1]) * dt;
+      ne[j] = s;
+    }
+    carry[tid] = s;
+    __syncthreads();
+    // inclusive suffix scan of the per-thread totals
+    for (int o = 1; o < nth; o <<= 1) {
+      double v = (tid + o < nth) ? carry[tid + o] : 0.0;
+      __syncthreads();
+      carry[tid] += v;
+      __syncthreads();
+    }
+    double c = (tid + 1 < nth) ? carry[tid + 1] : 0.0;
+#pragma unroll
+    for (int j = 0; j < SPT; j++) {
+      int n = a0 + j;
+      double v = ngp * (1.0 - exp(-(ne[j] + c)));
+      // SoftTimeWindow(4, 64)
+      if (n < 4 || n > N - 5) v = 0.0;
+      if (n >= 4 && n < 68) {
+        double qq = (double)(n - 4) / 64.0;
+        v *= 10 * qq * qq * qq - 15 * qq * qq * qq * qq + 6 * qq * qq * qq * qq * qq;
+      }
+      if (n <= N - 5 && n > N - 69) {
+        double qq = (double)(N - 5 - n) / 64.0;
+        v *= 10 * qq * qq * qq - 15 * qq * qq * qq * qq + 6 * qq * qq * qq * qq * qq;
+      }
+      ne[j] = v;
+    }
+  } else {
+#pragma unroll
+    for (int j = 0; j < SPT; j++) ne[j] = 0.0;
+  }
+
+  // 3. Kerr chi = ng chi3/nref E^2 (uppe.cl:132-151), P = chi E, J_p = ne E, packed as P + i J_p
+  const T kc = (T)(ngp * redchi3);
+#pragma unroll
+  for (int j = 0; j < SPT; j++) {
+    int n = a0 + j;
+    T chi = kc * e[j] * e[j];
+    T nej = (T)ne[j];
+    Y[n] = cmk(chi * e[j], nej * e[j]);
+    if (DIAG) {
+      chi_out[p * N + n] = chi;
+      ne_out[p * N + n] = nej;
+    }
+  }
+  __syncthreads();
+  lds_fft_radix2(Y, N, lgn, tw, N, -1);
+
+  // 4. J = F_NL [ rfft(J_p)/(0.1 - i w) + (-i w) rfft(P) ] (fft.cl:282-329, uppe_kernel.py:170-186)
+  for (int k = tid; k <= M; k += nth) {
+    cplx<T> yk = Y[k], ym = Y[(N - k) & (N - 1)];
+    cplx<T> pk = cscale(cadd(yk, cconj(ym)), T(0.5));
+    cplx<T> d = csub(yk, cconj(ym));
+    cplx<T> jp = cmk(T(0.5) * d.im, T(-0.5) * d.re);
+    T wk = w[k];
+    cplx<T> out;
+    if (ion.kind != UPPE_ION_NONE) {
+      T cr = T(0.1), ci = -wk;
+      T den = cr * cr + ci * ci;
+      out = cmk((jp.re * cr + jp.im * ci) / den, (jp.im * cr - jp.re * ci) / den);
+      out = cadd(out, cmk(wk * pk.im, -wk * pk.re));
+    } else {
+      out = cmk(wk * pk.im, -wk * pk.re);
+    }
+    jr[p * nwp + k] = cscale(out, nlf[k]);
+  }
+}
+
+// ---------------------------------------------------------------------------------------------------
+// control, layout and reduction kernels
+// ---------------------------------------------------------------------------------------------------
+
+// step size after the first RHS (uppe_kernel.py:221-227)
+__global__ void f_ctl_dz(StepState* st, const double* sched, double* rec) {
+  if (!st->active) return;
+  double dz;
+  if (st->sched_n > 0) {
+    if (st->sched_pos >= st->sched_n) {
+      st->error = 1;
+      st->active = 0;
+      return;
+    }
+    dz = sched[st->sched_pos++];
+  } else if (st->fixed) {
+    dz = st->L - st->z;
+  } else {
+    dz = bitsd(st->dzraw);
+    if (dz < st->dzmin) dz = st->dzmin;
+    if (st->z + dz > st->L) dz = st->L - st->z;
+  }
+  if (rec && st->steps < st->rec_cap) rec[st->steps] = dz;
+  st->dz = dz;
+}
+
+// end of step: advance z, subcycle bookkeeping (uppe_kernel.py:299-325)
+__global__ void f_ctl_end(StepState* st, unsigned long long* qmax, unsigned long long* qmax_next) {
+  if (!st->active) return;
+  st->z = st->z + st->dz;
+  st->steps++;
+  if (st->nanflag) {
+    st->nan_steps++;
+    st->nanflag = 0;
+  }
+  *qmax = *qmax_next;
+  *qmax_next = 0ull;
+  st->dzraw = dbits(INFINITY);
+  const double z = st->z, L = st->L;
+  if (st->max_steps > 0 && st->steps >= st->max_steps) {
+    st->z_last = z;
+    st->active = 0;
+    return;
+  }
+  if (!(z < L && (z - L) * (z - L) > (L / 1e6) * (L / 1e6))) {
+    st->z_last = z;
+    st->sub++;
+    if (st->sub >= st->sub_end) {
+      st->active = 0;
+    } else {
+      st->z = 0.0;
+      st->zoff = (st->zi + (double)st->sub * L) - st->zi;
+    }
+  }
+}
+
+// max |q| over the state (set_field)
+template <typename T>
+__global__ void f_qmax(const cplx<T>* q, long n, unsigned long long* out) {
+  double m = 0.0;
+  for (long i = blockIdx.x * (long)blockDim.x + threadIdx.x; i < n; i += (long)gridDim.x * blockDim.x) {
+    cplx<T> v = q[i];
+    m = fmax(m, sqrt((double)v.re * v.re + (double)v.im * v.im));
+  }
+  for (int o = 32; o > 0; o >>= 1) m = fmax(m, __shfl_xor(m, o));
+  if ((threadIdx.x & 63) == 0) atomicMax(out, dbits(m));
+}
+
+// host layout (nw, P) complex128 -> [P][nwp] complex T (zero padding)
+template <typename T>
+__global__ void f_from_host(const double* src, cplx<T>* dst, int nw, long P, int nwp) {
+  long i = blockIdx.x * (long)blockDim.x + threadIdx.x;
+  if (i >= P * nwp) return;
+  long p = i / nwp;
+  int f = (int)(i % nwp);
+  if (f < nw)
+    dst[i] = cmk((T)src[2 * (f * P + p)], (T)src[2 * (f * P + p) + 1]);
+  else
+    dst[i] = cmk(T(0), T(0));
+}
+
+template <typename T>
+__global__ void f_to_host(const cplx<T>* src, double* dst, int nw, long P, int nwp) {
+  long i = blockIdx.x * (long)blockDim.x + threadIdx.x;
+  if (i >= P * nw) return;
+  long f = i / P, p = i % P;
+  cplx<T> v = src[p * nwp + f];
+  dst[2 * i] = (double)v.re;
+  dst[2 * i + 1] = (double)v.im;
+}
+
+template <typename T>
+__global__ void f_time_to_host(const T* src, double* dst, int nt, long P) {
+  long i = blockIdx.x * (long)blockDim.x + threadIdx.x;
+  if (i >= P * nt) return;
+  long n = i / P, p = i % P;
+  dst[i] = (double)src[p * nt + n];
+}
+
+// medium tables: phs = Re kz + kg (FP64, as uppe.cl:69), kz as T; padding gets kz = 1, phs = 0
+template <typename T>
+__global__ void f_medium(const double* kz_host, const double* kg, double* phs, cplx<T>* kzc, int nw, long P,
+                         int nwp) {
+  long i = blockIdx.x * (long)blockDim.x + threadIdx.x;
+  if (i >= P * nwp) return;
+  long p = i / nwp;
+  int f = (int)(i % nwp);
+  if (f < nw) {
+    double kr = kz_host[2 * (f * P + p)], ki = kz_host[2 * (f * P + p) + 1];
+    phs[i] = kr + kg[f];
+    kzc[i] = cmk((T)kr, (T)ki);
+  } else {
+    phs[i] = 0.0;
+    kzc[i] = cmk(T(1), T(0));
+  }
+}
+
+// E = i kz A in place with kz indexed by the r-space point (finish, uppe_kernel.py:252-253)
+template <typename T>
+__global__ void f_vp_to_e(cplx<T>* a, const cplx<T>* kzc, long n) {
+  long i = blockIdx.x * (long)blockDim.x + threadIdx.x;
+  if (i >= n) return;
+  cplx<T> kz = kzc[i];
+  a[i] = cmul(cmk(-kz.im, kz.re), a[i]);
+}
+
+// ---------------------------------------------------------------------------------------------------
+// host driver
+// ---------------------------------------------------------------------------------------------------
+
+static int ilog2i(long v) {
+  int l = 0;
+  while ((1L << l) < v) l++;
+  return l;
+}
+
+template <typename K, typename... A>
+static void launchf(uppe_handle* h, int kc, K kernel, dim3 grid, dim3 block, size_t shm, A... args) {
+  ProfScope ps(h, kc);
+  hipLaunchKernelGGL(kernel, grid, block, shm, h->stream, args...);
+  UPPE_LAUNCH_CHECK();
+}
+
+template <typename T>
+struct Fused {
+  using C = cplx<T>;
+
+  static void create(uppe_handle* h, FusedState* s) {
+    const long P = h->P;
+    s->nwp = ((h->nw + 31) / 32) * 32;
+    s->el = sizeof(T);
+    const size_t cs = sizeof(C) * P * s->nwp;
+    UPPE_HIP(hipMalloc(&s->phs, sizeof(double) * P * s->nwp));
+    UPPE_HIP(hipMalloc(&s->kzc, cs));
+    UPPE_HIP(hipMalloc(&s->w, sizeof(T) * s->nwp));
+    UPPE_HIP(hipMalloc(&s->nlf, sizeof(T) * s->nwp));
+    UPPE_HIP(hipMalloc(&s->cond, sizeof(T) * s->nwp));
+    UPPE_HIP(hipMalloc(&s->ng, sizeof(double) * P));
+    void** bufs[] = {&s->q, &s->k1, &s->k2, &s->k3, &s->er, &s->jr};
+    for (auto b : bufs) {
+      UPPE_HIP(hipMalloc(b, cs));
+      UPPE_HIP(hipMemset(*b, 0, cs));
+    }
+    UPPE_HIP(hipMalloc(&s->chi_t, sizeof(T) * h->ntt));
+    UPPE_HIP(hipMalloc(&s->ne_t, sizeof(T) * h->ntt));
+    UPPE_HIP(hipMalloc(&s->stage, sizeof(double) * std::max(2 * h->npt, h->ntt)));
+    UPPE_HIP(hipMalloc(&s->qmax, sizeof(unsigned long long) * 2));
+    s->qmax_next = s->qmax + 1;
+    UPPE_HIP(hipMemset(s->qmax, 0, sizeof(unsigned long long) * 2));
+    UPPE_HIP(hipMalloc(&s->st, sizeof(StepState)));
+    UPPE_HIP(hipMemset(s->st, 0, sizeof(StepState)));
+    UPPE_HIP(hipHostMalloc(&s->st_host, sizeof(StepState), hipHostMallocDefault));
+    std::memset(s->st_host, 0, sizeof(StepState));
+    // twiddles exp(-2 pi i k/Nt), computed in FP64 and rounded to T
+    std::vector<C> t(h->nt / 2);
+    for (int k = 0; k < h->nt / 2; k++) {
+      double s_, c_;
+      sincos(2.0 * M_PI * k / h->nt, &s_, &c_);
+      t[k] = cmk((T)c_, (T)(-s_));
+    }
+    UPPE_HIP(hipMalloc(&s->tw, sizeof(C) * t.size()));
+    UPPE_HIP(hipMemcpy(s->tw, t.data(), sizeof(C) * t.size(), hipMemcpyHostToDevice));
+    if (h->d.geometry == UPPE_RADIAL) {
+      UPPE_HIP(hipMalloc(&s->Hb, sizeof(T) * (size_t)h->n1 * h->nk));
+      UPPE_HIP(hipMalloc(&s->Hf, sizeof(T) * (size_t)h->n1 * h->nk));
+    }
+  }
+
+  static void set_hankel(uppe_handle* h, FusedState* s) {
+    const int nr = h->n1, nk = h->nk;
+    std::vector<T> hb((size_t)nr * nk), hf((size_t)nk * nr);
+    for (int k = 0; k < nk; k++)
+      for (int j = 0; j < nr; j++) {
+        double v = h->H[(size_t)k * nr + j];
+        hb[(size_t)j * nk + k] = (T)(v * h->lam[k] / h->lam[j]);
+        hf[(size_t)k * nr + j] = (T)(v * h->lam[j] / h->lam[k]);
+      }
+    UPPE_HIP(hipMemcpy(s->Hb, hb.data(), sizeof(T) * hb.size(), hipMemcpyHostToDevice));
+    UPPE_HIP(hipMemcpy(s->Hf, hf.data(), sizeof(T) * hf.size(), hipMemcpyHostToDevice));
+  }
+
+  static void set_medium(uppe_handle* h, FusedState* s, const uppe_medium* m) {
+    const long P = h->P;
+    std::vector<T> w(s->nwp, T(0)), nl(s->nwp, T(0)), cd(s->nwp, T(0));
+    for (int f = 0; f < h->nw; f++) {
+      w[f] = (T)m->w[f];
+      nl[f] = (T)m->nl_filter[f];
+      cd[f] = (T)m->cond_filter[f];
+    }
+    UPPE_HIP(hipMemcpy(s->w, w.data(), sizeof(T) * s->nwp, hipMemcpyHostToDevice));
+    UPPE_HIP(hipMemcpy(s->nlf, nl.data(), sizeof(T) * s->nwp, hipMemcpyHostToDevice));
+    UPPE_HIP(hipMemcpy(s->cond, cd.data(), sizeof(T) * s->nwp, hipMemcpyHostToDevice));
+    UPPE_HIP(hipMemcpy(s->ng, m->ngas, sizeof(double) * P, hipMemcpyHostToDevice));
+    double* kg = nullptr;
+    UPPE_HIP(hipMalloc(&kg, sizeof(double) * h->nw));
+    UPPE_HIP(hipMemcpy(kg, m->kg, sizeof(double) * h->nw, hipMemcpyHostToDevice));
+    UPPE_HIP(hipMemcpy(s->stage, m->kz, sizeof(double) * 2 * h->npt, hipMemcpyHostToDevice));
+    launchf(h, UPPE_KC_POINT, f_medium<T>, dim3(grid_for(P * s->nwp)), dim3(kBlock), 0, (const double*)s->stage,
+            (const double*)kg, s->phs, (C*)s->kzc, h->nw, P, s->nwp);
+    UPPE_HIP(hipStreamSynchronize(h->stream));
+    UPPE_HIP(hipFree(kg));
+    s->redchi3 = m->redchi3;
+    s->dt = m->dt;
+    s->ion = m->ion;
+  }
+
+  // k-space <- r-space (plain transform)
+  static void r2k_plain(uppe_handle* h, FusedState* s, const C* in, C* out) {
+    dim3 grid(2 * s->nwp / BN, (h->nk + BM - 1) / BM);
+    launchf(h, UPPE_KC_HANKEL, f_gemm<T, PRO_NONE, EPI_NONE>, grid, dim3(256), 0, (const T*)s->Hf, in, out, h->nk,
+            h->n1, s->nwp, (const double*)nullptr, (const C*)nullptr, (const T*)nullptr, (C*)nullptr,
+            (const C*)nullptr, (C*)nullptr, (C*)nullptr, (C*)nullptr, (StepState*)nullptr,
+            (unsigned long long*)nullptr, (unsigned long long*)nullptr, 0, 0.0);
+  }
+
+  static void source(uppe_handle* h, FusedState* s, const C* er, C* jr, bool diag, StepState* st) {
+    const int N = h->nt;
+    const int nth = std::min(N, 256);
+    const int spt = N / nth;
+    const size_t shm = sizeof(C) * N;
+    const int lgn = ilog2i(N);
+    dim3 grid((unsigned)h->P), block(nth);
+#define UPPE_SRC(S)                                                                                              \
+  if (spt == S) {                                                                                                \
+    if (diag)                                                                                                    \
+      launchf(h, UPPE_KC_SOURCE, f_source<T, S, true>, grid, block, shm, er, jr, (T*)s->chi_t, (T*)s->ne_t,       \
+              (const C*)s->tw, (const T*)s->w, (const T*)s->nlf, (const double*)s->ng, N, lgn, s->nwp, s->redchi3, \
+              s->dt, s->ion, (const StepState*)st);                                                              \
+    else                                                                                                         \
+      launchf(h, UPPE_KC_SOURCE, f_source<T, S, false>, grid, block, shm, er, jr, (T*)nullptr, (T*)nullptr,       \
+              (const C*)s->tw, (const T*)s->w, (const T*)s->nlf, (const double*)s->ng, N, lgn, s->nwp, s->redchi3, \
+              s->dt, s->ion, (const StepState*)st);                                                              \
+    return;                                                                                                      \
+  }
+    UPPE_SRC(1) UPPE_SRC(2) UPPE_SRC(4) UPPE_SRC(8) UPPE_SRC(16)
+#undef UPPE_SRC
+    throw Error{UPPE_ERR_UNSUPPORTED, "fused source kernel: unsupported Nt"};
+  }
+
+  // one RK4 step: 4 x (k2r, source, r2k) + 2 control kernels
+  static void step(uppe_handle* h, FusedState* s) {
+    dim3 g_k2r(2 * s->nwp / BN, (h->n1 + BM - 1) / BM), g_r2k(2 * s->nwp / BN, (h->nk + BM - 1) / BM);
+    C* kp[5] = {nullptr, nullptr, (C*)s->k1, (C*)s->k2, (C*)s->k3};
+    for (int stage = 1; stage <= 4; stage++) {
+      launchf(h, UPPE_KC_HANKEL, f_gemm<T, PRO_STAGE, EPI_NONE>, g_k2r, dim3(256), 0, (const T*)s->Hb,
+              (const C*)s->q, (C*)s->er, h->n1, h->nk, s->nwp, (const double*)s->phs, (const C*)s->kzc,
+              (const T*)s->cond, (C*)nullptr, (const C*)kp[stage], (C*)nullptr, (C*)nullptr, (C*)nullptr, s->st,
+              s->qmax, s->qmax_next, stage, 0.0);
+      source(h, s, (const C*)s->er, (C*)s->jr, false, s->st);
+      launchf(h, UPPE_KC_HANKEL, f_gemm<T, PRO_NONE, EPI_STAGE>, g_r2k, dim3(256), 0, (const T*)s->Hf,
+              (const C*)s->jr, (C*)nullptr, h->nk, h->n1, s->nwp, (const double*)s->phs, (const C*)s->kzc,
+              (const T*)s->cond, (C*)s->q, (const C*)nullptr, (C*)s->k1, (C*)s->k2, (C*)s->k3, s->st, s->qmax,
+              s->qmax_next, stage, 0.0);
+      if (stage == 1)
+        launchf(h, UPPE_KC_REDUCE, f_ctl_dz, dim3(1), dim3(1), 0, s->st, (const double*)s->sched, s->rec_active);
+    }
+    launchf(h, UPPE_KC_REDUCE, f_ctl_end, dim3(1), dim3(1), 0, s->st, s->qmax, s->qmax_next);
+  }
+
+  static void set_field(uppe_handle* h, FusedState* s, const double* a_r) {
+    const long P = h->P;
+    UPPE_HIP(hipMemcpy(s->stage, a_r, sizeof(double) * 2 * h->npt, hipMemcpyHostToDevice));
+    launchf(h, UPPE_KC_POINT, f_from_host<T>, dim3(grid_for(P * s->nwp)), dim3(kBlock), 0, (const double*)s->stage,
+            (C*)s->er, h->nw, P, s->nwp);
+    UPPE_HIP(hipMemsetAsync(s->q, 0, sizeof(C) * P * s->nwp, h->stream));
+    r2k_plain(h, s, (const C*)s->er, (C*)s->q);
+    UPPE_HIP(hipMemsetAsync(s->qmax, 0, sizeof(unsigned long long) * 2, h->stream));
+    launchf(h, UPPE_KC_REDUCE, f_qmax<T>, dim3(256), dim3(kBlock), 0, (const C*)s->q, P * s->nwp, s->qmax);
+    UPPE_HIP(hipStreamSynchronize(h->stream));
+  }
+
+  static int advance(uppe_handle* h, FusedState* s, const uppe_step_control* c) {
+    StepState* sh = s->st_host;
+    std::memset(sh, 0, sizeof(StepState));
+    sh->zi = c->zi;
+    sh->L = c->length;
+    sh->dzmin = c->dzmin;
+    sh->dphi = c->dphi;
+    sh->rel_amin = c->rel_amin;
+    sh->sub = c->sub0;
+    sh->sub_end = c->sub0 + c->n_sub;
+    sh->z = 0.0;
+    sh->zoff = (c->zi + (double)c->sub0 * c->length) - c->zi;
+    sh->active = 1;
+    sh->sched_n = c->n_schedule;
+    sh->fixed = (c->n_schedule == 0 && c->dzmin >= c->length) ? 1 : 0;
+    sh->dzraw = 0x7ff0000000000000ull;  // +inf
+    sh->rec_cap = c->dz_record ? c->record_cap : 0;
+    sh->max_steps = c->max_steps > 0 ? c->max_steps : 0;
+    if (c->n_schedule > s->sched_cap) {
+      if (s->sched) UPPE_HIP(hipFree(s->sched));
+      UPPE_HIP(hipMalloc(&s->sched, sizeof(double) * c->n_schedule));
+      s->sched_cap = c->n_schedule;
+    }
+    if (c->n_schedule > 0)
+      UPPE_HIP(hipMemcpy(s->sched, c->dz_schedule, sizeof(double) * c->n_schedule, hipMemcpyHostToDevice));
+    if (sh->rec_cap > s->rec_cap) {
+      if (s->rec) UPPE_HIP(hipFree(s->rec));
+      UPPE_HIP(hipMalloc(&s->rec, sizeof(double) * sh->rec_cap));
+      s->rec_cap = sh->rec_cap;
+    }
+    double* rec_dev = sh->rec_cap > 0 ? s->rec : nullptr;
+    s->rec_active = rec_dev;
+    UPPE_HIP(hipMemcpyAsync(s->st, sh, sizeof(StepState), hipMemcpyHostToDevice, h->stream));
+    long long guard = 0;
+    int chunk = c->n_sub;
+    if (sh->max_steps > 0) chunk = std::min(chunk, sh->max_steps);
+    while (true) {
+      for (int i = 0; i < chunk; i++) step(h, s);
+      UPPE_HIP(hipMemcpyAsync(sh, s->st, sizeof(StepState), hipMemcpyDeviceToHost, h->stream));
+      UPPE_HIP(hipStreamSynchronize(h->stream));
+      guard += chunk;
+      if (!sh->active) break;
+      if (guard > 100000000LL) throw Error{UPPE_ERR_STATE, "step loop did not terminate"};
+      // remaining subcycles times the observed steps per subcycle
+      int done_sub = sh->sub - c->sub0;
+      double per = done_sub > 0 ? (double)sh->steps / done_sub : (double)sh->steps + 1.0;
+      chunk = std::max(1, (int)std::min(1e6, per * (sh->sub_end - sh->sub) * 0.5));
+      if (sh->max_steps > 0) chunk = std::max(1, std::min(chunk, sh->max_steps - sh->steps));
+    }
+    if (sh->error) throw Error{UPPE_ERR_ARG, "dz schedule exhausted"};
+    if (rec_dev && c->dz_record) {
+      int n = std::min(sh->steps, sh->rec_cap);
+      UPPE_HIP(hipMemcpy(c->dz_record, rec_dev, sizeof(double) * n, hipMemcpyDeviceToHost));
+    }
+    h->steps += sh->steps;
+    h->evals += 4LL * sh->steps;
+    h->nan_steps += sh->nan_steps;
+    h->z_last = sh->z_last;
+    return sh->steps;
+  }
+
+  static void finish(uppe_handle* h, FusedState* s, double z_total, double* a_r, double* j_r, double* chi_t,
+                     double* ne_t) {
+    const long P = h->P, n = P * s->nwp;
+    dim3 g_k2r(2 * s->nwp / BN, (h->n1 + BM - 1) / BM);
+    launchf(h, UPPE_KC_HANKEL, f_gemm<T, PRO_FINISH, EPI_NONE>, g_k2r, dim3(256), 0, (const T*)s->Hb,
+            (const C*)s->q, (C*)s->er, h->n1, h->nk, s->nwp, (const double*)s->phs, (const C*)s->kzc,
+            (const T*)s->cond, (C*)nullptr, (const C*)nullptr, (C*)nullptr, (C*)nullptr, (C*)nullptr,
+            (StepState*)nullptr, (unsigned long long*)nullptr, (unsigned long long*)nullptr, 0, z_total);
+    UPPE_HIP(hipMemcpyAsync(s->k1, s->er, sizeof(C) * n, hipMemcpyDeviceToDevice, h->stream));
+    launchf(h, UPPE_KC_POINT, f_vp_to_e<T>, dim3(grid_for(n)), dim3(kBlock), 0, (C*)s->k1, (const C*)s->kzc, n);
+    source(h, s, (const C*)s->k1, (C*)s->jr, true, nullptr);
+    auto out_c = [&](const void* src, double* dst) {
+      launchf(h, UPPE_KC_POINT, f_to_host<T>, dim3(grid_for(P * h->nw)), dim3(kBlock), 0, (const C*)src, s->stage,
+              h->nw, P, s->nwp);
+      UPPE_HIP(hipMemcpyAsync(dst, s->stage, sizeof(double) * 2 * h->npt, hipMemcpyDeviceToHost, h->stream));
+      UPPE_HIP(hipStreamSynchronize(h->stream));
+    };
+    auto out_t = [&](const void* src, double* dst) {
+      launchf(h, UPPE_KC_POINT, f_time_to_host<T>, dim3(grid_for(P * h->nt)), dim3(kBlock), 0, (const T*)src,
+              s->stage, h->nt, P);
+      UPPE_HIP(hipMemcpyAsync(dst, s->stage, sizeof(double) * h->ntt, hipMemcpyDeviceToHost, h->stream));
+      UPPE_HIP(hipStreamSynchronize(h->stream));
+    };
+    if (a_r) out_c(s->er, a_r);
+    if (j_r) out_c(s->jr, j_r);
+    if (chi_t) out_t(s->chi_t, chi_t);
+    if (ne_t) out_t(s->ne_t, ne_t);
+    UPPE_HIP(hipStreamSynchronize(h->stream));
+  }
+
+  static void get_state(uppe_handle* h, FusedState* s, double* q) {
+    const long P = h->P;
+    launchf(h, UPPE_KC_POINT, f_to_host<T>, dim3(grid_for(P * h->nw)), dim3(kBlock), 0, (const C*)s->q, s->stage,
+            h->nw, P, s->nwp);
+    UPPE_HIP(hipMemcpyAsync(q, s->stage, sizeof(double) * 2 * h->npt, hipMemcpyDeviceToHost, h->stream));
+    UPPE_HIP(hipStreamSynchronize(h->stream));
+  }
+};
+
+static bool is_fp64(uppe_handle* h) { return h->fused->prec == UPPE_PREC_FP64; }
+
+void fused_create(uppe_handle* h) {
+  if (h->d.geometry != UPPE_RADIAL || h->n2 != 1)
+    throw Error{UPPE_ERR_UNSUPPORTED, "fused path: radial geometry with one azimuthal mode only"};
+  if (h->nt > kMaxNtFused || h->nt < 4) throw Error{UPPE_ERR_UNSUPPORTED, "fused path supports 4 <= Nt <= 4096"};
+  auto* s = new FusedState();
+  h->fused = s;
+  s->prec = h->d.precision;
+  if (s->prec == UPPE_PREC_BF16X3) s->prec = UPPE_PREC_FP32;  // BF16x3 Hankel contraction not yet enabled
+  if (is_fp64(h))
+    Fused<double>::create(h, s);
+  else
+    Fused<float>::create(h, s);
+}
+
+void fused_destroy(uppe_handle* h) {
+  FusedState* s = h->fused;
+  if (!s) return;
+  void* ptrs[] = {s->phs, s->kzc, s->w, s->nlf, s->cond, s->ng, s->Hb, s->Hf, s->tw, s->q, s->k1, s->k2, s->k3,
+                  s->er, s->jr, s->chi_t, s->ne_t, s->stage, s->qmax, s->st, s->sched, s->rec};
+  for (void* p : ptrs)
+    if (p) (void)hipFree(p);
+  if (s->st_host) (void)hipHostFree(s->st_host);
+  delete s;
+  h->fused = nullptr;
+}
+
+void fused_set_hankel(uppe_handle* h) {
+  if (is_fp64(h))
+    Fused<double>::set_hankel(h, h->fused);
+  else
+    Fused<float>::set_hankel(h, h->fused);
+}
+
+void fused_set_medium(uppe_handle* h, const uppe_medium* m) {
+  if (is_fp64(h))
+    Fused<double>::set_medium(h, h->fused, m);
+  else
+    Fused<float>::set_medium(h, h->fused, m);
+}
+
+void fused_set_field(uppe_handle* h, const double* a_r) {
+  if (is_fp64(h))
+    Fused<double>::set_field(h, h->fused, a_r);
+  else
+    Fused<float>::set_field(h, h->fused, a_r);
+}
+
+int fused_advance(uppe_handle* h, const uppe_step_control* c) {
+  return is_fp64(h) ? Fused<double>::advance(h, h->fused, c) : Fused<float>::advance(h, h->fused, c);
+}
+
+void fused_finish(uppe_handle* h, double z_total, double* a_r, double* j_r, double* chi_t, double* ne_t) {
+  if (is_fp64(h))
+    Fused<double>::finish(h, h->fused, z_total, a_r, j_r, chi_t, ne_t);
+  else
+    Fused<float>::finish(h, h->fused, z_total, a_r, j_r, chi_t, ne_t);
+}
+
+void fused_get_state(uppe_handle* h, double* q) {
+  if (is_fp64(h))
+    Fused<double>::get_state(h, h->fused, q);
+  else
+    Fused<float>::get_state(h, h->fused, q);
+}
+
+}  // namespace uppe

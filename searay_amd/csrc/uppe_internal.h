@@ -1,0 +1,176 @@
+// uppe_internal.h -- shared declarations of the MI355X UPPE stepper library (not part of the public ABI).
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+#include <cmath>
+#include <cstdint>
+#include <cstdio>
+#include <string>
+#include <vector>
+
+#include "../../include/searay_uppe.h"
+
+namespace uppe {
+
+constexpr int kBlock = 256;        // threads per workgroup for point and strip kernels (4 wavefronts)
+constexpr int kMaxNtFused = 4096;  // fused source kernel keeps one Nt-point complex strip in LDS
+constexpr int kMaxNtPlain = 16384; // plain strip FFT keeps Nt/2 complex points in LDS (FP64)
+
+struct Error {
+  int code;
+  std::string msg;
+};
+
+#define UPPE_HIP(call)                                                                          \
+  do {                                                                                          \
+    hipError_t e_ = (call);                                                                     \
+    if (e_ != hipSuccess)                                                                       \
+      throw ::uppe::Error{UPPE_ERR_DEVICE, std::string(#call) + ": " + hipGetErrorString(e_)}; \
+  } while (0)
+
+#define UPPE_LAUNCH_CHECK() UPPE_HIP(hipGetLastError())
+
+// ---------------------------------------------------------------------------------------------------
+// complex helpers (interleaved re, im)
+// ---------------------------------------------------------------------------------------------------
+template <typename T>
+struct cplx {
+  T re, im;
+};
+
+template <typename T>
+__host__ __device__ inline cplx<T> cmk(T r, T i) {
+  return cplx<T>{r, i};
+}
+template <typename T>
+__host__ __device__ inline cplx<T> cmul(cplx<T> a, cplx<T> b) {
+  return cplx<T>{a.re * b.re - a.im * b.im, a.re * b.im + a.im * b.re};
+}
+template <typename T>
+__host__ __device__ inline cplx<T> cadd(cplx<T> a, cplx<T> b) {
+  return cplx<T>{a.re + b.re, a.im + b.im};
+}
+template <typename T>
+__host__ __device__ inline cplx<T> csub(cplx<T> a, cplx<T> b) {
+  return cplx<T>{a.re - b.re, a.im - b.im};
+}
+template <typename T>
+__host__ __device__ inline cplx<T> cconj(cplx<T> a) {
+  return cplx<T>{a.re, -a.im};
+}
+template <typename T>
+__host__ __device__ inline cplx<T> cscale(cplx<T> a, T s) {
+  return cplx<T>{a.re * s, a.im * s};
+}
+
+// ---------------------------------------------------------------------------------------------------
+// Block-cooperative radix-2 FFT in LDS. Data is n = 2^lg complex points (interleaved) at `buf`.
+// Twiddle table tw[k] = exp(-2 pi i k / Nbase), k < Nbase/2, with Nbase >= 2n; sign = -1 forward,
+// +1 inverse; no normalisation. All threads of the block must call it; it ends with a barrier.
+// ---------------------------------------------------------------------------------------------------
+template <typename T>
+__device__ inline void lds_fft_radix2(cplx<T>* buf, int n, int lg, const cplx<T>* __restrict__ tw, int nbase,
+                                      int sign) {
+  if (n <= 1) return;
+  for (int i = threadIdx.x; i < n; i += blockDim.x) {
+    int j = (int)(__brev((unsigned)i) >> (32 - lg));
+    if (j > i) {
+      cplx<T> t = buf[i];
+      buf[i] = buf[j];
+      buf[j] = t;
+    }
+  }
+  __syncthreads();
+  for (int s = 0; s < lg; s++) {
+    const int half = 1 << s;
+    const int tstep = nbase >> (s + 1);  // exp(-2 pi i pos / (2 half)) = tw[pos * nbase/(2 half)]
+    for (int b = threadIdx.x; b < (n >> 1); b += blockDim.x) {
+      const int pos = b & (half - 1);
+      const int a = ((b >> s) << (s + 1)) + pos;
+      const int c = a + half;
+      cplx<T> w = tw[pos * tstep];
+      if (sign > 0) w.im = -w.im;
+      cplx<T> u = buf[a];
+      cplx<T> v = cmul(buf[c], w);
+      buf[a] = cadd(u, v);
+      buf[c] = csub(u, v);
+    }
+    __syncthreads();
+  }
+}
+
+// ---------------------------------------------------------------------------------------------------
+// Profiling of kernel classes with HIP events on the handle's stream
+// ---------------------------------------------------------------------------------------------------
+struct ProfRecord {
+  int kc;
+  hipEvent_t a, b;
+};
+
+struct PlainState;
+struct FusedState;
+
+}  // namespace uppe
+
+struct uppe_handle {
+  uppe_desc d;
+  int nw = 0, n1 = 0, n2 = 0, nk = 0, nm = 0, nt = 0;
+  long P = 0;     // transverse points n1*n2
+  long npt = 0;   // nw * P
+  long ntt = 0;   // nt * P
+  hipStream_t stream = nullptr;
+  std::string err;
+  bool have_hankel = false, have_medium = false, have_field = false;
+  long long steps = 0, evals = 0;
+  int nan_steps = 0;
+  double z_last = 0.0;
+  // host copies of the tables (the reference keeps these on the host too)
+  std::vector<double> H, lam, w;
+  // profiling
+  bool prof = false;
+  std::vector<uppe::ProfRecord> prof_pending;
+  std::vector<hipEvent_t> prof_free;
+  double prof_ms[UPPE_KC_COUNT] = {0};
+  long long prof_n[UPPE_KC_COUNT] = {0};
+  // implementation state (exactly one is non-null)
+  uppe::PlainState* plain = nullptr;
+  uppe::FusedState* fused = nullptr;
+};
+
+namespace uppe {
+
+// event pair around one launch when profiling is on
+struct ProfScope {
+  uppe_handle* h;
+  int kc;
+  hipEvent_t a = nullptr, b = nullptr;
+  ProfScope(uppe_handle* h_, int kc_);
+  ~ProfScope() noexcept(false);
+};
+
+void prof_collect(uppe_handle* h);  // synchronise and fold finished events into totals
+
+inline unsigned grid_for(long n, int block = kBlock) { return (unsigned)((n + block - 1) / block); }
+
+// plain FP64 path (reference layout and pass structure)
+void plain_create(uppe_handle* h);
+void plain_destroy(uppe_handle* h);
+void plain_set_hankel(uppe_handle* h);
+void plain_set_medium(uppe_handle* h, const uppe_medium* m);
+void plain_set_field(uppe_handle* h, const double* a_r);
+int plain_advance(uppe_handle* h, const uppe_step_control* c);
+void plain_finish(uppe_handle* h, double z_total, double* a_r, double* j_r, double* chi_t, double* ne_t);
+void plain_get_state(uppe_handle* h, double* q);
+
+// fused path (strip-major layout, FP64 / FP32 / BF16x3 Hankel)
+void fused_create(uppe_handle* h);
+void fused_destroy(uppe_handle* h);
+void fused_set_hankel(uppe_handle* h);
+void fused_set_medium(uppe_handle* h, const uppe_medium* m);
+void fused_set_field(uppe_handle* h, const double* a_r);
+int fused_advance(uppe_handle* h, const uppe_step_control* c);
+void fused_finish(uppe_handle* h, double z_total, double* a_r, double* j_r, double* chi_t, double* ne_t);
+void fused_get_state(uppe_handle* h, double* q);
+
+}  // namespace uppe
