@@ -32,6 +32,7 @@ def library():
         lib.orc_uppe_source.argtypes = [ct.c_void_p, dp, dp, dp, dp]
         lib.orc_uppe_transverse.argtypes = [ct.c_void_p, dp, ct.c_int]
         lib.orc_num_threads.restype = ct.c_int
+        lib.orc_uppe_set_linear_rhs.argtypes = [ct.c_void_p, ct.c_int, ct.c_double, ct.c_double]
         _lib = lib
     return _lib
 
@@ -61,6 +62,11 @@ class OracleStepper(_abi.Stepper):
         self._check(self.lib.orc_uppe_source(self.h, _p(e_r.view(np.float64)), _p(j.view(np.float64)), _p(chi),
                                              _p(ne)), 'source')
         return j, chi, ne
+
+    def set_linear_rhs(self, lam, enable=True):
+        '''Test hook: the stepper integrates dq/dz = lam q with the same RK4 and step control.'''
+        self._check(self.lib.orc_uppe_set_linear_rhs(self.h, int(enable), float(np.real(lam)), float(np.imag(lam))),
+                    'set_linear_rhs')
 
     def transverse(self, a, direction):
         a = np.array(a, dtype=np.complex128, copy=True, order='C')

@@ -179,6 +179,8 @@ typedef struct orc_handle {
   double *qw, *qi, *qf, *Ew, *Jw, *scratchw, *hscratch, *dzb;
   double *Et, *ne, *chi, *Jt;
   int have_field, have_medium, have_hankel;
+  int linear_rhs;          /* test hook: S = lambda q in place of the UPPE right-hand side */
+  double lam_re, lam_im;
   long long steps, evals;
   int nan_steps;
   double z_last;
@@ -548,13 +550,21 @@ static double reduce_min(const double* v, long n) {
 /* RHS S(z, qw) into Jw; with return_dz also the step estimate (uppe_kernel.py:188-232) */
 static double load_source(orc_handle* h, double zphase, int return_dz, double zlocal, double L, double dzmin,
                           double dphi, double rel_amin) {
-  memcpy(h->Ew, h->qw, sizeof(double) * 2 * h->npt);
-  k_propagate_linear(h, h->Ew, zphase);
-  k_vp_to_e(h, h->Ew);
-  rspacex(h, h->Ew);
-  update_current(h);
-  kspacex(h, h->Jw);
-  k_current_to_rhs(h, h->Jw, zphase);
+  if (h->linear_rhs) {
+    for (long i = 0; i < h->npt; i++) {
+      double qr = h->qw[2 * i], qi = h->qw[2 * i + 1];
+      h->Jw[2 * i] = h->lam_re * qr - h->lam_im * qi;
+      h->Jw[2 * i + 1] = h->lam_re * qi + h->lam_im * qr;
+    }
+  } else {
+    memcpy(h->Ew, h->qw, sizeof(double) * 2 * h->npt);
+    k_propagate_linear(h, h->Ew, zphase);
+    k_vp_to_e(h, h->Ew);
+    rspacex(h, h->Ew);
+    update_current(h);
+    kspacex(h, h->Jw);
+    k_current_to_rhs(h, h->Jw, zphase);
+  }
   h->evals++;
   if (!return_dz) return 0.0;
   k_load_modulus(h);
@@ -787,6 +797,16 @@ int orc_uppe_transverse(orc_handle* h, double* a, int dir) {
     kspacex(h, a);
   else
     rspacex(h, a);
+  return UPPE_OK;
+}
+
+/* test hook: replace the UPPE right-hand side by S = lambda q, keeping the RK4 loop, conditioner and
+ * step-size control; enable = 0 restores the UPPE right-hand side */
+int orc_uppe_set_linear_rhs(orc_handle* h, int enable, double lam_re, double lam_im) {
+  if (!h) return UPPE_ERR_ARG;
+  h->linear_rhs = enable;
+  h->lam_re = lam_re;
+  h->lam_im = lam_im;
   return UPPE_OK;
 }
 
