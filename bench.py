@@ -1,0 +1,192 @@
+'''Benchmark of the MI355X UPPE stepper on the radial photoionisation workload.
+
+Workload: radial UPPE with Kerr, O2 PPT photoionisation (Ip = 12.07 eV, 5.4e18 cm^-3) and the plasma current;
+800 nm, 40 fs, w0 = 100 um, 3 P_cr; Nw = 2049 frequencies over (0, 8), Nr = 256 Hankel modes; 400 adaptive
+subcycles over 20 cm. A "step" is one adaptive RK4 z-step (four right-hand-side evaluations) over the whole
+grid; the timed region runs the first K steps of that run from the initial field, on inputs resident in HBM.
+
+Printed JSON line (one):
+  value         grid-point z-steps per second = Nw * Nr * K / T
+  roofline      the dominant kernel class (largest share of kernel time) against its bound; kernel durations
+                come from HIP events recorded on the library's stream around every launch, in a repeat of the
+                timed steps (the events themselves lengthen the step, so `value` is taken without them)
+  cpu_baseline  the FP64 oracle (reference algorithm and pass structure, OpenMP) on a bounded sample
+  extras        speedup over the plain FP64 GPU stepper, relative L2 error of the state against the fused FP64
+                stepper on the same step schedule
+'''
+import argparse
+import json
+import os
+import sys
+import time
+
+import numpy as np
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+if ROOT not in sys.path:
+    sys.path.insert(0, ROOT)
+
+from searay_amd import configs, uppe_kernel  # noqa: E402
+
+METRIC = 'UPPE grid-pt z-steps/s (1 GPU) vs FP64 plain at 1e-5 rel err; adjoint/fwd cost'
+HBM_PEAK_GBS = 8000.0
+PEAK_TFLOPS = {'fp64': 78.6, 'fp32': 157.3, 'bf16x3': 2500.0}
+DTYPE = {'fp64': 'f64', 'fp32': 'f32', 'bf16x3': 'f32 (Hankel: bf16x3 MFMA, f32 accumulate)', 'fp64-plain': 'f64'}
+
+
+def log(msg):
+    print('[bench] ' + msg, file=sys.stderr, flush=True)
+
+
+def run_steps(rs, st, K, schedule=None, record=False):
+    zi, zf, L = rs.plane(0)
+    st.set_field(rs.A0)
+    st.synchronize()
+    t0 = time.perf_counter()
+    n, rec = st.advance(zi, L, 0, rs.subcycles, rs.dzmin, max_steps=K, dz_schedule=schedule,
+                        record_cap=K if record else 0)
+    st.synchronize()
+    t1 = time.perf_counter()
+    return n, t1 - t0, rec
+
+
+def hankel_flops(rs):
+    '''Algorithmic flops of one radial transform launch: 4 Nw Nk Nr (real contraction of complex data).'''
+    T = rs.field_tool.GetTransverseTool()
+    return 4.0 * rs.N[0] * T.Nk * rs.N[1]
+
+
+def source_bytes(rs, el):
+    '''Algorithmic bytes of one source launch: read E_r and write J_r (Nw complex per strip).'''
+    return 2.0 * rs.N[0] * rs.N[1] * 2 * el
+
+
+def cpu_baseline(vol, steps):
+    from oracle import oracle
+    rs = uppe_kernel.RunSetup(vol)
+    st = rs.make_stepper(oracle.oracle_factory)
+    zi, zf, L = rs.plane(0)
+    st.set_field(rs.A0)
+    st.advance(zi, L, 0, rs.subcycles, rs.dzmin, max_steps=1)
+    st.set_field(rs.A0)
+    t0 = time.perf_counter()
+    n, _ = st.advance(zi, L, 0, rs.subcycles, rs.dzmin, max_steps=steps)
+    dt = time.perf_counter() - t0
+    st.close()
+    return rs.N[0] * rs.N[1] * n / dt, n, dt, oracle.num_threads()
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument('--gpus', type=int, default=1)
+    ap.add_argument('--steps', type=int, default=400)
+    ap.add_argument('--warmup', type=int, default=20)
+    ap.add_argument('--precision', default='fp32', choices=['bf16x3', 'fp32', 'fp64', 'fp64-plain'])
+    ap.add_argument('--rate', default='explicit', choices=['explicit', 'series'])
+    ap.add_argument('--plain-steps', type=int, default=20)
+    ap.add_argument('--cpu-steps', type=int, default=10)
+    ap.add_argument('--no-cpu', action='store_true')
+    ap.add_argument('--no-compare', action='store_true')
+    args = ap.parse_args()
+    if args.gpus != 1:
+        print('bench.py runs on one GPU only: this stepper has no multi-GPU path (use --gpus 1)', file=sys.stderr)
+        sys.exit(2)
+
+    vol = configs.radial_o2_ionization(rate=args.rate)
+    rs = uppe_kernel.RunSetup(vol)
+    pts = rs.N[0] * rs.N[1]
+    K, W = args.steps, args.warmup
+    prec = args.precision
+
+    st = rs.make_stepper(uppe_kernel.hip_stepper_factory(prec))
+    log('warmup %d steps (%s)' % (W, prec))
+    if W > 0:
+        run_steps(rs, st, W)
+    log('timed region: %d steps' % K)
+    n, secs, _ = run_steps(rs, st, K)
+    value = pts * n / secs
+    ms_per_step = 1e3 * secs / n
+    info = st.info()
+    log('%d steps in %.3f s: %.3f ms/step, %.4e pt-steps/s, NaN steps %d' % (n, secs, ms_per_step, value,
+                                                                           info['nan_steps']))
+    log('same %d steps again with HIP events around every launch' % K)
+    st.profile_enable(True)
+    n2, secs2, _ = run_steps(rs, st, K)
+    classes = {}
+    for kc in ('hankel', 'source', 'point', 'reduce', 'transverse_fft'):
+        ms, cnt = st.profile_read(kc)
+        classes[kc] = (ms, cnt)
+    st.profile_enable(False)
+    st.close()
+
+    dom = max(classes, key=lambda k: classes[k][0])
+    ms, cnt = classes[dom]
+    avg_s = ms / max(cnt, 1) * 1e-3
+    el = 8 if prec in ('fp64', 'fp64-plain') else 4
+    if dom == 'hankel':
+        F = hankel_flops(rs)
+        pk = PEAK_TFLOPS['fp64' if el == 8 else ('bf16x3' if prec == 'bf16x3' else 'fp32')]
+        issued = 3.0 if prec == 'bf16x3' else 1.0
+        achieved = F * issued / avg_s / 1e12
+        roofline = dict(bound='mfma', achieved=achieved, peak=pk, unit='TFLOP/s', frac=achieved / pk,
+                        traffic=None, kernel='hankel (radial transform GEMM)', avg_launch_us=avg_s * 1e6,
+                        launches=cnt, work_per_launch='%.4g flop (4 Nw Nk Nr%s)'
+                        % (F * issued, ', x3 bf16 products' if issued > 1 else ''))
+    else:
+        B = source_bytes(rs, el)
+        achieved = B / avg_s / 1e9
+        roofline = dict(bound='hbm', achieved=achieved, peak=HBM_PEAK_GBS, unit='GB/s', frac=achieved / HBM_PEAK_GBS,
+                        traffic=None, kernel=dom, avg_launch_us=avg_s * 1e6, launches=cnt,
+                        work_per_launch='%.4g B' % B)
+    share = {k: round(v[0] / max(1e-12, sum(c[0] for c in classes.values())), 4) for k, v in classes.items()
+             if v[1] > 0}
+    roofline['class_share'] = share
+
+    extras = {}
+    if not args.no_compare and prec != 'fp64-plain':
+        log('plain FP64 stepper, %d steps' % args.plain_steps)
+        sp = rs.make_stepper(uppe_kernel.hip_stepper_factory('fp64-plain'))
+        npl, tpl, _ = run_steps(rs, sp, min(args.plain_steps, K))
+        sp.close()
+        plain_value = pts * npl / tpl
+        extras['fp64_plain_pt_steps_per_s'] = plain_value
+        extras['speedup_vs_fp64_plain'] = value / plain_value
+        log('fused FP64 reference for the error, %d steps' % K)
+        s64 = rs.make_stepper(uppe_kernel.hip_stepper_factory('fp64'))
+        n64, t64, sched = run_steps(rs, s64, K, record=True)
+        q64 = s64.get_state()
+        s64.close()
+        extras['fp64_fused_pt_steps_per_s'] = pts * n64 / t64
+        sx = rs.make_stepper(uppe_kernel.hip_stepper_factory(prec))
+        nx, tx, _ = run_steps(rs, sx, K, schedule=sched)
+        qx = sx.get_state()
+        sx.close()
+        extras['rel_l2_err_vs_fp64'] = float(np.linalg.norm(qx - q64) / np.linalg.norm(q64))
+        extras['error_note'] = 'k-space state after %d steps on the FP64 step schedule' % K
+
+    cpu = None
+    if not args.no_cpu:
+        log('CPU baseline: FP64 oracle, %d steps' % args.cpu_steps)
+        cv, cn, ct_, threads = cpu_baseline(vol, args.cpu_steps)
+        cpu = dict(value=cv, unit='grid-pt z-steps/s', cores=threads, kind='port',
+                   sample='%d RK4 steps of this workload (2049 x 256) from the initial field, FP64, %.1f s'
+                   % (cn, ct_))
+
+    out = {
+        'metric': METRIC, 'value': value, 'unit': 'grid-pt z-steps/s', 'n_gpus': 1, 'steps': n, 'warmup': W,
+        'ms_per_step': ms_per_step, 'higher_is_better': True, 'scaling': 'weak', 'vs_baseline': None,
+        'dtype': DTYPE[prec], 'data': 'synthetic (seeded Gaussian pulse with 1% spectral-phase noise)',
+        'config': {'workload': 'radial UPPE, Kerr + O2 PPT photoionisation + plasma current, Nw=2049, Nr=256, '
+                               '400 adaptive subcycles over 20 cm; first %d RK4 steps' % K,
+                   'precision': prec, 'rate_model': args.rate, 'grid': [rs.N[0], rs.N[1]]},
+        'roofline': roofline,
+        'cpu_baseline': cpu,
+        'ms_per_step_with_events': 1e3 * secs2 / n2,
+        'nan_steps': info['nan_steps'],
+    }
+    out.update(extras)
+    print(json.dumps(out))
+
+
+if __name__ == '__main__':
+    main()

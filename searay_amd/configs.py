@@ -112,7 +112,7 @@ def o2_ionizer(kind='explicit'):
 
 
 def radial_o2_ionization(scale=1, steps=None, seed=1001, rate='explicit', precision=None, power=3.0,
-                         length='20 cm'):
+                         length='20 cm', uv_opacity=False):
     helper = input_tools.InputHelper(MKS_LENGTH)
     nw = (2048 // scale) + 1
     nr = max(8, 256 // scale)
@@ -120,7 +120,10 @@ def radial_o2_ionization(scale=1, steps=None, seed=1001, rate='explicit', precis
     L = helper.dnum(length)
     subcycles = 400 if steps is None else steps
     air = dispersion.SimpleAir(MKS_LENGTH)
-    air.add_opacity_region(helper.dnum('0.1 cm'), helper.dnum('0.01 um'), helper.dnum('0.3 um'))
+    if uv_opacity:
+        # the UV absorption band of examples/uppe/air_fil.py:36 (Im kz ~ 1.3e-4: exp(Im kz z) in the RHS
+        # leaves the FP32 range after ~9 cm, so FP32 runs need more diagnostic planes with it)
+        air.add_opacity_region(helper.dnum('0.1 cm'), helper.dnum('0.01 um'), helper.dnum('0.3 um'))
     vol = _base(helper, band, (helper.dnum('2 mm'), helper.dnum('2 mm'), L), (nw, nr, 1, 2),
                 (0.0, min(4.0, band[1])), subcycles, 1.0, 'cylindrical', nr)
     vol['dispersion inside'] = air

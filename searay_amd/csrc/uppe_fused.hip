@@ -50,6 +50,7 @@ struct FusedState {
   void* Hf = nullptr;      // [Nk][Nr] T
   void* tw = nullptr;      // [Nt/2] complex T
   double redchi3 = 0, dt = 0;
+  double max_ki = 0;  // largest Im kz of the loaded medium
   uppe_ionization ion{};
   // state and work buffers: complex T [P][nwp]
   void *q = nullptr, *k1 = nullptr, *k2 = nullptr, *k3 = nullptr, *er = nullptr, *jr = nullptr;
@@ -61,6 +62,13 @@ struct FusedState {
   double *sched = nullptr, *rec = nullptr;
   double* rec_active = nullptr;  // rec while the current advance records steps, else null
   int sched_cap = 0, rec_cap = 0;
+  // matrix-core contraction (cm < 0: SIMT GEMM)
+  int cm = -1;
+  void *frb_hi = nullptr, *frb_lo = nullptr;  // fragments of Hb (k2r)
+  void *frf_hi = nullptr, *frf_lo = nullptr;  // fragments of Hf (r2k)
+  int kp_b = 0, kp_f = 0;                     // padded contraction lengths
+  int rw_b = 16, rw_f = 16, bn_b = 16, bn_f = 16;
+  int f1 = 0;  // the r2k direction covers complex columns [0, f1): where the nonlinear filter is nonzero
 };
 
 constexpr int BM = 64, BN = 64, BK = 16;
@@ -262,6 +270,282 @@ __global__ __launch_bounds__(256) void f_gemm(const T* __restrict__ A, const cpl
 }
 
 // ---------------------------------------------------------------------------------------------------
+// Hankel contraction on the matrix cores
+//
+// out[M][2 nwp] = A[M][K] . B[K][2 nwp] with the same prologue/epilogue as f_gemm. One workgroup (4 waves)
+// owns an output tile of 4*rw rows x BN real columns and stages its whole B slab (K x BN) in LDS once, so the
+// prologue runs exactly once per element. A is read as pre-swizzled MFMA fragments straight from L2/HBM:
+//   CM_F64 / CM_F32: v_mfma_{f64,f32}_16x16x4, lane l holds A[16 rt + l%16][16 kq + 4 j + l/16], j < 4;
+//   CM_BF16X3:       v_mfma_f32_16x16x32_bf16, lane l holds A[16 rt + l%16][32 kc + 8 (l/16) + j], j < 8,
+//                    as hi = bf16(a) and lo = bf16(a - hi); the product is hi.hi + hi.lo + lo.hi in FP32.
+// The accumulator of a 16x16 tile sits in lane l, element i (i < 4), at column l % 16 and row 4 (l/16) + i for
+// the f32-output instructions but row 4 i + l/16 for v_mfma_f64_16x16x4 (both measured with one-hot operands).
+// ---------------------------------------------------------------------------------------------------
+enum { CM_F64 = 0, CM_F32 = 1, CM_BF16X3 = 2 };
+
+typedef float f32x4 __attribute__((ext_vector_type(4)));
+typedef double f64x4 __attribute__((ext_vector_type(4)));
+typedef __bf16 bf16x8 __attribute__((ext_vector_type(8)));
+
+template <int CM>
+struct MfmaTraits;
+template <>
+struct MfmaTraits<CM_F64> {
+  using T = double;
+  using acc_t = f64x4;
+  using frag_t = f64x4;  // 4 consecutive K steps of one lane
+};
+template <>
+struct MfmaTraits<CM_F32> {
+  using T = float;
+  using acc_t = f32x4;
+  using frag_t = f32x4;
+};
+template <>
+struct MfmaTraits<CM_BF16X3> {
+  using T = float;
+  using acc_t = f32x4;
+  using frag_t = bf16x8;
+};
+
+// one point of the r2k epilogue: S from the contracted current, then the stage bookkeeping
+template <typename T>
+__device__ inline void stage_epilogue(cplx<T> J, long idx, int f, int stage, double zph, double dz,
+                                      const double* __restrict__ phs, const cplx<T>* __restrict__ kzc,
+                                      const T* __restrict__ cond, cplx<T>* __restrict__ q,
+                                      cplx<T>* __restrict__ k1, cplx<T>* __restrict__ k2,
+                                      cplx<T>* __restrict__ k3, const StepState* st, double amin, double& loc,
+                                      bool& nan) {
+  cplx<T> kz = kzc[idx];
+  cplx<T> t = cmul(linear_factor<T>(phs[idx], kz.im, zph, -1), J);
+  cplx<T> S = cmk((T(-0.5) * t.im) / kz.re, (T(0.5) * t.re) / kz.re);
+  if (stage == 1) {
+    k1[idx] = S;
+    cplx<T> Q = q[idx];
+    double qr = (double)Q.re, qi = (double)Q.im;
+    double mod = sqrt(qr * qr + qi * qi);
+    if (mod != mod) nan = true;
+    if (!st->fixed) {
+      double sr = (double)S.re, si = (double)S.im;
+      double den = qr * qr + qi * qi + amin * amin;
+      double alpha = (qr * sr + qi * si) / den;
+      double keff = (qr * si - qi * sr) / den;
+      double Kr = 0.5 * st->dphi / st->L + fabs(keff) - alpha * (double)(alpha < 0.0);
+      loc = fmin(loc, st->dphi / Kr);
+    }
+  } else if (stage == 2) {
+    k2[idx] = S;
+  } else if (stage == 3) {
+    k3[idx] = S;
+  } else {
+    T c6 = (T)(dz / 6), c3 = (T)(dz / 3);
+    cplx<T> Q = q[idx], a1 = k1[idx], a2 = k2[idx], a3 = k3[idx];
+    T re = Q.re + c6 * a1.re;
+    T im = Q.im + c6 * a1.im;
+    re = re + c3 * a2.re;
+    im = im + c3 * a2.im;
+    re = re + c3 * a3.re;
+    im = im + c3 * a3.im;
+    re = re + c6 * S.re;
+    im = im + c6 * S.im;
+    T cf = cond[f];
+    cplx<T> qn = cmk(re * cf, im * cf);
+    q[idx] = qn;
+    loc = fmax(loc, sqrt((double)qn.re * qn.re + (double)qn.im * qn.im));
+  }
+}
+
+// B-operand element (k, f) with the prologue of the k2r direction applied
+template <typename T, int PRO>
+__device__ inline cplx<T> load_b(const cplx<T>* __restrict__ B, long idx, int f, int stage, double zph, double cu,
+                                 const double* __restrict__ phs, const cplx<T>* __restrict__ kzc,
+                                 const T* __restrict__ cond, const cplx<T>* __restrict__ kprev) {
+  cplx<T> v = B[idx];
+  if (PRO == PRO_STAGE) {
+    if (stage > 1) {
+      cplx<T> kp = kprev[idx];
+      T c = (T)cu;
+      v = cscale(cmk(v.re + c * kp.re, v.im + c * kp.im), cond[f]);
+    }
+    cplx<T> kz = kzc[idx];
+    cplx<T> a = cmul(linear_factor<T>(phs[idx], kz.im, zph, +1), v);
+    v = cmul(cmk(-kz.im, kz.re), a);
+  } else if (PRO == PRO_FINISH) {
+    cplx<T> kz = kzc[idx];
+    v = cmul(linear_factor<T>(phs[idx], kz.im, zph, +1), v);
+  }
+  return v;
+}
+
+template <int CM, int PRO, int EPI, int BN>
+__global__ __launch_bounds__(256) void f_hankel(
+    const typename MfmaTraits<CM>::frag_t* __restrict__ Ahi, const typename MfmaTraits<CM>::frag_t* __restrict__ Alo,
+    const cplx<typename MfmaTraits<CM>::T>* __restrict__ B, cplx<typename MfmaTraits<CM>::T>* __restrict__ out,
+    int M, int K, int Kp, int rw, int nwp, int f0, const double* __restrict__ phs,
+    const cplx<typename MfmaTraits<CM>::T>* __restrict__ kzc, const typename MfmaTraits<CM>::T* __restrict__ cond,
+    cplx<typename MfmaTraits<CM>::T>* __restrict__ q, const cplx<typename MfmaTraits<CM>::T>* __restrict__ kprev,
+    cplx<typename MfmaTraits<CM>::T>* __restrict__ k1, cplx<typename MfmaTraits<CM>::T>* __restrict__ k2,
+    cplx<typename MfmaTraits<CM>::T>* __restrict__ k3, StepState* __restrict__ st,
+    unsigned long long* __restrict__ qmax, unsigned long long* __restrict__ qmax_next, int stage, double zfix) {
+  using T = typename MfmaTraits<CM>::T;
+  using acc_t = typename MfmaTraits<CM>::acc_t;
+  if (st && !st->active) return;
+  extern __shared__ char smem[];
+  __shared__ double red[4];
+  const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
+  const int bm = 4 * rw;
+  const int row0 = blockIdx.y * bm;
+  const int fc0 = f0 + blockIdx.x * (BN / 2);  // first complex column of this tile
+  double zph = zfix, cu = 0.0, dz = 0.0;
+  if (st) {
+    dz = st->dz;
+    double cs = stage == 1 ? 0.0 : (stage == 4 ? 1.0 : 0.5);
+    zph = st->zoff + (st->z + cs * dz);
+    cu = stage == 1 ? 0.0 : (stage == 4 ? dz : dz / 2);
+  }
+
+  // ---- stage the B slab (Kp x BN) in LDS with the prologue applied
+  constexpr int HC = BN / 2;  // complex columns per tile
+  const int ldb = Kp + 8;     // bf16 row stride (k contiguous), padded
+  for (int e = tid; e < Kp * HC; e += 256) {
+    const int k = e / HC, cf = e % HC, f = fc0 + cf;
+    cplx<T> v = cmk(T(0), T(0));
+    if (k < K && f < nwp) v = load_b<T, PRO>(B, (long)k * nwp + f, f, stage, zph, cu, phs, kzc, cond, kprev);
+    if constexpr (CM == CM_BF16X3) {
+      __bf16* bhi = reinterpret_cast<__bf16*>(smem);
+      __bf16* blo = bhi + BN * ldb;
+      __bf16 hr = (__bf16)v.re, hi_ = (__bf16)v.im;
+      bhi[(2 * cf) * ldb + k] = hr;
+      bhi[(2 * cf + 1) * ldb + k] = hi_;
+      blo[(2 * cf) * ldb + k] = (__bf16)((float)v.re - (float)hr);
+      blo[(2 * cf + 1) * ldb + k] = (__bf16)((float)v.im - (float)hi_);
+    } else {
+      T* bs = reinterpret_cast<T*>(smem);
+      bs[k * BN + 2 * cf] = v.re;
+      bs[k * BN + 2 * cf + 1] = v.im;
+    }
+  }
+  __syncthreads();
+
+  // ---- MFMA main loop: wave w owns rows [row0 + w rw, row0 + (w+1) rw)
+  constexpr int CT = BN / 16;
+  const int rt_n = rw / 16;
+  const int mt0 = (row0 + wave * rw) / 16;  // first 16-row tile of this wave
+  const int M16 = (M + 15) / 16;
+  acc_t acc[4][CT];
+#pragma unroll
+  for (int r = 0; r < 4; r++)
+#pragma unroll
+    for (int c = 0; c < CT; c++) acc[r][c] = acc_t{0, 0, 0, 0};
+
+  if constexpr (CM == CM_BF16X3) {
+    const __bf16* bhi = reinterpret_cast<const __bf16*>(smem);
+    const __bf16* blo = bhi + BN * ldb;
+    const int KC = Kp / 32;
+    for (int kc = 0; kc < KC; kc++) {
+      bf16x8 bh[CT], bl[CT];
+#pragma unroll
+      for (int c = 0; c < CT; c++) {
+        const int col = c * 16 + (lane & 15), ko = kc * 32 + 8 * (lane >> 4);
+        bh[c] = *reinterpret_cast<const bf16x8*>(bhi + col * ldb + ko);
+        bl[c] = *reinterpret_cast<const bf16x8*>(blo + col * ldb + ko);
+      }
+#pragma unroll
+      for (int r = 0; r < 4; r++) {
+        if (r >= rt_n || mt0 + r >= M16) break;
+        const long fi = ((long)(mt0 + r) * KC + kc) * 64 + lane;
+        bf16x8 ah = reinterpret_cast<const bf16x8*>(Ahi)[fi];
+        bf16x8 al = reinterpret_cast<const bf16x8*>(Alo)[fi];
+#pragma unroll
+        for (int c = 0; c < CT; c++) {
+          acc[r][c] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(ah, bh[c], acc[r][c], 0, 0, 0);
+          acc[r][c] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(ah, bl[c], acc[r][c], 0, 0, 0);
+          acc[r][c] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(al, bh[c], acc[r][c], 0, 0, 0);
+        }
+      }
+    }
+  } else {
+    const T* bs = reinterpret_cast<const T*>(smem);
+    const int KQ = Kp / 16;
+    for (int kq = 0; kq < KQ; kq++) {
+      typename MfmaTraits<CM>::frag_t af[4];
+#pragma unroll
+      for (int r = 0; r < 4; r++)
+        if (r < rt_n && mt0 + r < M16) af[r] = Ahi[((long)(mt0 + r) * KQ + kq) * 64 + lane];
+#pragma unroll
+      for (int j = 0; j < 4; j++) {
+        const int krow = kq * 16 + 4 * j + (lane >> 4);
+        T b[CT];
+#pragma unroll
+        for (int c = 0; c < CT; c++) b[c] = bs[krow * BN + c * 16 + (lane & 15)];
+#pragma unroll
+        for (int r = 0; r < 4; r++) {
+          if (r >= rt_n || mt0 + r >= M16) break;
+#pragma unroll
+          for (int c = 0; c < CT; c++) {
+            if constexpr (CM == CM_F64)
+              acc[r][c] = __builtin_amdgcn_mfma_f64_16x16x4f64(af[r][j], b[c], acc[r][c], 0, 0, 0);
+            else
+              acc[r][c] = __builtin_amdgcn_mfma_f32_16x16x4f32(af[r][j], b[c], acc[r][c], 0, 0, 0);
+          }
+        }
+      }
+    }
+  }
+  __syncthreads();  // B slab no longer needed: reuse LDS for the output tile
+
+  // ---- accumulators -> LDS tile [bm][BN]
+  T* os = reinterpret_cast<T*>(smem);
+#pragma unroll
+  for (int r = 0; r < 4; r++) {
+    if (r >= rt_n) break;
+#pragma unroll
+    for (int c = 0; c < CT; c++)
+#pragma unroll
+      for (int i = 0; i < 4; i++) {
+        const int rin = CM == CM_F64 ? 4 * i + (lane >> 4) : 4 * (lane >> 4) + i;
+        const int row = wave * rw + r * 16 + rin, col = c * 16 + (lane & 15);
+        os[row * BN + col] = (T)acc[r][c][i];
+      }
+  }
+  __syncthreads();
+
+  // ---- epilogue over complex points of the tile
+  const double amin = (EPI == EPI_STAGE && stage == 1 && !st->fixed) ? st->rel_amin * bitsd(*qmax) : 0.0;
+  double loc = (EPI == EPI_STAGE && stage == 4) ? 0.0 : INFINITY;
+  bool nan = false;
+  for (int e = tid; e < bm * HC; e += 256) {
+    const int row = e / HC, cf = e % HC, r = row0 + row, f = fc0 + cf;
+    if (r >= M || f >= nwp) continue;
+    const long idx = (long)r * nwp + f;
+    cplx<T> J = cmk(os[row * BN + 2 * cf], os[row * BN + 2 * cf + 1]);
+    if (EPI == EPI_NONE)
+      out[idx] = J;
+    else
+      stage_epilogue<T>(J, idx, f, stage, zph, dz, phs, kzc, cond, q, k1, k2, k3, st, amin, loc, nan);
+  }
+  if (EPI == EPI_NONE) return;
+  if (stage == 1 && nan) atomicOr(&st->nanflag, 1);
+  if (stage == 2 || stage == 3) return;
+  if (stage == 1 && st->fixed) return;
+  const bool is_min = stage == 1;
+  for (int o = 32; o > 0; o >>= 1) {
+    double v = __shfl_xor(loc, o);
+    loc = is_min ? fmin(loc, v) : fmax(loc, v);
+  }
+  if (lane == 0) red[wave] = loc;
+  __syncthreads();
+  if (tid == 0) {
+    double v = red[0];
+    for (int i = 1; i < 4; i++) v = is_min ? fmin(v, red[i]) : fmax(v, red[i]);
+    if (is_min)
+      atomicMin(&st->dzraw, dbits(v));
+    else
+      atomicMax(qmax_next, dbits(v));
+  }
+}
+
+// ---------------------------------------------------------------------------------------------------
 // source kernel: one workgroup per strip
 // ---------------------------------------------------------------------------------------------------
 template <typename T, int SPT, bool DIAG>
@@ -424,6 +708,7 @@ __global__ void f_ctl_dz(StepState* st, const double* sched, double* rec) {
 }
 
 // end of step: advance z, subcycle bookkeeping (uppe_kernel.py:299-325)
+// qmax[2] holds max|q| over the columns the r2k direction never updates (the current is zero there)
 __global__ void f_ctl_end(StepState* st, unsigned long long* qmax, unsigned long long* qmax_next) {
   if (!st->active) return;
   st->z = st->z + st->dz;
@@ -432,7 +717,7 @@ __global__ void f_ctl_end(StepState* st, unsigned long long* qmax, unsigned long
     st->nan_steps++;
     st->nanflag = 0;
   }
-  *qmax = *qmax_next;
+  *qmax = *qmax_next > qmax[2] ? *qmax_next : qmax[2];
   *qmax_next = 0ull;
   st->dzraw = dbits(INFINITY);
   const double z = st->z, L = st->L;
@@ -461,6 +746,20 @@ __global__ void f_qmax(const cplx<T>* q, long n, unsigned long long* out) {
     cplx<T> v = q[i];
     m = fmax(m, sqrt((double)v.re * v.re + (double)v.im * v.im));
   }
+  for (int o = 32; o > 0; o >>= 1) m = fmax(m, __shfl_xor(m, o));
+  if ((threadIdx.x & 63) == 0) atomicMax(out, dbits(m));
+}
+
+// max |q| over complex columns [f_from, nwp) of every row
+template <typename T>
+__global__ void f_qmax_cols(const cplx<T>* q, long P, int nwp, int f_from, unsigned long long* out) {
+  const int w = nwp - f_from;
+  double m = 0.0;
+  if (w > 0)
+    for (long i = blockIdx.x * (long)blockDim.x + threadIdx.x; i < P * w; i += (long)gridDim.x * blockDim.x) {
+      cplx<T> v = q[(i / w) * nwp + f_from + (i % w)];
+      m = fmax(m, sqrt((double)v.re * v.re + (double)v.im * v.im));
+    }
   for (int o = 32; o > 0; o >>= 1) m = fmax(m, __shfl_xor(m, o));
   if ((threadIdx.x & 63) == 0) atomicMax(out, dbits(m));
 }
@@ -540,6 +839,72 @@ static void launchf(uppe_handle* h, int kc, K kernel, dim3 grid, dim3 block, siz
   UPPE_LAUNCH_CHECK();
 }
 
+// ---- host-side MFMA fragment packing -------------------------------------------------------------
+
+static uint16_t bf16_rne(float x) {
+  uint32_t b;
+  std::memcpy(&b, &x, 4);
+  if ((b & 0x7f800000u) == 0x7f800000u) return (uint16_t)(b >> 16);  // inf / nan
+  b += 0x7fffu + ((b >> 16) & 1u);
+  return (uint16_t)(b >> 16);
+}
+
+static double bf16_value(uint16_t h) {
+  uint32_t b = (uint32_t)h << 16;
+  float f;
+  std::memcpy(&f, &b, 4);
+  return (double)f;
+}
+
+// A is M x K row-major FP64. K4 layout (f32/f64 16x16x4): element j of lane l in block (rt, kq) is
+// A[16 rt + l%16][16 kq + 4 j + l/16]. BF16 layout (16x16x32): element j of lane l in block (rt, kc) is
+// A[16 rt + l%16][32 kc + 8 (l/16) + j], split into hi = bf16(a) and lo = bf16(a - hi).
+template <typename T>
+static void pack_k4(const std::vector<double>& A, int M, int K, std::vector<T>& out, int* kp) {
+  const int M16 = (M + 15) / 16, Kp = ((K + 15) / 16) * 16, KQ = Kp / 16;
+  out.assign((size_t)M16 * KQ * 64 * 4, T(0));
+  for (int rt = 0; rt < M16; rt++)
+    for (int kq = 0; kq < KQ; kq++)
+      for (int l = 0; l < 64; l++)
+        for (int j = 0; j < 4; j++) {
+          const int r = 16 * rt + (l & 15), k = 16 * kq + 4 * j + (l >> 4);
+          out[(((size_t)rt * KQ + kq) * 64 + l) * 4 + j] = (r < M && k < K) ? (T)A[(size_t)r * K + k] : T(0);
+        }
+  *kp = Kp;
+}
+
+static void pack_bf16x3(const std::vector<double>& A, int M, int K, std::vector<uint16_t>& hi,
+                        std::vector<uint16_t>& lo, int* kp) {
+  const int M16 = (M + 15) / 16, Kp = ((K + 31) / 32) * 32, KC = Kp / 32;
+  hi.assign((size_t)M16 * KC * 64 * 8, 0);
+  lo.assign(hi.size(), 0);
+  for (int rt = 0; rt < M16; rt++)
+    for (int kc = 0; kc < KC; kc++)
+      for (int l = 0; l < 64; l++)
+        for (int j = 0; j < 8; j++) {
+          const int r = 16 * rt + (l & 15), k = 32 * kc + 8 * (l >> 4) + j;
+          const double a = (r < M && k < K) ? A[(size_t)r * K + k] : 0.0;
+          const uint16_t h = bf16_rne((float)a);
+          const size_t o = (((size_t)rt * KC + kc) * 64 + l) * 8 + j;
+          hi[o] = h;
+          lo[o] = bf16_rne((float)(a - bf16_value(h)));
+        }
+  *kp = Kp;
+}
+
+template <typename V>
+static void* upload(const std::vector<V>& v) {
+  void* d = nullptr;
+  UPPE_HIP(hipMalloc(&d, sizeof(V) * std::max<size_t>(1, v.size())));
+  UPPE_HIP(hipMemcpy(d, v.data(), sizeof(V) * v.size(), hipMemcpyHostToDevice));
+  return d;
+}
+
+static int rows_per_wave(int M) {
+  int rw = ((M + 3) / 4 + 15) / 16 * 16;
+  return std::max(16, std::min(64, rw));
+}
+
 template <typename T>
 struct Fused {
   using C = cplx<T>;
@@ -548,6 +913,7 @@ struct Fused {
     const long P = h->P;
     s->nwp = ((h->nw + 31) / 32) * 32;
     s->el = sizeof(T);
+    s->f1 = s->nwp;
     const size_t cs = sizeof(C) * P * s->nwp;
     UPPE_HIP(hipMalloc(&s->phs, sizeof(double) * P * s->nwp));
     UPPE_HIP(hipMalloc(&s->kzc, cs));
@@ -563,9 +929,9 @@ struct Fused {
     UPPE_HIP(hipMalloc(&s->chi_t, sizeof(T) * h->ntt));
     UPPE_HIP(hipMalloc(&s->ne_t, sizeof(T) * h->ntt));
     UPPE_HIP(hipMalloc(&s->stage, sizeof(double) * std::max(2 * h->npt, h->ntt)));
-    UPPE_HIP(hipMalloc(&s->qmax, sizeof(unsigned long long) * 2));
+    UPPE_HIP(hipMalloc(&s->qmax, sizeof(unsigned long long) * 3));
     s->qmax_next = s->qmax + 1;
-    UPPE_HIP(hipMemset(s->qmax, 0, sizeof(unsigned long long) * 2));
+    UPPE_HIP(hipMemset(s->qmax, 0, sizeof(unsigned long long) * 3));
     UPPE_HIP(hipMalloc(&s->st, sizeof(StepState)));
     UPPE_HIP(hipMemset(s->st, 0, sizeof(StepState)));
     UPPE_HIP(hipHostMalloc(&s->st_host, sizeof(StepState), hipHostMallocDefault));
@@ -587,24 +953,50 @@ struct Fused {
 
   static void set_hankel(uppe_handle* h, FusedState* s) {
     const int nr = h->n1, nk = h->nk;
-    std::vector<T> hb((size_t)nr * nk), hf((size_t)nk * nr);
+    std::vector<double> hb((size_t)nr * nk), hf((size_t)nk * nr);
     for (int k = 0; k < nk; k++)
       for (int j = 0; j < nr; j++) {
         double v = h->H[(size_t)k * nr + j];
-        hb[(size_t)j * nk + k] = (T)(v * h->lam[k] / h->lam[j]);
-        hf[(size_t)k * nr + j] = (T)(v * h->lam[j] / h->lam[k]);
+        hb[(size_t)j * nk + k] = v * h->lam[k] / h->lam[j];
+        hf[(size_t)k * nr + j] = v * h->lam[j] / h->lam[k];
       }
-    UPPE_HIP(hipMemcpy(s->Hb, hb.data(), sizeof(T) * hb.size(), hipMemcpyHostToDevice));
-    UPPE_HIP(hipMemcpy(s->Hf, hf.data(), sizeof(T) * hf.size(), hipMemcpyHostToDevice));
+    std::vector<T> hbt(hb.begin(), hb.end()), hft(hf.begin(), hf.end());
+    UPPE_HIP(hipMemcpy(s->Hb, hbt.data(), sizeof(T) * hbt.size(), hipMemcpyHostToDevice));
+    UPPE_HIP(hipMemcpy(s->Hf, hft.data(), sizeof(T) * hft.size(), hipMemcpyHostToDevice));
+    void* old[] = {s->frb_hi, s->frb_lo, s->frf_hi, s->frf_lo};
+    for (void* p : old)
+      if (p) UPPE_HIP(hipFree(p));
+    s->frb_hi = s->frb_lo = s->frf_hi = s->frf_lo = nullptr;
+    if (s->cm < 0) return;
+    if (s->cm == CM_BF16X3) {
+      std::vector<uint16_t> hi, lo;
+      pack_bf16x3(hb, nr, nk, hi, lo, &s->kp_b);
+      s->frb_hi = upload(hi);
+      s->frb_lo = upload(lo);
+      pack_bf16x3(hf, nk, nr, hi, lo, &s->kp_f);
+      s->frf_hi = upload(hi);
+      s->frf_lo = upload(lo);
+    } else {
+      std::vector<T> fr;
+      pack_k4<T>(hb, nr, nk, fr, &s->kp_b);
+      s->frb_hi = upload(fr);
+      pack_k4<T>(hf, nk, nr, fr, &s->kp_f);
+      s->frf_hi = upload(fr);
+    }
+    s->rw_b = rows_per_wave(nr);
+    s->rw_f = rows_per_wave(nk);
+    s->bn_b = s->bn_f = 16;
   }
 
   static void set_medium(uppe_handle* h, FusedState* s, const uppe_medium* m) {
     const long P = h->P;
     std::vector<T> w(s->nwp, T(0)), nl(s->nwp, T(0)), cd(s->nwp, T(0));
+    int last_nl = -1;
     for (int f = 0; f < h->nw; f++) {
       w[f] = (T)m->w[f];
       nl[f] = (T)m->nl_filter[f];
       cd[f] = (T)m->cond_filter[f];
+      if (m->nl_filter[f] != 0.0) last_nl = f;
     }
     UPPE_HIP(hipMemcpy(s->w, w.data(), sizeof(T) * s->nwp, hipMemcpyHostToDevice));
     UPPE_HIP(hipMemcpy(s->nlf, nl.data(), sizeof(T) * s->nwp, hipMemcpyHostToDevice));
@@ -621,10 +1013,31 @@ struct Fused {
     s->redchi3 = m->redchi3;
     s->dt = m->dt;
     s->ion = m->ion;
+    s->max_ki = 0.0;
+    for (long i = 0; i < h->npt; i++) s->max_ki = std::max(s->max_ki, m->kz[2 * i + 1]);
+    // columns at and above f1 carry zero current: the r2k contraction and the stage updates skip them.
+    // f1 stays inside the padded row and covers the conditioner's four bins.
+    int f1 = std::max(last_nl + 1, 4);
+    f1 = std::min(s->nwp, ((f1 + 7) / 8) * 8);
+    if (f1 != s->f1) {
+      s->f1 = f1;
+      const size_t cs = sizeof(C) * P * s->nwp;
+      UPPE_HIP(hipMemset(s->k1, 0, cs));
+      UPPE_HIP(hipMemset(s->k2, 0, cs));
+      UPPE_HIP(hipMemset(s->k3, 0, cs));
+      if (h->have_field) static_qmax(h, s);
+    }
   }
 
-  // k-space <- r-space (plain transform)
-  static void r2k_plain(uppe_handle* h, FusedState* s, const C* in, C* out) {
+  static void static_qmax(uppe_handle* h, FusedState* s) {
+    UPPE_HIP(hipMemsetAsync(s->qmax + 2, 0, sizeof(unsigned long long), h->stream));
+    launchf(h, UPPE_KC_REDUCE, f_qmax_cols<T>, dim3(256), dim3(kBlock), 0, (const C*)s->q, h->P, s->nwp, s->f1,
+            s->qmax + 2);
+    UPPE_HIP(hipStreamSynchronize(h->stream));
+  }
+
+  // SIMT contraction (the FP64 path and the one-off transforms of set_field/finish)
+  static void gemm_r2k_plain(uppe_handle* h, FusedState* s, const C* in, C* out) {
     dim3 grid(2 * s->nwp / BN, (h->nk + BM - 1) / BM);
     launchf(h, UPPE_KC_HANKEL, f_gemm<T, PRO_NONE, EPI_NONE>, grid, dim3(256), 0, (const T*)s->Hf, in, out, h->nk,
             h->n1, s->nwp, (const double*)nullptr, (const C*)nullptr, (const T*)nullptr, (C*)nullptr,
@@ -656,9 +1069,60 @@ struct Fused {
     throw Error{UPPE_ERR_UNSUPPORTED, "fused source kernel: unsupported Nt"};
   }
 
+  // matrix-core contraction of one RHS stage; dir 0 = k2r (prologue), 1 = r2k (epilogue)
+  template <int CM, int PRO, int EPI>
+  static void hankel_mfma(uppe_handle* h, FusedState* s, int dir, const C* B, C* out, const C* kprev, int stage,
+                          double zfix, StepState* st) {
+    const int M = dir == 0 ? h->n1 : h->nk, K = dir == 0 ? h->nk : h->n1;
+    const int kp = dir == 0 ? s->kp_b : s->kp_f, rw = dir == 0 ? s->rw_b : s->rw_f;
+    const int bn = dir == 0 ? s->bn_b : s->bn_f;
+    const int ncols = dir == 0 ? s->nwp : s->f1;  // complex columns covered
+    const void* ahi = dir == 0 ? s->frb_hi : s->frf_hi;
+    const void* alo = dir == 0 ? s->frb_lo : s->frf_lo;
+    using F = typename MfmaTraits<CM>::frag_t;
+    const int bm = 4 * rw;
+    dim3 grid((unsigned)((ncols + bn / 2 - 1) / (bn / 2)), (unsigned)((M + bm - 1) / bm));
+    size_t slab = CM == CM_BF16X3 ? 2ull * bn * (kp + 8) * 2 : (size_t)kp * bn * sizeof(T);
+    size_t tile = (size_t)bm * bn * sizeof(T);
+    size_t shm = std::max(slab, tile);
+#define UPPE_HK(BNV)                                                                                             \
+  launchf(h, UPPE_KC_HANKEL, f_hankel<CM, PRO, EPI, BNV>, grid, dim3(256), shm, (const F*)ahi, (const F*)alo, B, out, \
+          M, K, kp, rw, s->nwp, 0, (const double*)s->phs, (const C*)s->kzc, (const T*)s->cond, (C*)s->q, kprev,      \
+          (C*)s->k1, (C*)s->k2, (C*)s->k3, st, s->qmax, s->qmax_next, stage, zfix)
+    if (bn == 16)
+      UPPE_HK(16);
+    else
+      UPPE_HK(32);
+#undef UPPE_HK
+  }
+
+  template <int CM>
+  static void step_mfma(uppe_handle* h, FusedState* s) {
+    const C* kp[5] = {nullptr, nullptr, (const C*)s->k1, (const C*)s->k2, (const C*)s->k3};
+    for (int stage = 1; stage <= 4; stage++) {
+      hankel_mfma<CM, PRO_STAGE, EPI_NONE>(h, s, 0, (const C*)s->q, (C*)s->er, kp[stage], stage, 0.0, s->st);
+      source(h, s, (const C*)s->er, (C*)s->jr, false, s->st);
+      hankel_mfma<CM, PRO_NONE, EPI_STAGE>(h, s, 1, (const C*)s->jr, (C*)nullptr, nullptr, stage, 0.0, s->st);
+      if (stage == 1)
+        launchf(h, UPPE_KC_REDUCE, f_ctl_dz, dim3(1), dim3(1), 0, s->st, (const double*)s->sched, s->rec_active);
+    }
+    launchf(h, UPPE_KC_REDUCE, f_ctl_end, dim3(1), dim3(1), 0, s->st, s->qmax, s->qmax_next);
+  }
+
   // one RK4 step: 4 x (k2r, source, r2k) + 2 control kernels
   static void step(uppe_handle* h, FusedState* s) {
-    dim3 g_k2r(2 * s->nwp / BN, (h->n1 + BM - 1) / BM), g_r2k(2 * s->nwp / BN, (h->nk + BM - 1) / BM);
+    if (s->cm >= 0) {
+      if constexpr (sizeof(T) == 8) {
+        step_mfma<CM_F64>(h, s);
+      } else {
+        if (s->cm == CM_BF16X3)
+          step_mfma<CM_BF16X3>(h, s);
+        else
+          step_mfma<CM_F32>(h, s);
+      }
+      return;
+    }
+    dim3 g_k2r(2 * s->nwp / BN, (h->n1 + BM - 1) / BM), g_r2k((2 * s->f1 + BN - 1) / BN, (h->nk + BM - 1) / BM);
     C* kp[5] = {nullptr, nullptr, (C*)s->k1, (C*)s->k2, (C*)s->k3};
     for (int stage = 1; stage <= 4; stage++) {
       launchf(h, UPPE_KC_HANKEL, f_gemm<T, PRO_STAGE, EPI_NONE>, g_k2r, dim3(256), 0, (const T*)s->Hb,
@@ -682,13 +1146,23 @@ struct Fused {
     launchf(h, UPPE_KC_POINT, f_from_host<T>, dim3(grid_for(P * s->nwp)), dim3(kBlock), 0, (const double*)s->stage,
             (C*)s->er, h->nw, P, s->nwp);
     UPPE_HIP(hipMemsetAsync(s->q, 0, sizeof(C) * P * s->nwp, h->stream));
-    r2k_plain(h, s, (const C*)s->er, (C*)s->q);
-    UPPE_HIP(hipMemsetAsync(s->qmax, 0, sizeof(unsigned long long) * 2, h->stream));
+    gemm_r2k_plain(h, s, (const C*)s->er, (C*)s->q);
+    UPPE_HIP(hipMemsetAsync(s->qmax, 0, sizeof(unsigned long long) * 3, h->stream));
     launchf(h, UPPE_KC_REDUCE, f_qmax<T>, dim3(256), dim3(kBlock), 0, (const C*)s->q, P * s->nwp, s->qmax);
+    launchf(h, UPPE_KC_REDUCE, f_qmax_cols<T>, dim3(256), dim3(kBlock), 0, (const C*)s->q, P, s->nwp, s->f1,
+            s->qmax + 2);
     UPPE_HIP(hipStreamSynchronize(h->stream));
   }
 
   static int advance(uppe_handle* h, FusedState* s, const uppe_step_control* c) {
+    if (sizeof(T) == 4) {
+      // the right-hand side carries exp(+Im kz z) (uppe.cl:86-90); keep it inside the FP32 range
+      const double zmax = ((c->zi + (double)(c->sub0 + c->n_sub) * c->length) - c->zi);
+      if (s->max_ki * zmax > 80.0)
+        throw Error{UPPE_ERR_UNSUPPORTED,
+                    "FP32 range: exp(Im kz z) would exceed 3e38 in this interval (max Im kz * z = " +
+                        std::to_string(s->max_ki * zmax) + "); use FP64 or more diagnostic planes"};
+    }
     StepState* sh = s->st_host;
     std::memset(sh, 0, sizeof(StepState));
     sh->zi = c->zi;
@@ -777,6 +1251,8 @@ struct Fused {
     if (chi_t) out_t(s->chi_t, chi_t);
     if (ne_t) out_t(s->ne_t, ne_t);
     UPPE_HIP(hipStreamSynchronize(h->stream));
+    // finish leaves k1 holding the r-space field: clear it so the next run starts from zero stage buffers
+    UPPE_HIP(hipMemset(s->k1, 0, sizeof(C) * n));
   }
 
   static void get_state(uppe_handle* h, FusedState* s, double* q) {
@@ -794,10 +1270,18 @@ void fused_create(uppe_handle* h) {
   if (h->d.geometry != UPPE_RADIAL || h->n2 != 1)
     throw Error{UPPE_ERR_UNSUPPORTED, "fused path: radial geometry with one azimuthal mode only"};
   if (h->nt > kMaxNtFused || h->nt < 4) throw Error{UPPE_ERR_UNSUPPORTED, "fused path supports 4 <= Nt <= 4096"};
+  if (h->n1 > 512) throw Error{UPPE_ERR_UNSUPPORTED, "fused path supports up to 512 radial cells"};
   auto* s = new FusedState();
   h->fused = s;
   s->prec = h->d.precision;
-  if (s->prec == UPPE_PREC_BF16X3) s->prec = UPPE_PREC_FP32;  // BF16x3 Hankel contraction not yet enabled
+  const char* env = getenv("SEARAY_UPPE_SIMT_HANKEL");
+  const bool simt = env && env[0] == '1';
+  if (s->prec == UPPE_PREC_FP64)
+    s->cm = simt ? -1 : CM_F64;
+  else if (s->prec == UPPE_PREC_BF16X3)
+    s->cm = CM_BF16X3;
+  else
+    s->cm = simt ? -1 : CM_F32;
   if (is_fp64(h))
     Fused<double>::create(h, s);
   else
@@ -808,7 +1292,8 @@ void fused_destroy(uppe_handle* h) {
   FusedState* s = h->fused;
   if (!s) return;
   void* ptrs[] = {s->phs, s->kzc, s->w, s->nlf, s->cond, s->ng, s->Hb, s->Hf, s->tw, s->q, s->k1, s->k2, s->k3,
-                  s->er, s->jr, s->chi_t, s->ne_t, s->stage, s->qmax, s->st, s->sched, s->rec};
+                  s->er, s->jr, s->chi_t, s->ne_t, s->stage, s->qmax, s->st, s->sched, s->rec, s->frb_hi,
+                  s->frb_lo, s->frf_hi, s->frf_lo};
   for (void* p : ptrs)
     if (p) (void)hipFree(p);
   if (s->st_host) (void)hipHostFree(s->st_host);

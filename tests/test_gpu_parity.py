@@ -44,7 +44,7 @@ def test_gaussian_vacuum(oracle_mod, coords, precision):
     assert got[5] == ref[5]
 
 
-@pytest.mark.parametrize('precision', ['fp64-plain', 'fp64', 'fp32'])
+@pytest.mark.parametrize('precision', ['fp64-plain', 'fp64', 'fp32', 'bf16x3'])
 def test_radial_kerr(oracle_mod, precision):
     vol = configs.radial_kerr_air(scale=4, steps=6)
     _say('radial_kerr scale=4 steps=6 %s' % precision)
@@ -58,7 +58,7 @@ def test_radial_kerr(oracle_mod, precision):
 
 
 @pytest.mark.parametrize('rate', ['explicit', 'series'])
-@pytest.mark.parametrize('precision', ['fp64-plain', 'fp64', 'fp32'])
+@pytest.mark.parametrize('precision', ['fp64-plain', 'fp64', 'fp32', 'bf16x3'])
 def test_radial_ionization_adaptive(oracle_mod, rate, precision):
     vol = configs.radial_o2_ionization(scale=4, steps=4, rate=rate, length='1 mm', power=3.0)
     _say('radial_ionization scale=4 %s %s' % (rate, precision))
@@ -74,3 +74,23 @@ def test_radial_ionization_adaptive(oracle_mod, rate, precision):
         tol, tol_ne = 1e-4, 1e-3   # adaptive schedules may differ slightly; exact replay is tested below
     assert rel_l2(got[0][..., 1], ref[0][..., 1]) < tol
     assert rel_l2(got[3][..., 1], ref[3][..., 1]) < tol_ne
+
+
+@pytest.mark.parametrize('precision', ['fp32', 'bf16x3'])
+def test_radial_ionization_replayed_schedule(oracle_mod, precision):
+    '''FP32 / BF16x3 on the oracle's own dz schedule: field within 1e-5, electron density within 1e-4.'''
+    vol = configs.radial_o2_ionization(scale=4, steps=4, rate='explicit', length='1 mm', power=3.0)
+    _say('radial_ionization replayed schedule %s' % precision)
+    rec_ref = []
+    ref = oracle_mod.oracle_track(vol, record=rec_ref)
+    from searay_amd import uppe_kernel as uk
+    rs = uk.RunSetup(vol)
+    st = rs.make_stepper(uk.hip_stepper_factory(precision))
+    zi, zf, L = rs.plane(0)
+    st.set_field(rs.A0)
+    n, _ = st.advance(zi, L, 0, rs.subcycles, rs.dzmin, dz_schedule=np.array(rec_ref))
+    a, J, chi_t, ne_t = st.finish(zf - zi)
+    st.close()
+    assert n == len(rec_ref)
+    assert rel_l2(a, ref[0][..., 1]) < FP32_FIELD_TOL
+    assert rel_l2(np.fft.rfft(ne_t, axis=0), ref[3][..., 1]) < FP32_NE_TOL
