@@ -378,7 +378,7 @@ __device__ inline cplx<T> load_b(const cplx<T>* __restrict__ B, long idx, int f,
 }
 
 template <int CM, int PRO, int EPI, int BN>
-__global__ __launch_bounds__(256) void f_hankel(
+__global__ __launch_bounds__(1024) void f_hankel(
     const typename MfmaTraits<CM>::frag_t* __restrict__ Ahi, const typename MfmaTraits<CM>::frag_t* __restrict__ Alo,
     const cplx<typename MfmaTraits<CM>::T>* __restrict__ B, cplx<typename MfmaTraits<CM>::T>* __restrict__ out,
     int M, int K, int Kp, int rw, int nwp, int f0, const double* __restrict__ phs,
@@ -388,12 +388,14 @@ __global__ __launch_bounds__(256) void f_hankel(
     cplx<typename MfmaTraits<CM>::T>* __restrict__ k3, StepState* __restrict__ st,
     unsigned long long* __restrict__ qmax, unsigned long long* __restrict__ qmax_next, int stage, double zfix) {
   using T = typename MfmaTraits<CM>::T;
+  using C = cplx<T>;
   using acc_t = typename MfmaTraits<CM>::acc_t;
   if (st && !st->active) return;
   extern __shared__ char smem[];
-  __shared__ double red[4];
+  __shared__ double red[16];
+  const int nth = blockDim.x, nwv = nth >> 6;
   const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
-  const int bm = 4 * rw;
+  const int bm = nwv * rw;
   const int row0 = blockIdx.y * bm;
   const int fc0 = f0 + blockIdx.x * (BN / 2);  // first complex column of this tile
   double zph = zfix, cu = 0.0, dz = 0.0;
@@ -404,37 +406,78 @@ __global__ __launch_bounds__(256) void f_hankel(
     cu = stage == 1 ? 0.0 : (stage == 4 ? dz : dz / 2);
   }
 
-  // ---- stage the B slab (Kp x BN) in LDS with the prologue applied
+  // ---- stage the B slab (Kp x BN) in LDS with the prologue applied; loads of a batch are issued together
   constexpr int HC = BN / 2;  // complex columns per tile
+  constexpr int EB = 4;       // elements per thread per batch
   const int ldb = Kp + 8;     // bf16 row stride (k contiguous), padded
-  for (int e = tid; e < Kp * HC; e += 256) {
-    const int k = e / HC, cf = e % HC, f = fc0 + cf;
-    cplx<T> v = cmk(T(0), T(0));
-    if (k < K && f < nwp) v = load_b<T, PRO>(B, (long)k * nwp + f, f, stage, zph, cu, phs, kzc, cond, kprev);
-    if constexpr (CM == CM_BF16X3) {
-      __bf16* bhi = reinterpret_cast<__bf16*>(smem);
-      __bf16* blo = bhi + BN * ldb;
-      __bf16 hr = (__bf16)v.re, hi_ = (__bf16)v.im;
-      bhi[(2 * cf) * ldb + k] = hr;
-      bhi[(2 * cf + 1) * ldb + k] = hi_;
-      blo[(2 * cf) * ldb + k] = (__bf16)((float)v.re - (float)hr);
-      blo[(2 * cf + 1) * ldb + k] = (__bf16)((float)v.im - (float)hi_);
-    } else {
-      T* bs = reinterpret_cast<T*>(smem);
-      bs[k * BN + 2 * cf] = v.re;
-      bs[k * BN + 2 * cf + 1] = v.im;
+  const int ne = Kp * HC;
+  for (int base = 0; base < ne; base += EB * nth) {
+    C vb[EB], vk[EB], vz[EB];
+    double vp[EB];
+    T vc[EB];
+#pragma unroll
+    for (int j = 0; j < EB; j++) {
+      const int e = base + tid + j * nth;
+      const int k = e / HC, f = fc0 + e % HC;
+      vb[j] = cmk(T(0), T(0));
+      vk[j] = cmk(T(0), T(0));
+      vz[j] = cmk(T(1), T(0));
+      vp[j] = 0.0;
+      vc[j] = T(1);
+      if (e < ne && k < K && f < nwp) {
+        const long idx = (long)k * nwp + f;
+        vb[j] = B[idx];
+        if (PRO != PRO_NONE) {
+          vz[j] = kzc[idx];
+          vp[j] = phs[idx];
+        }
+        if (PRO == PRO_STAGE && stage > 1) {
+          vk[j] = kprev[idx];
+          vc[j] = cond[f];
+        }
+      }
+    }
+#pragma unroll
+    for (int j = 0; j < EB; j++) {
+      const int e = base + tid + j * nth;
+      if (e >= ne) break;
+      const int k = e / HC, cf = e % HC;
+      C v = vb[j];
+      if (PRO == PRO_STAGE) {
+        if (stage > 1) {
+          T c = (T)cu;
+          v = cscale(cmk(v.re + c * vk[j].re, v.im + c * vk[j].im), vc[j]);
+        }
+        C a = cmul(linear_factor<T>(vp[j], vz[j].im, zph, +1), v);
+        v = cmul(cmk(-vz[j].im, vz[j].re), a);  // E = i kz A
+      } else if (PRO == PRO_FINISH) {
+        v = cmul(linear_factor<T>(vp[j], vz[j].im, zph, +1), v);
+      }
+      if constexpr (CM == CM_BF16X3) {
+        __bf16* bhi = reinterpret_cast<__bf16*>(smem);
+        __bf16* blo = bhi + BN * ldb;
+        __bf16 hr = (__bf16)v.re, hi_ = (__bf16)v.im;
+        bhi[(2 * cf) * ldb + k] = hr;
+        bhi[(2 * cf + 1) * ldb + k] = hi_;
+        blo[(2 * cf) * ldb + k] = (__bf16)((float)v.re - (float)hr);
+        blo[(2 * cf + 1) * ldb + k] = (__bf16)((float)v.im - (float)hi_);
+      } else {
+        T* bs = reinterpret_cast<T*>(smem);
+        bs[k * BN + 2 * cf] = v.re;
+        bs[k * BN + 2 * cf + 1] = v.im;
+      }
     }
   }
   __syncthreads();
 
-  // ---- MFMA main loop: wave w owns rows [row0 + w rw, row0 + (w+1) rw)
+  // ---- MFMA main loop: wave w owns rows [row0 + w rw, row0 + (w+1) rw), rw <= 32
   constexpr int CT = BN / 16;
   const int rt_n = rw / 16;
   const int mt0 = (row0 + wave * rw) / 16;  // first 16-row tile of this wave
   const int M16 = (M + 15) / 16;
-  acc_t acc[4][CT];
+  acc_t acc[2][CT];
 #pragma unroll
-  for (int r = 0; r < 4; r++)
+  for (int r = 0; r < 2; r++)
 #pragma unroll
     for (int c = 0; c < CT; c++) acc[r][c] = acc_t{0, 0, 0, 0};
 
@@ -443,6 +486,14 @@ __global__ __launch_bounds__(256) void f_hankel(
     const __bf16* blo = bhi + BN * ldb;
     const int KC = Kp / 32;
     for (int kc = 0; kc < KC; kc++) {
+      bf16x8 ah[2], al[2];
+#pragma unroll
+      for (int r = 0; r < 2; r++)
+        if (r < rt_n && mt0 + r < M16) {
+          const long fi = ((long)(mt0 + r) * KC + kc) * 64 + lane;
+          ah[r] = reinterpret_cast<const bf16x8*>(Ahi)[fi];
+          al[r] = reinterpret_cast<const bf16x8*>(Alo)[fi];
+        }
       bf16x8 bh[CT], bl[CT];
 #pragma unroll
       for (int c = 0; c < CT; c++) {
@@ -451,16 +502,13 @@ __global__ __launch_bounds__(256) void f_hankel(
         bl[c] = *reinterpret_cast<const bf16x8*>(blo + col * ldb + ko);
       }
 #pragma unroll
-      for (int r = 0; r < 4; r++) {
+      for (int r = 0; r < 2; r++) {
         if (r >= rt_n || mt0 + r >= M16) break;
-        const long fi = ((long)(mt0 + r) * KC + kc) * 64 + lane;
-        bf16x8 ah = reinterpret_cast<const bf16x8*>(Ahi)[fi];
-        bf16x8 al = reinterpret_cast<const bf16x8*>(Alo)[fi];
 #pragma unroll
         for (int c = 0; c < CT; c++) {
-          acc[r][c] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(ah, bh[c], acc[r][c], 0, 0, 0);
-          acc[r][c] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(ah, bl[c], acc[r][c], 0, 0, 0);
-          acc[r][c] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(al, bh[c], acc[r][c], 0, 0, 0);
+          acc[r][c] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(ah[r], bh[c], acc[r][c], 0, 0, 0);
+          acc[r][c] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(ah[r], bl[c], acc[r][c], 0, 0, 0);
+          acc[r][c] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(al[r], bh[c], acc[r][c], 0, 0, 0);
         }
       }
     }
@@ -468,9 +516,9 @@ __global__ __launch_bounds__(256) void f_hankel(
     const T* bs = reinterpret_cast<const T*>(smem);
     const int KQ = Kp / 16;
     for (int kq = 0; kq < KQ; kq++) {
-      typename MfmaTraits<CM>::frag_t af[4];
+      typename MfmaTraits<CM>::frag_t af[2];
 #pragma unroll
-      for (int r = 0; r < 4; r++)
+      for (int r = 0; r < 2; r++)
         if (r < rt_n && mt0 + r < M16) af[r] = Ahi[((long)(mt0 + r) * KQ + kq) * 64 + lane];
 #pragma unroll
       for (int j = 0; j < 4; j++) {
@@ -479,7 +527,7 @@ __global__ __launch_bounds__(256) void f_hankel(
 #pragma unroll
         for (int c = 0; c < CT; c++) b[c] = bs[krow * BN + c * 16 + (lane & 15)];
 #pragma unroll
-        for (int r = 0; r < 4; r++) {
+        for (int r = 0; r < 2; r++) {
           if (r >= rt_n || mt0 + r >= M16) break;
 #pragma unroll
           for (int c = 0; c < CT; c++) {
@@ -497,7 +545,7 @@ __global__ __launch_bounds__(256) void f_hankel(
   // ---- accumulators -> LDS tile [bm][BN]
   T* os = reinterpret_cast<T*>(smem);
 #pragma unroll
-  for (int r = 0; r < 4; r++) {
+  for (int r = 0; r < 2; r++) {
     if (r >= rt_n) break;
 #pragma unroll
     for (int c = 0; c < CT; c++)
@@ -514,11 +562,11 @@ __global__ __launch_bounds__(256) void f_hankel(
   const double amin = (EPI == EPI_STAGE && stage == 1 && !st->fixed) ? st->rel_amin * bitsd(*qmax) : 0.0;
   double loc = (EPI == EPI_STAGE && stage == 4) ? 0.0 : INFINITY;
   bool nan = false;
-  for (int e = tid; e < bm * HC; e += 256) {
+  for (int e = tid; e < bm * HC; e += nth) {
     const int row = e / HC, cf = e % HC, r = row0 + row, f = fc0 + cf;
     if (r >= M || f >= nwp) continue;
     const long idx = (long)r * nwp + f;
-    cplx<T> J = cmk(os[row * BN + 2 * cf], os[row * BN + 2 * cf + 1]);
+    C J = cmk(os[row * BN + 2 * cf], os[row * BN + 2 * cf + 1]);
     if (EPI == EPI_NONE)
       out[idx] = J;
     else
@@ -537,7 +585,7 @@ __global__ __launch_bounds__(256) void f_hankel(
   __syncthreads();
   if (tid == 0) {
     double v = red[0];
-    for (int i = 1; i < 4; i++) v = is_min ? fmin(v, red[i]) : fmax(v, red[i]);
+    for (int i = 1; i < nwv; i++) v = is_min ? fmin(v, red[i]) : fmax(v, red[i]);
     if (is_min)
       atomicMin(&st->dzraw, dbits(v));
     else
@@ -548,21 +596,53 @@ __global__ __launch_bounds__(256) void f_hankel(
 // ---------------------------------------------------------------------------------------------------
 // source kernel: one workgroup per strip
 // ---------------------------------------------------------------------------------------------------
+// ionisation rate of one sample (ionization.cl:22-44, 71-97). The FP32 path forms the closed form as one
+// exponential exp(ln(rate_conv C_pre) + C_pow ln E + C_exp/E) in FP32 and keeps the series polynomial in FP64
+// (its terms cancel) before an FP32 exponential; the FP64 path follows the reference's expressions.
+template <typename T>
+__device__ inline double ion_rate(double ea, const uppe_ionization& ion, double log_pre);
+
+template <>
+__device__ inline double ion_rate<double>(double ea, const uppe_ionization& ion, double log_pre) {
+  if (ion.kind == UPPE_ION_EXPLICIT) {
+    double emag = ion.e_conv * ea + ion.e_cutoff;
+    return ion.rate_conv * ion.c_pre * pow(emag, ion.c_pow) * exp(ion.c_exp / emag);
+  }
+  double lg = log(ion.e_conv * ea + ion.e_cutoff);
+  return ion.rate_conv * exp(ion.c[0] + ion.c[1] * lg + ion.c[2] * lg * lg + ion.c[3] * (lg * lg * lg) +
+                             ion.c[4] * (lg * lg * lg * lg) + ion.c[5] * (lg * lg * lg * lg * lg));
+}
+
+template <>
+__device__ inline double ion_rate<float>(double ea, const uppe_ionization& ion, double log_pre) {
+  const float emag = (float)(ion.e_conv * ea + ion.e_cutoff);
+  const float lg = logf(emag);
+  if (ion.kind == UPPE_ION_EXPLICIT)
+    return (double)expf((float)log_pre + (float)ion.c_pow * lg + (float)ion.c_exp / emag);
+  const double l = (double)lg;
+  const double poly = ion.c[0] + l * (ion.c[1] + l * (ion.c[2] + l * (ion.c[3] + l * (ion.c[4] + l * ion.c[5]))));
+  return ion.rate_conv * (double)expf((float)poly);
+}
+
+// one workgroup (up to 1024 threads) per strip; thread t owns time samples [t SPT, (t+1) SPT)
 template <typename T, int SPT, bool DIAG>
-__global__ __launch_bounds__(256) void f_source(const cplx<T>* __restrict__ er, cplx<T>* __restrict__ jr,
-                                                T* __restrict__ chi_out, T* __restrict__ ne_out,
-                                                const cplx<T>* __restrict__ tw, const T* __restrict__ w,
-                                                const T* __restrict__ nlf, const double* __restrict__ ng, int nt,
-                                                int lgn, int nwp, double redchi3, double dt, uppe_ionization ion,
-                                                const StepState* __restrict__ st) {
+__global__ __launch_bounds__(1024) void f_source(const cplx<T>* __restrict__ er, cplx<T>* __restrict__ jr,
+                                                 T* __restrict__ chi_out, T* __restrict__ ne_out,
+                                                 const cplx<T>* __restrict__ tw, const T* __restrict__ w,
+                                                 const T* __restrict__ nlf, const double* __restrict__ ng, int nt,
+                                                 int lgn, int nwp, double redchi3, double dt, uppe_ionization ion,
+                                                 const StepState* __restrict__ st) {
   if (st && !st->active) return;
   extern __shared__ char smem[];
   cplx<T>* Y = reinterpret_cast<cplx<T>*>(smem);
-  __shared__ double carry[256];
+  cplx<T>* tws = Y + nt;  // twiddles exp(-2 pi i k/Nt), k < Nt/2
+  __shared__ double wsum[16];
   const int N = nt, M = nt / 2, lgm = lgn - 1;
   const long p = blockIdx.x;
-  const int tid = threadIdx.x, nth = blockDim.x;
+  const int tid = threadIdx.x, nth = blockDim.x, lane = tid & 63, wave = tid >> 6, nwv = nth >> 6;
   const cplx<T>* X = er + p * nwp;
+  for (int k = tid; k < M; k += nth) tws[k] = tw[k];
+  __syncthreads();
 
   // 1. irfft (numpy semantics; imaginary parts of the DC and Nyquist bins are ignored, fft.cl:271-278)
   for (int k = tid; k < M; k += nth) {
@@ -572,11 +652,11 @@ __global__ __launch_bounds__(256) void f_source(const cplx<T>* __restrict__ er, 
       xm.im = T(0);
     }
     cplx<T> ze = cscale(cadd(xk, cconj(xm)), T(0.5));
-    cplx<T> zo = cmul(cscale(csub(xk, cconj(xm)), T(0.5)), cconj(tw[k]));
+    cplx<T> zo = cmul(cscale(csub(xk, cconj(xm)), T(0.5)), cconj(tws[k]));
     Y[k] = cmk(ze.re - zo.im, ze.im + zo.re);
   }
   __syncthreads();
-  lds_fft_radix2(Y, M, lgm, tw, N, +1);
+  lds_fft_stockham(Y, M, lgm, tws, N, +1);
   const T invm = T(1) / T(M);
   const T* ys = reinterpret_cast<const T*>(Y);
   const int a0 = tid * SPT;
@@ -592,20 +672,12 @@ __global__ __launch_bounds__(256) void f_source(const cplx<T>* __restrict__ er, 
   const double ngp = ng[p];
   double ne[SPT];
   if (ion.kind != UPPE_ION_NONE) {
+    const double log_pre = log(ion.rate_conv * ion.c_pre);
     double r[SPT + 1];
 #pragma unroll
     for (int j = 0; j <= SPT; j++) {
       int n = a0 + j;
-      double ea = fabs((double)e[j]);
-      double v;
-      if (ion.kind == UPPE_ION_EXPLICIT) {
-        double emag = ion.e_conv * ea + ion.e_cutoff;
-        v = ion.rate_conv * ion.c_pre * pow(emag, ion.c_pow) * exp(ion.c_exp / emag);
-      } else {
-        double lg = log(ion.e_conv * ea + ion.e_cutoff);
-        v = ion.rate_conv * exp(ion.c[0] + ion.c[1] * lg + ion.c[2] * lg * lg + ion.c[3] * (lg * lg * lg) +
-                                ion.c[4] * (lg * lg * lg * lg) + ion.c[5] * (lg * lg * lg * lg * lg));
-      }
+      double v = ion_rate<T>(fabs((double)e[j]), ion, log_pre);
       r[j] = (n >= N - 1) ? 0.0 : v;  // last sample zeroed (ionization.cl:147); r[N] is the initial 0
     }
     double s = 0.0;
@@ -614,31 +686,32 @@ __global__ __launch_bounds__(256) void f_source(const cplx<T>* __restrict__ er, 
       s += 0.5 * (r[j] + r[j + 1]) * dt;
       ne[j] = s;
     }
-    carry[tid] = s;
-    __syncthreads();
-    // inclusive suffix scan of the per-thread totals
-    for (int o = 1; o < nth; o <<= 1) {
-      double v = (tid + o < nth) ? carry[tid + o] : 0.0;
-      __syncthreads();
-      carry[tid] += v;
-      __syncthreads();
+    // suffix sums of the per-thread totals: within the wave by shuffles, across waves through LDS
+    double v = s;
+    for (int o = 1; o < 64; o <<= 1) {
+      double t = __shfl_down(v, o);
+      if (lane + o < 64) v += t;
     }
-    double c = (tid + 1 < nth) ? carry[tid + 1] : 0.0;
+    if (lane == 0) wsum[wave] = v;
+    __syncthreads();
+    double after = 0.0;
+    for (int w2 = wave + 1; w2 < nwv; w2++) after += wsum[w2];
+    const double c = (v - s) + after;
 #pragma unroll
     for (int j = 0; j < SPT; j++) {
       int n = a0 + j;
-      double v = ngp * (1.0 - exp(-(ne[j] + c)));
+      double x = ngp * (1.0 - exp(-(ne[j] + c)));
       // SoftTimeWindow(4, 64)
-      if (n < 4 || n > N - 5) v = 0.0;
+      if (n < 4 || n > N - 5) x = 0.0;
       if (n >= 4 && n < 68) {
         double qq = (double)(n - 4) / 64.0;
-        v *= 10 * qq * qq * qq - 15 * qq * qq * qq * qq + 6 * qq * qq * qq * qq * qq;
+        x *= 10 * qq * qq * qq - 15 * qq * qq * qq * qq + 6 * qq * qq * qq * qq * qq;
       }
       if (n <= N - 5 && n > N - 69) {
         double qq = (double)(N - 5 - n) / 64.0;
-        v *= 10 * qq * qq * qq - 15 * qq * qq * qq * qq + 6 * qq * qq * qq * qq * qq;
+        x *= 10 * qq * qq * qq - 15 * qq * qq * qq * qq + 6 * qq * qq * qq * qq * qq;
       }
-      ne[j] = v;
+      ne[j] = x;
     }
   } else {
 #pragma unroll
@@ -659,7 +732,7 @@ __global__ __launch_bounds__(256) void f_source(const cplx<T>* __restrict__ er, 
     }
   }
   __syncthreads();
-  lds_fft_radix2(Y, N, lgn, tw, N, -1);
+  lds_fft_stockham(Y, N, lgn, tws, N, -1);
 
   // 4. J = F_NL [ rfft(J_p)/(0.1 - i w) + (-i w) rfft(P) ] (fft.cl:282-329, uppe_kernel.py:170-186)
   for (int k = tid; k <= M; k += nth) {
@@ -900,9 +973,11 @@ static void* upload(const std::vector<V>& v) {
   return d;
 }
 
+// one workgroup covers all M rows with up to 16 waves of at most two 16-row tiles each
+static int hankel_waves(int M) { return std::max(1, std::min(16, (M + 15) / 16)); }
 static int rows_per_wave(int M) {
-  int rw = ((M + 3) / 4 + 15) / 16 * 16;
-  return std::max(16, std::min(64, rw));
+  const int m16 = (M + 15) / 16, w = hankel_waves(M);
+  return 16 * ((m16 + w - 1) / w);
 }
 
 template <typename T>
@@ -1047,9 +1122,9 @@ struct Fused {
 
   static void source(uppe_handle* h, FusedState* s, const C* er, C* jr, bool diag, StepState* st) {
     const int N = h->nt;
-    const int nth = std::min(N, 256);
+    const int nth = std::min(N, 1024);
     const int spt = N / nth;
-    const size_t shm = sizeof(C) * N;
+    const size_t shm = sizeof(C) * (N + N / 2);
     const int lgn = ilog2i(N);
     dim3 grid((unsigned)h->P), block(nth);
 #define UPPE_SRC(S)                                                                                              \
@@ -1080,13 +1155,14 @@ struct Fused {
     const void* ahi = dir == 0 ? s->frb_hi : s->frf_hi;
     const void* alo = dir == 0 ? s->frb_lo : s->frf_lo;
     using F = typename MfmaTraits<CM>::frag_t;
-    const int bm = 4 * rw;
+    const int nwv = hankel_waves(M);
+    const int bm = nwv * rw;
     dim3 grid((unsigned)((ncols + bn / 2 - 1) / (bn / 2)), (unsigned)((M + bm - 1) / bm));
     size_t slab = CM == CM_BF16X3 ? 2ull * bn * (kp + 8) * 2 : (size_t)kp * bn * sizeof(T);
     size_t tile = (size_t)bm * bn * sizeof(T);
     size_t shm = std::max(slab, tile);
 #define UPPE_HK(BNV)                                                                                             \
-  launchf(h, UPPE_KC_HANKEL, f_hankel<CM, PRO, EPI, BNV>, grid, dim3(256), shm, (const F*)ahi, (const F*)alo, B, out, \
+  launchf(h, UPPE_KC_HANKEL, f_hankel<CM, PRO, EPI, BNV>, grid, dim3(64 * nwv), shm, (const F*)ahi, (const F*)alo, B, out, \
           M, K, kp, rw, s->nwp, 0, (const double*)s->phs, (const C*)s->kzc, (const T*)s->cond, (C*)s->q, kprev,      \
           (C*)s->k1, (C*)s->k2, (C*)s->k3, st, s->qmax, s->qmax_next, stage, zfix)
     if (bn == 16)

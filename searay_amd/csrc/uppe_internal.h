@@ -101,6 +101,73 @@ __device__ inline void lds_fft_radix2(cplx<T>* buf, int n, int lg, const cplx<T>
 }
 
 // ---------------------------------------------------------------------------------------------------
+// Block-cooperative in-place Stockham FFT in LDS: radix-4 passes and one radix-2 pass when log2(n) is odd.
+// Each pass loads its butterfly inputs into registers, waits for the whole block, then stores the outputs
+// in autosorted order, so no second buffer is needed. Pass with span Ns and radix R: butterfly j reads
+// buf[j + r n/R], multiplies input r by exp(sign 2 pi i r (j mod Ns)/(Ns R)), forms the R-point DFT and
+// writes buf[(j - j mod Ns) R + j mod Ns + r Ns]. Twiddles come from tw[k] = exp(-2 pi i k/nbase), k < nbase/2,
+// with nbase a multiple of n. At most two butterflies per thread per pass (n/R <= 2 blockDim.x).
+// ---------------------------------------------------------------------------------------------------
+template <typename T>
+__device__ inline cplx<T> stockham_twiddle(const cplx<T>* tw, int t, int nbase, int sign) {
+  const int h = nbase >> 1;
+  cplx<T> w = t < h ? tw[t] : cmk(-tw[t - h].re, -tw[t - h].im);
+  if (sign > 0) w.im = -w.im;
+  return w;
+}
+
+template <typename T>
+__device__ inline void lds_fft_stockham(cplx<T>* buf, int n, int lg, const cplx<T>* tw, int nbase, int sign) {
+  const int nth = blockDim.x, tid = threadIdx.x;
+  int Ns = 1;
+  int rem = lg;
+  while (rem > 0) {
+    const int R = rem >= 2 ? 4 : 2;
+    const int nb = n / R;
+    cplx<T> v[2][4];
+#pragma unroll
+    for (int b = 0; b < 2; b++) {
+      const int j = tid + b * nth;
+      if (j < nb)
+#pragma unroll
+        for (int r = 0; r < 4; r++)
+          if (r < R) v[b][r] = buf[j + r * nb];
+    }
+    __syncthreads();
+#pragma unroll
+    for (int b = 0; b < 2; b++) {
+      const int j = tid + b * nth;
+      if (j >= nb) continue;
+      const int k = j & (Ns - 1);
+      const int tstep = k * (nbase / (Ns * R));
+      cplx<T> y[4];
+      if (R == 4) {
+        cplx<T> v1 = cmul(v[b][1], stockham_twiddle(tw, tstep, nbase, sign));
+        cplx<T> v2 = cmul(v[b][2], stockham_twiddle(tw, 2 * tstep, nbase, sign));
+        cplx<T> v3 = cmul(v[b][3], stockham_twiddle(tw, 3 * tstep, nbase, sign));
+        cplx<T> a0 = cadd(v[b][0], v2), a1 = csub(v[b][0], v2), a2 = cadd(v1, v3), d = csub(v1, v3);
+        cplx<T> a3 = sign < 0 ? cmk(d.im, -d.re) : cmk(-d.im, d.re);
+        y[0] = cadd(a0, a2);
+        y[1] = cadd(a1, a3);
+        y[2] = csub(a0, a2);
+        y[3] = csub(a1, a3);
+      } else {
+        cplx<T> v1 = cmul(v[b][1], stockham_twiddle(tw, tstep, nbase, sign));
+        y[0] = cadd(v[b][0], v1);
+        y[1] = csub(v[b][0], v1);
+      }
+      const int o = (j - k) * R + k;
+#pragma unroll
+      for (int r = 0; r < 4; r++)
+        if (r < R) buf[o + r * Ns] = y[r];
+    }
+    __syncthreads();
+    Ns *= R;
+    rem -= R == 4 ? 2 : 1;
+  }
+}
+
+// ---------------------------------------------------------------------------------------------------
 // Profiling of kernel classes with HIP events on the handle's stream
 // ---------------------------------------------------------------------------------------------------
 struct ProfRecord {
