@@ -87,7 +87,21 @@ def main():
     ap.add_argument('--cpu-steps', type=int, default=10)
     ap.add_argument('--no-cpu', action='store_true')
     ap.add_argument('--no-compare', action='store_true')
+    ap.add_argument('--calibrate-copy', type=int, default=0, metavar='MIB',
+                    help='only run a plain device-to-device copy of MIB MiB (for scaling HBM byte counters)')
     args = ap.parse_args()
+    if args.calibrate_copy:
+        import torch
+        n = args.calibrate_copy * (1 << 20) // 4
+        src = torch.ones(n, dtype=torch.float32, device='cuda')
+        dst = torch.empty_like(src)
+        for _ in range(3):
+            dst.copy_(src)
+        torch.cuda.synchronize()
+        log('copied %d MiB three times (read %d MiB + write %d MiB per copy)' % (args.calibrate_copy,
+                                                                              args.calibrate_copy,
+                                                                              args.calibrate_copy))
+        return
     if args.gpus != 1:
         print('bench.py runs on one GPU only: this stepper has no multi-GPU path (use --gpus 1)', file=sys.stderr)
         sys.exit(2)

@@ -377,7 +377,7 @@ __device__ inline cplx<T> load_b(const cplx<T>* __restrict__ B, long idx, int f,
   return v;
 }
 
-template <int CM, int PRO, int EPI, int BN>
+template <int CM, int PRO, int EPI, int BN, int RT>
 __global__ __launch_bounds__(1024) void f_hankel(
     const typename MfmaTraits<CM>::frag_t* __restrict__ Ahi, const typename MfmaTraits<CM>::frag_t* __restrict__ Alo,
     const cplx<typename MfmaTraits<CM>::T>* __restrict__ B, cplx<typename MfmaTraits<CM>::T>* __restrict__ out,
@@ -406,9 +406,31 @@ __global__ __launch_bounds__(1024) void f_hankel(
     cu = stage == 1 ? 0.0 : (stage == 4 ? dz : dz / 2);
   }
 
+  // ---- A fragments: a ring of RING contraction chunks per row tile, the first RING issued before the B
+  // slab is staged so their latency overlaps it
+  using F = typename MfmaTraits<CM>::frag_t;
+  constexpr int RING = (CM == CM_F64 || RT == 2) ? 2 : 4;
+  constexpr int rt_n = RT;  // 16-row tiles per wave (rw = 16 RT)
+  const int mt0 = (row0 + wave * rw) / 16;  // first 16-row tile of this wave
+  const int M16 = (M + 15) / 16;
+  const int NCH = CM == CM_BF16X3 ? Kp / 32 : Kp / 16;  // contraction chunks
+  F ring_h[RING][RT], ring_l[RING][CM == CM_BF16X3 ? RT : 1];
+  auto fetch_frag = [&](int ch, int slot) {
+    if (ch >= NCH) return;
+#pragma unroll
+    for (int r = 0; r < RT; r++)
+      if (mt0 + r < M16) {
+        const long fi = ((long)(mt0 + r) * NCH + ch) * 64 + lane;
+        ring_h[slot][r] = Ahi[fi];
+        if constexpr (CM == CM_BF16X3) ring_l[slot][r] = Alo[fi];
+      }
+  };
+#pragma unroll
+  for (int d = 0; d < RING; d++) fetch_frag(d, d);
+
   // ---- stage the B slab (Kp x BN) in LDS with the prologue applied; loads of a batch are issued together
   constexpr int HC = BN / 2;  // complex columns per tile
-  constexpr int EB = 4;       // elements per thread per batch
+  constexpr int EB = 2;       // elements per thread per batch
   const int ldb = Kp + 8;     // bf16 row stride (k contiguous), padded
   const int ne = Kp * HC;
   for (int base = 0; base < ne; base += EB * nth) {
@@ -472,12 +494,9 @@ __global__ __launch_bounds__(1024) void f_hankel(
 
   // ---- MFMA main loop: wave w owns rows [row0 + w rw, row0 + (w+1) rw), rw <= 32
   constexpr int CT = BN / 16;
-  const int rt_n = rw / 16;
-  const int mt0 = (row0 + wave * rw) / 16;  // first 16-row tile of this wave
-  const int M16 = (M + 15) / 16;
-  acc_t acc[2][CT];
+  acc_t acc[RT][CT];
 #pragma unroll
-  for (int r = 0; r < 2; r++)
+  for (int r = 0; r < RT; r++)
 #pragma unroll
     for (int c = 0; c < CT; c++) acc[r][c] = acc_t{0, 0, 0, 0};
 
@@ -485,58 +504,58 @@ __global__ __launch_bounds__(1024) void f_hankel(
     const __bf16* bhi = reinterpret_cast<const __bf16*>(smem);
     const __bf16* blo = bhi + BN * ldb;
     const int KC = Kp / 32;
-    for (int kc = 0; kc < KC; kc++) {
-      bf16x8 ah[2], al[2];
+    for (int kc0 = 0; kc0 < KC; kc0 += RING) {
 #pragma unroll
-      for (int r = 0; r < 2; r++)
-        if (r < rt_n && mt0 + r < M16) {
-          const long fi = ((long)(mt0 + r) * KC + kc) * 64 + lane;
-          ah[r] = reinterpret_cast<const bf16x8*>(Ahi)[fi];
-          al[r] = reinterpret_cast<const bf16x8*>(Alo)[fi];
-        }
-      bf16x8 bh[CT], bl[CT];
-#pragma unroll
-      for (int c = 0; c < CT; c++) {
-        const int col = c * 16 + (lane & 15), ko = kc * 32 + 8 * (lane >> 4);
-        bh[c] = *reinterpret_cast<const bf16x8*>(bhi + col * ldb + ko);
-        bl[c] = *reinterpret_cast<const bf16x8*>(blo + col * ldb + ko);
-      }
-#pragma unroll
-      for (int r = 0; r < 2; r++) {
-        if (r >= rt_n || mt0 + r >= M16) break;
+      for (int d = 0; d < RING; d++) {
+        const int kc = kc0 + d;
+        if (kc >= KC) break;
+        bf16x8 bh[CT], bl[CT];
 #pragma unroll
         for (int c = 0; c < CT; c++) {
-          acc[r][c] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(ah[r], bh[c], acc[r][c], 0, 0, 0);
-          acc[r][c] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(ah[r], bl[c], acc[r][c], 0, 0, 0);
-          acc[r][c] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(al[r], bh[c], acc[r][c], 0, 0, 0);
+          const int col = c * 16 + (lane & 15), ko = kc * 32 + 8 * (lane >> 4);
+          bh[c] = *reinterpret_cast<const bf16x8*>(bhi + col * ldb + ko);
+          bl[c] = *reinterpret_cast<const bf16x8*>(blo + col * ldb + ko);
         }
+#pragma unroll
+        for (int r = 0; r < RT; r++) {
+          if (mt0 + r >= M16) break;
+#pragma unroll
+          for (int c = 0; c < CT; c++) {
+            acc[r][c] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(ring_h[d][r], bh[c], acc[r][c], 0, 0, 0);
+            acc[r][c] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(ring_h[d][r], bl[c], acc[r][c], 0, 0, 0);
+            acc[r][c] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(ring_l[d][r], bh[c], acc[r][c], 0, 0, 0);
+          }
+        }
+        fetch_frag(kc + RING, d);
       }
     }
   } else {
     const T* bs = reinterpret_cast<const T*>(smem);
     const int KQ = Kp / 16;
-    for (int kq = 0; kq < KQ; kq++) {
-      typename MfmaTraits<CM>::frag_t af[2];
+    for (int kq0 = 0; kq0 < KQ; kq0 += RING) {
 #pragma unroll
-      for (int r = 0; r < 2; r++)
-        if (r < rt_n && mt0 + r < M16) af[r] = Ahi[((long)(mt0 + r) * KQ + kq) * 64 + lane];
+      for (int d = 0; d < RING; d++) {
+        const int kq = kq0 + d;
+        if (kq >= KQ) break;
 #pragma unroll
-      for (int j = 0; j < 4; j++) {
-        const int krow = kq * 16 + 4 * j + (lane >> 4);
-        T b[CT];
+        for (int j = 0; j < 4; j++) {
+          const int krow = kq * 16 + 4 * j + (lane >> 4);
+          T b[CT];
 #pragma unroll
-        for (int c = 0; c < CT; c++) b[c] = bs[krow * BN + c * 16 + (lane & 15)];
+          for (int c = 0; c < CT; c++) b[c] = bs[krow * BN + c * 16 + (lane & 15)];
 #pragma unroll
-        for (int r = 0; r < 2; r++) {
-          if (r >= rt_n || mt0 + r >= M16) break;
+          for (int r = 0; r < 2; r++) {
+            if (r >= rt_n || mt0 + r >= M16) break;
 #pragma unroll
-          for (int c = 0; c < CT; c++) {
-            if constexpr (CM == CM_F64)
-              acc[r][c] = __builtin_amdgcn_mfma_f64_16x16x4f64(af[r][j], b[c], acc[r][c], 0, 0, 0);
-            else
-              acc[r][c] = __builtin_amdgcn_mfma_f32_16x16x4f32(af[r][j], b[c], acc[r][c], 0, 0, 0);
+            for (int c = 0; c < CT; c++) {
+              if constexpr (CM == CM_F64)
+                acc[r][c] = __builtin_amdgcn_mfma_f64_16x16x4f64(ring_h[d][r][j], b[c], acc[r][c], 0, 0, 0);
+              else
+                acc[r][c] = __builtin_amdgcn_mfma_f32_16x16x4f32(ring_h[d][r][j], b[c], acc[r][c], 0, 0, 0);
+            }
           }
         }
+        fetch_frag(kq + RING, d);
       }
     }
   }
@@ -545,8 +564,7 @@ __global__ __launch_bounds__(1024) void f_hankel(
   // ---- accumulators -> LDS tile [bm][BN]
   T* os = reinterpret_cast<T*>(smem);
 #pragma unroll
-  for (int r = 0; r < 2; r++) {
-    if (r >= rt_n) break;
+  for (int r = 0; r < RT; r++) {
 #pragma unroll
     for (int c = 0; c < CT; c++)
 #pragma unroll
@@ -624,6 +642,43 @@ __device__ inline double ion_rate<float>(double ea, const uppe_ionization& ion, 
   return ion.rate_conv * (double)expf((float)poly);
 }
 
+// contiguous per-thread runs of LDS elements moved as 16-byte vectors (the run start is 16-byte aligned
+// whenever the run is at least 16 bytes long); shorter runs fall back to scalar accesses
+template <typename T, int RUN>
+__device__ inline void lds_read_run(const T* src, T* dst) {
+  constexpr int V = 16 / sizeof(T);
+  if constexpr (RUN % V == 0) {
+    typedef T vec __attribute__((ext_vector_type(V)));
+#pragma unroll
+    for (int i = 0; i < RUN / V; i++) {
+      vec v = reinterpret_cast<const vec*>(src)[i];
+#pragma unroll
+      for (int k = 0; k < V; k++) dst[i * V + k] = v[k];
+    }
+  } else {
+#pragma unroll
+    for (int j = 0; j < RUN; j++) dst[j] = src[j];
+  }
+}
+
+template <typename T, int RUN>
+__device__ inline void lds_write_run(T* dst, const T* src) {
+  constexpr int V = 16 / sizeof(T);
+  if constexpr (RUN % V == 0) {
+    typedef T vec __attribute__((ext_vector_type(V)));
+#pragma unroll
+    for (int i = 0; i < RUN / V; i++) {
+      vec v;
+#pragma unroll
+      for (int k = 0; k < V; k++) v[k] = src[i * V + k];
+      reinterpret_cast<vec*>(dst)[i] = v;
+    }
+  } else {
+#pragma unroll
+    for (int j = 0; j < RUN; j++) dst[j] = src[j];
+  }
+}
+
 // one workgroup (up to 1024 threads) per strip; thread t owns time samples [t SPT, (t+1) SPT)
 template <typename T, int SPT, bool DIAG>
 __global__ __launch_bounds__(1024) void f_source(const cplx<T>* __restrict__ er, cplx<T>* __restrict__ jr,
@@ -635,14 +690,20 @@ __global__ __launch_bounds__(1024) void f_source(const cplx<T>* __restrict__ er,
   if (st && !st->active) return;
   extern __shared__ char smem[];
   cplx<T>* Y = reinterpret_cast<cplx<T>*>(smem);
-  cplx<T>* tws = Y + nt;  // twiddles exp(-2 pi i k/Nt), k < Nt/2
   __shared__ double wsum[16];
   const int N = nt, M = nt / 2, lgm = lgn - 1;
   const long p = blockIdx.x;
   const int tid = threadIdx.x, nth = blockDim.x, lane = tid & 63, wave = tid >> 6, nwv = nth >> 6;
   const cplx<T>* X = er + p * nwp;
-  for (int k = tid; k < M; k += nth) tws[k] = tw[k];
-  __syncthreads();
+  // per-pass twiddle tables (N + N/2 entries): in LDS for FP32, read from L2 for FP64 (LDS would not fit)
+  const cplx<T>* tws = tw;
+  if constexpr (sizeof(T) == 4) {
+    cplx<T>* tl = Y + nt;
+    for (int k = tid; k < N + M; k += nth) tl[k] = tw[k];
+    tws = tl;
+    __syncthreads();
+  }
+  const bool lg_odd = (lgn & 1) != 0;
 
   // 1. irfft (numpy semantics; imaginary parts of the DC and Nyquist bins are ignored, fft.cl:271-278)
   for (int k = tid; k < M; k += nth) {
@@ -652,19 +713,23 @@ __global__ __launch_bounds__(1024) void f_source(const cplx<T>* __restrict__ er,
       xm.im = T(0);
     }
     cplx<T> ze = cscale(cadd(xk, cconj(xm)), T(0.5));
-    cplx<T> zo = cmul(cscale(csub(xk, cconj(xm)), T(0.5)), cconj(tws[k]));
+    cplx<T> zo = cmul(cscale(csub(xk, cconj(xm)), T(0.5)), cconj(stockham_twn(tws, N, lg_odd, k)));
     Y[k] = cmk(ze.re - zo.im, ze.im + zo.re);
   }
   __syncthreads();
-  lds_fft_stockham(Y, M, lgm, tws, N, +1);
+  lds_fft_stockham_tab(Y, M, lgm, tws, N, +1);
   const T invm = T(1) / T(M);
   const T* ys = reinterpret_cast<const T*>(Y);
   const int a0 = tid * SPT;
   T e[SPT + 1];
+  lds_read_run<T, SPT>(ys + a0, e);
 #pragma unroll
-  for (int j = 0; j <= SPT; j++) {
-    int n = a0 + j;
-    e[j] = n < N ? ys[n] * invm : T(0);
+  for (int j = 0; j < SPT; j++) e[j] *= invm;
+  {
+    // the sample after this thread's run is the next thread's first one
+    T nxt = __shfl_down(e[0], 1);
+    if (lane == 63) nxt = a0 + SPT < N ? ys[a0 + SPT] * invm : T(0);
+    e[SPT] = nxt;
   }
   __syncthreads();
 
@@ -720,19 +785,22 @@ __global__ __launch_bounds__(1024) void f_source(const cplx<T>* __restrict__ er,
 
   // 3. Kerr chi = ng chi3/nref E^2 (uppe.cl:132-151), P = chi E, J_p = ne E, packed as P + i J_p
   const T kc = (T)(ngp * redchi3);
+  T yrun[2 * SPT];
 #pragma unroll
   for (int j = 0; j < SPT; j++) {
     int n = a0 + j;
     T chi = kc * e[j] * e[j];
     T nej = (T)ne[j];
-    Y[n] = cmk(chi * e[j], nej * e[j]);
+    yrun[2 * j] = chi * e[j];
+    yrun[2 * j + 1] = nej * e[j];
     if (DIAG) {
       chi_out[p * N + n] = chi;
       ne_out[p * N + n] = nej;
     }
   }
+  lds_write_run<T, 2 * SPT>(reinterpret_cast<T*>(Y + a0), yrun);
   __syncthreads();
-  lds_fft_stockham(Y, N, lgn, tws, N, -1);
+  lds_fft_stockham_tab(Y, N, lgn, tws, N, -1);
 
   // 4. J = F_NL [ rfft(J_p)/(0.1 - i w) + (-i w) rfft(P) ] (fft.cl:282-329, uppe_kernel.py:170-186)
   for (int k = tid; k <= M; k += nth) {
@@ -1011,13 +1079,8 @@ struct Fused {
     UPPE_HIP(hipMemset(s->st, 0, sizeof(StepState)));
     UPPE_HIP(hipHostMalloc(&s->st_host, sizeof(StepState), hipHostMallocDefault));
     std::memset(s->st_host, 0, sizeof(StepState));
-    // twiddles exp(-2 pi i k/Nt), computed in FP64 and rounded to T
-    std::vector<C> t(h->nt / 2);
-    for (int k = 0; k < h->nt / 2; k++) {
-      double s_, c_;
-      sincos(2.0 * M_PI * k / h->nt, &s_, &c_);
-      t[k] = cmk((T)c_, (T)(-s_));
-    }
+    // per-pass Stockham twiddles for the Nt- and Nt/2-point transforms, computed in FP64 and rounded to T
+    std::vector<C> t = stockham_tables<T>(h->nt);
     UPPE_HIP(hipMalloc(&s->tw, sizeof(C) * t.size()));
     UPPE_HIP(hipMemcpy(s->tw, t.data(), sizeof(C) * t.size(), hipMemcpyHostToDevice));
     if (h->d.geometry == UPPE_RADIAL) {
@@ -1124,7 +1187,7 @@ struct Fused {
     const int N = h->nt;
     const int nth = std::min(N, 1024);
     const int spt = N / nth;
-    const size_t shm = sizeof(C) * (N + N / 2);
+    const size_t shm = sizeof(C) * (sizeof(T) == 4 ? N + N + N / 2 : N);
     const int lgn = ilog2i(N);
     dim3 grid((unsigned)h->P), block(nth);
 #define UPPE_SRC(S)                                                                                              \
@@ -1161,14 +1224,21 @@ struct Fused {
     size_t slab = CM == CM_BF16X3 ? 2ull * bn * (kp + 8) * 2 : (size_t)kp * bn * sizeof(T);
     size_t tile = (size_t)bm * bn * sizeof(T);
     size_t shm = std::max(slab, tile);
-#define UPPE_HK(BNV)                                                                                             \
-  launchf(h, UPPE_KC_HANKEL, f_hankel<CM, PRO, EPI, BNV>, grid, dim3(64 * nwv), shm, (const F*)ahi, (const F*)alo, B, out, \
+#define UPPE_HK(BNV, RTV)                                                                                        \
+  launchf(h, UPPE_KC_HANKEL, f_hankel<CM, PRO, EPI, BNV, RTV>, grid, dim3(64 * nwv), shm, (const F*)ahi, (const F*)alo, B, out, \
           M, K, kp, rw, s->nwp, 0, (const double*)s->phs, (const C*)s->kzc, (const T*)s->cond, (C*)s->q, kprev,      \
           (C*)s->k1, (C*)s->k2, (C*)s->k3, st, s->qmax, s->qmax_next, stage, zfix)
-    if (bn == 16)
-      UPPE_HK(16);
-    else
-      UPPE_HK(32);
+    if (rw == 16) {
+      if (bn == 16)
+        UPPE_HK(16, 1);
+      else
+        UPPE_HK(32, 1);
+    } else {
+      if (bn == 16)
+        UPPE_HK(16, 2);
+      else
+        UPPE_HK(32, 2);
+    }
 #undef UPPE_HK
   }
 

@@ -168,6 +168,100 @@ __device__ inline void lds_fft_stockham(cplx<T>* buf, int n, int lg, const cplx<
 }
 
 // ---------------------------------------------------------------------------------------------------
+// Stockham FFT with per-pass twiddle tables, so that the threads of a pass read consecutive table entries
+// (the single exp(-2 pi i k/nbase) table is read at power-of-two strides, which puts a wave on one LDS bank).
+// Layout of `tab` (built by stockham_tables): the radix-4 pass of span Ns uses tab[Ns - 1 + (r-1) Ns + k] =
+// exp(-2 pi i r k/(4 Ns)), k < Ns, r = 1..3; the radix-2 pass of an n-point transform (log2 n odd) uses
+// tab[r2off + k] = exp(-2 pi i k/n), k < n/2. Inverse transforms conjugate.
+// ---------------------------------------------------------------------------------------------------
+template <typename T>
+__device__ inline void lds_fft_stockham_tab(cplx<T>* buf, int n, int lg, const cplx<T>* tab, int r2off, int sign) {
+  const int nth = blockDim.x, tid = threadIdx.x;
+  int Ns = 1;
+  int rem = lg;
+  while (rem > 0) {
+    const int R = rem >= 2 ? 4 : 2;
+    const int nb = n / R;
+    cplx<T> v[2][4];
+#pragma unroll
+    for (int b = 0; b < 2; b++) {
+      const int j = tid + b * nth;
+      if (j < nb)
+#pragma unroll
+        for (int r = 0; r < 4; r++)
+          if (r < R) v[b][r] = buf[j + r * nb];
+    }
+    __syncthreads();
+#pragma unroll
+    for (int b = 0; b < 2; b++) {
+      const int j = tid + b * nth;
+      if (j >= nb) continue;
+      const int k = j & (Ns - 1);
+      cplx<T> y[4];
+      if (R == 4) {
+        cplx<T> w1 = tab[Ns - 1 + k], w2 = tab[2 * Ns - 1 + k], w3 = tab[3 * Ns - 1 + k];
+        if (sign > 0) {
+          w1.im = -w1.im;
+          w2.im = -w2.im;
+          w3.im = -w3.im;
+        }
+        cplx<T> v1 = cmul(v[b][1], w1), v2 = cmul(v[b][2], w2), v3 = cmul(v[b][3], w3);
+        cplx<T> a0 = cadd(v[b][0], v2), a1 = csub(v[b][0], v2), a2 = cadd(v1, v3), d = csub(v1, v3);
+        cplx<T> a3 = sign < 0 ? cmk(d.im, -d.re) : cmk(-d.im, d.re);
+        y[0] = cadd(a0, a2);
+        y[1] = cadd(a1, a3);
+        y[2] = csub(a0, a2);
+        y[3] = csub(a1, a3);
+      } else {
+        cplx<T> w = tab[r2off + k];
+        if (sign > 0) w.im = -w.im;
+        cplx<T> v1 = cmul(v[b][1], w);
+        y[0] = cadd(v[b][0], v1);
+        y[1] = csub(v[b][0], v1);
+      }
+      const int o = (j - k) * R + k;
+#pragma unroll
+      for (int r = 0; r < 4; r++)
+        if (r < R) buf[o + r * Ns] = y[r];
+    }
+    __syncthreads();
+    Ns *= R;
+    rem -= R == 4 ? 2 : 1;
+  }
+}
+
+// host: per-pass tables for an N-point and an N/2-point transform (N a power of two >= 4), FP64 -> T,
+// N + N/2 entries with r2off = N
+template <typename T>
+inline std::vector<cplx<T>> stockham_tables(int N) {
+  std::vector<cplx<T>> tab((size_t)N + N / 2, cplx<T>{T(0), T(0)});
+  int lgN = 0;
+  while ((1 << lgN) < N) lgN++;
+  for (int Ns = 1; 4 * Ns <= N; Ns *= 4)
+    for (int r = 1; r <= 3; r++)
+      for (int k = 0; k < Ns; k++) {
+        const double a = -2.0 * M_PI * r * k / (4.0 * Ns);
+        tab[(size_t)Ns - 1 + (size_t)(r - 1) * Ns + k] = cplx<T>{(T)std::cos(a), (T)std::sin(a)};
+      }
+  const int nodd = (lgN & 1) ? N : N / 2;
+  for (int k = 0; k < nodd / 2; k++) {
+    const double a = -2.0 * M_PI * k / nodd;
+    tab[(size_t)N + k] = cplx<T>{(T)std::cos(a), (T)std::sin(a)};
+  }
+  return tab;
+}
+
+// exp(-2 pi i k/N), k < N/2, read from the per-pass tables
+template <typename T>
+__device__ inline cplx<T> stockham_twn(const cplx<T>* tab, int N, bool lg_odd, int k) {
+  if (lg_odd) return tab[N + k];
+  const int q = N / 4;
+  if (k < q) return tab[q - 1 + k];
+  cplx<T> w = tab[q - 1 + k - q];
+  return cmk(w.im, -w.re);  // -i w
+}
+
+// ---------------------------------------------------------------------------------------------------
 // Profiling of kernel classes with HIP events on the handle's stream
 // ---------------------------------------------------------------------------------------------------
 struct ProfRecord {
