@@ -377,6 +377,13 @@ __device__ inline cplx<T> load_b(const cplx<T>* __restrict__ B, long idx, int f,
   return v;
 }
 
+// Workgroups are dealt to the 8 XCDs round-robin; renumber them so that each XCD owns a contiguous run of
+// column tiles, whose partial 128-byte lines then meet in that XCD's L2.
+__device__ inline int xcd_tile(int b, int G) {
+  const int q = G >> 3, r = G & 7, x = b & 7, slot = b >> 3;
+  return x < r ? x * (q + 1) + slot : r * (q + 1) + (x - r) * q + slot;
+}
+
 template <int CM, int PRO, int EPI, int BN, int RT>
 __global__ __launch_bounds__(1024) void f_hankel(
     const typename MfmaTraits<CM>::frag_t* __restrict__ Ahi, const typename MfmaTraits<CM>::frag_t* __restrict__ Alo,
@@ -397,7 +404,7 @@ __global__ __launch_bounds__(1024) void f_hankel(
   const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
   const int bm = nwv * rw;
   const int row0 = blockIdx.y * bm;
-  const int fc0 = f0 + blockIdx.x * (BN / 2);  // first complex column of this tile
+  const int fc0 = f0 + xcd_tile(blockIdx.x, gridDim.x) * (BN / 2);  // first complex column of this tile
   double zph = zfix, cu = 0.0, dz = 0.0;
   if (st) {
     dz = st->dz;
@@ -1041,11 +1048,12 @@ static void* upload(const std::vector<V>& v) {
   return d;
 }
 
-// one workgroup covers all M rows with up to 16 waves of at most two 16-row tiles each
+// one workgroup covers all M rows with up to 16 waves of at most two 16-row tiles each (fewer waves per
+// workgroup, i.e. several workgroups per CU, measured no faster at Nr = 256)
 static int hankel_waves(int M) { return std::max(1, std::min(16, (M + 15) / 16)); }
 static int rows_per_wave(int M) {
   const int m16 = (M + 15) / 16, w = hankel_waves(M);
-  return 16 * ((m16 + w - 1) / w);
+  return std::min(32, 16 * ((m16 + w - 1) / w));
 }
 
 template <typename T>
@@ -1123,6 +1131,8 @@ struct Fused {
     }
     s->rw_b = rows_per_wave(nr);
     s->rw_f = rows_per_wave(nk);
+    // 8 complex columns per workgroup: at Nw = 2049 that is ~260 workgroups, one per CU (wider tiles halve
+    // the grid below the CU count and measured slower in FP32/FP64)
     s->bn_b = s->bn_f = 16;
   }
 
