@@ -7,9 +7,15 @@ grid; the timed region runs the first K steps of that run from the initial field
 
 Printed JSON line (one):
   value         grid-point z-steps per second = Nw * Nr * K / T
-  roofline      the dominant kernel class (largest share of kernel time) against its bound; kernel durations
-                come from HIP events recorded on the library's stream around every launch, in a repeat of the
-                timed steps (the events themselves lengthen the step, so `value` is taken without them)
+  roofline      the dominant kernel class (largest share of kernel time) against HBM bandwidth: algorithmic
+                bytes per launch (per grid point: 40 B for a radial transform launch -- the k->r pass, 16 B, and
+                the r->k pass with the RK stage, 40 B, plus 24 B of kz tables, averaged over the two -- and 16 B
+                for the source pass; doubled in FP64) over the average launch duration. Durations come from HIP
+                events recorded on the library's stream around every launch, in a repeat of the timed steps (the
+                events lengthen the step, so `value` is taken without them). The transform is also reported
+                against the matrix-core peak of the type it issues. `traffic` is the measured HBM bytes per
+                launch (rocprofv3 FETCH_SIZE + WRITE_SIZE, copy-calibrated) read from the committed profile
+                summary of the same kernel and workload, or null
   cpu_baseline  the FP64 oracle (reference algorithm and pass structure, OpenMP) on a bounded sample
   extras        speedup over the plain FP64 GPU stepper, relative L2 error of the state against the fused FP64
                 stepper on the same step schedule
@@ -50,6 +56,28 @@ def run_steps(rs, st, K, schedule=None, record=False):
     return n, t1 - t0, rec
 
 
+def hankel_bytes(rs, el):
+    '''Algorithmic bytes of one radial transform launch, averaged over k->r and r->k: 40 B per grid point in
+    FP32 (16 + 40 + 24 over two launches), twice that in FP64.'''
+    return 40.0 * (el / 4) * rs.N[0] * rs.N[1]
+
+
+def step_bytes(rs, el):
+    '''Algorithmic bytes of one RK4 step: 4 right-hand sides x 96 B per grid point in FP32.'''
+    return 384.0 * (el / 4) * rs.N[0] * rs.N[1]
+
+
+def measured_traffic(prec, kernel):
+    '''HBM bytes per launch from the committed rocprofv3 counter summary of this workload, if present.'''
+    path = os.path.join(ROOT, 'profiles', 'traffic_radial_o2_2049x256.json')
+    try:
+        with open(path) as f:
+            tab = json.load(f)
+        return tab[prec][kernel]['bytes_per_launch']
+    except (OSError, KeyError, ValueError):
+        return None
+
+
 def hankel_flops(rs):
     '''Algorithmic flops of one radial transform launch: 4 Nw Nk Nr (real contraction of complex data).'''
     T = rs.field_tool.GetTransverseTool()
@@ -58,7 +86,45 @@ def hankel_flops(rs):
 
 def source_bytes(rs, el):
     '''Algorithmic bytes of one source launch: read E_r and write J_r (Nw complex per strip).'''
-    return 2.0 * rs.N[0] * rs.N[1] * 2 * el
+    return 2.0 * rs.N[0] * rs.N[1] * 2 * el  # 16 B per grid point in FP32
+
+
+def pcie_inclusive(rs, prec, K):
+    '''Grid-pt z-steps/s with the host field upload (set_field) and the finish outputs (host arrays) inside the
+    timed region.'''
+    st = rs.make_stepper(uppe_kernel.hip_stepper_factory(prec))
+    zi, zf, L = rs.plane(0)
+    run_steps(rs, st, 2)
+    st.synchronize()
+    t0 = time.perf_counter()
+    st.set_field(rs.A0)
+    n, _ = st.advance(zi, L, 0, rs.subcycles, rs.dzmin, max_steps=K)
+    st.finish(zf - zi)
+    t1 = time.perf_counter()
+    st.close()
+    return rs.N[0] * rs.N[1] * n / (t1 - t0)
+
+
+def config1_speedup(prec, steps_plain):
+    '''Radial Kerr workload (Nw = 513, Nr = 128, 100 fixed steps over 1 m): the fused stepper against the plain
+    FP64 stepper, and its field error against the fused FP64 stepper (fixed steps: identical schedules).'''
+    vol = configs.radial_kerr_air()
+    rs = uppe_kernel.RunSetup(vol)
+    pts = rs.N[0] * rs.N[1]
+    out = {}
+    states = {}
+    for p, K in ((prec, rs.subcycles), ('fp64', rs.subcycles), ('fp64-plain', steps_plain)):
+        st = rs.make_stepper(uppe_kernel.hip_stepper_factory(p))
+        run_steps(rs, st, 2)
+        n, secs, _ = run_steps(rs, st, K)
+        out[p] = pts * n / secs
+        if p != 'fp64-plain':
+            states[p] = st.get_state()
+        st.close()
+    err = float(np.linalg.norm(states[prec] - states['fp64']) / np.linalg.norm(states['fp64']))
+    return dict(workload='radial Kerr in air, Nw=513, Nr=128, 100 fixed steps over 1 m',
+                pt_steps_per_s=out[prec], fp64_plain_pt_steps_per_s=out['fp64-plain'],
+                speedup_vs_fp64_plain=out[prec] / out['fp64-plain'], rel_l2_err_vs_fp64=err)
 
 
 def cpu_baseline(vol, steps):
@@ -81,10 +147,10 @@ def main():
     ap.add_argument('--gpus', type=int, default=1)
     ap.add_argument('--steps', type=int, default=400)
     ap.add_argument('--warmup', type=int, default=20)
-    ap.add_argument('--precision', default='fp32', choices=['bf16x3', 'fp32', 'fp64', 'fp64-plain'])
+    ap.add_argument('--precision', default='bf16x3', choices=['bf16x3', 'fp32', 'fp64', 'fp64-plain'])
     ap.add_argument('--rate', default='explicit', choices=['explicit', 'series'])
     ap.add_argument('--plain-steps', type=int, default=20)
-    ap.add_argument('--cpu-steps', type=int, default=10)
+    ap.add_argument('--cpu-steps', type=int, default=60)
     ap.add_argument('--no-cpu', action='store_true')
     ap.add_argument('--no-compare', action='store_true')
     ap.add_argument('--calibrate-copy', type=int, default=0, metavar='MIB',
@@ -138,20 +204,23 @@ def main():
     avg_s = ms / max(cnt, 1) * 1e-3
     el = 8 if prec in ('fp64', 'fp64-plain') else 4
     if dom == 'hankel':
+        B = hankel_bytes(rs, el)
         F = hankel_flops(rs)
         pk = PEAK_TFLOPS['fp64' if el == 8 else ('bf16x3' if prec == 'bf16x3' else 'fp32')]
         issued = 3.0 if prec == 'bf16x3' else 1.0
-        achieved = F * issued / avg_s / 1e12
-        roofline = dict(bound='mfma', achieved=achieved, peak=pk, unit='TFLOP/s', frac=achieved / pk,
-                        traffic=None, kernel='hankel (radial transform GEMM)', avg_launch_us=avg_s * 1e6,
-                        launches=cnt, work_per_launch='%.4g flop (4 Nw Nk Nr%s)'
-                        % (F * issued, ', x3 bf16 products' if issued > 1 else ''))
+        mfma = F * issued / avg_s / 1e12
+        kname = 'hankel (radial transform: MFMA contraction with fused stage prologue / RK epilogue)'
+        extra = dict(mfma_achieved_tflops=mfma, mfma_peak_tflops=pk, mfma_frac=mfma / pk,
+                     flop_per_launch=F * issued)
     else:
         B = source_bytes(rs, el)
-        achieved = B / avg_s / 1e9
-        roofline = dict(bound='hbm', achieved=achieved, peak=HBM_PEAK_GBS, unit='GB/s', frac=achieved / HBM_PEAK_GBS,
-                        traffic=None, kernel=dom, avg_launch_us=avg_s * 1e6, launches=cnt,
-                        work_per_launch='%.4g B' % B)
+        kname = dom
+        extra = {}
+    achieved = B / avg_s / 1e9
+    roofline = dict(bound='hbm', achieved=achieved, peak=HBM_PEAK_GBS, unit='GB/s', frac=achieved / HBM_PEAK_GBS,
+                    traffic=measured_traffic(prec, dom), kernel=kname, avg_launch_us=avg_s * 1e6, launches=cnt,
+                    bytes_per_launch=B, **extra)
+    roofline['step_hbm_frac'] = step_bytes(rs, el) / (ms_per_step * 1e-3) / 1e9 / HBM_PEAK_GBS
     share = {k: round(v[0] / max(1e-12, sum(c[0] for c in classes.values())), 4) for k, v in classes.items()
              if v[1] > 0}
     roofline['class_share'] = share
@@ -177,6 +246,10 @@ def main():
         sx.close()
         extras['rel_l2_err_vs_fp64'] = float(np.linalg.norm(qx - q64) / np.linalg.norm(q64))
         extras['error_note'] = 'k-space state after %d steps on the FP64 step schedule' % K
+        log('host-inclusive rate (set_field + %d steps + finish)' % K)
+        extras['pcie_inclusive_pt_steps_per_s'] = pcie_inclusive(rs, prec, K)
+        log('radial Kerr workload (Nw=513, Nr=128): %s vs plain FP64' % prec)
+        extras['config1'] = config1_speedup(prec, args.plain_steps)
 
     cpu = None
     if not args.no_cpu:

@@ -134,8 +134,10 @@ int uppe_get_state(uppe_handle* h, double* q_k);
 /* Counters since the last uppe_set_field. */
 int uppe_get_info(uppe_handle* h, uppe_info* info);
 
-/* Per-kernel-class timing with HIP events on the handle's stream (for measurement). enable != 0 starts
- * recording and clears totals; uppe_profile_read returns total milliseconds and launch count of a class. */
+/* Per-kernel-class timing with HIP events on the handle's stream (for measurement): in the fused paths each
+ * dispatch stamps its own start and end events, the plain path records an event pair around each launch.
+ * enable != 0 starts recording and clears totals; uppe_profile_read returns total milliseconds and launch
+ * count of a class. */
 enum { UPPE_KC_HANKEL = 0, UPPE_KC_SOURCE = 1, UPPE_KC_POINT = 2, UPPE_KC_REDUCE = 3, UPPE_KC_TRANSVERSE_FFT = 4,
        UPPE_KC_COUNT = 5 };
 int uppe_profile_enable(uppe_handle* h, int enable);

@@ -12,6 +12,13 @@
  *     including every per-step reduction (max, sum, min) of the step-size control.
  * Each function cites the reference lines it restates. Exported entry points mirror include/searay_uppe.h
  * with the prefix orc_.
+ *
+ * Pinning (tests/test_oracle.py): the strip FFTs against numpy; the linear propagator and transforms against
+ * the exact interaction-picture solution and the reference's Gaussian-beam test; RK4 against the Taylor
+ * polynomial of a linear right-hand side; the step-size rule against a brute-force restatement; the Kerr and
+ * plasma currents, the density scan and the rates against closed forms and the host models.
+ * Parity unpinned: the full nonlinear run as a whole (the reference stores no nonlinear output to compare
+ * with); it is pinned only through the parts above.
  */
 #include "../include/searay_uppe.h"
 

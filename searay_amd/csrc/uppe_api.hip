@@ -7,25 +7,33 @@ using namespace uppe;
 
 namespace uppe {
 
-ProfScope::ProfScope(uppe_handle* h_, int kc_) : h(h_), kc(kc_) {
-  if (!h->prof) return;
+void prof_take(uppe_handle* h, hipEvent_t* a, hipEvent_t* b) {
   if (h->prof_free.size() >= 2) {
-    a = h->prof_free.back();
+    *a = h->prof_free.back();
     h->prof_free.pop_back();
-    b = h->prof_free.back();
+    *b = h->prof_free.back();
     h->prof_free.pop_back();
   } else {
-    UPPE_HIP(hipEventCreate(&a));
-    UPPE_HIP(hipEventCreate(&b));
+    UPPE_HIP(hipEventCreate(a));
+    UPPE_HIP(hipEventCreate(b));
   }
+}
+
+void prof_push(uppe_handle* h, int kc, hipEvent_t a, hipEvent_t b) {
+  h->prof_pending.push_back(ProfRecord{kc, a, b});
+  if (h->prof_pending.size() > 4096) prof_collect(h);
+}
+
+ProfScope::ProfScope(uppe_handle* h_, int kc_) : h(h_), kc(kc_) {
+  if (!h->prof) return;
+  prof_take(h, &a, &b);
   UPPE_HIP(hipEventRecord(a, h->stream));
 }
 
 ProfScope::~ProfScope() noexcept(false) {
   if (!a) return;
   UPPE_HIP(hipEventRecord(b, h->stream));
-  h->prof_pending.push_back(ProfRecord{kc, a, b});
-  if (h->prof_pending.size() > 4096) prof_collect(h);
+  prof_push(h, kc, a, b);
 }
 
 void prof_collect(uppe_handle* h) {

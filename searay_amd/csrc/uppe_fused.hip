@@ -14,6 +14,8 @@
 //      per-point step estimate (paraxial.cl:40-58) and the field conditioner (uppe_kernel.py:131-137).
 // Step control (uppe_kernel.py:218-227, 299-325) runs on the device: two single-thread control kernels per
 // step, so a whole subcycle runs without host synchronisation.
+#include <hip/hip_ext.h>
+
 #include "uppe_internal.h"
 
 #include <algorithm>
@@ -272,9 +274,9 @@ __global__ __launch_bounds__(256) void f_gemm(const T* __restrict__ A, const cpl
 // ---------------------------------------------------------------------------------------------------
 // Hankel contraction on the matrix cores
 //
-// out[M][2 nwp] = A[M][K] . B[K][2 nwp] with the same prologue/epilogue as f_gemm. One workgroup (4 waves)
-// owns an output tile of 4*rw rows x BN real columns and stages its whole B slab (K x BN) in LDS once, so the
-// prologue runs exactly once per element. A is read as pre-swizzled MFMA fragments straight from L2/HBM:
+// out[M][2 nwp] = A[M][K] . B[K][2 nwp] with the same prologue/epilogue as f_gemm. One workgroup (up to 16
+// waves of rw = 16 or 32 rows) owns an output tile of all M rows x BN real columns and stages its whole B slab
+// (K x BN) in LDS once, so the prologue runs exactly once per element. A is read as pre-swizzled MFMA fragments straight from L2/HBM:
 //   CM_F64 / CM_F32: v_mfma_{f64,f32}_16x16x4, lane l holds A[16 rt + l%16][16 kq + 4 j + l/16], j < 4;
 //   CM_BF16X3:       v_mfma_f32_16x16x32_bf16, lane l holds A[16 rt + l%16][32 kc + 8 (l/16) + j], j < 8,
 //                    as hi = bf16(a) and lo = bf16(a - hi); the product is hi.hi + hi.lo + lo.hi in FP32.
@@ -982,8 +984,15 @@ static int ilog2i(long v) {
 
 template <typename K, typename... A>
 static void launchf(uppe_handle* h, int kc, K kernel, dim3 grid, dim3 block, size_t shm, A... args) {
-  ProfScope ps(h, kc);
-  hipLaunchKernelGGL(kernel, grid, block, shm, h->stream, args...);
+  if (!h->prof) {
+    hipLaunchKernelGGL(kernel, grid, block, shm, h->stream, args...);
+  } else {
+    // events stamped by the dispatch itself: they bracket the kernel's execution, not the gaps around it
+    hipEvent_t a, b;
+    prof_take(h, &a, &b);
+    hipExtLaunchKernelGGL(kernel, grid, block, (std::uint32_t)shm, h->stream, a, b, 0u, args...);
+    prof_push(h, kc, a, b);
+  }
   UPPE_LAUNCH_CHECK();
 }
 

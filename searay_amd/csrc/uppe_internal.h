@@ -311,6 +311,8 @@ struct ProfScope {
 };
 
 void prof_collect(uppe_handle* h);  // synchronise and fold finished events into totals
+void prof_take(uppe_handle* h, hipEvent_t* a, hipEvent_t* b);      // an unused event pair
+void prof_push(uppe_handle* h, int kc, hipEvent_t a, hipEvent_t b);  // queue a recorded pair for collection
 
 inline unsigned grid_for(long n, int block = kBlock) { return (unsigned)((n + block - 1) / block); }
 
