@@ -112,10 +112,10 @@ def o2_ionizer(kind='explicit'):
 
 
 def radial_o2_ionization(scale=1, steps=None, seed=1001, rate='explicit', precision=None, power=3.0,
-                         length='20 cm', uv_opacity=False):
+                         length='20 cm', uv_opacity=False, nr=None):
     helper = input_tools.InputHelper(MKS_LENGTH)
     nw = (2048 // scale) + 1
-    nr = max(8, 256 // scale)
+    nr = max(8, 256 // scale) if nr is None else nr
     band = (0.0, 8.0 / scale)
     L = helper.dnum(length)
     subcycles = 400 if steps is None else steps

@@ -69,7 +69,12 @@ struct FusedState {
   void *frb_hi = nullptr, *frb_lo = nullptr;  // fragments of Hb (k2r)
   void *frf_hi = nullptr, *frf_lo = nullptr;  // fragments of Hf (r2k)
   int kp_b = 0, kp_f = 0;                     // padded contraction lengths
-  int rw_b = 16, rw_f = 16, bn_b = 16, bn_f = 16;
+  // launch plan of each contraction direction (0 = k2r, 1 = r2k): column tiles of 8 complex columns (the
+  // last one also takes the remainder), row blocks per tile, waves per workgroup, 16-row tiles per wave
+  struct HkPlan {
+    int tiles = 1, split = 1, nwv = 1, rt = 1;
+  } plan[2];
+  int n_cu = 256;
   int f1 = 0;  // the r2k direction covers complex columns [0, f1): where the nonlinear filter is nonzero
 };
 
@@ -386,11 +391,14 @@ __device__ inline int xcd_tile(int b, int G) {
   return x < r ? x * (q + 1) + slot : r * (q + 1) + (x - r) * q + slot;
 }
 
+// B-slab elements each thread loads per batch (all loads of a batch are in flight together)
+constexpr int hankel_batch(int cm, int rt) { return (cm == CM_F64 || rt == 2) ? 2 : 4; }
+
 template <int CM, int PRO, int EPI, int BN, int RT>
 __global__ __launch_bounds__(1024) void f_hankel(
     const typename MfmaTraits<CM>::frag_t* __restrict__ Ahi, const typename MfmaTraits<CM>::frag_t* __restrict__ Alo,
     const cplx<typename MfmaTraits<CM>::T>* __restrict__ B, cplx<typename MfmaTraits<CM>::T>* __restrict__ out,
-    int M, int K, int Kp, int rw, int nwp, int f0, const double* __restrict__ phs,
+    int M, int K, int Kp, int rw, int nwp, int f0, int fend, const double* __restrict__ phs,
     const cplx<typename MfmaTraits<CM>::T>* __restrict__ kzc, const typename MfmaTraits<CM>::T* __restrict__ cond,
     cplx<typename MfmaTraits<CM>::T>* __restrict__ q, const cplx<typename MfmaTraits<CM>::T>* __restrict__ kprev,
     cplx<typename MfmaTraits<CM>::T>* __restrict__ k1, cplx<typename MfmaTraits<CM>::T>* __restrict__ k2,
@@ -399,14 +407,22 @@ __global__ __launch_bounds__(1024) void f_hankel(
   using T = typename MfmaTraits<CM>::T;
   using C = cplx<T>;
   using acc_t = typename MfmaTraits<CM>::acc_t;
-  if (st && !st->active) return;
+  // no early exit for a finished run: the contraction only writes scratch (E_r) or stops before its
+  // epilogue, so its loads need not wait for the step state
   extern __shared__ char smem[];
   __shared__ double red[16];
   const int nth = blockDim.x, nwv = nth >> 6;
   const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
   const int bm = nwv * rw;
   const int row0 = blockIdx.y * bm;
-  const int fc0 = f0 + xcd_tile(blockIdx.x, gridDim.x) * (BN / 2);  // first complex column of this tile
+  // column tiles of HC complex columns; the last tile of the grid also takes the remainder (fewer than HC
+  // columns), so that the grid never needs a workgroup for a sliver. ct_n = 16-column MFMA tiles in use.
+  constexpr int HC = BN / 2;
+  const int tile = xcd_tile(blockIdx.x, gridDim.x);
+  const int fc0 = f0 + tile * HC;  // first complex column of this tile
+  const int ct_n = (tile == (int)gridDim.x - 1 && fend - fc0 > HC) ? 2 : 1;
+  const int hcw = ct_n * HC;       // complex columns staged (columns >= fend are zero)
+  constexpr int BW = 2 * BN;       // LDS row width in real columns (room for two MFMA column tiles)
   double zph = zfix, cu = 0.0, dz = 0.0;
   if (st) {
     dz = st->dz;
@@ -437,25 +453,26 @@ __global__ __launch_bounds__(1024) void f_hankel(
 #pragma unroll
   for (int d = 0; d < RING; d++) fetch_frag(d, d);
 
-  // ---- stage the B slab (Kp x BN) in LDS with the prologue applied; loads of a batch are issued together
-  constexpr int HC = BN / 2;  // complex columns per tile
-  constexpr int EB = 2;       // elements per thread per batch
+  // ---- stage the B slab (Kp x hcw) in LDS with the prologue applied; loads of a batch are issued together
+  constexpr int EB = hankel_batch(CM, RT);  // register room: elements per thread per batch
   const int ldb = Kp + 8;     // bf16 row stride (k contiguous), padded
-  const int ne = Kp * HC;
-  for (int base = 0; base < ne; base += EB * nth) {
+  const int lgh = ct_n == 2 ? __builtin_ctz(HC) + 1 : __builtin_ctz(HC);
+  const int ne = Kp * hcw;
+  const int eb = min(EB, (ne + nth - 1) / nth);  // elements per thread per batch: as few as cover the slab
+  for (int base = 0; base < ne; base += eb * nth) {
     C vb[EB], vk[EB], vz[EB];
     double vp[EB];
     T vc[EB];
 #pragma unroll
     for (int j = 0; j < EB; j++) {
       const int e = base + tid + j * nth;
-      const int k = e / HC, f = fc0 + e % HC;
+      const int k = e >> lgh, f = fc0 + (e & (hcw - 1));
       vb[j] = cmk(T(0), T(0));
       vk[j] = cmk(T(0), T(0));
       vz[j] = cmk(T(1), T(0));
       vp[j] = 0.0;
       vc[j] = T(1);
-      if (e < ne && k < K && f < nwp) {
+      if (j < eb && e < ne && k < K && f < fend) {
         const long idx = (long)k * nwp + f;
         vb[j] = B[idx];
         if (PRO != PRO_NONE) {
@@ -471,8 +488,8 @@ __global__ __launch_bounds__(1024) void f_hankel(
 #pragma unroll
     for (int j = 0; j < EB; j++) {
       const int e = base + tid + j * nth;
-      if (e >= ne) break;
-      const int k = e / HC, cf = e % HC;
+      if (j >= eb || e >= ne) break;
+      const int k = e >> lgh, cf = e & (hcw - 1);
       C v = vb[j];
       if (PRO == PRO_STAGE) {
         if (stage > 1) {
@@ -486,7 +503,7 @@ __global__ __launch_bounds__(1024) void f_hankel(
       }
       if constexpr (CM == CM_BF16X3) {
         __bf16* bhi = reinterpret_cast<__bf16*>(smem);
-        __bf16* blo = bhi + BN * ldb;
+        __bf16* blo = bhi + BW * ldb;
         __bf16 hr = (__bf16)v.re, hi_ = (__bf16)v.im;
         bhi[(2 * cf) * ldb + k] = hr;
         bhi[(2 * cf + 1) * ldb + k] = hi_;
@@ -494,15 +511,15 @@ __global__ __launch_bounds__(1024) void f_hankel(
         blo[(2 * cf + 1) * ldb + k] = (__bf16)((float)v.im - (float)hi_);
       } else {
         T* bs = reinterpret_cast<T*>(smem);
-        bs[k * BN + 2 * cf] = v.re;
-        bs[k * BN + 2 * cf + 1] = v.im;
+        bs[k * BW + 2 * cf] = v.re;
+        bs[k * BW + 2 * cf + 1] = v.im;
       }
     }
   }
   __syncthreads();
 
   // ---- MFMA main loop: wave w owns rows [row0 + w rw, row0 + (w+1) rw), rw <= 32
-  constexpr int CT = BN / 16;
+  constexpr int CT = BW / 16;  // column tiles the accumulators can hold; ct_n of them are in use
   acc_t acc[RT][CT];
 #pragma unroll
   for (int r = 0; r < RT; r++)
@@ -511,7 +528,7 @@ __global__ __launch_bounds__(1024) void f_hankel(
 
   if constexpr (CM == CM_BF16X3) {
     const __bf16* bhi = reinterpret_cast<const __bf16*>(smem);
-    const __bf16* blo = bhi + BN * ldb;
+    const __bf16* blo = bhi + BW * ldb;
     const int KC = Kp / 32;
     for (int kc0 = 0; kc0 < KC; kc0 += RING) {
 #pragma unroll
@@ -521,6 +538,7 @@ __global__ __launch_bounds__(1024) void f_hankel(
         bf16x8 bh[CT], bl[CT];
 #pragma unroll
         for (int c = 0; c < CT; c++) {
+          if (c >= ct_n) break;
           const int col = c * 16 + (lane & 15), ko = kc * 32 + 8 * (lane >> 4);
           bh[c] = *reinterpret_cast<const bf16x8*>(bhi + col * ldb + ko);
           bl[c] = *reinterpret_cast<const bf16x8*>(blo + col * ldb + ko);
@@ -530,6 +548,7 @@ __global__ __launch_bounds__(1024) void f_hankel(
           if (mt0 + r >= M16) break;
 #pragma unroll
           for (int c = 0; c < CT; c++) {
+            if (c >= ct_n) break;
             acc[r][c] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(ring_h[d][r], bh[c], acc[r][c], 0, 0, 0);
             acc[r][c] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(ring_h[d][r], bl[c], acc[r][c], 0, 0, 0);
             acc[r][c] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(ring_l[d][r], bh[c], acc[r][c], 0, 0, 0);
@@ -551,12 +570,13 @@ __global__ __launch_bounds__(1024) void f_hankel(
           const int krow = kq * 16 + 4 * j + (lane >> 4);
           T b[CT];
 #pragma unroll
-          for (int c = 0; c < CT; c++) b[c] = bs[krow * BN + c * 16 + (lane & 15)];
+          for (int c = 0; c < CT; c++) b[c] = c < ct_n ? bs[krow * BW + c * 16 + (lane & 15)] : T(0);
 #pragma unroll
           for (int r = 0; r < 2; r++) {
             if (r >= rt_n || mt0 + r >= M16) break;
 #pragma unroll
             for (int c = 0; c < CT; c++) {
+              if (c >= ct_n) break;
               if constexpr (CM == CM_F64)
                 acc[r][c] = __builtin_amdgcn_mfma_f64_16x16x4f64(ring_h[d][r][j], b[c], acc[r][c], 0, 0, 0);
               else
@@ -570,30 +590,34 @@ __global__ __launch_bounds__(1024) void f_hankel(
   }
   __syncthreads();  // B slab no longer needed: reuse LDS for the output tile
 
-  // ---- accumulators -> LDS tile [bm][BN]
+  // ---- accumulators -> LDS tile [bm][BW]
   T* os = reinterpret_cast<T*>(smem);
 #pragma unroll
   for (int r = 0; r < RT; r++) {
 #pragma unroll
-    for (int c = 0; c < CT; c++)
+    for (int c = 0; c < CT; c++) {
+      if (c >= ct_n) break;
 #pragma unroll
       for (int i = 0; i < 4; i++) {
         const int rin = CM == CM_F64 ? 4 * i + (lane >> 4) : 4 * (lane >> 4) + i;
         const int row = wave * rw + r * 16 + rin, col = c * 16 + (lane & 15);
-        os[row * BN + col] = (T)acc[r][c][i];
+        os[row * BW + col] = (T)acc[r][c][i];
       }
+    }
   }
   __syncthreads();
 
-  // ---- epilogue over complex points of the tile
+  // ---- epilogue over complex points of the tile (the r2k direction leaves the state alone once the run
+  // has finished)
+  if (EPI == EPI_STAGE && !st->active) return;
   const double amin = (EPI == EPI_STAGE && stage == 1 && !st->fixed) ? st->rel_amin * bitsd(*qmax) : 0.0;
   double loc = (EPI == EPI_STAGE && stage == 4) ? 0.0 : INFINITY;
   bool nan = false;
-  for (int e = tid; e < bm * HC; e += nth) {
-    const int row = e / HC, cf = e % HC, r = row0 + row, f = fc0 + cf;
-    if (r >= M || f >= nwp) continue;
+  for (int e = tid; e < bm * hcw; e += nth) {
+    const int row = e >> lgh, cf = e & (hcw - 1), r = row0 + row, f = fc0 + cf;
+    if (r >= M || f >= fend) continue;
     const long idx = (long)r * nwp + f;
-    C J = cmk(os[row * BN + 2 * cf], os[row * BN + 2 * cf + 1]);
+    C J = cmk(os[row * BW + 2 * cf], os[row * BW + 2 * cf + 1]);
     if (EPI == EPI_NONE)
       out[idx] = J;
     else
@@ -624,8 +648,10 @@ __global__ __launch_bounds__(1024) void f_hankel(
 // source kernel: one workgroup per strip
 // ---------------------------------------------------------------------------------------------------
 // ionisation rate of one sample (ionization.cl:22-44, 71-97). The FP32 path forms the closed form as one
-// exponential exp(ln(rate_conv C_pre) + C_pow ln E + C_exp/E) in FP32 and keeps the series polynomial in FP64
-// (its terms cancel) before an FP32 exponential; the FP64 path follows the reference's expressions.
+// base-2 exponential 2^(log2(rate_conv C_pre) + C_pow log2 E + C_exp log2(e) / E) with the hardware exp2,
+// log2 and reciprocal (about 1 ulp each; the exponent's own rounding, |x| 2^-24, dominates), and keeps the
+// series polynomial in FP64 (its terms cancel) before an FP32 exponential; the FP64 path follows the
+// reference's expressions.
 template <typename T>
 __device__ inline double ion_rate(double ea, const uppe_ionization& ion, double log_pre);
 
@@ -643,35 +669,46 @@ __device__ inline double ion_rate<double>(double ea, const uppe_ionization& ion,
 template <>
 __device__ inline double ion_rate<float>(double ea, const uppe_ionization& ion, double log_pre) {
   const float emag = (float)(ion.e_conv * ea + ion.e_cutoff);
+  if (ion.kind == UPPE_ION_EXPLICIT) {
+    // log_pre here is log2(rate_conv C_pre)
+    const float x = (float)log_pre + (float)ion.c_pow * __builtin_amdgcn_logf(emag) +
+                    (float)(ion.c_exp * 1.4426950408889634) * __builtin_amdgcn_rcpf(emag);
+    return (double)__builtin_amdgcn_exp2f(x);
+  }
   const float lg = logf(emag);
-  if (ion.kind == UPPE_ION_EXPLICIT)
-    return (double)expf((float)log_pre + (float)ion.c_pow * lg + (float)ion.c_exp / emag);
   const double l = (double)lg;
   const double poly = ion.c[0] + l * (ion.c[1] + l * (ion.c[2] + l * (ion.c[3] + l * (ion.c[4] + l * ion.c[5]))));
   return ion.rate_conv * (double)expf((float)poly);
 }
 
-// contiguous per-thread runs of LDS elements moved as 16-byte vectors (the run start is 16-byte aligned
-// whenever the run is at least 16 bytes long); shorter runs fall back to scalar accesses
+// real sample n of a strip buffer in the lds_swz layout (the real and imaginary parts of complex element
+// n / 2 hold samples n and n + 1 for even n)
+template <typename T>
+__device__ inline T& lds_sample(cplx<T>* Y, int n) {
+  return reinterpret_cast<T*>(Y + lds_swz<T>(n >> 1))[n & 1];
+}
+
+// RUN consecutive real samples from n0 (a multiple of RUN), as 16-byte vectors when RUN allows: a vector holds
+// whole elements that lds_swz keeps together
 template <typename T, int RUN>
-__device__ inline void lds_read_run(const T* src, T* dst) {
+__device__ inline void lds_read_reals(const cplx<T>* Y, int n0, T* dst) {
   constexpr int V = 16 / sizeof(T);
   if constexpr (RUN % V == 0) {
     typedef T vec __attribute__((ext_vector_type(V)));
 #pragma unroll
     for (int i = 0; i < RUN / V; i++) {
-      vec v = reinterpret_cast<const vec*>(src)[i];
+      vec v = *reinterpret_cast<const vec*>(Y + lds_swz<T>(n0 / 2 + i * (V / 2)));
 #pragma unroll
       for (int k = 0; k < V; k++) dst[i * V + k] = v[k];
     }
   } else {
 #pragma unroll
-    for (int j = 0; j < RUN; j++) dst[j] = src[j];
+    for (int j = 0; j < RUN; j++) dst[j] = lds_sample(const_cast<cplx<T>*>(Y), n0 + j);
   }
 }
 
 template <typename T, int RUN>
-__device__ inline void lds_write_run(T* dst, const T* src) {
+__device__ inline void lds_write_reals(cplx<T>* Y, int n0, const T* src) {
   constexpr int V = 16 / sizeof(T);
   if constexpr (RUN % V == 0) {
     typedef T vec __attribute__((ext_vector_type(V)));
@@ -680,11 +717,11 @@ __device__ inline void lds_write_run(T* dst, const T* src) {
       vec v;
 #pragma unroll
       for (int k = 0; k < V; k++) v[k] = src[i * V + k];
-      reinterpret_cast<vec*>(dst)[i] = v;
+      *reinterpret_cast<vec*>(Y + lds_swz<T>(n0 / 2 + i * (V / 2))) = v;
     }
   } else {
 #pragma unroll
-    for (int j = 0; j < RUN; j++) dst[j] = src[j];
+    for (int j = 0; j < RUN; j++) lds_sample(Y, n0 + j) = src[j];
   }
 }
 
@@ -695,8 +732,8 @@ __global__ __launch_bounds__(1024) void f_source(const cplx<T>* __restrict__ er,
                                                  const cplx<T>* __restrict__ tw, const T* __restrict__ w,
                                                  const T* __restrict__ nlf, const double* __restrict__ ng, int nt,
                                                  int lgn, int nwp, double redchi3, double dt, uppe_ionization ion,
-                                                 const StepState* __restrict__ st) {
-  if (st && !st->active) return;
+                                                 double log_pre) {
+  // no early exit for a finished run: the kernel only writes scratch (J_r, diagnostics)
   extern __shared__ char smem[];
   cplx<T>* Y = reinterpret_cast<cplx<T>*>(smem);
   __shared__ double wsum[16];
@@ -704,40 +741,70 @@ __global__ __launch_bounds__(1024) void f_source(const cplx<T>* __restrict__ er,
   const long p = blockIdx.x;
   const int tid = threadIdx.x, nth = blockDim.x, lane = tid & 63, wave = tid >> 6, nwv = nth >> 6;
   const cplx<T>* X = er + p * nwp;
+  // every global load the kernel makes up front, issued together: this strip's spectrum, the frequency and
+  // filter of the bins this thread writes at the end, and (FP32) the twiddle tables bound for LDS
+  constexpr int KPT = SPT / 2 + 1;  // spectral bins per thread (M / nth rounded up, plus the Nyquist bin)
+  cplx<T> xk[KPT], xm[KPT];
+  T wv[KPT], fv[KPT];
+#pragma unroll
+  for (int i = 0; i < KPT; i++) {
+    const int k = tid + i * nth;
+    if (k < M) {
+      xk[i] = X[k];
+      xm[i] = X[M - k];
+    }
+    if (k <= M) {
+      wv[i] = w[k];
+      fv[i] = nlf[k];
+    }
+  }
   // per-pass twiddle tables (N + N/2 entries): in LDS for FP32, read from L2 for FP64 (LDS would not fit)
   const cplx<T>* tws = tw;
   if constexpr (sizeof(T) == 4) {
+    constexpr int TPT = (3 * SPT + 1) / 2;  // table entries per thread, N + N/2 = 1.5 SPT nth
     cplx<T>* tl = Y + nt;
-    for (int k = tid; k < N + M; k += nth) tl[k] = tw[k];
+    cplx<T> tv[TPT];
+#pragma unroll
+    for (int i = 0; i < TPT; i++) {
+      const int k = tid + i * nth;
+      if (k < N + M) tv[i] = tw[k];
+    }
+#pragma unroll
+    for (int i = 0; i < TPT; i++) {
+      const int k = tid + i * nth;
+      if (k < N + M) tl[k] = tv[i];
+    }
     tws = tl;
     __syncthreads();
   }
   const bool lg_odd = (lgn & 1) != 0;
 
   // 1. irfft (numpy semantics; imaginary parts of the DC and Nyquist bins are ignored, fft.cl:271-278)
-  for (int k = tid; k < M; k += nth) {
-    cplx<T> xk = X[k], xm = X[M - k];
+#pragma unroll
+  for (int i = 0; i < KPT; i++) {
+    const int k = tid + i * nth;
+    if (k >= M) break;
+    cplx<T> xa = xk[i], xb = xm[i];
     if (k == 0) {
-      xk.im = T(0);
-      xm.im = T(0);
+      xa.im = T(0);
+      xb.im = T(0);
     }
-    cplx<T> ze = cscale(cadd(xk, cconj(xm)), T(0.5));
-    cplx<T> zo = cmul(cscale(csub(xk, cconj(xm)), T(0.5)), cconj(stockham_twn(tws, N, lg_odd, k)));
-    Y[k] = cmk(ze.re - zo.im, ze.im + zo.re);
+    cplx<T> ze = cscale(cadd(xa, cconj(xb)), T(0.5));
+    cplx<T> zo = cmul(cscale(csub(xa, cconj(xb)), T(0.5)), cconj(stockham_twn(tws, N, lg_odd, k)));
+    Y[lds_swz<T>(k)] = cmk(ze.re - zo.im, ze.im + zo.re);
   }
   __syncthreads();
   lds_fft_stockham_tab(Y, M, lgm, tws, N, +1);
   const T invm = T(1) / T(M);
-  const T* ys = reinterpret_cast<const T*>(Y);
   const int a0 = tid * SPT;
   T e[SPT + 1];
-  lds_read_run<T, SPT>(ys + a0, e);
+  lds_read_reals<T, SPT>(Y, a0, e);
 #pragma unroll
   for (int j = 0; j < SPT; j++) e[j] *= invm;
   {
     // the sample after this thread's run is the next thread's first one
     T nxt = __shfl_down(e[0], 1);
-    if (lane == 63) nxt = a0 + SPT < N ? ys[a0 + SPT] * invm : T(0);
+    if (lane == 63) nxt = a0 + SPT < N ? lds_sample(Y, a0 + SPT) * invm : T(0);
     e[SPT] = nxt;
   }
   __syncthreads();
@@ -746,7 +813,6 @@ __global__ __launch_bounds__(1024) void f_source(const cplx<T>* __restrict__ er,
   const double ngp = ng[p];
   double ne[SPT];
   if (ion.kind != UPPE_ION_NONE) {
-    const double log_pre = log(ion.rate_conv * ion.c_pre);
     double r[SPT + 1];
 #pragma unroll
     for (int j = 0; j <= SPT; j++) {
@@ -774,16 +840,23 @@ __global__ __launch_bounds__(1024) void f_source(const cplx<T>* __restrict__ er,
 #pragma unroll
     for (int j = 0; j < SPT; j++) {
       int n = a0 + j;
-      double x = ngp * (1.0 - exp(-(ne[j] + c)));
+      // n_e = n_g (1 - exp(-integral)): the integral stays FP64; the FP32 path evaluates the depletion as
+      // -expm1 in FP32 (accurate also where the integral is tiny), the FP64 path as the reference does
+      using R = T;
+      R x;
+      if constexpr (sizeof(T) == 4)
+        x = (float)ngp * -expm1f(-(float)(ne[j] + c));
+      else
+        x = ngp * (1.0 - exp(-(ne[j] + c)));
       // SoftTimeWindow(4, 64)
-      if (n < 4 || n > N - 5) x = 0.0;
+      if (n < 4 || n > N - 5) x = R(0);
       if (n >= 4 && n < 68) {
-        double qq = (double)(n - 4) / 64.0;
-        x *= 10 * qq * qq * qq - 15 * qq * qq * qq * qq + 6 * qq * qq * qq * qq * qq;
+        R qq = (R)(n - 4) / R(64);
+        x *= R(10) * qq * qq * qq - R(15) * qq * qq * qq * qq + R(6) * qq * qq * qq * qq * qq;
       }
       if (n <= N - 5 && n > N - 69) {
-        double qq = (double)(N - 5 - n) / 64.0;
-        x *= 10 * qq * qq * qq - 15 * qq * qq * qq * qq + 6 * qq * qq * qq * qq * qq;
+        R qq = (R)(N - 5 - n) / R(64);
+        x *= R(10) * qq * qq * qq - R(15) * qq * qq * qq * qq + R(6) * qq * qq * qq * qq * qq;
       }
       ne[j] = x;
     }
@@ -807,17 +880,20 @@ __global__ __launch_bounds__(1024) void f_source(const cplx<T>* __restrict__ er,
       ne_out[p * N + n] = nej;
     }
   }
-  lds_write_run<T, 2 * SPT>(reinterpret_cast<T*>(Y + a0), yrun);
+  lds_write_reals<T, 2 * SPT>(Y, 2 * a0, yrun);
   __syncthreads();
   lds_fft_stockham_tab(Y, N, lgn, tws, N, -1);
 
   // 4. J = F_NL [ rfft(J_p)/(0.1 - i w) + (-i w) rfft(P) ] (fft.cl:282-329, uppe_kernel.py:170-186)
-  for (int k = tid; k <= M; k += nth) {
-    cplx<T> yk = Y[k], ym = Y[(N - k) & (N - 1)];
+#pragma unroll
+  for (int i = 0; i < KPT; i++) {
+    const int k = tid + i * nth;
+    if (k > M) break;
+    cplx<T> yk = Y[lds_swz<T>(k)], ym = Y[lds_swz<T>((N - k) & (N - 1))];
     cplx<T> pk = cscale(cadd(yk, cconj(ym)), T(0.5));
     cplx<T> d = csub(yk, cconj(ym));
     cplx<T> jp = cmk(T(0.5) * d.im, T(-0.5) * d.re);
-    T wk = w[k];
+    T wk = wv[i];
     cplx<T> out;
     if (ion.kind != UPPE_ION_NONE) {
       T cr = T(0.1), ci = -wk;
@@ -827,7 +903,7 @@ __global__ __launch_bounds__(1024) void f_source(const cplx<T>* __restrict__ er,
     } else {
       out = cmk(wk * pk.im, -wk * pk.re);
     }
-    jr[p * nwp + k] = cscale(out, nlf[k]);
+    jr[p * nwp + k] = cscale(out, fv[i]);
   }
 }
 
@@ -1057,12 +1133,34 @@ static void* upload(const std::vector<V>& v) {
   return d;
 }
 
-// one workgroup covers all M rows with up to 16 waves of at most two 16-row tiles each (fewer waves per
-// workgroup, i.e. several workgroups per CU, measured no faster at Nr = 256)
-static int hankel_waves(int M) { return std::max(1, std::min(16, (M + 15) / 16)); }
-static int rows_per_wave(int M) {
-  const int m16 = (M + 15) / 16, w = hankel_waves(M);
-  return std::min(32, 16 * ((m16 + w - 1) / w));
+// Launch plan of one contraction: ncols complex columns in tiles of 8, split into row blocks of at least 128
+// rows while that keeps one workgroup per CU. When a grid of ceil(ncols/8) tiles would not fit the CUs, the
+// last tile takes the remainder instead (a 2049-column transform is 256 tiles, not 257), so that no
+// workgroup waits for a second round.
+static FusedState::HkPlan plan_hankel(int M, int ncols, int n_cu) {
+  FusedState::HkPlan p;
+  const int m16 = (M + 15) / 16;
+  const int narrow = (ncols + 7) / 8, merged = std::max(1, ncols / 8);
+  p.tiles = narrow;
+  p.split = 1;
+  for (int sp = 8; sp >= 1; sp /= 2) {
+    if (sp > 1 && m16 / sp < 8) continue;
+    if ((long)narrow * sp <= n_cu) {
+      p.tiles = narrow;
+      p.split = sp;
+      break;
+    }
+    if ((long)merged * sp <= n_cu) {
+      p.tiles = merged;
+      p.split = sp;
+      break;
+    }
+  }
+  const int per = (m16 + p.split - 1) / p.split;  // 16-row tiles per row block
+  p.nwv = std::min(16, per);
+  p.rt = (per + p.nwv - 1) / p.nwv;
+  if (p.rt > 2) throw Error{UPPE_ERR_UNSUPPORTED, "radial contraction: too many rows"};
+  return p;
 }
 
 template <typename T>
@@ -1073,7 +1171,11 @@ struct Fused {
     const long P = h->P;
     s->nwp = ((h->nw + 31) / 32) * 32;
     s->el = sizeof(T);
-    s->f1 = s->nwp;
+    s->f1 = h->nw;
+    int dev = 0, ncu = 0;
+    UPPE_HIP(hipGetDevice(&dev));
+    UPPE_HIP(hipDeviceGetAttribute(&ncu, hipDeviceAttributeMultiprocessorCount, dev));
+    s->n_cu = std::max(1, ncu);
     const size_t cs = sizeof(C) * P * s->nwp;
     UPPE_HIP(hipMalloc(&s->phs, sizeof(double) * P * s->nwp));
     UPPE_HIP(hipMalloc(&s->kzc, cs));
@@ -1138,11 +1240,8 @@ struct Fused {
       pack_k4<T>(hf, nk, nr, fr, &s->kp_f);
       s->frf_hi = upload(fr);
     }
-    s->rw_b = rows_per_wave(nr);
-    s->rw_f = rows_per_wave(nk);
-    // 8 complex columns per workgroup: at Nw = 2049 that is ~260 workgroups, one per CU (wider tiles halve
-    // the grid below the CU count and measured slower in FP32/FP64)
-    s->bn_b = s->bn_f = 16;
+    s->plan[0] = plan_hankel(nr, h->nw, s->n_cu);
+    s->plan[1] = plan_hankel(nk, s->f1, s->n_cu);
   }
 
   static void set_medium(uppe_handle* h, FusedState* s, const uppe_medium* m) {
@@ -1173,11 +1272,11 @@ struct Fused {
     s->max_ki = 0.0;
     for (long i = 0; i < h->npt; i++) s->max_ki = std::max(s->max_ki, m->kz[2 * i + 1]);
     // columns at and above f1 carry zero current: the r2k contraction and the stage updates skip them.
-    // f1 stays inside the padded row and covers the conditioner's four bins.
-    int f1 = std::max(last_nl + 1, 4);
-    f1 = std::min(s->nwp, ((f1 + 7) / 8) * 8);
+    // f1 covers the conditioner's four bins.
+    const int f1 = std::min(h->nw, std::max(last_nl + 1, 4));
     if (f1 != s->f1) {
       s->f1 = f1;
+      if (h->have_hankel) s->plan[1] = plan_hankel(h->nk, s->f1, s->n_cu);
       const size_t cs = sizeof(C) * P * s->nwp;
       UPPE_HIP(hipMemset(s->k1, 0, cs));
       UPPE_HIP(hipMemset(s->k2, 0, cs));
@@ -1208,17 +1307,22 @@ struct Fused {
     const int spt = N / nth;
     const size_t shm = sizeof(C) * (sizeof(T) == 4 ? N + N + N / 2 : N);
     const int lgn = ilog2i(N);
+    // the explicit-rate prefactor, in the base of the exponential each precision uses
+    const double log_pre = s->ion.kind != UPPE_ION_EXPLICIT ? 0.0
+                           : sizeof(T) == 4                  ? std::log2(s->ion.rate_conv * s->ion.c_pre)
+                                                             : std::log(s->ion.rate_conv * s->ion.c_pre);
+    (void)st;
     dim3 grid((unsigned)h->P), block(nth);
 #define UPPE_SRC(S)                                                                                              \
   if (spt == S) {                                                                                                \
     if (diag)                                                                                                    \
       launchf(h, UPPE_KC_SOURCE, f_source<T, S, true>, grid, block, shm, er, jr, (T*)s->chi_t, (T*)s->ne_t,       \
               (const C*)s->tw, (const T*)s->w, (const T*)s->nlf, (const double*)s->ng, N, lgn, s->nwp, s->redchi3, \
-              s->dt, s->ion, (const StepState*)st);                                                              \
+              s->dt, s->ion, log_pre);                                                                           \
     else                                                                                                         \
       launchf(h, UPPE_KC_SOURCE, f_source<T, S, false>, grid, block, shm, er, jr, (T*)nullptr, (T*)nullptr,       \
               (const C*)s->tw, (const T*)s->w, (const T*)s->nlf, (const double*)s->ng, N, lgn, s->nwp, s->redchi3, \
-              s->dt, s->ion, (const StepState*)st);                                                              \
+              s->dt, s->ion, log_pre);                                                                           \
     return;                                                                                                      \
   }
     UPPE_SRC(1) UPPE_SRC(2) UPPE_SRC(4) UPPE_SRC(8) UPPE_SRC(16)
@@ -1231,33 +1335,30 @@ struct Fused {
   static void hankel_mfma(uppe_handle* h, FusedState* s, int dir, const C* B, C* out, const C* kprev, int stage,
                           double zfix, StepState* st) {
     const int M = dir == 0 ? h->n1 : h->nk, K = dir == 0 ? h->nk : h->n1;
-    const int kp = dir == 0 ? s->kp_b : s->kp_f, rw = dir == 0 ? s->rw_b : s->rw_f;
-    const int bn = dir == 0 ? s->bn_b : s->bn_f;
-    const int ncols = dir == 0 ? s->nwp : s->f1;  // complex columns covered
+    const int kp = dir == 0 ? s->kp_b : s->kp_f;
+    const FusedState::HkPlan& pl = s->plan[dir];
+    const int ncols = dir == 0 ? h->nw : s->f1;  // complex columns covered
     const void* ahi = dir == 0 ? s->frb_hi : s->frf_hi;
     const void* alo = dir == 0 ? s->frb_lo : s->frf_lo;
     using F = typename MfmaTraits<CM>::frag_t;
-    const int nwv = hankel_waves(M);
-    const int bm = nwv * rw;
-    dim3 grid((unsigned)((ncols + bn / 2 - 1) / (bn / 2)), (unsigned)((M + bm - 1) / bm));
-    size_t slab = CM == CM_BF16X3 ? 2ull * bn * (kp + 8) * 2 : (size_t)kp * bn * sizeof(T);
-    size_t tile = (size_t)bm * bn * sizeof(T);
+    constexpr int BNV = 16, BW = 2 * BNV;
+    const int rw = 16 * pl.rt;
+    const int bm = pl.nwv * rw;
+    if ((long)pl.tiles * 8 + 8 <= ncols || (long)pl.tiles * 8 >= ncols + 8)
+      throw Error{UPPE_ERR_STATE, "radial contraction plan out of date"};
+    dim3 grid((unsigned)pl.tiles, (unsigned)((M + bm - 1) / bm));
+    size_t slab = CM == CM_BF16X3 ? 2ull * BW * (kp + 8) * 2 : (size_t)kp * BW * sizeof(T);
+    size_t tile = (size_t)bm * BW * sizeof(T);
     size_t shm = std::max(slab, tile);
-#define UPPE_HK(BNV, RTV)                                                                                        \
-  launchf(h, UPPE_KC_HANKEL, f_hankel<CM, PRO, EPI, BNV, RTV>, grid, dim3(64 * nwv), shm, (const F*)ahi, (const F*)alo, B, out, \
-          M, K, kp, rw, s->nwp, 0, (const double*)s->phs, (const C*)s->kzc, (const T*)s->cond, (C*)s->q, kprev,      \
-          (C*)s->k1, (C*)s->k2, (C*)s->k3, st, s->qmax, s->qmax_next, stage, zfix)
-    if (rw == 16) {
-      if (bn == 16)
-        UPPE_HK(16, 1);
-      else
-        UPPE_HK(32, 1);
-    } else {
-      if (bn == 16)
-        UPPE_HK(16, 2);
-      else
-        UPPE_HK(32, 2);
-    }
+#define UPPE_HK(RTV)                                                                                             \
+  launchf(h, UPPE_KC_HANKEL, f_hankel<CM, PRO, EPI, BNV, RTV>, grid, dim3(64 * pl.nwv), shm, (const F*)ahi,    \
+          (const F*)alo, B, out, M, K, kp, rw, s->nwp, 0, ncols, (const double*)s->phs, (const C*)s->kzc,         \
+          (const T*)s->cond, (C*)s->q, kprev, (C*)s->k1, (C*)s->k2, (C*)s->k3, st, s->qmax, s->qmax_next, stage, \
+          zfix)
+    if (pl.rt == 1)
+      UPPE_HK(1);
+    else
+      UPPE_HK(2);
 #undef UPPE_HK
   }
 

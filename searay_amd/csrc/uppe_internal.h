@@ -101,74 +101,28 @@ __device__ inline void lds_fft_radix2(cplx<T>* buf, int n, int lg, const cplx<T>
 }
 
 // ---------------------------------------------------------------------------------------------------
-// Block-cooperative in-place Stockham FFT in LDS: radix-4 passes and one radix-2 pass when log2(n) is odd.
-// Each pass loads its butterfly inputs into registers, waits for the whole block, then stores the outputs
-// in autosorted order, so no second buffer is needed. Pass with span Ns and radix R: butterfly j reads
-// buf[j + r n/R], multiplies input r by exp(sign 2 pi i r (j mod Ns)/(Ns R)), forms the R-point DFT and
-// writes buf[(j - j mod Ns) R + j mod Ns + r Ns]. Twiddles come from tw[k] = exp(-2 pi i k/nbase), k < nbase/2,
-// with nbase a multiple of n. At most two butterflies per thread per pass (n/R <= 2 blockDim.x).
+// LDS layout of the strip buffers of the FP32 source kernel: complex element i lives at slot
+// i ^ (((i >> 3) & 15) << 1). The map permutes each aligned block of 128 elements, keeps element pairs
+// (2m, 2m + 1) together (16-byte vector accesses stay aligned) and spreads the power-of-two-strided writes
+// of the early Stockham passes (spans 1, 4, 16) and the 32-byte per-thread runs over the LDS banks. FP64
+// elements are 16 bytes wide and keep the identity layout.
 // ---------------------------------------------------------------------------------------------------
 template <typename T>
-__device__ inline cplx<T> stockham_twiddle(const cplx<T>* tw, int t, int nbase, int sign) {
-  const int h = nbase >> 1;
-  cplx<T> w = t < h ? tw[t] : cmk(-tw[t - h].re, -tw[t - h].im);
-  if (sign > 0) w.im = -w.im;
-  return w;
-}
-
-template <typename T>
-__device__ inline void lds_fft_stockham(cplx<T>* buf, int n, int lg, const cplx<T>* tw, int nbase, int sign) {
-  const int nth = blockDim.x, tid = threadIdx.x;
-  int Ns = 1;
-  int rem = lg;
-  while (rem > 0) {
-    const int R = rem >= 2 ? 4 : 2;
-    const int nb = n / R;
-    cplx<T> v[2][4];
-#pragma unroll
-    for (int b = 0; b < 2; b++) {
-      const int j = tid + b * nth;
-      if (j < nb)
-#pragma unroll
-        for (int r = 0; r < 4; r++)
-          if (r < R) v[b][r] = buf[j + r * nb];
-    }
-    __syncthreads();
-#pragma unroll
-    for (int b = 0; b < 2; b++) {
-      const int j = tid + b * nth;
-      if (j >= nb) continue;
-      const int k = j & (Ns - 1);
-      const int tstep = k * (nbase / (Ns * R));
-      cplx<T> y[4];
-      if (R == 4) {
-        cplx<T> v1 = cmul(v[b][1], stockham_twiddle(tw, tstep, nbase, sign));
-        cplx<T> v2 = cmul(v[b][2], stockham_twiddle(tw, 2 * tstep, nbase, sign));
-        cplx<T> v3 = cmul(v[b][3], stockham_twiddle(tw, 3 * tstep, nbase, sign));
-        cplx<T> a0 = cadd(v[b][0], v2), a1 = csub(v[b][0], v2), a2 = cadd(v1, v3), d = csub(v1, v3);
-        cplx<T> a3 = sign < 0 ? cmk(d.im, -d.re) : cmk(-d.im, d.re);
-        y[0] = cadd(a0, a2);
-        y[1] = cadd(a1, a3);
-        y[2] = csub(a0, a2);
-        y[3] = csub(a1, a3);
-      } else {
-        cplx<T> v1 = cmul(v[b][1], stockham_twiddle(tw, tstep, nbase, sign));
-        y[0] = cadd(v[b][0], v1);
-        y[1] = csub(v[b][0], v1);
-      }
-      const int o = (j - k) * R + k;
-#pragma unroll
-      for (int r = 0; r < 4; r++)
-        if (r < R) buf[o + r * Ns] = y[r];
-    }
-    __syncthreads();
-    Ns *= R;
-    rem -= R == 4 ? 2 : 1;
-  }
+__device__ inline int lds_swz(int i) {
+  if constexpr (sizeof(T) == 4)
+    return i ^ (((i >> 3) & 15) << 1);
+  else
+    return i;
 }
 
 // ---------------------------------------------------------------------------------------------------
-// Stockham FFT with per-pass twiddle tables, so that the threads of a pass read consecutive table entries
+// Block-cooperative in-place Stockham FFT in LDS (buffer in the lds_swz layout): radix-4 passes and one
+// radix-2 pass when log2(n) is odd. Each pass loads its butterfly inputs into registers, waits for the whole
+// block, then stores the outputs in autosorted order, so no second buffer is needed. Pass with span Ns and
+// radix R: butterfly j reads element j + r n/R, multiplies input r by exp(sign 2 pi i r (j mod Ns)/(Ns R)),
+// forms the R-point DFT and writes element (j - j mod Ns) R + j mod Ns + r Ns. At most two butterflies per
+// thread per pass (n/R <= 2 blockDim.x).
+// Twiddles come from per-pass tables, so that the threads of a pass read consecutive table entries
 // (the single exp(-2 pi i k/nbase) table is read at power-of-two strides, which puts a wave on one LDS bank).
 // Layout of `tab` (built by stockham_tables): the radix-4 pass of span Ns uses tab[Ns - 1 + (r-1) Ns + k] =
 // exp(-2 pi i r k/(4 Ns)), k < Ns, r = 1..3; the radix-2 pass of an n-point transform (log2 n odd) uses
@@ -189,7 +143,7 @@ __device__ inline void lds_fft_stockham_tab(cplx<T>* buf, int n, int lg, const c
       if (j < nb)
 #pragma unroll
         for (int r = 0; r < 4; r++)
-          if (r < R) v[b][r] = buf[j + r * nb];
+          if (r < R) v[b][r] = buf[lds_swz<T>(j + r * nb)];
     }
     __syncthreads();
 #pragma unroll
@@ -222,7 +176,7 @@ __device__ inline void lds_fft_stockham_tab(cplx<T>* buf, int n, int lg, const c
       const int o = (j - k) * R + k;
 #pragma unroll
       for (int r = 0; r < 4; r++)
-        if (r < R) buf[o + r * Ns] = y[r];
+        if (r < R) buf[lds_swz<T>(o + r * Ns)] = y[r];
     }
     __syncthreads();
     Ns *= R;

@@ -1,0 +1,78 @@
+'''Full-size properties on the photoionisation workload (Nw = 2049, Nr = 256), where the oracle is too slow to
+run to completion: agreement of the reduced-precision paths with the fused FP64 path on one step schedule,
+FP64 fused against FP64 plain over the first steps, and conservation of energy in the linear regime.'''
+import sys
+
+import numpy as np
+import pytest
+
+from conftest import rel_l2
+from searay_amd import configs, uppe_kernel
+
+pytestmark = pytest.mark.gpu
+
+
+def _say(msg):
+    print(msg, flush=True)
+    sys.stdout.flush()
+
+
+@pytest.fixture(scope='module')
+def c2():
+    return uppe_kernel.RunSetup(configs.radial_o2_ionization())
+
+
+def _steps(rs, precision, K, schedule=None, record=False):
+    st = rs.make_stepper(uppe_kernel.hip_stepper_factory(precision))
+    zi, zf, L = rs.plane(0)
+    st.set_field(rs.A0)
+    n, rec = st.advance(zi, L, 0, rs.subcycles, rs.dzmin, max_steps=K, dz_schedule=schedule,
+                        record_cap=K if record else 0)
+    q = st.get_state()
+    info = st.info()
+    st.close()
+    return n, np.array(rec), q, info
+
+
+def test_fp64_fused_matches_plain(c2):
+    _say('full size: fp64 fused vs fp64 plain, 6 steps')
+    n0, rec0, q0, _ = _steps(c2, 'fp64-plain', 6, record=True)
+    n1, rec1, q1, _ = _steps(c2, 'fp64', 6, record=True)
+    assert n0 == n1 == 6
+    assert np.allclose(rec1, rec0, rtol=1e-8, atol=0)
+    assert rel_l2(q1, q0) < 1e-10
+
+
+@pytest.mark.parametrize('precision', ['fp32', 'bf16x3'])
+def test_reduced_precision_on_fp64_schedule(c2, precision):
+    '''200 steps: relative L2 of the k-space state against fused FP64 on its own schedule within 1e-5.'''
+    _say('full size: %s vs fp64 on the fp64 schedule, 200 steps' % precision)
+    n64, rec, q64, info64 = _steps(c2, 'fp64', 200, record=True)
+    n, _, q, info = _steps(c2, precision, 200, schedule=rec)
+    assert n == n64 == 200
+    assert info['nan_steps'] == 0 and info64['nan_steps'] == 0
+    assert rel_l2(q, q64) < 1e-5
+
+
+@pytest.mark.parametrize('precision', ['fp64', 'fp32', 'bf16x3'])
+def test_linear_energy_conservation(precision):
+    '''Kerr and ionisation off, lossless air: sum_w sum_r w^2 |A|^2 Lambda_r^2 after 20 steps and the finish
+    (linear phase, inverse Hankel transform) equals its initial value within 1e-6 (1e-12 in FP64).'''
+    vol = configs.radial_o2_ionization()
+    vol['chi3'] = 0.0
+    del vol['ionizer']
+    rs = uppe_kernel.RunSetup(vol)
+    _say('full size linear energy %s' % precision)
+    st = rs.make_stepper(uppe_kernel.hip_stepper_factory(precision))
+    zi, zf, L = rs.plane(0)
+    st.set_field(rs.A0)
+    n, _ = st.advance(zi, L, 0, rs.subcycles, rs.dzmin, max_steps=20)
+    z = st.info()['z_last']
+    a, _, _, _ = st.finish(z, want_diagnostics=False)
+    st.close()
+    lam = rs.field_tool.GetTransverseTool().Lambda
+    wgt = (rs.w[:, None, None] ** 2) * (lam[None, :, None] ** 2)
+    e0 = np.sum(wgt * np.abs(rs.A0) ** 2)
+    e1 = np.sum(wgt * np.abs(a) ** 2)
+    assert n == 20
+    assert abs(e1 / e0 - 1.0) < (1e-12 if precision == 'fp64' else 1e-6)

@@ -94,3 +94,44 @@ def test_radial_ionization_replayed_schedule(oracle_mod, precision):
     assert n == len(rec_ref)
     assert rel_l2(a, ref[0][..., 1]) < FP32_FIELD_TOL
     assert rel_l2(np.fft.rfft(ne_t, axis=0), ref[3][..., 1]) < FP32_NE_TOL
+
+
+# ---- committed golden fixtures (tests/golden) --------------------------------------------------------------
+
+import golden_cases  # noqa: E402
+
+
+@pytest.mark.parametrize('name', golden_cases.NAMES)
+@pytest.mark.parametrize('precision', ['fp64-plain', 'fp64'])
+def test_golden_fp64(name, precision):
+    '''FP64 paths against the frozen oracle output: field and diagnostics 1e-10, same dz schedule.'''
+    make, fx = golden_cases.load(name)
+    _say('golden %s %s' % (name, precision))
+    rec = []
+    A, J, chi, ne, dom4d, evals = _run(make(), precision, rec)
+    assert list(evals) == list(fx['evals'])
+    assert np.allclose(rec, fx['dz'], rtol=1e-8, atol=0)
+    assert rel_l2(A[..., -1], fx['A']) < FP64_TOL
+    for key, arr in (('J', J), ('chi', chi), ('ne', ne)):
+        if key in fx:
+            assert rel_l2(arr[..., -1], fx[key]) < FP64_TOL
+
+
+@pytest.mark.parametrize('name', golden_cases.NAMES)
+@pytest.mark.parametrize('precision', ['fp32', 'bf16x3'])
+def test_golden_fp32_replayed(name, precision):
+    '''FP32 / BF16x3 on the fixture's dz schedule: field within 1e-5, electron density within 1e-4.'''
+    make, fx = golden_cases.load(name)
+    _say('golden replay %s %s' % (name, precision))
+    rs = uppe_kernel.RunSetup(make())
+    assert rs.diagnostic_steps == 1
+    st = rs.make_stepper(uppe_kernel.hip_stepper_factory(precision))
+    zi, zf, L = rs.plane(0)
+    st.set_field(rs.A0)
+    n, _ = st.advance(zi, L, 0, rs.subcycles, rs.dzmin, dz_schedule=fx['dz'])
+    a, J, chi_t, ne_t = st.finish(zf - zi)
+    st.close()
+    assert n == len(fx['dz'])
+    assert rel_l2(a, fx['A']) < FP32_FIELD_TOL
+    if 'ne' in fx:
+        assert rel_l2(np.fft.rfft(ne_t, axis=0), fx['ne']) < FP32_NE_TOL
