@@ -681,6 +681,22 @@ __device__ inline double ion_rate<float>(double ea, const uppe_ionization& ion, 
   return ion.rate_conv * (double)expf((float)poly);
 }
 
+// exp(+2 pi i k/N), the split twiddle of the half-length inverse real transform: FP32 from the hardware
+// sine and cosine of k/N turns (about 2 ulp), FP64 from sincospi
+template <typename T>
+__device__ inline cplx<T> twiddle_inv(int k, int N);
+template <>
+__device__ inline cplx<float> twiddle_inv<float>(int k, int N) {
+  const float t = (float)k / (float)N;
+  return cmk(__builtin_amdgcn_cosf(t), __builtin_amdgcn_sinf(t));
+}
+template <>
+__device__ inline cplx<double> twiddle_inv<double>(int k, int N) {
+  double sn, cs;
+  sincospi(2.0 * (double)k / (double)N, &sn, &cs);
+  return cmk(cs, sn);
+}
+
 // real sample n of a strip buffer in the lds_swz layout (the real and imaginary parts of complex element
 // n / 2 hold samples n and n + 1 for even n)
 template <typename T>
@@ -736,7 +752,7 @@ __global__ __launch_bounds__(1024) void f_source(const cplx<T>* __restrict__ er,
   // no early exit for a finished run: the kernel only writes scratch (J_r, diagnostics)
   extern __shared__ char smem[];
   cplx<T>* Y = reinterpret_cast<cplx<T>*>(smem);
-  __shared__ double wsum[16];
+  __shared__ T wsum[16];
   const int N = nt, M = nt / 2, lgm = lgn - 1;
   const long p = blockIdx.x;
   const int tid = threadIdx.x, nth = blockDim.x, lane = tid & 63, wave = tid >> 6, nwv = nth >> 6;
@@ -758,7 +774,8 @@ __global__ __launch_bounds__(1024) void f_source(const cplx<T>* __restrict__ er,
       fv[i] = nlf[k];
     }
   }
-  // per-pass twiddle tables (N + N/2 entries): in LDS for FP32, read from L2 for FP64 (LDS would not fit)
+  // per-pass twiddle tables of the N- and N/2-point transforms (fewer than N + N/2 entries): in LDS for
+  // FP32, read from L2 for FP64 (LDS would not fit)
   const cplx<T>* tws = tw;
   if constexpr (sizeof(T) == 4) {
     constexpr int TPT = (3 * SPT + 1) / 2;  // table entries per thread, N + N/2 = 1.5 SPT nth
@@ -777,7 +794,6 @@ __global__ __launch_bounds__(1024) void f_source(const cplx<T>* __restrict__ er,
     tws = tl;
     __syncthreads();
   }
-  const bool lg_odd = (lgn & 1) != 0;
 
   // 1. irfft (numpy semantics; imaginary parts of the DC and Nyquist bins are ignored, fft.cl:271-278)
 #pragma unroll
@@ -790,11 +806,11 @@ __global__ __launch_bounds__(1024) void f_source(const cplx<T>* __restrict__ er,
       xb.im = T(0);
     }
     cplx<T> ze = cscale(cadd(xa, cconj(xb)), T(0.5));
-    cplx<T> zo = cmul(cscale(csub(xa, cconj(xb)), T(0.5)), cconj(stockham_twn(tws, N, lg_odd, k)));
+    cplx<T> zo = cmul(cscale(csub(xa, cconj(xb)), T(0.5)), twiddle_inv<T>(k, N));
     Y[lds_swz<T>(k)] = cmk(ze.re - zo.im, ze.im + zo.re);
   }
   __syncthreads();
-  lds_fft_stockham_tab(Y, M, lgm, tws, N, +1);
+  lds_fft_r8(Y, M, lgm, tws + r8_table_len(lgn), +1);
   const T invm = T(1) / T(M);
   const int a0 = tid * SPT;
   T e[SPT + 1];
@@ -809,43 +825,46 @@ __global__ __launch_bounds__(1024) void f_source(const cplx<T>* __restrict__ er,
   }
   __syncthreads();
 
-  // 2. rate, backward trapezoid with depletion (ionization.cl:127-155), soft window (fft.cl:331-355)
-  const double ngp = ng[p];
-  double ne[SPT];
+  // 2. rate, backward trapezoid with depletion (ionization.cl:127-155), soft window (fft.cl:331-355). The
+  // integral is a sum of positive terms formed as a tree (per thread, then a shuffle scan, then across
+  // waves), in the path's precision: FP32 rounding grows with the tree depth, ~12 ulp at Nt = 4096.
+  using R = T;
+  const R ngp = (R)ng[p];
+  R ne[SPT];
   if (ion.kind != UPPE_ION_NONE) {
-    double r[SPT + 1];
+    const R dtr = (R)dt;
+    R r[SPT + 1];
 #pragma unroll
     for (int j = 0; j <= SPT; j++) {
       int n = a0 + j;
-      double v = ion_rate<T>(fabs((double)e[j]), ion, log_pre);
-      r[j] = (n >= N - 1) ? 0.0 : v;  // last sample zeroed (ionization.cl:147); r[N] is the initial 0
+      R v = (R)ion_rate<T>(fabs((double)e[j]), ion, log_pre);
+      r[j] = (n >= N - 1) ? R(0) : v;  // last sample zeroed (ionization.cl:147); r[N] is the initial 0
     }
-    double s = 0.0;
+    R s = R(0);
 #pragma unroll
     for (int j = SPT - 1; j >= 0; j--) {
-      s += 0.5 * (r[j] + r[j + 1]) * dt;
+      s += R(0.5) * (r[j] + r[j + 1]) * dtr;
       ne[j] = s;
     }
     // suffix sums of the per-thread totals: within the wave by shuffles, across waves through LDS
-    double v = s;
+    R v = s;
     for (int o = 1; o < 64; o <<= 1) {
-      double t = __shfl_down(v, o);
+      R t = __shfl_down(v, o);
       if (lane + o < 64) v += t;
     }
     if (lane == 0) wsum[wave] = v;
     __syncthreads();
-    double after = 0.0;
+    R after = R(0);
     for (int w2 = wave + 1; w2 < nwv; w2++) after += wsum[w2];
-    const double c = (v - s) + after;
+    const R c = (v - s) + after;
 #pragma unroll
     for (int j = 0; j < SPT; j++) {
       int n = a0 + j;
-      // n_e = n_g (1 - exp(-integral)): the integral stays FP64; the FP32 path evaluates the depletion as
-      // -expm1 in FP32 (accurate also where the integral is tiny), the FP64 path as the reference does
-      using R = T;
+      // n_e = n_g (1 - exp(-integral)); the FP32 path evaluates the depletion as -expm1 (accurate also
+      // where the integral is tiny), the FP64 path as the reference does
       R x;
       if constexpr (sizeof(T) == 4)
-        x = (float)ngp * -expm1f(-(float)(ne[j] + c));
+        x = ngp * -expm1f(-(ne[j] + c));
       else
         x = ngp * (1.0 - exp(-(ne[j] + c)));
       // SoftTimeWindow(4, 64)
@@ -882,7 +901,7 @@ __global__ __launch_bounds__(1024) void f_source(const cplx<T>* __restrict__ er,
   }
   lds_write_reals<T, 2 * SPT>(Y, 2 * a0, yrun);
   __syncthreads();
-  lds_fft_stockham_tab(Y, N, lgn, tws, N, -1);
+  lds_fft_r8(Y, N, lgn, tws, -1);
 
   // 4. J = F_NL [ rfft(J_p)/(0.1 - i w) + (-i w) rfft(P) ] (fft.cl:282-329, uppe_kernel.py:170-186)
 #pragma unroll
@@ -1199,7 +1218,7 @@ struct Fused {
     UPPE_HIP(hipHostMalloc(&s->st_host, sizeof(StepState), hipHostMallocDefault));
     std::memset(s->st_host, 0, sizeof(StepState));
     // per-pass Stockham twiddles for the Nt- and Nt/2-point transforms, computed in FP64 and rounded to T
-    std::vector<C> t = stockham_tables<T>(h->nt);
+    std::vector<C> t = r8_tables<T>(h->nt);
     UPPE_HIP(hipMalloc(&s->tw, sizeof(C) * t.size()));
     UPPE_HIP(hipMemcpy(s->tw, t.data(), sizeof(C) * t.size(), hipMemcpyHostToDevice));
     if (h->d.geometry == UPPE_RADIAL) {
@@ -1325,7 +1344,7 @@ struct Fused {
               s->dt, s->ion, log_pre);                                                                           \
     return;                                                                                                      \
   }
-    UPPE_SRC(1) UPPE_SRC(2) UPPE_SRC(4) UPPE_SRC(8) UPPE_SRC(16)
+    UPPE_SRC(1) UPPE_SRC(2) UPPE_SRC(4)
 #undef UPPE_SRC
     throw Error{UPPE_ERR_UNSUPPORTED, "fused source kernel: unsupported Nt"};
   }

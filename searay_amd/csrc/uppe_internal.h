@@ -116,103 +116,133 @@ __device__ inline int lds_swz(int i) {
 }
 
 // ---------------------------------------------------------------------------------------------------
-// Block-cooperative in-place Stockham FFT in LDS (buffer in the lds_swz layout): radix-4 passes and one
-// radix-2 pass when log2(n) is odd. Each pass loads its butterfly inputs into registers, waits for the whole
-// block, then stores the outputs in autosorted order, so no second buffer is needed. Pass with span Ns and
-// radix R: butterfly j reads element j + r n/R, multiplies input r by exp(sign 2 pi i r (j mod Ns)/(Ns R)),
-// forms the R-point DFT and writes element (j - j mod Ns) R + j mod Ns + r Ns. At most two butterflies per
-// thread per pass (n/R <= 2 blockDim.x).
-// Twiddles come from per-pass tables, so that the threads of a pass read consecutive table entries
-// (the single exp(-2 pi i k/nbase) table is read at power-of-two strides, which puts a wave on one LDS bank).
-// Layout of `tab` (built by stockham_tables): the radix-4 pass of span Ns uses tab[Ns - 1 + (r-1) Ns + k] =
-// exp(-2 pi i r k/(4 Ns)), k < Ns, r = 1..3; the radix-2 pass of an n-point transform (log2 n odd) uses
-// tab[r2off + k] = exp(-2 pi i k/n), k < n/2. Inverse transforms conjugate.
+// Block-cooperative in-place Stockham FFT in LDS (buffer in the lds_swz layout). An n = 2^lg point transform
+// runs one twiddle-free radix-2^(lg mod 3) pass (span 1) when lg is not a multiple of 3, then radix-8 passes.
+// Pass with span Ns and radix R: butterfly j < n/R reads element j + r n/R, multiplies input r by
+// exp(sign 2 pi i r (j mod Ns)/(R Ns)), forms the R-point DFT and writes element
+// (j - j mod Ns) R + j mod Ns + r Ns (autosorted). Each pass loads its inputs into registers, waits for the
+// block, then stores, so no second buffer is needed; one butterfly per thread (n/R <= blockDim.x).
+// Twiddles: per-pass tables, the radix-8 pass of span Ns holding exp(-2 pi i r k/(8 Ns)) at
+// [(Ns - R0) + (r - 1) Ns + k], r = 1..7, k < Ns, with R0 = 2^(lg mod 3) the span of the first radix-8
+// pass; the table of an n-point transform has R0 (8^(lg/3) - 1) entries. Threads of a pass read consecutive
+// entries. Inverse transforms conjugate.
 // ---------------------------------------------------------------------------------------------------
+__host__ __device__ inline int r8_table_len(int lg) {
+  const int r0 = 1 << (lg % 3);
+  int p8 = 1;
+  for (int i = 0; i < lg / 3; i++) p8 *= 8;
+  return r0 * (p8 - 1);
+}
+
+// 4-point DFT in place (sign < 0 forward)
 template <typename T>
-__device__ inline void lds_fft_stockham_tab(cplx<T>* buf, int n, int lg, const cplx<T>* tab, int r2off, int sign) {
-  const int nth = blockDim.x, tid = threadIdx.x;
+__device__ inline void dft4(cplx<T>& a0, cplx<T>& a1, cplx<T>& a2, cplx<T>& a3, int sign) {
+  const cplx<T> A0 = cadd(a0, a2), A1 = csub(a0, a2), A2 = cadd(a1, a3), d = csub(a1, a3);
+  const cplx<T> A3 = sign < 0 ? cmk(d.im, -d.re) : cmk(-d.im, d.re);  // -i d forward, +i d inverse
+  a0 = cadd(A0, A2);
+  a2 = csub(A0, A2);
+  a1 = cadd(A1, A3);
+  a3 = csub(A1, A3);
+}
+
+// 8-point DFT of v[0..7] into y[0..7]: radix-2 split into sums and twiddled differences, then two 4-point DFTs
+template <typename T>
+__device__ inline void dft8(const cplx<T>* v, cplx<T>* y, int sign) {
+  const T h = T(0.70710678118654752440);
+  cplx<T> a0 = cadd(v[0], v[4]), a1 = cadd(v[1], v[5]), a2 = cadd(v[2], v[6]), a3 = cadd(v[3], v[7]);
+  cplx<T> b0 = csub(v[0], v[4]), b1 = csub(v[1], v[5]), b2 = csub(v[2], v[6]), b3 = csub(v[3], v[7]);
+  if (sign < 0) {  // times exp(-i pi m/4), m = 1, 2, 3
+    b1 = cmk(h * (b1.re + b1.im), h * (b1.im - b1.re));
+    b2 = cmk(b2.im, -b2.re);
+    b3 = cmk(h * (b3.im - b3.re), -h * (b3.re + b3.im));
+  } else {  // times exp(+i pi m/4)
+    b1 = cmk(h * (b1.re - b1.im), h * (b1.im + b1.re));
+    b2 = cmk(-b2.im, b2.re);
+    b3 = cmk(-h * (b3.re + b3.im), h * (b3.re - b3.im));
+  }
+  dft4(a0, a1, a2, a3, sign);
+  dft4(b0, b1, b2, b3, sign);
+  y[0] = a0;
+  y[2] = a1;
+  y[4] = a2;
+  y[6] = a3;
+  y[1] = b0;
+  y[3] = b1;
+  y[5] = b2;
+  y[7] = b3;
+}
+
+template <typename T>
+__device__ inline void lds_fft_r8(cplx<T>* buf, int n, int lg, const cplx<T>* tab, int sign) {
+  const int tid = threadIdx.x;
+  const int r0 = lg % 3;
   int Ns = 1;
-  int rem = lg;
-  while (rem > 0) {
-    const int R = rem >= 2 ? 4 : 2;
-    const int nb = n / R;
-    cplx<T> v[2][4];
-#pragma unroll
-    for (int b = 0; b < 2; b++) {
-      const int j = tid + b * nth;
-      if (j < nb)
-#pragma unroll
-        for (int r = 0; r < 4; r++)
-          if (r < R) v[b][r] = buf[lds_swz<T>(j + r * nb)];
-    }
-    __syncthreads();
-#pragma unroll
-    for (int b = 0; b < 2; b++) {
-      const int j = tid + b * nth;
-      if (j >= nb) continue;
-      const int k = j & (Ns - 1);
-      cplx<T> y[4];
-      if (R == 4) {
-        cplx<T> w1 = tab[Ns - 1 + k], w2 = tab[2 * Ns - 1 + k], w3 = tab[3 * Ns - 1 + k];
-        if (sign > 0) {
-          w1.im = -w1.im;
-          w2.im = -w2.im;
-          w3.im = -w3.im;
-        }
-        cplx<T> v1 = cmul(v[b][1], w1), v2 = cmul(v[b][2], w2), v3 = cmul(v[b][3], w3);
-        cplx<T> a0 = cadd(v[b][0], v2), a1 = csub(v[b][0], v2), a2 = cadd(v1, v3), d = csub(v1, v3);
-        cplx<T> a3 = sign < 0 ? cmk(d.im, -d.re) : cmk(-d.im, d.re);
-        y[0] = cadd(a0, a2);
-        y[1] = cadd(a1, a3);
-        y[2] = csub(a0, a2);
-        y[3] = csub(a1, a3);
-      } else {
-        cplx<T> w = tab[r2off + k];
-        if (sign > 0) w.im = -w.im;
-        cplx<T> v1 = cmul(v[b][1], w);
-        y[0] = cadd(v[b][0], v1);
-        y[1] = csub(v[b][0], v1);
-      }
-      const int o = (j - k) * R + k;
+  if (r0 > 0) {
+    const int R = 1 << r0, nb = n >> r0;
+    cplx<T> v[4];
+    if (tid < nb)
 #pragma unroll
       for (int r = 0; r < 4; r++)
-        if (r < R) buf[lds_swz<T>(o + r * Ns)] = y[r];
+        if (r < R) v[r] = buf[lds_swz<T>(tid + r * nb)];
+    __syncthreads();
+    if (tid < nb) {
+      if (R == 4) {
+        dft4(v[0], v[1], v[2], v[3], sign);
+      } else {
+        const cplx<T> s0 = cadd(v[0], v[1]), s1 = csub(v[0], v[1]);
+        v[0] = s0;
+        v[1] = s1;
+      }
+#pragma unroll
+      for (int r = 0; r < 4; r++)
+        if (r < R) buf[lds_swz<T>(tid * R + r)] = v[r];
     }
     __syncthreads();
-    Ns *= R;
-    rem -= R == 4 ? 2 : 1;
+    Ns = R;
+  }
+  const cplx<T>* tp = tab;
+  for (; Ns < n; Ns *= 8) {
+    const int nb = n >> 3;
+    cplx<T> v[8], y[8];
+    if (tid < nb)
+#pragma unroll
+      for (int r = 0; r < 8; r++) v[r] = buf[lds_swz<T>(tid + r * nb)];
+    __syncthreads();
+    if (tid < nb) {
+      const int k = tid & (Ns - 1);
+#pragma unroll
+      for (int r = 1; r < 8; r++) {
+        cplx<T> w = tp[(r - 1) * Ns + k];
+        if (sign > 0) w.im = -w.im;
+        v[r] = cmul(v[r], w);
+      }
+      dft8(v, y, sign);
+      const int o = (tid - k) * 8 + k;
+#pragma unroll
+      for (int r = 0; r < 8; r++) buf[lds_swz<T>(o + r * Ns)] = y[r];
+    }
+    __syncthreads();
+    tp += 7 * Ns;
   }
 }
 
-// host: per-pass tables for an N-point and an N/2-point transform (N a power of two >= 4), FP64 -> T,
-// N + N/2 entries with r2off = N
+// host: the radix-8 pass tables of an N-point transform followed by those of the N/2-point transform,
+// FP64 -> T (see lds_fft_r8 for the layout)
 template <typename T>
-inline std::vector<cplx<T>> stockham_tables(int N) {
-  std::vector<cplx<T>> tab((size_t)N + N / 2, cplx<T>{T(0), T(0)});
+inline std::vector<cplx<T>> r8_tables(int N) {
   int lgN = 0;
   while ((1 << lgN) < N) lgN++;
-  for (int Ns = 1; 4 * Ns <= N; Ns *= 4)
-    for (int r = 1; r <= 3; r++)
-      for (int k = 0; k < Ns; k++) {
-        const double a = -2.0 * M_PI * r * k / (4.0 * Ns);
-        tab[(size_t)Ns - 1 + (size_t)(r - 1) * Ns + k] = cplx<T>{(T)std::cos(a), (T)std::sin(a)};
-      }
-  const int nodd = (lgN & 1) ? N : N / 2;
-  for (int k = 0; k < nodd / 2; k++) {
-    const double a = -2.0 * M_PI * k / nodd;
-    tab[(size_t)N + k] = cplx<T>{(T)std::cos(a), (T)std::sin(a)};
+  std::vector<cplx<T>> tab;
+  for (int lg : {lgN, lgN - 1}) {
+    const int n = 1 << lg;
+    for (int Ns = 1 << (lg % 3); Ns < n; Ns *= 8)
+      for (int r = 1; r < 8; r++)
+        for (int k = 0; k < Ns; k++) {
+          const double a = -2.0 * M_PI * r * k / (8.0 * Ns);
+          tab.push_back(cplx<T>{(T)std::cos(a), (T)std::sin(a)});
+        }
   }
   return tab;
-}
-
-// exp(-2 pi i k/N), k < N/2, read from the per-pass tables
-template <typename T>
-__device__ inline cplx<T> stockham_twn(const cplx<T>* tab, int N, bool lg_odd, int k) {
-  if (lg_odd) return tab[N + k];
-  const int q = N / 4;
-  if (k < q) return tab[q - 1 + k];
-  cplx<T> w = tab[q - 1 + k - q];
-  return cmk(w.im, -w.re);  // -i w
 }
 
 // ---------------------------------------------------------------------------------------------------
