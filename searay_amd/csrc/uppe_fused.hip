@@ -315,20 +315,19 @@ struct MfmaTraits<CM_BF16X3> {
   using frag_t = bf16x8;
 };
 
-// one point of the r2k epilogue: S from the contracted current, then the stage bookkeeping
+// one point of the r2k epilogue, from values already loaded: S from the contracted current, then the stage
+// bookkeeping (stage 1: k1, NaN check and the per-point step estimate; stages 2, 3: k2, k3; stage 4: the RK
+// update of q with the conditioner and max|q|)
 template <typename T>
-__device__ inline void stage_epilogue(cplx<T> J, long idx, int f, int stage, double zph, double dz,
-                                      const double* __restrict__ phs, const cplx<T>* __restrict__ kzc,
-                                      const T* __restrict__ cond, cplx<T>* __restrict__ q,
-                                      cplx<T>* __restrict__ k1, cplx<T>* __restrict__ k2,
-                                      cplx<T>* __restrict__ k3, const StepState* st, double amin, double& loc,
-                                      bool& nan) {
-  cplx<T> kz = kzc[idx];
-  cplx<T> t = cmul(linear_factor<T>(phs[idx], kz.im, zph, -1), J);
+__device__ inline void stage_point(cplx<T> J, cplx<T> kz, double ph, cplx<T> Q, cplx<T> a1, cplx<T> a2,
+                                   cplx<T> a3, T cf, long idx, int stage, double zph, double dz,
+                                   cplx<T>* __restrict__ q, cplx<T>* __restrict__ k1, cplx<T>* __restrict__ k2,
+                                   cplx<T>* __restrict__ k3, const StepState* st, double amin, double& loc,
+                                   bool& nan) {
+  cplx<T> t = cmul(linear_factor<T>(ph, kz.im, zph, -1), J);
   cplx<T> S = cmk((T(-0.5) * t.im) / kz.re, (T(0.5) * t.re) / kz.re);
   if (stage == 1) {
     k1[idx] = S;
-    cplx<T> Q = q[idx];
     double qr = (double)Q.re, qi = (double)Q.im;
     double mod = sqrt(qr * qr + qi * qi);
     if (mod != mod) nan = true;
@@ -346,7 +345,6 @@ __device__ inline void stage_epilogue(cplx<T> J, long idx, int f, int stage, dou
     k3[idx] = S;
   } else {
     T c6 = (T)(dz / 6), c3 = (T)(dz / 3);
-    cplx<T> Q = q[idx], a1 = k1[idx], a2 = k2[idx], a3 = k3[idx];
     T re = Q.re + c6 * a1.re;
     T im = Q.im + c6 * a1.im;
     re = re + c3 * a2.re;
@@ -355,11 +353,32 @@ __device__ inline void stage_epilogue(cplx<T> J, long idx, int f, int stage, dou
     im = im + c3 * a3.im;
     re = re + c6 * S.re;
     im = im + c6 * S.im;
-    T cf = cond[f];
     cplx<T> qn = cmk(re * cf, im * cf);
     q[idx] = qn;
     loc = fmax(loc, sqrt((double)qn.re * qn.re + (double)qn.im * qn.im));
   }
+}
+
+// one point of the r2k epilogue (loads, then stage_point)
+template <typename T>
+__device__ inline void stage_epilogue(cplx<T> J, long idx, int f, int stage, double zph, double dz,
+                                      const double* __restrict__ phs, const cplx<T>* __restrict__ kzc,
+                                      const T* __restrict__ cond, cplx<T>* __restrict__ q,
+                                      cplx<T>* __restrict__ k1, cplx<T>* __restrict__ k2,
+                                      cplx<T>* __restrict__ k3, const StepState* st, double amin, double& loc,
+                                      bool& nan) {
+  const cplx<T> z0 = cmk(T(0), T(0));
+  cplx<T> Q = z0, a1 = z0, a2 = z0, a3 = z0;
+  T cf = T(1);
+  if (stage == 1 || stage == 4) Q = q[idx];
+  if (stage == 4) {
+    a1 = k1[idx];
+    a2 = k2[idx];
+    a3 = k3[idx];
+    cf = cond[f];
+  }
+  stage_point<T>(J, kzc[idx], phs[idx], Q, a1, a2, a3, cf, idx, stage, zph, dz, q, k1, k2, k3, st, amin, loc,
+                 nan);
 }
 
 // B-operand element (k, f) with the prologue of the k2r direction applied
@@ -613,15 +632,45 @@ __global__ __launch_bounds__(1024) void f_hankel(
   const double amin = (EPI == EPI_STAGE && stage == 1 && !st->fixed) ? st->rel_amin * bitsd(*qmax) : 0.0;
   double loc = (EPI == EPI_STAGE && stage == 4) ? 0.0 : INFINITY;
   bool nan = false;
-  for (int e = tid; e < bm * hcw; e += nth) {
-    const int row = e >> lgh, cf = e & (hcw - 1), r = row0 + row, f = fc0 + cf;
-    if (r >= M || f >= fend) continue;
-    const long idx = (long)r * nwp + f;
-    C J = cmk(os[row * BW + 2 * cf], os[row * BW + 2 * cf + 1]);
-    if (EPI == EPI_NONE)
-      out[idx] = J;
-    else
-      stage_epilogue<T>(J, idx, f, stage, zph, dz, phs, kzc, cond, q, k1, k2, k3, st, amin, loc, nan);
+  // points are taken EPB per thread per batch, all of a batch's loads issued before its arithmetic
+  constexpr int EPB = sizeof(T) == 8 ? 1 : 2;
+  const int npts = bm * hcw;
+  for (int base = 0; base < npts; base += EPB * nth) {
+    long idx[EPB];
+    int fo[EPB];
+    bool ok[EPB];
+    C J[EPB], kz[EPB], Q[EPB], a1[EPB], a2[EPB], a3[EPB];
+    double ph[EPB];
+    T cfv[EPB];
+#pragma unroll
+    for (int j = 0; j < EPB; j++) {
+      const int e = base + tid + j * nth;
+      const int row = e >> lgh, cf = e & (hcw - 1), r = row0 + row, f = fc0 + cf;
+      ok[j] = e < npts && r < M && f < fend;
+      idx[j] = (long)r * nwp + f;
+      fo[j] = f;
+      if (!ok[j]) continue;
+      J[j] = cmk(os[row * BW + 2 * cf], os[row * BW + 2 * cf + 1]);
+      if (EPI == EPI_NONE) {
+        out[idx[j]] = J[j];
+        continue;
+      }
+      kz[j] = kzc[idx[j]];
+      ph[j] = phs[idx[j]];
+      if (stage == 1 || stage == 4) Q[j] = q[idx[j]];
+      if (stage == 4) {
+        a1[j] = k1[idx[j]];
+        a2[j] = k2[idx[j]];
+        a3[j] = k3[idx[j]];
+        cfv[j] = cond[fo[j]];
+      }
+    }
+    if (EPI == EPI_NONE) continue;
+#pragma unroll
+    for (int j = 0; j < EPB; j++)
+      if (ok[j])
+        stage_point<T>(J[j], kz[j], ph[j], Q[j], a1[j], a2[j], a3[j], cfv[j], idx[j], stage, zph, dz, q, k1, k2,
+                       k3, st, amin, loc, nan);
   }
   if (EPI == EPI_NONE) return;
   if (stage == 1 && nan) atomicOr(&st->nanflag, 1);
@@ -741,6 +790,34 @@ __device__ inline void lds_write_reals(cplx<T>* Y, int n0, const T* src) {
   }
 }
 
+// DPP lane move (gfx9 control codes: row_shr:n = 0x110 + n, wave_shr:1 = 0x138, row_bcast:15 = 0x142,
+// row_bcast:31 = 0x143); lanes without a source, or in rows outside RM, read 0
+template <int CTRL, int RM, typename T>
+__device__ inline T dpp_move(T v) {
+  if constexpr (sizeof(T) == 4) {
+    return __int_as_float(__builtin_amdgcn_update_dpp(0, __float_as_int(v), CTRL, RM, 0xf, false));
+  } else {
+    const long long b = __double_as_longlong(v);
+    const int lo = __builtin_amdgcn_update_dpp(0, (int)b, CTRL, RM, 0xf, false);
+    const int hi = __builtin_amdgcn_update_dpp(0, (int)(b >> 32), CTRL, RM, 0xf, false);
+    return __longlong_as_double(((long long)hi << 32) | (unsigned int)lo);
+  }
+}
+
+// exclusive prefix sum over the 64 lanes of a wave: shift by one lane, then an inclusive scan (log steps within
+// each 16-lane row, then the row totals carried by row_bcast)
+template <typename T>
+__device__ inline T wave_excl_prefix(T v) {
+  v = dpp_move<0x138, 0xf>(v);
+  v += dpp_move<0x111, 0xf>(v);
+  v += dpp_move<0x112, 0xf>(v);
+  v += dpp_move<0x114, 0xf>(v);
+  v += dpp_move<0x118, 0xf>(v);
+  v += dpp_move<0x142, 0xa>(v);
+  v += dpp_move<0x143, 0xc>(v);
+  return v;
+}
+
 // one workgroup (up to 1024 threads) per strip; thread t owns time samples [t SPT, (t+1) SPT)
 template <typename T, int SPT, bool DIAG>
 __global__ __launch_bounds__(1024) void f_source(const cplx<T>* __restrict__ er, cplx<T>* __restrict__ jr,
@@ -812,61 +889,49 @@ __global__ __launch_bounds__(1024) void f_source(const cplx<T>* __restrict__ er,
   __syncthreads();
   lds_fft_r8(Y, M, lgm, tws + r8_table_len(lgn), +1);
   const T invm = T(1) / T(M);
-  const int a0 = tid * SPT;
-  T e[SPT + 1];
+  // thread t owns samples [a0, a0 + SPT) with a0 = (nth - 1 - t) SPT: the depletion integral runs from the
+  // last sample backward, which makes it a prefix sum over the threads in lane order
+  const int a0 = (nth - 1 - tid) * SPT;
+  T e[SPT];
   lds_read_reals<T, SPT>(Y, a0, e);
 #pragma unroll
   for (int j = 0; j < SPT; j++) e[j] *= invm;
-  {
-    // the sample after this thread's run is the next thread's first one
-    T nxt = __shfl_down(e[0], 1);
-    if (lane == 63) nxt = a0 + SPT < N ? lds_sample(Y, a0 + SPT) * invm : T(0);
-    e[SPT] = nxt;
-  }
-  __syncthreads();
 
-  // 2. rate, backward trapezoid with depletion (ionization.cl:127-155), soft window (fft.cl:331-355). The
-  // integral is a sum of positive terms formed as a tree (per thread, then a shuffle scan, then across
-  // waves), in the path's precision: FP32 rounding grows with the tree depth, ~12 ulp at Nt = 4096.
+  // 2. rate, backward trapezoid with depletion (ionization.cl:127-155), soft window (fft.cl:331-355). With the
+  // last rate sample zeroed, the trapezoid from sample n to the end is I_n = dt (r_n / 2 + sum_{m > n} r_m).
+  // The sum is formed as a tree of positive terms (per thread, a DPP scan over the wave, then across
+  // waves) in the path's precision: FP32 rounding grows with the tree depth, ~12 ulp at Nt = 4096.
   using R = T;
   const R ngp = (R)ng[p];
   R ne[SPT];
   if (ion.kind != UPPE_ION_NONE) {
     const R dtr = (R)dt;
-    R r[SPT + 1];
-#pragma unroll
-    for (int j = 0; j <= SPT; j++) {
-      int n = a0 + j;
-      R v = (R)ion_rate<T>(fabs((double)e[j]), ion, log_pre);
-      r[j] = (n >= N - 1) ? R(0) : v;  // last sample zeroed (ionization.cl:147); r[N] is the initial 0
-    }
-    R s = R(0);
-#pragma unroll
-    for (int j = SPT - 1; j >= 0; j--) {
-      s += R(0.5) * (r[j] + r[j + 1]) * dtr;
-      ne[j] = s;
-    }
-    // suffix sums of the per-thread totals: within the wave by shuffles, across waves through LDS
-    R v = s;
-    for (int o = 1; o < 64; o <<= 1) {
-      R t = __shfl_down(v, o);
-      if (lane + o < 64) v += t;
-    }
-    if (lane == 0) wsum[wave] = v;
-    __syncthreads();
-    R after = R(0);
-    for (int w2 = wave + 1; w2 < nwv; w2++) after += wsum[w2];
-    const R c = (v - s) + after;
+    R r[SPT];
+    R tot = R(0);
 #pragma unroll
     for (int j = 0; j < SPT; j++) {
-      int n = a0 + j;
+      const int n = a0 + j;
+      const R v = (R)ion_rate<T>(fabs((double)e[j]), ion, log_pre);
+      r[j] = (n >= N - 1) ? R(0) : v;  // last sample zeroed (ionization.cl:147)
+      tot += r[j];
+    }
+    const R ex = wave_excl_prefix(tot);  // rates of the later samples held by lower lanes of this wave
+    if (lane == 63) wsum[wave] = ex + tot;
+    __syncthreads();
+    R later = ex;
+    for (int w2 = 0; w2 < wave; w2++) later += wsum[w2];
+#pragma unroll
+    for (int j = SPT - 1; j >= 0; j--) {
+      const int n = a0 + j;
+      const R integral = (later + R(0.5) * r[j]) * dtr;
+      later += r[j];
       // n_e = n_g (1 - exp(-integral)); the FP32 path evaluates the depletion as -expm1 (accurate also
       // where the integral is tiny), the FP64 path as the reference does
       R x;
       if constexpr (sizeof(T) == 4)
-        x = ngp * -expm1f(-(ne[j] + c));
+        x = ngp * -expm1f(-integral);
       else
-        x = ngp * (1.0 - exp(-(ne[j] + c)));
+        x = ngp * (1.0 - exp(-integral));
       // SoftTimeWindow(4, 64)
       if (n < 4 || n > N - 5) x = R(0);
       if (n >= 4 && n < 68) {
@@ -881,7 +946,8 @@ __global__ __launch_bounds__(1024) void f_source(const cplx<T>* __restrict__ er,
     }
   } else {
 #pragma unroll
-    for (int j = 0; j < SPT; j++) ne[j] = 0.0;
+    for (int j = 0; j < SPT; j++) ne[j] = R(0);
+    __syncthreads();  // every thread has read its samples before the strip is overwritten
   }
 
   // 3. Kerr chi = ng chi3/nref E^2 (uppe.cl:132-151), P = chi E, J_p = ne E, packed as P + i J_p
