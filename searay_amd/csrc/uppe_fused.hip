@@ -76,6 +76,9 @@ struct FusedState {
   } plan[2];
   int n_cu = 256;
   int f1 = 0;  // the r2k direction covers complex columns [0, f1): where the nonlinear filter is nonzero
+  // Cartesian: radix-8 pass tables of the x and y transforms (complex T)
+  void *tw_x = nullptr, *tw_y = nullptr;
+  int twlen_x = 0, twlen_y = 0;
 };
 
 constexpr int BM = 64, BN = 64, BK = 16;
@@ -694,6 +697,171 @@ __global__ __launch_bounds__(1024) void f_hankel(
 }
 
 // ---------------------------------------------------------------------------------------------------
+// Cartesian transverse transforms (grid_tools.py:218-250; FFT_axis1/2, IFFT_axis1/2: fft.cl:387-441)
+//
+// A 2-D transverse transform is two passes of 1-D FFTs, one per axis. A workgroup takes CW consecutive
+// frequency columns of one line (all Na points along the axis at one index of the other axis), so every
+// global access is a run of CW complex values; the tile sits in LDS as [Na][CW] and its CW columns are
+// transformed together (consecutive threads touch consecutive columns, so the LDS accesses of every pass are
+// contiguous). The first pass of k2r applies the stage prologue on load, the last pass of r2k the RK stage
+// epilogue on store. Forward transforms are unnormalised, inverse ones scale by 1/Na per pass (numpy).
+// ---------------------------------------------------------------------------------------------------
+// twiddle-free radix-R first pass (span 1) over the CW columns: butterfly j reads elements j + r Na/R and
+// writes element j R + r; nj threads per column take 8/R butterflies each
+template <typename T, int CW, int R>
+__device__ inline void cols_first_pass(cplx<T>* buf, int na, int nj, int sign) {
+  const int t = threadIdx.x, c = t % CW, jt = t / CW, nb = na / R;
+  constexpr int IT = 8 / R;
+  cplx<T> v[IT][R];
+#pragma unroll
+  for (int i = 0; i < IT; i++) {
+    const int j = jt + i * nj;
+    if (j < nb)
+#pragma unroll
+      for (int r = 0; r < R; r++) v[i][r] = buf[(j + r * nb) * CW + c];
+  }
+  __syncthreads();
+#pragma unroll
+  for (int i = 0; i < IT; i++) {
+    const int j = jt + i * nj;
+    if (j >= nb) continue;
+    if constexpr (R == 4) {
+      dft4(v[i][0], v[i][1], v[i][2], v[i][3], sign);
+    } else {
+      const cplx<T> a = cadd(v[i][0], v[i][1]), b = csub(v[i][0], v[i][1]);
+      v[i][0] = a;
+      v[i][1] = b;
+    }
+#pragma unroll
+    for (int r = 0; r < R; r++) buf[(j * R + r) * CW + c] = v[i][r];
+  }
+  __syncthreads();
+}
+
+// Stockham FFT of the CW columns of an [na][CW] LDS tile (the pass sequence and table layout of lds_fft_r8)
+template <typename T, int CW>
+__device__ inline void lds_fft_cols(cplx<T>* buf, int na, int lg, const cplx<T>* tab, int sign) {
+  const int t = threadIdx.x, c = t % CW, jt = t / CW, nj = blockDim.x / CW;
+  int Ns = 1;
+  if (lg % 3 == 2) {
+    cols_first_pass<T, CW, 4>(buf, na, nj, sign);
+    Ns = 4;
+  } else if (lg % 3 == 1) {
+    cols_first_pass<T, CW, 2>(buf, na, nj, sign);
+    Ns = 2;
+  }
+  const cplx<T>* tp = tab;
+  for (; Ns < na; Ns *= 8) {
+    const int nb = na >> 3;
+    const bool act = jt < nb;
+    cplx<T> v[8], y[8];
+    if (act)
+#pragma unroll
+      for (int r = 0; r < 8; r++) v[r] = buf[(jt + r * nb) * CW + c];
+    __syncthreads();
+    if (act) {
+      const int k = jt & (Ns - 1);
+#pragma unroll
+      for (int r = 1; r < 8; r++) {
+        cplx<T> w = tp[(r - 1) * Ns + k];
+        if (sign > 0) w.im = -w.im;
+        v[r] = cmul(v[r], w);
+      }
+      dft8(v, y, sign);
+      const int o = (jt - k) * 8 + k;
+#pragma unroll
+      for (int r = 0; r < 8; r++) buf[(o + r * Ns) * CW + c] = y[r];
+    }
+    __syncthreads();
+    tp += 7 * Ns;
+  }
+}
+
+// one 1-D pass of a transverse transform: axis 1 (x, lines strided by Ny Nwp) or 2 (y, strided by Nwp); grid
+// (column tiles, index along the other axis); block >= 64 threads, CW per column line
+template <typename T, int CW, int PRO, int EPI>
+__global__ __launch_bounds__(1024) void f_tfft(
+    const cplx<T>* __restrict__ src, cplx<T>* __restrict__ dst, int axis, int na, int lg, int ny, int nwp, int fend,
+    const cplx<T>* __restrict__ tab_g, int tab_len, int sign, T scale, const double* __restrict__ phs,
+    const cplx<T>* __restrict__ kzc, const T* __restrict__ cond, cplx<T>* __restrict__ q,
+    const cplx<T>* __restrict__ kprev, cplx<T>* __restrict__ k1, cplx<T>* __restrict__ k2,
+    cplx<T>* __restrict__ k3, StepState* __restrict__ st, unsigned long long* __restrict__ qmax,
+    unsigned long long* __restrict__ qmax_next, int stage, double zfix) {
+  using C = cplx<T>;
+  extern __shared__ char smem[];
+  C* buf = reinterpret_cast<C*>(smem);
+  C* tab = buf + na * CW;
+  __shared__ double red[16];
+  const int t = threadIdx.x, nth = blockDim.x, lane = t & 63, wave = t >> 6, nwv = nth >> 6;
+  const int f0 = blockIdx.x * CW;
+  const long sa = axis == 1 ? (long)ny * nwp : (long)nwp;
+  const long base = (axis == 1 ? (long)blockIdx.y : (long)blockIdx.y * ny) * nwp + f0;
+  double zph = zfix, cu = 0.0, dz = 0.0;
+  if (st) {
+    dz = st->dz;
+    const double cs = stage == 1 ? 0.0 : (stage == 4 ? 1.0 : 0.5);
+    zph = st->zoff + (st->z + cs * dz);
+    cu = stage == 1 ? 0.0 : (stage == 4 ? dz : dz / 2);
+  }
+  for (int i = t; i < tab_len; i += nth) tab[i] = tab_g[i];
+  // the tile, with the prologue applied; four elements per thread per batch, their loads issued together
+  const int ne = na * CW;
+  for (int b0 = 0; b0 < ne; b0 += 4 * nth) {
+    C v[4];
+#pragma unroll
+    for (int j = 0; j < 4; j++) {
+      const int e = b0 + t + j * nth, a = e / CW, c = e % CW, f = f0 + c;
+      v[j] = cmk(T(0), T(0));
+      if (e < ne && f < fend) {
+        const long idx = base + a * sa + c;
+        v[j] = PRO == PRO_NONE ? src[idx] : load_b<T, PRO>(src, idx, f, stage, zph, cu, phs, kzc, cond, kprev);
+      }
+    }
+#pragma unroll
+    for (int j = 0; j < 4; j++) {
+      const int e = b0 + t + j * nth;
+      if (e < ne) buf[e] = v[j];
+    }
+  }
+  __syncthreads();
+  lds_fft_cols<T, CW>(buf, na, lg, tab, sign);
+  // store, or the RK stage epilogue (left out once the run has finished)
+  if (EPI == EPI_STAGE && !st->active) return;
+  const double amin = (EPI == EPI_STAGE && stage == 1 && !st->fixed) ? st->rel_amin * bitsd(*qmax) : 0.0;
+  double loc = (EPI == EPI_STAGE && stage == 4) ? 0.0 : INFINITY;
+  bool nan = false;
+  for (int e = t; e < ne; e += nth) {
+    const int a = e / CW, c = e % CW, f = f0 + c;
+    if (f >= fend) continue;
+    const long idx = base + a * sa + c;
+    const C v = cscale(buf[e], scale);
+    if (EPI == EPI_NONE)
+      dst[idx] = v;
+    else
+      stage_epilogue<T>(v, idx, f, stage, zph, dz, phs, kzc, cond, q, k1, k2, k3, st, amin, loc, nan);
+  }
+  if (EPI == EPI_NONE) return;
+  if (stage == 1 && nan) atomicOr(&st->nanflag, 1);
+  if (stage == 2 || stage == 3) return;
+  if (stage == 1 && st->fixed) return;
+  const bool is_min = stage == 1;
+  for (int o = 32; o > 0; o >>= 1) {
+    double v = __shfl_xor(loc, o);
+    loc = is_min ? fmin(loc, v) : fmax(loc, v);
+  }
+  if (lane == 0) red[wave] = loc;
+  __syncthreads();
+  if (t == 0) {
+    double v = red[0];
+    for (int i = 1; i < nwv; i++) v = is_min ? fmin(v, red[i]) : fmax(v, red[i]);
+    if (is_min)
+      atomicMin(&st->dzraw, dbits(v));
+    else
+      atomicMax(qmax_next, dbits(v));
+  }
+}
+
+// ---------------------------------------------------------------------------------------------------
 // source kernel: one workgroup per strip
 // ---------------------------------------------------------------------------------------------------
 // ionisation rate of one sample (ionization.cl:22-44, 71-97). The FP32 path forms the closed form as one
@@ -818,7 +986,7 @@ __device__ inline T wave_excl_prefix(T v) {
   return v;
 }
 
-// one workgroup (up to 1024 threads) per strip; thread t owns time samples [t SPT, (t+1) SPT)
+// one workgroup per strip, Nt/4 threads (at least min(Nt, 64)); thread t owns SPT = Nt / threads samples
 template <typename T, int SPT, bool DIAG>
 __global__ __launch_bounds__(1024) void f_source(const cplx<T>* __restrict__ er, cplx<T>* __restrict__ jr,
                                                  T* __restrict__ chi_out, T* __restrict__ ne_out,
@@ -887,7 +1055,7 @@ __global__ __launch_bounds__(1024) void f_source(const cplx<T>* __restrict__ er,
     Y[lds_swz<T>(k)] = cmk(ze.re - zo.im, ze.im + zo.re);
   }
   __syncthreads();
-  lds_fft_r8(Y, M, lgm, tws + r8_table_len(lgn), +1);
+  lds_fft_r8(Y, M, lgm, tws + r8_table_len(lgn), +1, tid);
   const T invm = T(1) / T(M);
   // thread t owns samples [a0, a0 + SPT) with a0 = (nth - 1 - t) SPT: the depletion integral runs from the
   // last sample backward, which makes it a prefix sum over the threads in lane order
@@ -967,7 +1135,7 @@ __global__ __launch_bounds__(1024) void f_source(const cplx<T>* __restrict__ er,
   }
   lds_write_reals<T, 2 * SPT>(Y, 2 * a0, yrun);
   __syncthreads();
-  lds_fft_r8(Y, N, lgn, tws, -1);
+  lds_fft_r8(Y, N, lgn, tws, -1, tid);
 
   // 4. J = F_NL [ rfft(J_p)/(0.1 - i w) + (-i w) rfft(P) ] (fft.cl:282-329, uppe_kernel.py:170-186)
 #pragma unroll
@@ -1290,6 +1458,14 @@ struct Fused {
     if (h->d.geometry == UPPE_RADIAL) {
       UPPE_HIP(hipMalloc(&s->Hb, sizeof(T) * (size_t)h->n1 * h->nk));
       UPPE_HIP(hipMalloc(&s->Hf, sizeof(T) * (size_t)h->n1 * h->nk));
+    } else {
+      std::vector<C> tx = r8_table_one<T>(h->n1), ty = r8_table_one<T>(h->n2);
+      s->twlen_x = (int)tx.size();
+      s->twlen_y = (int)ty.size();
+      UPPE_HIP(hipMalloc(&s->tw_x, sizeof(C) * std::max<size_t>(1, tx.size())));
+      UPPE_HIP(hipMalloc(&s->tw_y, sizeof(C) * std::max<size_t>(1, ty.size())));
+      if (!tx.empty()) UPPE_HIP(hipMemcpy(s->tw_x, tx.data(), sizeof(C) * tx.size(), hipMemcpyHostToDevice));
+      if (!ty.empty()) UPPE_HIP(hipMemcpy(s->tw_y, ty.data(), sizeof(C) * ty.size(), hipMemcpyHostToDevice));
     }
   }
 
@@ -1388,7 +1564,8 @@ struct Fused {
 
   static void source(uppe_handle* h, FusedState* s, const C* er, C* jr, bool diag, StepState* st) {
     const int N = h->nt;
-    const int nth = std::min(N, 1024);
+    // four samples per thread (Nt >= 256): short strips then get short workgroups, several per CU
+    const int nth = std::max(std::min(N, 64), N / 4);
     const int spt = N / nth;
     const size_t shm = sizeof(C) * (sizeof(T) == 4 ? N + N + N / 2 : N);
     const int lgn = ilog2i(N);
@@ -1397,17 +1574,13 @@ struct Fused {
                            : sizeof(T) == 4                  ? std::log2(s->ion.rate_conv * s->ion.c_pre)
                                                              : std::log(s->ion.rate_conv * s->ion.c_pre);
     (void)st;
-    dim3 grid((unsigned)h->P), block(nth);
+    const dim3 grid((unsigned)h->P), block(nth);
 #define UPPE_SRC(S)                                                                                              \
   if (spt == S) {                                                                                                \
-    if (diag)                                                                                                    \
-      launchf(h, UPPE_KC_SOURCE, f_source<T, S, true>, grid, block, shm, er, jr, (T*)s->chi_t, (T*)s->ne_t,       \
-              (const C*)s->tw, (const T*)s->w, (const T*)s->nlf, (const double*)s->ng, N, lgn, s->nwp, s->redchi3, \
-              s->dt, s->ion, log_pre);                                                                           \
-    else                                                                                                         \
-      launchf(h, UPPE_KC_SOURCE, f_source<T, S, false>, grid, block, shm, er, jr, (T*)nullptr, (T*)nullptr,       \
-              (const C*)s->tw, (const T*)s->w, (const T*)s->nlf, (const double*)s->ng, N, lgn, s->nwp, s->redchi3, \
-              s->dt, s->ion, log_pre);                                                                           \
+    auto kern = diag ? f_source<T, S, true> : f_source<T, S, false>;                                            \
+    launchf(h, UPPE_KC_SOURCE, kern, grid, block, shm, er, jr, diag ? (T*)s->chi_t : (T*)nullptr,                 \
+            diag ? (T*)s->ne_t : (T*)nullptr, (const C*)s->tw, (const T*)s->w, (const T*)s->nlf,                  \
+            (const double*)s->ng, N, lgn, s->nwp, s->redchi3, s->dt, s->ion, log_pre);                            \
     return;                                                                                                      \
   }
     UPPE_SRC(1) UPPE_SRC(2) UPPE_SRC(4)
@@ -1461,7 +1634,46 @@ struct Fused {
   }
 
   // one RK4 step: 4 x (k2r, source, r2k) + 2 control kernels
+  // one 1-D pass of a Cartesian transverse transform over columns [0, fend); sign +1 inverse (scaled 1/Na)
+  template <int PRO, int EPI>
+  static void tpass(uppe_handle* h, FusedState* s, int axis, const C* src, C* dst, int fend, int sign,
+                    const C* kprev, int stage, double zfix, StepState* st) {
+    constexpr int CW = sizeof(T) == 4 ? 16 : 8;  // columns per line: 128-byte runs
+    const int na = axis == 1 ? h->n1 : h->n2, nother = axis == 1 ? h->n2 : h->n1;
+    const int nth = std::max(64, CW * std::max(1, na / 8));
+    const void* tab = axis == 1 ? s->tw_x : s->tw_y;
+    const int tl = axis == 1 ? s->twlen_x : s->twlen_y;
+    dim3 grid((unsigned)((fend + CW - 1) / CW), (unsigned)nother);
+    const size_t shm = sizeof(C) * ((size_t)na * CW + tl);
+    const T scale = sign > 0 ? T(1.0 / na) : T(1);
+    launchf(h, UPPE_KC_TRANSVERSE_FFT, f_tfft<T, CW, PRO, EPI>, grid, dim3(nth), shm, src, dst, axis, na, ilog2i(na),
+            h->n2, s->nwp, fend, (const C*)tab, tl, sign, scale, (const double*)s->phs, (const C*)s->kzc,
+            (const T*)s->cond, (C*)s->q, kprev, (C*)s->k1, (C*)s->k2, (C*)s->k3, st, s->qmax, s->qmax_next, stage,
+            zfix);
+  }
+
+  // one RK4 step on a Cartesian grid: per RHS, the inverse transform (y with the stage prologue, then x), the
+  // source, the forward transform over the nonlinear band (x, then y with the RK stage epilogue)
+  static void step_cart(uppe_handle* h, FusedState* s) {
+    const C* kp[5] = {nullptr, nullptr, (const C*)s->k1, (const C*)s->k2, (const C*)s->k3};
+    C *q = (C*)s->q, *er = (C*)s->er, *jr = (C*)s->jr;
+    for (int stage = 1; stage <= 4; stage++) {
+      tpass<PRO_STAGE, EPI_NONE>(h, s, 2, q, er, h->nw, +1, kp[stage], stage, 0.0, s->st);
+      tpass<PRO_NONE, EPI_NONE>(h, s, 1, er, er, h->nw, +1, nullptr, stage, 0.0, s->st);
+      source(h, s, er, jr, false, s->st);
+      tpass<PRO_NONE, EPI_NONE>(h, s, 1, jr, jr, s->f1, -1, nullptr, stage, 0.0, s->st);
+      tpass<PRO_NONE, EPI_STAGE>(h, s, 2, jr, (C*)nullptr, s->f1, -1, nullptr, stage, 0.0, s->st);
+      if (stage == 1)
+        launchf(h, UPPE_KC_REDUCE, f_ctl_dz, dim3(1), dim3(1), 0, s->st, (const double*)s->sched, s->rec_active);
+    }
+    launchf(h, UPPE_KC_REDUCE, f_ctl_end, dim3(1), dim3(1), 0, s->st, s->qmax, s->qmax_next);
+  }
+
   static void step(uppe_handle* h, FusedState* s) {
+    if (h->d.geometry == UPPE_CARTESIAN) {
+      step_cart(h, s);
+      return;
+    }
     if (s->cm >= 0) {
       if constexpr (sizeof(T) == 8) {
         step_mfma<CM_F64>(h, s);
@@ -1497,7 +1709,13 @@ struct Fused {
     launchf(h, UPPE_KC_POINT, f_from_host<T>, dim3(grid_for(P * s->nwp)), dim3(kBlock), 0, (const double*)s->stage,
             (C*)s->er, h->nw, P, s->nwp);
     UPPE_HIP(hipMemsetAsync(s->q, 0, sizeof(C) * P * s->nwp, h->stream));
-    gemm_r2k_plain(h, s, (const C*)s->er, (C*)s->q);
+    if (h->d.geometry == UPPE_CARTESIAN) {
+      // numpy fft over x, then over y (grid_tools.py:220-229)
+      tpass<PRO_NONE, EPI_NONE>(h, s, 1, (const C*)s->er, (C*)s->er, h->nw, -1, nullptr, 0, 0.0, nullptr);
+      tpass<PRO_NONE, EPI_NONE>(h, s, 2, (const C*)s->er, (C*)s->q, h->nw, -1, nullptr, 0, 0.0, nullptr);
+    } else {
+      gemm_r2k_plain(h, s, (const C*)s->er, (C*)s->q);
+    }
     UPPE_HIP(hipMemsetAsync(s->qmax, 0, sizeof(unsigned long long) * 3, h->stream));
     launchf(h, UPPE_KC_REDUCE, f_qmax<T>, dim3(256), dim3(kBlock), 0, (const C*)s->q, P * s->nwp, s->qmax);
     launchf(h, UPPE_KC_REDUCE, f_qmax_cols<T>, dim3(256), dim3(kBlock), 0, (const C*)s->q, P, s->nwp, s->f1,
@@ -1577,11 +1795,17 @@ struct Fused {
   static void finish(uppe_handle* h, FusedState* s, double z_total, double* a_r, double* j_r, double* chi_t,
                      double* ne_t) {
     const long P = h->P, n = P * s->nwp;
-    dim3 g_k2r(2 * s->nwp / BN, (h->n1 + BM - 1) / BM);
-    launchf(h, UPPE_KC_HANKEL, f_gemm<T, PRO_FINISH, EPI_NONE>, g_k2r, dim3(256), 0, (const T*)s->Hb,
-            (const C*)s->q, (C*)s->er, h->n1, h->nk, s->nwp, (const double*)s->phs, (const C*)s->kzc,
-            (const T*)s->cond, (C*)nullptr, (const C*)nullptr, (C*)nullptr, (C*)nullptr, (C*)nullptr,
-            (StepState*)nullptr, (unsigned long long*)nullptr, (unsigned long long*)nullptr, 0, z_total);
+    if (h->d.geometry == UPPE_CARTESIAN) {
+      // linear propagation to z_total on load, then numpy ifft over y and over x
+      tpass<PRO_FINISH, EPI_NONE>(h, s, 2, (const C*)s->q, (C*)s->er, h->nw, +1, nullptr, 0, z_total, nullptr);
+      tpass<PRO_NONE, EPI_NONE>(h, s, 1, (const C*)s->er, (C*)s->er, h->nw, +1, nullptr, 0, z_total, nullptr);
+    } else {
+      dim3 g_k2r(2 * s->nwp / BN, (h->n1 + BM - 1) / BM);
+      launchf(h, UPPE_KC_HANKEL, f_gemm<T, PRO_FINISH, EPI_NONE>, g_k2r, dim3(256), 0, (const T*)s->Hb,
+              (const C*)s->q, (C*)s->er, h->n1, h->nk, s->nwp, (const double*)s->phs, (const C*)s->kzc,
+              (const T*)s->cond, (C*)nullptr, (const C*)nullptr, (C*)nullptr, (C*)nullptr, (C*)nullptr,
+              (StepState*)nullptr, (unsigned long long*)nullptr, (unsigned long long*)nullptr, 0, z_total);
+    }
     UPPE_HIP(hipMemcpyAsync(s->k1, s->er, sizeof(C) * n, hipMemcpyDeviceToDevice, h->stream));
     launchf(h, UPPE_KC_POINT, f_vp_to_e<T>, dim3(grid_for(n)), dim3(kBlock), 0, (C*)s->k1, (const C*)s->kzc, n);
     source(h, s, (const C*)s->k1, (C*)s->jr, true, nullptr);
@@ -1618,10 +1842,13 @@ struct Fused {
 static bool is_fp64(uppe_handle* h) { return h->fused->prec == UPPE_PREC_FP64; }
 
 void fused_create(uppe_handle* h) {
-  if (h->d.geometry != UPPE_RADIAL || h->n2 != 1)
+  if (h->d.geometry == UPPE_RADIAL && h->n2 != 1)
     throw Error{UPPE_ERR_UNSUPPORTED, "fused path: radial geometry with one azimuthal mode only"};
+  if (h->d.geometry == UPPE_CARTESIAN && (h->n1 < 2 || h->n2 < 2 || h->n1 > 512 || h->n2 > 512))
+    throw Error{UPPE_ERR_UNSUPPORTED, "fused Cartesian path supports 2 <= Nx, Ny <= 512"};
   if (h->nt > kMaxNtFused || h->nt < 4) throw Error{UPPE_ERR_UNSUPPORTED, "fused path supports 4 <= Nt <= 4096"};
-  if (h->n1 > 512) throw Error{UPPE_ERR_UNSUPPORTED, "fused path supports up to 512 radial cells"};
+  if (h->d.geometry == UPPE_RADIAL && h->n1 > 512)
+    throw Error{UPPE_ERR_UNSUPPORTED, "fused path supports up to 512 radial cells"};
   auto* s = new FusedState();
   h->fused = s;
   s->prec = h->d.precision;
@@ -1644,7 +1871,7 @@ void fused_destroy(uppe_handle* h) {
   if (!s) return;
   void* ptrs[] = {s->phs, s->kzc, s->w, s->nlf, s->cond, s->ng, s->Hb, s->Hf, s->tw, s->q, s->k1, s->k2, s->k3,
                   s->er, s->jr, s->chi_t, s->ne_t, s->stage, s->qmax, s->st, s->sched, s->rec, s->frb_hi,
-                  s->frb_lo, s->frf_hi, s->frf_lo};
+                  s->frb_lo, s->frf_hi, s->frf_lo, s->tw_x, s->tw_y};
   for (void* p : ptrs)
     if (p) (void)hipFree(p);
   if (s->st_host) (void)hipHostFree(s->st_host);

@@ -121,7 +121,8 @@ __device__ inline int lds_swz(int i) {
 // Pass with span Ns and radix R: butterfly j < n/R reads element j + r n/R, multiplies input r by
 // exp(sign 2 pi i r (j mod Ns)/(R Ns)), forms the R-point DFT and writes element
 // (j - j mod Ns) R + j mod Ns + r Ns (autosorted). Each pass loads its inputs into registers, waits for the
-// block, then stores, so no second buffer is needed; one butterfly per thread (n/R <= blockDim.x).
+// block, then stores, so no second buffer is needed; the block has at least n/8 threads for the radix-8 passes
+// and n/4 for the first pass (one or two butterflies per thread).
 // Twiddles: per-pass tables, the radix-8 pass of span Ns holding exp(-2 pi i r k/(8 Ns)) at
 // [(Ns - R0) + (r - 1) Ns + k], r = 1..7, k < Ns, with R0 = 2^(lg mod 3) the span of the first radix-8
 // pass; the table of an n-point transform has R0 (8^(lg/3) - 1) entries. Threads of a pass read consecutive
@@ -173,29 +174,36 @@ __device__ inline void dft8(const cplx<T>* v, cplx<T>* y, int sign) {
 }
 
 template <typename T>
-__device__ inline void lds_fft_r8(cplx<T>* buf, int n, int lg, const cplx<T>* tab, int sign) {
-  const int tid = threadIdx.x;
+__device__ inline void lds_fft_r8(cplx<T>* buf, int n, int lg, const cplx<T>* tab, int sign, int tid) {
   const int r0 = lg % 3;
   int Ns = 1;
   if (r0 > 0) {
-    const int R = 1 << r0, nb = n >> r0;
-    cplx<T> v[4];
-    if (tid < nb)
+    // up to two butterflies per thread (a radix-2 pass has n/2 of them, the block at least n/4 threads)
+    const int R = 1 << r0, nb = n >> r0, nth = blockDim.x;
+    cplx<T> v[2][4];
 #pragma unroll
-      for (int r = 0; r < 4; r++)
-        if (r < R) v[r] = buf[lds_swz<T>(tid + r * nb)];
+    for (int b = 0; b < 2; b++) {
+      const int j = tid + b * nth;
+      if (j < nb)
+#pragma unroll
+        for (int r = 0; r < 4; r++)
+          if (r < R) v[b][r] = buf[lds_swz<T>(j + r * nb)];
+    }
     __syncthreads();
-    if (tid < nb) {
+#pragma unroll
+    for (int b = 0; b < 2; b++) {
+      const int j = tid + b * nth;
+      if (j >= nb) continue;
       if (R == 4) {
-        dft4(v[0], v[1], v[2], v[3], sign);
+        dft4(v[b][0], v[b][1], v[b][2], v[b][3], sign);
       } else {
-        const cplx<T> s0 = cadd(v[0], v[1]), s1 = csub(v[0], v[1]);
-        v[0] = s0;
-        v[1] = s1;
+        const cplx<T> s0 = cadd(v[b][0], v[b][1]), s1 = csub(v[b][0], v[b][1]);
+        v[b][0] = s0;
+        v[b][1] = s1;
       }
 #pragma unroll
       for (int r = 0; r < 4; r++)
-        if (r < R) buf[lds_swz<T>(tid * R + r)] = v[r];
+        if (r < R) buf[lds_swz<T>(j * R + r)] = v[b][r];
     }
     __syncthreads();
     Ns = R;
@@ -226,22 +234,26 @@ __device__ inline void lds_fft_r8(cplx<T>* buf, int n, int lg, const cplx<T>* ta
   }
 }
 
-// host: the radix-8 pass tables of an N-point transform followed by those of the N/2-point transform,
-// FP64 -> T (see lds_fft_r8 for the layout)
+// host: the radix-8 pass tables of an n-point transform, FP64 -> T (see lds_fft_r8 for the layout)
+template <typename T>
+inline std::vector<cplx<T>> r8_table_one(int n) {
+  int lg = 0;
+  while ((1 << lg) < n) lg++;
+  std::vector<cplx<T>> tab;
+  for (int Ns = 1 << (lg % 3); Ns < n; Ns *= 8)
+    for (int r = 1; r < 8; r++)
+      for (int k = 0; k < Ns; k++) {
+        const double a = -2.0 * M_PI * r * k / (8.0 * Ns);
+        tab.push_back(cplx<T>{(T)std::cos(a), (T)std::sin(a)});
+      }
+  return tab;
+}
+
+// host: the tables of an N-point transform followed by those of the N/2-point transform
 template <typename T>
 inline std::vector<cplx<T>> r8_tables(int N) {
-  int lgN = 0;
-  while ((1 << lgN) < N) lgN++;
-  std::vector<cplx<T>> tab;
-  for (int lg : {lgN, lgN - 1}) {
-    const int n = 1 << lg;
-    for (int Ns = 1 << (lg % 3); Ns < n; Ns *= 8)
-      for (int r = 1; r < 8; r++)
-        for (int k = 0; k < Ns; k++) {
-          const double a = -2.0 * M_PI * r * k / (8.0 * Ns);
-          tab.push_back(cplx<T>{(T)std::cos(a), (T)std::sin(a)});
-        }
-  }
+  std::vector<cplx<T>> tab = r8_table_one<T>(N), half = r8_table_one<T>(N / 2);
+  tab.insert(tab.end(), half.begin(), half.end());
   return tab;
 }
 

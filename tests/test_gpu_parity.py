@@ -29,10 +29,8 @@ def _run(vol, precision, record=None):
 
 
 @pytest.mark.parametrize('coords', ['cylindrical', 'cartesian'])
-@pytest.mark.parametrize('precision', ['fp64-plain', 'fp64', 'fp32'])
+@pytest.mark.parametrize('precision', ['fp64-plain', 'fp64', 'fp32', 'bf16x3'])
 def test_gaussian_vacuum(oracle_mod, coords, precision):
-    if coords == 'cartesian' and precision != 'fp64-plain':
-        pytest.skip('fused Cartesian path not built yet')
     n = 128 if coords == 'cylindrical' else 64
     vol, p = configs.gaussian_vacuum(coords, n=n)
     _say('gaussian_vacuum %s %s n=%d' % (coords, precision, n))
@@ -135,3 +133,17 @@ def test_golden_fp32_replayed(name, precision):
     assert rel_l2(a, fx['A']) < FP32_FIELD_TOL
     if 'ne' in fx:
         assert rel_l2(np.fft.rfft(ne_t, axis=0), fx['ne']) < FP32_NE_TOL
+
+
+def test_volume_propagate_and_report(tmp_path):
+    '''The volume's UPPE branch (volume.py:606-613, 635-641) through the HIP path: the files hold track's arrays.'''
+    from searay_amd import volume
+    vol = configs.radial_kerr_air(scale=8, steps=2, precision='fp32')
+    _say('volume propagate + report (radial Kerr, fp32)')
+    box = vol['object']
+    ret = box.Propagate(None, None, None)
+    base = str(tmp_path / 'run')
+    box.Report(base, configs.MKS_LENGTH)
+    for suffix, ref in zip(volume.UPPE_OUTPUTS, ret):
+        assert np.array_equal(np.load(base + '_' + box.name + '_' + suffix + '.npy'), ref)
+    assert np.all(np.isfinite(ret[0]))

@@ -255,3 +255,30 @@ class TestMaterial:
         assert np.all(nl[:int(0.25 / dw)] == 0) and np.all(nl[int(1.5 / dw):] == 0)
         assert np.all(nl[int(0.25 / dw):int(1.5 / dw)] == 1)
         assert np.all(cond[:4] == 0) and np.all(cond[4:] == 1)
+
+
+class TestReportFiles:
+    '''Output files under the reference's names (volume.py:635-641, rays.py:117-123).'''
+
+    def test_uppe_report_names_and_contents(self, tmp_path):
+        from searay_amd import volume
+        box = volume.AnalyticBox('air')
+        rng = np.random.default_rng(5)
+        shape = (9, 4, 1, 2)
+        arrs = [rng.standard_normal(shape) + 1j * rng.standard_normal(shape) for _ in range(4)]
+        box.uppe_wave, box.uppe_j, box.uppe_chi, box.uppe_plasma = arrs
+        box.dom4d = np.arange(8.0)
+        base = str(tmp_path / 'run')
+        box.Report(base, 1.0)
+        for suffix, ref in zip(volume.UPPE_OUTPUTS, arrs + [box.dom4d]):
+            got = np.load(base + '_air_' + suffix + '.npy')
+            assert got.dtype == ref.dtype and np.array_equal(got, ref)
+
+    def test_sim_json(self, tmp_path):
+        import json
+        from searay_amd import volume
+        base = str(tmp_path / 'run')
+        volume.write_sim_json(base, {'mks_length': 1.27e-7}, [{'rays': [1, 2]}], {'base filename': 'out/test'})
+        assert json.load(open(base + '_sim.json')) == {'mks_length': 1.27e-7}
+        assert json.load(open(base + '_sources.json')) == [{'rays': [1, 2]}]
+        assert json.load(open(base + '_diagnostics.json')) == {'base filename': 'out/test'}
