@@ -115,6 +115,21 @@ int uppe_set_hankel(uppe_handle* h, const double* H, const double* lambda_);
 /* Medium tables; may be called again between advance calls (uppe_kernel.py:303). */
 int uppe_set_medium(uppe_handle* h, const uppe_medium* m);
 
+/* Rotational (Raman) susceptibility added to chi in every source evaluation (update_current's
+ * rotator.AddChi, uppe_kernel.py:174-176; rotations.py:44-85; rotations.cl:80-129): `states` levels with the
+ * transition weights tjj[states] and W_j dt as complex128 wdt[states] (Rotator.Tj and Rotator.Wj * dt), the
+ * polarizability difference and the normalised hbar. A NULL rotator, or one with fewer than 3 states,
+ * switches it off. */
+typedef struct uppe_rotator {
+  int states;
+  int reserved;
+  const double* tjj;
+  const double* wdt;
+  double dalpha;
+  double hbar;
+} uppe_rotator;
+int uppe_set_rotator(uppe_handle* h, const uppe_rotator* r);
+
 /* Upload the r-space field a[nw][n1][n2] (complex128) and transform it to k-space
  * (uppe_kernel.py:290-291: Source(...) then T.kspacex(src.qw_dev)). */
 int uppe_set_field(uppe_handle* h, const double* a_r);

@@ -180,6 +180,9 @@ typedef struct orc_handle {
   double *w, *kz, *kg, *ng, *nlf, *cond;
   double redchi3, dt;
   uppe_ionization ion;
+  /* rotational susceptibility (rotations.py:44-85) */
+  int rot_states;
+  double *rot_tjj, *rot_wdt, rot_dalpha, rot_hbar;
   /* transverse basis */
   double *H, *lam;
   /* device-array counterparts of Source (uppe_kernel.py:108-119) and the Hankel scratch (grid_tools.py:302) */
@@ -327,6 +330,44 @@ static void k_add_kerr_chi(orc_handle* h) {
   for (long idx = 0; idx < h->ntt; idx++) {
     double e = h->Et[idx];
     h->chi[idx] += h->ng[idx % per] * h->redchi3 * e * e;
+  }
+}
+
+/* chi += rotational susceptibility, one strip per iteration, serially over the states and over time from
+ * the last sample (rotations.cl:80-129, AddChiUPPE) */
+static void k_add_chi_rotations(orc_handle* h) {
+  const long per = h->ntr, steps = h->nt;
+  const double dt = h->dt, dalpha = h->rot_dalpha, hbar = h->rot_hbar;
+#pragma omp parallel for schedule(static)
+  for (long xy = 0; xy < per; xy++) {
+    for (int j = 2; j < h->rot_states; j++) {
+      const double wr = h->rot_wdt[2 * j], wi = h->rot_wdt[2 * j + 1];
+      const double Sjj = 2.0 * j * (j - 1) / (15 * (2 * j - 1));
+      double rr = 0.0, ri = 0.0;                 /* rhojj */
+      double er = 1.0, ei = 0.0;                 /* exp_iwt */
+      const double hr = cos(0.5 * wr) * exp(-0.5 * wi), hi = sin(0.5 * wr) * exp(-0.5 * wi); /* exp_iwdth */
+      double cum = 0.0;
+      for (long k = steps - 1; k >= 0; --k) {
+        const long idx = k * per + xy;
+        /* term = Tjj Sjj (rho conj(exp_iwt)) wdt */
+        const double ar = rr * er + ri * ei, ai = -rr * ei + ri * er;
+        const double ti = h->rot_tjj[j] * Sjj * (ar * wi + ai * wr);
+        cum += 2 * ti * h->ng[xy] * dalpha;
+        h->chi[idx] += cum;
+        const double e = h->Et[idx];
+        const double F = 0.5 * (dalpha / hbar) * e * e;
+        double nr = er * hr - ei * hi, ni = er * hi + ei * hr;
+        er = nr;
+        ei = ni;
+        /* rho -= dt F i exp_iwt */
+        rr -= dt * F * (-ei);
+        ri -= dt * F * er;
+        nr = er * hr - ei * hi;
+        ni = er * hi + ei * hr;
+        er = nr;
+        ei = ni;
+      }
+    }
   }
 }
 
@@ -524,6 +565,7 @@ static void update_current(orc_handle* h) {
     memset(h->Jw, 0, sizeof(double) * 2 * h->npt);
   }
   memset(h->chi, 0, sizeof(double) * h->ntt);
+  if (h->rot_states > 2) k_add_chi_rotations(h);
   k_add_kerr_chi(h);
   k_source_polarization(h);
   rfft_all(h, h->Jt, h->scratchw);
@@ -594,7 +636,7 @@ static void condition_field(orc_handle* h) { k_filter(h, h->qw, h->cond); }
 static void free_all(orc_handle* h) {
   double** ptrs[] = {&h->w, &h->kz, &h->kg, &h->ng, &h->nlf, &h->cond, &h->H, &h->lam, &h->qw, &h->qi,
                      &h->qf, &h->Ew, &h->Jw, &h->scratchw, &h->hscratch, &h->dzb, &h->Et, &h->ne, &h->chi,
-                     &h->Jt};
+                     &h->Jt, &h->rot_tjj, &h->rot_wdt};
   for (size_t i = 0; i < sizeof(ptrs) / sizeof(ptrs[0]); i++) {
     free(*ptrs[i]);
     *ptrs[i] = NULL;
@@ -657,6 +699,27 @@ int orc_uppe_set_hankel(orc_handle* h, const double* H, const double* lam) {
   memcpy(h->H, H, sizeof(double) * h->nk * h->n1 * h->nm);
   memcpy(h->lam, lam, sizeof(double) * h->n1);
   h->have_hankel = 1;
+  return UPPE_OK;
+}
+
+/* Rotator.AddChi's tables (rotations.py:56-61); NULL or fewer than 3 states switches the rotator off */
+int orc_uppe_set_rotator(orc_handle* h, const uppe_rotator* r) {
+  if (!h) return UPPE_ERR_ARG;
+  free(h->rot_tjj);
+  free(h->rot_wdt);
+  h->rot_tjj = h->rot_wdt = NULL;
+  h->rot_states = 0;
+  if (!r || r->states < 3) return UPPE_OK;
+  if (!r->tjj || !r->wdt) return fail(h, UPPE_ERR_ARG, "null rotator table");
+  if (r->states > 4096) return fail(h, UPPE_ERR_ARG, "rotator: at most 4096 states");
+  h->rot_tjj = (double*)malloc(sizeof(double) * r->states);
+  h->rot_wdt = (double*)malloc(sizeof(double) * 2 * r->states);
+  if (!h->rot_tjj || !h->rot_wdt) return fail(h, UPPE_ERR_ARG, "out of host memory");
+  memcpy(h->rot_tjj, r->tjj, sizeof(double) * r->states);
+  memcpy(h->rot_wdt, r->wdt, sizeof(double) * 2 * r->states);
+  h->rot_states = r->states;
+  h->rot_dalpha = r->dalpha;
+  h->rot_hbar = r->hbar;
   return UPPE_OK;
 }
 

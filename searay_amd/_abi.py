@@ -38,6 +38,11 @@ class uppe_medium(ct.Structure):
                 ('redchi3', ct.c_double), ('dt', ct.c_double), ('ion', uppe_ionization)]
 
 
+class uppe_rotator(ct.Structure):
+    _fields_ = [('states', ct.c_int), ('reserved', ct.c_int), ('tjj', _dp), ('wdt', _dp), ('dalpha', ct.c_double),
+                ('hbar', ct.c_double)]
+
+
 class uppe_step_control(ct.Structure):
     _fields_ = [('zi', ct.c_double), ('length', ct.c_double), ('sub0', ct.c_int), ('max_steps', ct.c_int),
                 ('dzmin', ct.c_double), ('dphi', ct.c_double), ('rel_amin', ct.c_double),
@@ -50,7 +55,7 @@ class uppe_info(ct.Structure):
                 ('reserved', ct.c_int), ('z_last', ct.c_double)]
 
 
-ENTRY_POINTS = ('create', 'set_hankel', 'set_medium', 'set_field', 'advance', 'finish', 'get_state',
+ENTRY_POINTS = ('create', 'set_hankel', 'set_medium', 'set_rotator', 'set_field', 'advance', 'finish', 'get_state',
                 'get_info', 'destroy', 'last_error')
 HIP_ONLY_ENTRY_POINTS = ('profile_enable', 'profile_read', 'synchronize')
 
@@ -128,6 +133,17 @@ class Stepper:
         m = uppe_medium(_ptr(arrs['w']), _ptr(arrs['kz']), _ptr(arrs['kg']), _ptr(arrs['ngas']),
                         _ptr(arrs['nl']), _ptr(arrs['cond']), float(redchi3), float(dt), io)
         self._check(self._f['set_medium'](self.h, ct.byref(m)), 'set_medium')
+
+    def set_rotator(self, rotator, dt):
+        '''Load a rotations.Rotator's tables for source time step dt, or switch the rotator off (None).'''
+        if rotator is None:
+            self._check(self._f['set_rotator'](self.h, None), 'set_rotator')
+            return
+        states, tjj, wdt = rotator.kernel_tables(dt)
+        tjj = _f64(tjj)
+        wdt = _c128(wdt).view(np.float64)
+        r = uppe_rotator(int(states), 0, _ptr(tjj), _ptr(wdt), float(rotator.dalpha), float(rotator.hbar))
+        self._check(self._f['set_rotator'](self.h, ct.byref(r)), 'set_rotator')
 
     def set_field(self, a_r):
         a = _c128(a_r).reshape(self.shape)

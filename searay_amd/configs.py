@@ -21,6 +21,7 @@ from scipy import constants as C
 from . import dispersion
 from . import input_tools
 from . import ionization
+from . import rotations
 from . import volume
 
 MKS_LENGTH = 0.8e-6 / (2 * np.pi)
@@ -100,6 +101,27 @@ def radial_kerr_air(scale=1, steps=None, seed=1000, precision=None):
     vol['incoming wave'] = incoming
     if precision:
         vol['uppe precision'] = precision
+    return vol
+
+
+def n2_rotator(states=32):
+    '''The N2 rotator of the reference's examples (examples/uppe/rot_benchmark.py:50-55): B = 2 cm^-1,
+    polarizability difference 4 pi 6.7e-25 cm^3, kT = 0.025 eV, nuclear degeneracy 6 - 3 (j mod 2), level
+    damping 1/(100 ps), hbar normalised to the 800 nm carrier.'''
+    helper = input_tools.InputHelper(MKS_LENGTH)
+    j = np.arange(0, states)
+    damping = np.ones(states) / helper.dnum('100 ps')
+    dnuc = 6 - 3 * (j % 2)
+    w00 = 1.0  # 2 pi / (800 nm) in units of 1/MKS_LENGTH
+    hbar_norm = C.hbar * (w00 * C.c / MKS_LENGTH) / (C.m_e * C.c ** 2)
+    return rotations.Rotator(2 / helper.dnum('1 cm'), 4 * np.pi * 6.7e-25 / helper.dnum('1 cm-3'),
+                             helper.dnum('.025 eV'), dnuc, damping, hbar_norm)
+
+
+def radial_raman_air(scale=1, steps=None, seed=1003, precision=None, states=32):
+    '''The radial Kerr workload in N2 with the rotational (Raman) susceptibility added.'''
+    vol = radial_kerr_air(scale=scale, steps=steps, seed=seed, precision=precision)
+    vol['rotator'] = n2_rotator(states)
     return vol
 
 

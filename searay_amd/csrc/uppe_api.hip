@@ -168,6 +168,29 @@ int uppe_set_medium(uppe_handle* h, const uppe_medium* m) {
   });
 }
 
+int uppe_set_rotator(uppe_handle* h, const uppe_rotator* r) {
+  if (!h) return UPPE_ERR_ARG;
+  return guarded(h, [&] {
+    UPPE_HIP(hipSetDevice(h->d.device));
+    if (!r || r->states < 3) {
+      h->rot_states = 0;
+      h->rot_tjj.clear();
+      h->rot_wdt.clear();
+    } else {
+      if (!r->tjj || !r->wdt) throw Error{UPPE_ERR_ARG, "null rotator table"};
+      if (r->states > 4096) throw Error{UPPE_ERR_ARG, "rotator: at most 4096 states"};
+      if (!(r->hbar != 0.0)) throw Error{UPPE_ERR_ARG, "rotator: hbar must be nonzero"};
+      h->rot_states = r->states;
+      h->rot_tjj.assign(r->tjj, r->tjj + r->states);
+      h->rot_wdt.assign(r->wdt, r->wdt + 2 * r->states);
+      h->rot_dalpha = r->dalpha;
+      h->rot_hbar = r->hbar;
+    }
+    if (h->plain) plain_set_rotator(h);
+    if (h->fused) fused_set_rotator(h);
+  });
+}
+
 int uppe_set_field(uppe_handle* h, const double* a_r) {
   if (!h) return UPPE_ERR_ARG;
   if (!a_r) {

@@ -79,6 +79,7 @@ struct FusedState {
   // Cartesian: radix-8 pass tables of the x and y transforms (complex T)
   void *tw_x = nullptr, *tw_y = nullptr;
   int twlen_x = 0, twlen_y = 0;
+  double *rot_tjj = nullptr, *rot_wdt = nullptr;  // rotator tables (device), when the handle has a rotator
 };
 
 constexpr int BM = 64, BN = 64, BK = 16;
@@ -958,6 +959,25 @@ __device__ inline void lds_write_reals(cplx<T>* Y, int n0, const T* src) {
   }
 }
 
+// E^n with E = exp(i w / 2) for the complex W_j dt = w (rotations.cl:107: exp_iwdth = exp(i w_re / 2 - w_im / 2)):
+// the phase n w_re / 2 formed in FP64; FP32 reduces it to turns for the hardware sine and cosine
+template <typename R>
+__device__ inline cplx<R> rot_pow(double wr, double wi, int n);
+template <>
+__device__ inline cplx<float> rot_pow<float>(double wr, double wi, int n) {
+  const double turns = 0.5 * wr * (double)n * 0.15915494309189535;
+  const float f = (float)(turns - rint(turns));
+  const float mag = __builtin_amdgcn_exp2f((float)(-0.5 * wi * (double)n * 1.4426950408889634));
+  return cmk(__builtin_amdgcn_cosf(f) * mag, __builtin_amdgcn_sinf(f) * mag);
+}
+template <>
+__device__ inline cplx<double> rot_pow<double>(double wr, double wi, int n) {
+  double sn, cs;
+  sincos(0.5 * wr * (double)n, &sn, &cs);
+  const double mag = exp(-0.5 * wi * (double)n);
+  return cmk(cs * mag, sn * mag);
+}
+
 // DPP lane move (gfx9 control codes: row_shr:n = 0x110 + n, wave_shr:1 = 0x138, row_bcast:15 = 0x142,
 // row_bcast:31 = 0x143); lanes without a source, or in rows outside RM, read 0
 template <int CTRL, int RM, typename T>
@@ -993,11 +1013,15 @@ __global__ __launch_bounds__(1024) void f_source(const cplx<T>* __restrict__ er,
                                                  const cplx<T>* __restrict__ tw, const T* __restrict__ w,
                                                  const T* __restrict__ nlf, const double* __restrict__ ng, int nt,
                                                  int lgn, int nwp, double redchi3, double dt, uppe_ionization ion,
-                                                 double log_pre) {
+                                                 double log_pre, const double* __restrict__ rot_tjj,
+                                                 const double* __restrict__ rot_wdt, int rot_states,
+                                                 double rot_dalpha, double rot_hbar) {
   // no early exit for a finished run: the kernel only writes scratch (J_r, diagnostics)
   extern __shared__ char smem[];
   cplx<T>* Y = reinterpret_cast<cplx<T>*>(smem);
   __shared__ T wsum[16];
+  __shared__ cplx<T> rot_wsc[16];
+  __shared__ T rot_wsr[16];
   const int N = nt, M = nt / 2, lgm = lgn - 1;
   const long p = blockIdx.x;
   const int tid = threadIdx.x, nth = blockDim.x, lane = tid & 63, wave = tid >> 6, nwv = nth >> 6;
@@ -1118,13 +1142,63 @@ __global__ __launch_bounds__(1024) void f_source(const cplx<T>* __restrict__ er,
     __syncthreads();  // every thread has read its samples before the strip is overwritten
   }
 
+  // 2b. rotational (Raman) susceptibility (rotations.cl:80-129, AddChiUPPE). For each state j >= 2 the
+  // reference runs the recurrence rho <- rho - i dt F_m E^(2m+1) and cum <- cum + 2 Im(T_j S_j rho conj(E^(2m))
+  // W_j dt) n_g dalpha, adding cum to chi, over m = N-1-n (time from the last sample), E = exp(i W_j dt / 2),
+  // F_m = dalpha/(2 hbar) E(t)^2. Here both running sums are prefix sums over m: the thread order already
+  // follows m, so each is a DPP wave scan plus the sum over the earlier waves.
+  R dchi[SPT];
+#pragma unroll
+  for (int j = 0; j < SPT; j++) dchi[j] = R(0);
+  if (rot_states > 2) {
+    const R Fc = (R)(0.5 * rot_dalpha / rot_hbar), dtr = (R)dt;
+    const int m0 = N - a0 - SPT;  // m of this thread's last sample; m grows as the sample index falls
+    for (int sj = 2; sj < rot_states; sj++) {
+      const double wr = rot_wdt[2 * sj], wi = rot_wdt[2 * sj + 1];
+      const R cj = (R)(rot_tjj[sj] * (2.0 * sj * (sj - 1) / (15.0 * (2 * sj - 1))) * 2.0 * ng[p] * rot_dalpha);
+      const cplx<R> e1 = rot_pow<R>(wr, wi, 1), e2 = rot_pow<R>(wr, wi, 2);
+      cplx<R> pw = rot_pow<R>(wr, wi, 2 * m0), a[SPT], pe[SPT];
+      cplx<R> tot = cmk(R(0), R(0));
+#pragma unroll
+      for (int jj = SPT - 1; jj >= 0; jj--) {
+        pe[jj] = pw;  // E^(2m)
+        a[jj] = cscale(cmul(pw, e1), Fc * e[jj] * e[jj]);  // F_m E^(2m+1)
+        tot = cadd(tot, a[jj]);
+        pw = cmul(pw, e2);
+      }
+      cplx<R> run = cmk(wave_excl_prefix(tot.re), wave_excl_prefix(tot.im));
+      if (lane == 63) rot_wsc[wave] = cadd(run, tot);
+      __syncthreads();
+      for (int w2 = 0; w2 < wave; w2++) run = cadd(run, rot_wsc[w2]);
+      R g[SPT], gtot = R(0);
+#pragma unroll
+      for (int jj = SPT - 1; jj >= 0; jj--) {
+        // rho_m = -i dt (sum of a over earlier m); Im(rho conj(E^(2m)) W dt) with W dt = (wr, wi)
+        const cplx<R> rho = cmk(dtr * run.im, -dtr * run.re);
+        const cplx<R> q = cmul(rho, cconj(pe[jj]));
+        g[jj] = cj * (q.re * (R)wi + q.im * (R)wr);
+        gtot += g[jj];
+        run = cadd(run, a[jj]);
+      }
+      R cum = wave_excl_prefix(gtot);
+      if (lane == 63) rot_wsr[wave] = cum + gtot;
+      __syncthreads();
+      for (int w2 = 0; w2 < wave; w2++) cum += rot_wsr[w2];
+#pragma unroll
+      for (int jj = SPT - 1; jj >= 0; jj--) {
+        cum += g[jj];
+        dchi[jj] += cum;
+      }
+    }
+  }
+
   // 3. Kerr chi = ng chi3/nref E^2 (uppe.cl:132-151), P = chi E, J_p = ne E, packed as P + i J_p
   const T kc = (T)(ngp * redchi3);
   T yrun[2 * SPT];
 #pragma unroll
   for (int j = 0; j < SPT; j++) {
     int n = a0 + j;
-    T chi = kc * e[j] * e[j];
+    T chi = (T)dchi[j] + kc * e[j] * e[j];
     T nej = (T)ne[j];
     yrun[2 * j] = chi * e[j];
     yrun[2 * j + 1] = nej * e[j];
@@ -1580,7 +1654,8 @@ struct Fused {
     auto kern = diag ? f_source<T, S, true> : f_source<T, S, false>;                                            \
     launchf(h, UPPE_KC_SOURCE, kern, grid, block, shm, er, jr, diag ? (T*)s->chi_t : (T*)nullptr,                 \
             diag ? (T*)s->ne_t : (T*)nullptr, (const C*)s->tw, (const T*)s->w, (const T*)s->nlf,                  \
-            (const double*)s->ng, N, lgn, s->nwp, s->redchi3, s->dt, s->ion, log_pre);                            \
+            (const double*)s->ng, N, lgn, s->nwp, s->redchi3, s->dt, s->ion, log_pre, (const double*)s->rot_tjj,  \
+            (const double*)s->rot_wdt, h->rot_states, h->rot_dalpha, h->rot_hbar);                               \
     return;                                                                                                      \
   }
     UPPE_SRC(1) UPPE_SRC(2) UPPE_SRC(4)
@@ -1871,12 +1946,24 @@ void fused_destroy(uppe_handle* h) {
   if (!s) return;
   void* ptrs[] = {s->phs, s->kzc, s->w, s->nlf, s->cond, s->ng, s->Hb, s->Hf, s->tw, s->q, s->k1, s->k2, s->k3,
                   s->er, s->jr, s->chi_t, s->ne_t, s->stage, s->qmax, s->st, s->sched, s->rec, s->frb_hi,
-                  s->frb_lo, s->frf_hi, s->frf_lo, s->tw_x, s->tw_y};
+                  s->frb_lo, s->frf_hi, s->frf_lo, s->tw_x, s->tw_y, s->rot_tjj, s->rot_wdt};
   for (void* p : ptrs)
     if (p) (void)hipFree(p);
   if (s->st_host) (void)hipHostFree(s->st_host);
   delete s;
   h->fused = nullptr;
+}
+
+void fused_set_rotator(uppe_handle* h) {
+  FusedState* s = h->fused;
+  if (s->rot_tjj) UPPE_HIP(hipFree(s->rot_tjj));
+  if (s->rot_wdt) UPPE_HIP(hipFree(s->rot_wdt));
+  s->rot_tjj = s->rot_wdt = nullptr;
+  if (h->rot_states < 3) return;
+  UPPE_HIP(hipMalloc(&s->rot_tjj, sizeof(double) * h->rot_states));
+  UPPE_HIP(hipMalloc(&s->rot_wdt, sizeof(double) * 2 * h->rot_states));
+  UPPE_HIP(hipMemcpy(s->rot_tjj, h->rot_tjj.data(), sizeof(double) * h->rot_states, hipMemcpyHostToDevice));
+  UPPE_HIP(hipMemcpy(s->rot_wdt, h->rot_wdt.data(), sizeof(double) * 2 * h->rot_states, hipMemcpyHostToDevice));
 }
 
 void fused_set_hankel(uppe_handle* h) {

@@ -284,6 +284,10 @@ struct uppe_handle {
   double z_last = 0.0;
   // host copies of the tables (the reference keeps these on the host too)
   std::vector<double> H, lam, w;
+  // rotational susceptibility: states (0 = off), Tjj[states], W_j dt [states] complex, dalpha, hbar
+  int rot_states = 0;
+  std::vector<double> rot_tjj, rot_wdt;
+  double rot_dalpha = 0.0, rot_hbar = 1.0;
   // profiling
   bool prof = false;
   std::vector<uppe::ProfRecord> prof_pending;
@@ -317,6 +321,7 @@ void plain_create(uppe_handle* h);
 void plain_destroy(uppe_handle* h);
 void plain_set_hankel(uppe_handle* h);
 void plain_set_medium(uppe_handle* h, const uppe_medium* m);
+void plain_set_rotator(uppe_handle* h);
 void plain_set_field(uppe_handle* h, const double* a_r);
 int plain_advance(uppe_handle* h, const uppe_step_control* c);
 void plain_finish(uppe_handle* h, double z_total, double* a_r, double* j_r, double* chi_t, double* ne_t);
@@ -326,6 +331,7 @@ void plain_get_state(uppe_handle* h, double* q);
 void fused_create(uppe_handle* h);
 void fused_destroy(uppe_handle* h);
 void fused_set_hankel(uppe_handle* h);
+void fused_set_rotator(uppe_handle* h);
 void fused_set_medium(uppe_handle* h, const uppe_medium* m);
 void fused_set_field(uppe_handle* h, const double* a_r);
 int fused_advance(uppe_handle* h, const uppe_step_control* c);

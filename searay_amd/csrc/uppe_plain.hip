@@ -23,10 +23,44 @@ struct PlainState {
   double *qw = nullptr, *qi = nullptr, *qf = nullptr, *Ew = nullptr, *Jw = nullptr, *scratchw = nullptr,
          *hscratch = nullptr, *dzb = nullptr;
   double *Et = nullptr, *ne = nullptr, *chi = nullptr, *Jt = nullptr;
+  double *rot_tjj = nullptr, *rot_wdt = nullptr;  // rotator tables (device), when h->rot_states > 0
   double* part = nullptr;       // per-block reduction partials (device)
   double* part_host = nullptr;  // pinned host copy
   int nblk_red = 0;
 };
+
+// chi += rotational susceptibility (rotations.cl:80-129, AddChiUPPE): one thread per strip, serially over the
+// states and over time from the last sample, as the reference's work-item does
+__global__ void plain_add_chi_rotations(double* chi, const double* Et, const double* ng, const double* tjj,
+                                        const double* wdt, double dt, double dalpha, double hbar, int steps,
+                                        int states, long P) {
+  const long xy = blockIdx.x * (long)blockDim.x + threadIdx.x;
+  if (xy >= P) return;
+  for (int j = 2; j < states; j++) {
+    const double wr = wdt[2 * j], wi = wdt[2 * j + 1];
+    const double Sjj = 2.0 * j * (j - 1) / (15 * (2 * j - 1));
+    double rr = 0.0, ri = 0.0, er = 1.0, ei = 0.0;
+    const double hr = cos(0.5 * wr) * exp(-0.5 * wi), hi = sin(0.5 * wr) * exp(-0.5 * wi);
+    double cum = 0.0;
+    for (int k = steps - 1; k >= 0; --k) {
+      const long idx = (long)k * P + xy;
+      const double ar = rr * er + ri * ei, ai = -rr * ei + ri * er;  // rho conj(exp_iwt)
+      cum += 2 * (tjj[j] * Sjj * (ar * wi + ai * wr)) * ng[xy] * dalpha;
+      chi[idx] += cum;
+      const double e = Et[idx];
+      const double F = 0.5 * (dalpha / hbar) * e * e;
+      double nr = er * hr - ei * hi, ni = er * hi + ei * hr;
+      er = nr;
+      ei = ni;
+      rr -= dt * F * (-ei);  // rho -= dt F i exp_iwt
+      ri -= dt * F * er;
+      nr = er * hr - ei * hi;
+      ni = er * hi + ei * hr;
+      er = nr;
+      ei = ni;
+    }
+  }
+}
 
 // ---------------------------------------------------------------------------------------------------
 // point kernels (kernels/uppe.cl, fft.cl, paraxial.cl)
@@ -391,6 +425,18 @@ static void launch(uppe_handle* h, int kc, K kernel, unsigned grid, unsigned blo
   UPPE_LAUNCH_CHECK();
 }
 
+void plain_set_rotator(uppe_handle* h) {
+  PlainState* s = h->plain;
+  if (s->rot_tjj) UPPE_HIP(hipFree(s->rot_tjj));
+  if (s->rot_wdt) UPPE_HIP(hipFree(s->rot_wdt));
+  s->rot_tjj = s->rot_wdt = nullptr;
+  if (h->rot_states < 3) return;
+  UPPE_HIP(hipMalloc(&s->rot_tjj, sizeof(double) * h->rot_states));
+  UPPE_HIP(hipMalloc(&s->rot_wdt, sizeof(double) * 2 * h->rot_states));
+  UPPE_HIP(hipMemcpy(s->rot_tjj, h->rot_tjj.data(), sizeof(double) * h->rot_states, hipMemcpyHostToDevice));
+  UPPE_HIP(hipMemcpy(s->rot_wdt, h->rot_wdt.data(), sizeof(double) * 2 * h->rot_states, hipMemcpyHostToDevice));
+}
+
 void plain_create(uppe_handle* h) {
   if (h->nt > kMaxNtPlain) throw Error{UPPE_ERR_UNSUPPORTED, "plain path supports Nt <= 16384"};
   auto* s = new PlainState();
@@ -430,7 +476,7 @@ void plain_destroy(uppe_handle* h) {
   if (!s) return;
   void* ptrs[] = {s->w, s->kz, s->kg, s->ng, s->nlf, s->cond, s->H, s->lam, s->tw_t, s->tw1, s->tw2, s->qw,
                   s->qi, s->qf, s->Ew, s->Jw, s->scratchw, s->hscratch, s->dzb, s->Et, s->ne, s->chi, s->Jt,
-                  s->part};
+                  s->part, s->rot_tjj, s->rot_wdt};
   for (void* p : ptrs)
     if (p) (void)hipFree(p);
   if (s->part_host) (void)hipHostFree(s->part_host);
@@ -526,6 +572,10 @@ static void update_current(uppe_handle* h) {
     UPPE_HIP(hipMemsetAsync(s->Jw, 0, sizeof(double) * 2 * npt, h->stream));
   }
   UPPE_HIP(hipMemsetAsync(s->chi, 0, sizeof(double) * ntt, h->stream));
+  if (h->rot_states > 2)
+    launch(h, UPPE_KC_SOURCE, plain_add_chi_rotations, grid_for(P, 64), 64, 0, s->chi, (const double*)s->Et,
+           (const double*)s->ng, (const double*)s->rot_tjj, (const double*)s->rot_wdt, s->dt, h->rot_dalpha,
+           h->rot_hbar, h->nt, h->rot_states, P);
   launch(h, UPPE_KC_POINT, plain_kerr_chi, grid_for(ntt), kBlock, 0, s->chi, (const double*)s->Et,
          (const double*)s->ng, s->redchi3, ntt, P);
   launch(h, UPPE_KC_POINT, plain_mul, grid_for(ntt), kBlock, 0, s->Jt, (const double*)s->chi, (const double*)s->Et,

@@ -14,7 +14,8 @@ dictionary keys select the device path: 'uppe precision' in {'fp64', 'fp32', 'bf
 
 Differences from the reference, by design: the ray boundary (``GetBoundaryFields``) and ray relaunch
 (``RelaunchRays``) are not provided, so 'incoming wave' is required and xp/eikonal/vg are returned unchanged;
-the rotational Raman ``rotator`` is not provided (NotImplementedError).
+the rotational Raman ``rotator`` (a rotations.Rotator) is loaded into the stepper and added to chi in every
+source evaluation.
 '''
 import logging
 
@@ -110,9 +111,10 @@ class _RunContext:
         if geometry == _abi.UPPE_RADIAL:
             self.stepper.set_hankel(T.H, T.Lambda)
         self.medium_key = None
+        self.rotator_key = (id(None), None)
 
 
-def _propagate(ctx, ctool, vg, a, mat, ionizer, NL_band, subcycles, zi, zf, dzmin, record=None):
+def _propagate(ctx, ctool, vg, a, mat, ionizer, NL_band, subcycles, zi, zf, dzmin, record=None, rotator=None):
     w, x, y, ext = ctool.GetGridInfo()
     if w[0] != 0.0:
         raise ValueError('UPPE propagator requires frequency range [0,...]')
@@ -122,6 +124,9 @@ def _propagate(ctx, ctool, vg, a, mat, ionizer, NL_band, subcycles, zi, zf, dzmi
     L = (zf - zi) / subcycles
     dt, nl, cond = source_tables(w, NL_band)
     ion = ionizer.kernel_params() if ionizer is not None else ionization.no_ionization_params()
+    if ctx.rotator_key != (id(rotator), dt):
+        st.set_rotator(rotator, dt)
+        ctx.rotator_key = (id(rotator), dt)
     st.set_field(a)
     # density at every subcycle midpoint; consecutive subcycles with the same density share one device call
     dens = [mat.density(zi + sub * L + 0.5 * L) for sub in range(subcycles)]
@@ -149,15 +154,13 @@ def propagator(cl, ctool, vg, a, mat, ionizer, rotator, NL_band, subcycles, zi, 
     '''Advance the field a(w, x1, x2) from zi to the next diagnostic plane zf (uppe_kernel.py:260-331).
 
     Returns (a_r, J_r, rfft(chi), rfft(ne), RHS evaluations). The stepper handle is cached on ``mat``.'''
-    if rotator is not None:
-        raise NotImplementedError('rotational Raman source is not provided by this build')
     ctx = getattr(mat, '_uppe_ctx', None)
     if ctx is None:
         geometry = _abi.UPPE_CARTESIAN if isinstance(ctool, caustic_tools.FourierTool) else _abi.UPPE_RADIAL
         ctx = _RunContext(getattr(mat, 'stepper_factory', None) or hip_stepper_factory(), geometry, ctool,
                           mat.N)
         mat._uppe_ctx = ctx
-    return _propagate(ctx, ctool, vg, a, mat, ionizer, NL_band, subcycles, zi, zf, dzmin)
+    return _propagate(ctx, ctool, vg, a, mat, ionizer, NL_band, subcycles, zi, zf, dzmin, rotator=rotator)
 
 
 class RunSetup:
@@ -197,8 +200,7 @@ class RunSetup:
         self.ionizer = None
         if vol_dict.get('ionizer', None) is not None:
             self.ionizer = ionization.Ionizer(vol_dict['ionizer'])
-        if vol_dict.get('rotator', None) is not None:
-            raise NotImplementedError('rotational Raman source is not provided by this build')
+        self.rotator = vol_dict.get('rotator', None)
         self.mat = Material(None, self.field_tool, N, vol_dict['wave coordinates'], vol_dict['object'], chi,
                             chi3, vol_dict['density reference'], self.window_speed)
         dens_nodes = grid_tools.cell_centers(-size[3] / 2, size[3] / 2, steps)
@@ -225,6 +227,7 @@ class RunSetup:
         self.mat.UpdateMaterial(zi + 0.5 * L)
         st.set_medium(self.w, self.mat.kz, self.mat.kg, self.mat.ngas, self.nl_filter, self.cond_filter,
                       self.mat.redchi3, self.dt, self.ion)
+        st.set_rotator(self.rotator, self.dt)
         return st
 
 
@@ -245,7 +248,7 @@ def track_with(vol_dict, stepper_factory, record=None):
         zi, zf, _ = rs.plane(k)
         logging.info('Advancing to diagnostic plane %d', k + 1)
         A0, J0, chi0, ne0, ev = _propagate(ctx, rs.field_tool, rs.window_speed, A0, rs.mat, rs.ionizer, rs.NL_band,
-                                           rs.subcycles, zi, zf, rs.dzmin, record)
+                                           rs.subcycles, zi, zf, rs.dzmin, record, rotator=rs.rotator)
         logging.info('%d evaluations of j(w,kx,ky)', ev)
         evals.append(ev)
         A[..., k + 1] = A0

@@ -29,6 +29,8 @@ CASES = {
     # the same with the series (quintic-fit) rate: A
     'radial_o2_series_513x16': (lambda: configs.radial_o2_ionization(scale=4, steps=3, length='1 mm', nr=16,
                                                                      rate='series'), ()),
+    # radial Kerr in N2 with the rotational (Raman) susceptibility of 32 states, Nw = 65, Nr = 16: A, chiNL
+    'radial_raman_65x16': (lambda: configs.radial_raman_air(scale=8, steps=4), ('chi',)),
     # 3-D Cartesian filamentation (super-Gaussian with amplitude noise, Kerr + O2), Nw = 33, 16 x 16, 3 fixed
     # steps over 1 mm: A
     'cartesian_fil_33x16x16': (lambda: configs.cartesian_filamentation(scale=16, steps=3, length='1 mm'), ()),

@@ -53,6 +53,7 @@ int main(void) {
          sizeof(uppe_step_control), sizeof(uppe_info));
   printf("%zu %zu %zu %zu\n", offsetof(uppe_medium, ion), offsetof(uppe_step_control, dz_schedule),
          offsetof(uppe_step_control, record_cap), offsetof(uppe_ionization, c));
+  printf("%zu %zu %zu\n", sizeof(uppe_rotator), offsetof(uppe_rotator, wdt), offsetof(uppe_rotator, hbar));
   return 0;
 }
 '''
@@ -69,6 +70,8 @@ int main(void) {
                      ct.sizeof(_abi.uppe_step_control), ct.sizeof(_abi.uppe_info)]
     assert offs == [_abi.uppe_medium.ion.offset, _abi.uppe_step_control.dz_schedule.offset,
                     _abi.uppe_step_control.record_cap.offset, _abi.uppe_ionization.c.offset]
+    rot = [int(v) for v in lines[2].split()]
+    assert rot == [ct.sizeof(_abi.uppe_rotator), _abi.uppe_rotator.wdt.offset, _abi.uppe_rotator.hbar.offset]
 
 
 def test_missing_library_fails_loudly(monkeypatch, tmp_path):

@@ -55,6 +55,23 @@ def test_radial_kerr(oracle_mod, precision):
     assert got[5] == ref[5]
 
 
+@pytest.mark.parametrize('precision', ['fp64-plain', 'fp64', 'fp32', 'bf16x3'])
+def test_radial_raman(oracle_mod, precision):
+    '''Kerr plus the 32-state N2 rotator (rotations.cl:80-129 inside the source term).'''
+    vol = configs.radial_raman_air(scale=4, steps=6)
+    _say('radial_raman scale=4 steps=6 %s' % precision)
+    ref = oracle_mod.oracle_track(vol)
+    got = _run(vol, precision)
+    kerr = _run(configs.radial_kerr_air(scale=4, steps=6, seed=1003), precision)
+    fp64 = precision.startswith('fp64')
+    tol = FP64_TOL if fp64 else FP32_FIELD_TOL
+    assert rel_l2(got[0][..., 1], ref[0][..., 1]) < tol
+    assert rel_l2(got[2][..., 1], ref[2][..., 1]) < (tol if fp64 else 1e-4)
+    # the rotational term is a visible part of chiNL, not lost in the tolerance
+    assert rel_l2(kerr[2][..., 1], ref[2][..., 1]) > 1e3 * (tol if fp64 else 1e-4)
+    assert got[5] == ref[5]
+
+
 @pytest.mark.parametrize('rate', ['explicit', 'series'])
 @pytest.mark.parametrize('precision', ['fp64-plain', 'fp64', 'fp32', 'bf16x3'])
 def test_radial_ionization_adaptive(oracle_mod, rate, precision):

@@ -299,3 +299,67 @@ class TestOracleABI:
             st.set_field(np.zeros((65, 16, 1), complex))     # Hankel tables missing
         with pytest.raises(_abi.UppeError):
             st.advance(0.0, 1.0, 0, 1, 1.0)                # no field
+
+
+class TestRotationalRaman:
+    '''The oracle's rotational susceptibility (the serial recurrence of rotations.cl:80-129) against its closed
+    form: rho and cum are prefix sums over m = Nt-1-n, so for each state
+
+        dchi[n] = sum_{m'' <= m} 2 T_j S_j n_g dalpha Im(rho_m'' conj(E^(2m'')) W_j dt),
+        rho_m'' = -i dt sum_{m' < m''} F_m' E^(2m'+1),   E = exp(i W_j dt / 2),
+
+    evaluated here with explicit powers and double sums (no recurrence).'''
+
+    @staticmethod
+    def _closed_form(rot, Et, ng, dt):
+        N = Et.shape[0]
+        wdt = rot.Wj * dt
+        out = np.zeros(N)
+        F = 0.5 * (rot.dalpha / rot.hbar) * Et[::-1] ** 2  # indexed by m
+        m = np.arange(N)
+        for j in range(2, rot.states):
+            Sjj = 2.0 * j * (j - 1) / (15 * (2 * j - 1))
+            E = np.exp(0.5j * wdt[j].real - 0.5 * wdt[j].imag)
+            a = F * E ** (2 * m + 1)
+            rho = -1j * dt * np.concatenate(([0], np.cumsum(a)[:-1]))
+            g = 2 * rot.Tj[j] * Sjj * ng * rot.dalpha * np.imag(rho * np.conj(E ** (2 * m)) * wdt[j])
+            out += np.cumsum(g)[::-1]
+        return out
+
+    def test_against_closed_form(self, oracle_mod):
+        from searay_amd import configs
+        rot = configs.n2_rotator(states=12)
+        nw, n1, n2 = 33, 2, 2
+        st = oracle_mod.OracleStepper(_abi.UPPE_CARTESIAN, nw, n1, n2, n1, 0)
+        w = np.linspace(0.0, 2.0, nw)
+        kz = np.repeat(w[:, None, None], n1 * n2, axis=1).reshape(nw, n1, n2).astype(np.complex128)
+        kz[0] = 1.0
+        ng = np.array([[1.0, 0.5], [2.0, 0.25]])
+        dt = np.pi / w[-1]
+        ion = ionization.no_ionization_params()
+        st.set_medium(w, kz, np.zeros(nw), ng, np.ones(nw), np.ones(nw), 0.0, dt, ion)
+        st.set_rotator(rot, dt)
+        rng = np.random.default_rng(11)
+        spec = np.exp(-(w - 1.0) ** 2 / 0.05)[:, None, None] * (1 + 0.1 * rng.standard_normal((nw, n1, n2)))
+        A = 3e-3 * spec.astype(np.complex128)
+        st.set_field(A)
+        a, J, chi_t, ne_t = st.finish(0.0)
+        st.close()
+        E = 1j * kz * a
+        Et = np.fft.irfft(E, axis=0)
+        for x in range(n1):
+            for y in range(n2):
+                ref = self._closed_form(rot, Et[:, x, y], ng[x, y], dt)
+                assert np.abs(ref).max() > 0
+                assert rel_l2(chi_t[:, x, y], ref) < 1e-10
+
+    def test_switched_off(self, oracle_mod):
+        from searay_amd import configs
+        vol = configs.radial_raman_air(scale=16, steps=2)
+        vol_off = configs.radial_raman_air(scale=16, steps=2)
+        vol_off['rotator'] = None
+        a = oracle_mod.oracle_track(vol)
+        b = oracle_mod.oracle_track(vol_off)
+        c = oracle_mod.oracle_track(configs.radial_kerr_air(scale=16, steps=2, seed=1003))
+        assert rel_l2(b[0][..., -1], c[0][..., -1]) == 0.0
+        assert rel_l2(a[0][..., -1], c[0][..., -1]) > 1e-4
