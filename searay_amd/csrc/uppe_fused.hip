@@ -79,7 +79,18 @@ struct FusedState {
   // Cartesian: radix-8 pass tables of the x and y transforms (complex T)
   void *tw_x = nullptr, *tw_y = nullptr;
   int twlen_x = 0, twlen_y = 0;
-  double *rot_tjj = nullptr, *rot_wdt = nullptr;  // rotator tables (device), when the handle has a rotator
+  void* rot_tab = nullptr;  // rotator rows (device, RotRow<T> per state j >= 2), when the handle has a rotator
+};
+
+// per-state constants of the rotator for the source kernel: the coefficient T_j S_j 2 dalpha (times n_g in the
+// kernel), W_j dt, E = exp(i W_j dt / 2), E^2, and the binary powers E^(2 SPT 2^b) from which the thread with
+// index t forms E^(2 SPT t) (rotations.cl:80-129, rotations.py:56-61)
+constexpr int ROT_NB = 10;  // bits of the thread index (at most 1024 threads)
+template <typename T>
+struct RotRow {
+  double wr64, wi64;
+  T c, wr, wi, pad;
+  cplx<T> e[2 + ROT_NB];
 };
 
 constexpr int BM = 64, BN = 64, BK = 16;
@@ -959,25 +970,6 @@ __device__ inline void lds_write_reals(cplx<T>* Y, int n0, const T* src) {
   }
 }
 
-// E^n with E = exp(i w / 2) for the complex W_j dt = w (rotations.cl:107: exp_iwdth = exp(i w_re / 2 - w_im / 2)):
-// the phase n w_re / 2 formed in FP64; FP32 reduces it to turns for the hardware sine and cosine
-template <typename R>
-__device__ inline cplx<R> rot_pow(double wr, double wi, int n);
-template <>
-__device__ inline cplx<float> rot_pow<float>(double wr, double wi, int n) {
-  const double turns = 0.5 * wr * (double)n * 0.15915494309189535;
-  const float f = (float)(turns - rint(turns));
-  const float mag = __builtin_amdgcn_exp2f((float)(-0.5 * wi * (double)n * 1.4426950408889634));
-  return cmk(__builtin_amdgcn_cosf(f) * mag, __builtin_amdgcn_sinf(f) * mag);
-}
-template <>
-__device__ inline cplx<double> rot_pow<double>(double wr, double wi, int n) {
-  double sn, cs;
-  sincos(0.5 * wr * (double)n, &sn, &cs);
-  const double mag = exp(-0.5 * wi * (double)n);
-  return cmk(cs * mag, sn * mag);
-}
-
 // DPP lane move (gfx9 control codes: row_shr:n = 0x110 + n, wave_shr:1 = 0x138, row_bcast:15 = 0x142,
 // row_bcast:31 = 0x143); lanes without a source, or in rows outside RM, read 0
 template <int CTRL, int RM, typename T>
@@ -1006,21 +998,66 @@ __device__ inline T wave_excl_prefix(T v) {
   return v;
 }
 
+// sum of v over lanes 0 .. k-1 (k wave-uniform, at most 16; v held by lanes 0 .. 15): an inclusive scan within the
+// first row of 16 lanes, read at lane k-1
+template <typename T>
+__device__ inline T lanes_below(T v, int k) {
+  v += dpp_move<0x111, 0xf>(v);
+  v += dpp_move<0x112, 0xf>(v);
+  v += dpp_move<0x114, 0xf>(v);
+  v += dpp_move<0x118, 0xf>(v);
+  if (k == 0) return T(0);
+  if constexpr (sizeof(T) == 4) {
+    return __int_as_float(__builtin_amdgcn_readlane(__float_as_int(v), k - 1));
+  } else {
+    const long long b = __double_as_longlong(v);
+    const int lo = __builtin_amdgcn_readlane((int)b, k - 1), hi = __builtin_amdgcn_readlane((int)(b >> 32), k - 1);
+    return __longlong_as_double(((long long)hi << 32) | (unsigned int)lo);
+  }
+}
+
+// E^(2 m0) of a rotator state for the thread with index tid, m0 = SPT tid. FP32: the phase m0 W dt reduced to
+// turns in FP64, then the hardware sine, cosine and exp2; FP64: the product of the staged powers E^(2 SPT 2^b) over
+// the set bits of tid
+template <typename R>
+__device__ inline cplx<R> rot_start(const RotRow<R>& rw, int tid, int m0);
+template <>
+__device__ inline cplx<float> rot_start<float>(const RotRow<float>& rw, int tid, int m0) {
+  (void)tid;
+  const double turns = rw.wr64 * (double)m0 * 0.15915494309189535;
+  const float f = (float)(turns - rint(turns));
+  const float mag = __builtin_amdgcn_exp2f((float)(-rw.wi64 * (double)m0 * 1.4426950408889634));
+  return cmk(__builtin_amdgcn_cosf(f) * mag, __builtin_amdgcn_sinf(f) * mag);
+}
+template <>
+__device__ inline cplx<double> rot_start<double>(const RotRow<double>& rw, int tid, int m0) {
+  (void)m0;
+  cplx<double> pw = cmk(1.0, 0.0);
+#pragma unroll
+  for (int b = 0; b < ROT_NB; b++) {
+    if ((tid >> b) & 1) pw = cmul(pw, rw.e[2 + b]);
+  }
+  return pw;
+}
+
+// rotator states whose constants are staged in LDS at a time
+constexpr int ROT_CHUNK = 64;
+// rotator states per barrier in the source kernel
+constexpr int ROT_G32 = 8, ROT_G64 = 2;
+
 // one workgroup per strip, Nt/4 threads (at least min(Nt, 64)); thread t owns SPT = Nt / threads samples
-template <typename T, int SPT, bool DIAG>
+template <typename T, int SPT, bool DIAG, bool ROT>
 __global__ __launch_bounds__(1024) void f_source(const cplx<T>* __restrict__ er, cplx<T>* __restrict__ jr,
                                                  T* __restrict__ chi_out, T* __restrict__ ne_out,
                                                  const cplx<T>* __restrict__ tw, const T* __restrict__ w,
                                                  const T* __restrict__ nlf, const double* __restrict__ ng, int nt,
                                                  int lgn, int nwp, double redchi3, double dt, uppe_ionization ion,
-                                                 double log_pre, const double* __restrict__ rot_tjj,
-                                                 const double* __restrict__ rot_wdt, int rot_states,
-                                                 double rot_dalpha, double rot_hbar) {
+                                                 double log_pre, const RotRow<T>* __restrict__ rot_tab,
+                                                 int rot_states, double rot_fc) {
   // no early exit for a finished run: the kernel only writes scratch (J_r, diagnostics)
   extern __shared__ char smem[];
   cplx<T>* Y = reinterpret_cast<cplx<T>*>(smem);
   __shared__ T wsum[16];
-  __shared__ cplx<T> rot_wsc[16];
   __shared__ T rot_wsr[16];
   const int N = nt, M = nt / 2, lgm = lgn - 1;
   const long p = blockIdx.x;
@@ -1145,50 +1182,87 @@ __global__ __launch_bounds__(1024) void f_source(const cplx<T>* __restrict__ er,
   // 2b. rotational (Raman) susceptibility (rotations.cl:80-129, AddChiUPPE). For each state j >= 2 the
   // reference runs the recurrence rho <- rho - i dt F_m E^(2m+1) and cum <- cum + 2 Im(T_j S_j rho conj(E^(2m))
   // W_j dt) n_g dalpha, adding cum to chi, over m = N-1-n (time from the last sample), E = exp(i W_j dt / 2),
-  // F_m = dalpha/(2 hbar) E(t)^2. Here both running sums are prefix sums over m: the thread order already
-  // follows m, so each is a DPP wave scan plus the sum over the earlier waves.
+  // F_m = dalpha/(2 hbar) E(t)^2. Both running sums are prefix sums over m, and the thread order already follows
+  // m: rho_m is a DPP wave scan plus the totals of the earlier waves, one barrier per state. The sum over states
+  // commutes with the outer prefix sum, so the per-state terms g_j(m) are summed first and scanned once.
   R dchi[SPT];
 #pragma unroll
   for (int j = 0; j < SPT; j++) dchi[j] = R(0);
-  if (rot_states > 2) {
-    const R Fc = (R)(0.5 * rot_dalpha / rot_hbar), dtr = (R)dt;
-    const int m0 = N - a0 - SPT;  // m of this thread's last sample; m grows as the sample index falls
-    for (int sj = 2; sj < rot_states; sj++) {
-      const double wr = rot_wdt[2 * sj], wi = rot_wdt[2 * sj + 1];
-      const R cj = (R)(rot_tjj[sj] * (2.0 * sj * (sj - 1) / (15.0 * (2 * sj - 1))) * 2.0 * ng[p] * rot_dalpha);
-      const cplx<R> e1 = rot_pow<R>(wr, wi, 1), e2 = rot_pow<R>(wr, wi, 2);
-      cplx<R> pw = rot_pow<R>(wr, wi, 2 * m0), a[SPT], pe[SPT];
-      cplx<R> tot = cmk(R(0), R(0));
+  if (ROT) {
+    constexpr int G = sizeof(R) == 4 ? ROT_G32 : ROT_G64;  // states per barrier
+    const R dtr = (R)dt, fc = (R)rot_fc;
+    const int m0 = N - a0 - SPT;  // = SPT tid: m of this thread's last sample; m grows as the sample index falls
+    const int wv_u = __builtin_amdgcn_readfirstlane(wave);
+    R gs[SPT];
 #pragma unroll
-      for (int jj = SPT - 1; jj >= 0; jj--) {
-        pe[jj] = pw;  // E^(2m)
-        a[jj] = cscale(cmul(pw, e1), Fc * e[jj] * e[jj]);  // F_m E^(2m+1)
-        tot = cadd(tot, a[jj]);
-        pw = cmul(pw, e2);
+    for (int j = 0; j < SPT; j++) gs[j] = R(0);
+    // per-state constants, staged in LDS for ROT_CHUNK states at a time; the wave totals of G states per
+    // barrier, double-buffered
+    __shared__ RotRow<R> rot_r[ROT_CHUNK];
+    __shared__ cplx<R> rot_ws[2][G][16];
+    int par = 0;
+    for (int s0 = 2; s0 < rot_states; s0 += ROT_CHUNK) {
+      const int nc = min(ROT_CHUNK, rot_states - s0);
+      __syncthreads();  // the previous chunk's constants are no longer read
+      {
+        const unsigned* src = reinterpret_cast<const unsigned*>(rot_tab + (s0 - 2));
+        unsigned* dst = reinterpret_cast<unsigned*>(rot_r);
+        const int words = nc * (int)(sizeof(RotRow<R>) / sizeof(unsigned));
+        for (int i = tid; i < words; i += nth) dst[i] = src[i];
       }
-      cplx<R> run = cmk(wave_excl_prefix(tot.re), wave_excl_prefix(tot.im));
-      if (lane == 63) rot_wsc[wave] = cadd(run, tot);
       __syncthreads();
-      for (int w2 = 0; w2 < wave; w2++) run = cadd(run, rot_wsc[w2]);
-      R g[SPT], gtot = R(0);
+      for (int q0 = 0; q0 < nc; q0 += G) {
+        // pass 1: this thread's sum of F_m E^(2m+1), its exclusive scan over the wave, the wave totals
+        cplx<R> pw0[G], run[G];
 #pragma unroll
-      for (int jj = SPT - 1; jj >= 0; jj--) {
-        // rho_m = -i dt (sum of a over earlier m); Im(rho conj(E^(2m)) W dt) with W dt = (wr, wi)
-        const cplx<R> rho = cmk(dtr * run.im, -dtr * run.re);
-        const cplx<R> q = cmul(rho, cconj(pe[jj]));
-        g[jj] = cj * (q.re * (R)wi + q.im * (R)wr);
-        gtot += g[jj];
-        run = cadd(run, a[jj]);
-      }
-      R cum = wave_excl_prefix(gtot);
-      if (lane == 63) rot_wsr[wave] = cum + gtot;
-      __syncthreads();
-      for (int w2 = 0; w2 < wave; w2++) cum += rot_wsr[w2];
+        for (int g = 0; g < G; g++) {
+          const RotRow<R>& rw = rot_r[min(q0 + g, nc - 1)];  // past the chunk end: a repeat, not used
+          const cplx<R> e1 = rw.e[0], e2 = rw.e[1];
+          cplx<R> pw = rot_start(rw, tid, m0), tot = cmk(R(0), R(0));
+          pw0[g] = pw;
 #pragma unroll
-      for (int jj = SPT - 1; jj >= 0; jj--) {
-        cum += g[jj];
-        dchi[jj] += cum;
+          for (int jj = SPT - 1; jj >= 0; jj--) {
+            tot = cadd(tot, cscale(cmul(pw, e1), fc * e[jj] * e[jj]));
+            pw = cmul(pw, e2);
+          }
+          run[g] = cmk(wave_excl_prefix(tot.re), wave_excl_prefix(tot.im));
+          if (lane == 63) rot_ws[par][g][wave] = cadd(run[g], tot);
+        }
+        __syncthreads();
+        // pass 2: rho_m = -i dt (sum of F E^(2m+1) over earlier m); g_j(m) = c_j Im(rho conj(E^(2m)) W dt)
+#pragma unroll
+        for (int g = 0; g < G; g++) {
+          if (q0 + g < nc) {
+            const RotRow<R>& rw = rot_r[q0 + g];
+            const R cj = rw.c * ngp, wr = rw.wr, wi = rw.wi;
+            const cplx<R> e1 = rw.e[0], e2 = rw.e[1];
+            const cplx<R> wt = lane < nwv ? rot_ws[par][g][lane] : cmk(R(0), R(0));
+            cplx<R> r = cadd(run[g], cmk(lanes_below(wt.re, wv_u), lanes_below(wt.im, wv_u)));
+            cplx<R> pw = pw0[g];
+#pragma unroll
+            for (int jj = SPT - 1; jj >= 0; jj--) {
+              const cplx<R> rho = cmk(dtr * r.im, -dtr * r.re);
+              const cplx<R> qv = cmul(rho, cconj(pw));
+              gs[jj] += cj * (qv.re * wi + qv.im * wr);
+              r = cadd(r, cscale(cmul(pw, e1), fc * e[jj] * e[jj]));
+              pw = cmul(pw, e2);
+            }
+          }
+        }
+        par ^= 1;
       }
+    }
+    R gtot = R(0);
+#pragma unroll
+    for (int jj = 0; jj < SPT; jj++) gtot += gs[jj];
+    R cum = wave_excl_prefix(gtot);
+    if (lane == 63) rot_wsr[wave] = cum + gtot;
+    __syncthreads();
+    for (int w2 = 0; w2 < wave; w2++) cum += rot_wsr[w2];
+#pragma unroll
+    for (int jj = SPT - 1; jj >= 0; jj--) {
+      cum += gs[jj];
+      dchi[jj] = cum;
     }
   }
 
@@ -1648,14 +1722,16 @@ struct Fused {
                            : sizeof(T) == 4                  ? std::log2(s->ion.rate_conv * s->ion.c_pre)
                                                              : std::log(s->ion.rate_conv * s->ion.c_pre);
     (void)st;
+    const bool rot = h->rot_states > 2;
     const dim3 grid((unsigned)h->P), block(nth);
 #define UPPE_SRC(S)                                                                                              \
   if (spt == S) {                                                                                                \
-    auto kern = diag ? f_source<T, S, true> : f_source<T, S, false>;                                            \
+    auto kern = rot ? (diag ? f_source<T, S, true, true> : f_source<T, S, false, true>)                         \
+                    : (diag ? f_source<T, S, true, false> : f_source<T, S, false, false>);                      \
     launchf(h, UPPE_KC_SOURCE, kern, grid, block, shm, er, jr, diag ? (T*)s->chi_t : (T*)nullptr,                 \
             diag ? (T*)s->ne_t : (T*)nullptr, (const C*)s->tw, (const T*)s->w, (const T*)s->nlf,                  \
-            (const double*)s->ng, N, lgn, s->nwp, s->redchi3, s->dt, s->ion, log_pre, (const double*)s->rot_tjj,  \
-            (const double*)s->rot_wdt, h->rot_states, h->rot_dalpha, h->rot_hbar);                               \
+            (const double*)s->ng, N, lgn, s->nwp, s->redchi3, s->dt, s->ion, log_pre,                             \
+            (const RotRow<T>*)s->rot_tab, h->rot_states, 0.5 * h->rot_dalpha / h->rot_hbar);                    \
     return;                                                                                                      \
   }
     UPPE_SRC(1) UPPE_SRC(2) UPPE_SRC(4)
@@ -1946,7 +2022,7 @@ void fused_destroy(uppe_handle* h) {
   if (!s) return;
   void* ptrs[] = {s->phs, s->kzc, s->w, s->nlf, s->cond, s->ng, s->Hb, s->Hf, s->tw, s->q, s->k1, s->k2, s->k3,
                   s->er, s->jr, s->chi_t, s->ne_t, s->stage, s->qmax, s->st, s->sched, s->rec, s->frb_hi,
-                  s->frb_lo, s->frf_hi, s->frf_lo, s->tw_x, s->tw_y, s->rot_tjj, s->rot_wdt};
+                  s->frb_lo, s->frf_hi, s->frf_lo, s->tw_x, s->tw_y, s->rot_tab};
   for (void* p : ptrs)
     if (p) (void)hipFree(p);
   if (s->st_host) (void)hipHostFree(s->st_host);
@@ -1954,16 +2030,44 @@ void fused_destroy(uppe_handle* h) {
   h->fused = nullptr;
 }
 
+template <typename T>
+static void rot_rows(uppe_handle* h, FusedState* s) {
+  // the source kernel's launch shape: nth threads, SPT samples each
+  const int N = h->nt, nth = std::max(std::min(N, 64), N / 4), spt = N / nth;
+  const int n = h->rot_states - 2;
+  std::vector<RotRow<T>> rows((size_t)n);
+  for (int i = 0; i < n; i++) {
+    const int sj = i + 2;
+    const double wr = h->rot_wdt[2 * sj], wi = h->rot_wdt[2 * sj + 1];
+    // E^k = exp(i k W dt / 2) for the complex W dt = wr + i wi (rotations.cl:107)
+    auto epow = [&](double k) {
+      const double mag = std::exp(-0.5 * wi * k);
+      return cplx<T>{(T)(mag * std::cos(0.5 * wr * k)), (T)(mag * std::sin(0.5 * wr * k))};
+    };
+    RotRow<T>& r = rows[(size_t)i];
+    r.wr64 = wr;
+    r.wi64 = wi;
+    r.c = (T)(h->rot_tjj[sj] * (2.0 * sj * (sj - 1) / (15.0 * (2 * sj - 1))) * 2.0 * h->rot_dalpha);
+    r.wr = (T)wr;
+    r.wi = (T)wi;
+    r.pad = T(0);
+    r.e[0] = epow(1.0);
+    r.e[1] = epow(2.0);
+    for (int b = 0; b < ROT_NB; b++) r.e[2 + b] = epow(2.0 * spt * (double)(1 << b));
+  }
+  UPPE_HIP(hipMalloc(&s->rot_tab, sizeof(RotRow<T>) * (size_t)n));
+  UPPE_HIP(hipMemcpy(s->rot_tab, rows.data(), sizeof(RotRow<T>) * (size_t)n, hipMemcpyHostToDevice));
+}
+
 void fused_set_rotator(uppe_handle* h) {
   FusedState* s = h->fused;
-  if (s->rot_tjj) UPPE_HIP(hipFree(s->rot_tjj));
-  if (s->rot_wdt) UPPE_HIP(hipFree(s->rot_wdt));
-  s->rot_tjj = s->rot_wdt = nullptr;
+  if (s->rot_tab) UPPE_HIP(hipFree(s->rot_tab));
+  s->rot_tab = nullptr;
   if (h->rot_states < 3) return;
-  UPPE_HIP(hipMalloc(&s->rot_tjj, sizeof(double) * h->rot_states));
-  UPPE_HIP(hipMalloc(&s->rot_wdt, sizeof(double) * 2 * h->rot_states));
-  UPPE_HIP(hipMemcpy(s->rot_tjj, h->rot_tjj.data(), sizeof(double) * h->rot_states, hipMemcpyHostToDevice));
-  UPPE_HIP(hipMemcpy(s->rot_wdt, h->rot_wdt.data(), sizeof(double) * 2 * h->rot_states, hipMemcpyHostToDevice));
+  if (is_fp64(h))
+    rot_rows<double>(h, s);
+  else
+    rot_rows<float>(h, s);
 }
 
 void fused_set_hankel(uppe_handle* h) {
