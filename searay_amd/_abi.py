@@ -135,14 +135,18 @@ class Stepper:
         self._check(self._f['set_medium'](self.h, ct.byref(m)), 'set_medium')
 
     def set_rotator(self, rotator, dt):
-        '''Load a rotations.Rotator's tables for source time step dt, or switch the rotator off (None).'''
+        '''Load a rotator's tables for source time step dt, or switch the rotator off (None). Any object with the
+        attributes of the reference's Rotator works (rotations.py:27-35): Wj, Tj, dalpha, hbar; the device gets
+        Tj and W_j dt as AddChi uploads them (rotations.py:56-61).'''
         if rotator is None:
             self._check(self._f['set_rotator'](self.h, None), 'set_rotator')
             return
-        states, tjj, wdt = rotator.kernel_tables(dt)
-        tjj = _f64(tjj)
-        wdt = _c128(wdt).view(np.float64)
-        r = uppe_rotator(int(states), 0, _ptr(tjj), _ptr(wdt), float(rotator.dalpha), float(rotator.hbar))
+        wj = np.asarray(rotator.Wj)
+        tjj = _f64(rotator.Tj)
+        wdt = _c128(wj * dt).view(np.float64)
+        if tjj.shape[0] != wj.shape[0]:
+            raise ValueError('rotator: Tj and Wj differ in length')
+        r = uppe_rotator(int(wj.shape[0]), 0, _ptr(tjj), _ptr(wdt), float(rotator.dalpha), float(rotator.hbar))
         self._check(self._f['set_rotator'](self.h, ct.byref(r)), 'set_rotator')
 
     def set_field(self, a_r):

@@ -37,7 +37,3 @@ class Rotator:
     @property
     def states(self):
         return self.Wj.shape[0]
-
-    def kernel_tables(self, dt):
-        '''(states, Tjj, W_j dt) as the strip kernel takes them (rotations.py:56-61).'''
-        return self.states, np.ascontiguousarray(self.Tj), np.ascontiguousarray(self.Wj * dt)

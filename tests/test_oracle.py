@@ -353,6 +353,20 @@ class TestRotationalRaman:
                 assert np.abs(ref).max() > 0
                 assert rel_l2(chi_t[:, x, y], ref) < 1e-10
 
+    def test_reference_shaped_rotator(self, oracle_mod):
+        '''Any object with the reference Rotator's attributes (rotations.py:27-35) is accepted.'''
+        import types
+        from searay_amd import configs
+        rot = configs.n2_rotator(states=16)
+        bare = types.SimpleNamespace(Wj=rot.Wj.copy(), Tj=rot.Tj.copy(), dalpha=rot.dalpha, hbar=rot.hbar)
+        vol = configs.radial_raman_air(scale=16, steps=2)
+        vol['rotator'] = rot
+        vol_bare = configs.radial_raman_air(scale=16, steps=2)
+        vol_bare['rotator'] = bare
+        a = oracle_mod.oracle_track(vol)
+        b = oracle_mod.oracle_track(vol_bare)
+        assert rel_l2(b[2][..., -1], a[2][..., -1]) == 0.0
+
     def test_switched_off(self, oracle_mod):
         from searay_amd import configs
         vol = configs.radial_raman_air(scale=16, steps=2)
@@ -363,3 +377,73 @@ class TestRotationalRaman:
         c = oracle_mod.oracle_track(configs.radial_kerr_air(scale=16, steps=2, seed=1003))
         assert rel_l2(b[0][..., -1], c[0][..., -1]) == 0.0
         assert rel_l2(a[0][..., -1], c[0][..., -1]) > 1e-4
+
+
+class TestRotationalRamanContinuum:
+    '''The oracle's rotational susceptibility against the continuous equation its recurrence discretises
+    (rotations.cl:80-129), evaluated by quadrature. With m dt = t (m = Nt-1-n: the strip runs backwards in time),
+    rho_j(t) = -i int_0^t F(t') exp(i W_j t') dt' and dchi_j/dt = 2 T_j S_j n_g dalpha Im(W_j rho_j conj(exp(i W_j t))),
+    W_j = w_j - w_{j-2} + i damp_j. Undamped this is the impulsive Raman response
+    chi(t) = -2 n_g dalpha sum_j T_j S_j int_0^t F(t') sin(w_j (t - t')) dt'; the recurrence must converge to it at
+    second order in dt. Damped, the reference's factor is exp(-damp (t + t')) (both exponentials carry the decay),
+    not the exp(-damp (t - t')) of a decaying coherence: the oracle must converge to the former and stay clear of
+    the latter.'''
+
+    T, T0, TAU, B, DALPHA, KT, HBAR, NS, NG = 40.0, 8.0, 1.5, 0.004, 0.7, 0.2, 1.0, 10, 1.3
+
+    def _field(self, t):
+        return np.exp(-((t - self.T0) / self.TAU) ** 2)
+
+    def _rotator(self, damp):
+        from searay_amd import rotations
+        j = np.arange(self.NS)
+        return rotations.Rotator(self.B, self.DALPHA, self.KT, 6 - 3 * (j % 2), damp * np.ones(self.NS), self.HBAR)
+
+    def _oracle_chi(self, oracle_mod, nw, rot):
+        n = 2 * (nw - 1)
+        dt = self.T / n
+        w = np.arange(nw) * (2 * np.pi / self.T)
+        st = oracle_mod.OracleStepper(_abi.UPPE_CARTESIAN, nw, 1, 1, 1, 0)
+        kz = np.ones((nw, 1, 1), np.complex128)  # E = i kz A = i A
+        st.set_medium(w, kz, np.zeros(nw), np.array([[self.NG]]), np.ones(nw), np.ones(nw), 0.0, dt,
+                      ionization.no_ionization_params())
+        st.set_rotator(rot, dt)
+        t = np.arange(n) * dt
+        st.set_field((np.fft.rfft(self._field(t)[::-1]) / 1j)[:, None, None])
+        _, _, chi_t, _ = st.finish(0.0)
+        st.close()
+        return t, chi_t[::-1, 0, 0]
+
+    def _continuum(self, rot, t, decay_sum):
+        from scipy import integrate
+        out = 0.0
+        for j in range(2, rot.states):
+            s = 2.0 * j * (j - 1) / (15 * (2 * j - 1))
+            om, g = rot.Wj[j].real, rot.Wj[j].imag
+
+            def f(tp):
+                # int_tp^t exp(-g s') (om cos(om (s'-tp)) + g sin(om (s'-tp))) ds', times the decay of rho
+                z = (np.exp((-g + 1j * om) * (t - tp)) - 1) / (-g + 1j * om)
+                decay = np.exp(-2 * g * tp) if decay_sum else 1.0
+                return 0.5 * self.DALPHA / self.HBAR * self._field(tp) ** 2 * decay * (om * z.real + g * z.imag)
+            out += -2 * rot.Tj[j] * s * self.NG * self.DALPHA * integrate.quad(f, 0, t, limit=200)[0]
+        return out
+
+    def _error(self, oracle_mod, nw, rot, decay_sum=True):
+        t, chi = self._oracle_chi(oracle_mod, nw, rot)
+        k = max(1, (nw - 1) // 16)
+        ref = np.array([self._continuum(rot, x, decay_sum) for x in t[::k]])
+        return rel_l2(chi[::k], ref)
+
+    def test_undamped_second_order(self, oracle_mod):
+        rot = self._rotator(0.0)
+        assert 0.1 < rot.Wj.real.max() * self.T / 256 < 0.2  # W dt of the coarse grid
+        e1, e2 = self._error(oracle_mod, 129, rot), self._error(oracle_mod, 257, rot)
+        assert e1 < 5e-4
+        assert 3.6 < e1 / e2 < 4.4
+
+    def test_damping_form(self, oracle_mod):
+        rot = self._rotator(0.01)
+        e1, e2 = self._error(oracle_mod, 129, rot), self._error(oracle_mod, 257, rot)
+        assert e2 < 1e-3 and 1.8 < e1 / e2 < 2.2
+        assert self._error(oracle_mod, 257, rot, decay_sum=False) > 0.1
