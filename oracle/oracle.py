@@ -78,9 +78,9 @@ def oracle_factory(geometry, nw, n1, n2, nk):
     return OracleStepper(geometry, nw, n1, n2, nk)
 
 
-def oracle_track(vol_dict, record=None):
-    '''The reference's track() on the oracle: (A, J, chiNL, ne, dom4d, evals).'''
-    return uppe_kernel.track_with(vol_dict, oracle_factory, record)
+def oracle_track(vol_dict, record=None, xp=None, eikonal=None, vg=None):
+    '''The reference's track() on the oracle: (A, J, chiNL, ne, dom4d, evals); ray data as track() takes it.'''
+    return uppe_kernel.track_with(vol_dict, oracle_factory, record, xp, eikonal, vg)
 
 
 # -- strip FFT restatements (kernels/fft.cl) exposed for their own checks --------------------------------

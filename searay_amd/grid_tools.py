@@ -1,11 +1,15 @@
-'''Grid generators and transverse mode tools for the UPPE propagator (host side, FP64).
+'''Grid generators, ray-grid interpolation and transverse mode tools for the UPPE propagator (host side, FP64).
 
-Only the parts of SeaRay's ``modules/grid_tools.py`` that the UPPE path needs are provided:
-node/wall generators (grid_tools.py:25-49), the finite-difference Fourier tool's kr^2
+Only the parts of SeaRay's ``modules/grid_tools.py`` that the UPPE path and its ray boundary need are provided:
+node/wall generators (grid_tools.py:25-49), the interpolation between ray data and the wave grid that the ray
+boundary and the ray relaunch use (grid_tools.py:51-177), the finite-difference Fourier tool's kr^2
 (grid_tools.py:218-250) and the discrete Hankel tool's eigenbasis (grid_tools.py:252-298,357-358).
 The device-side transforms themselves live in the HIP library; this module only builds tables.
 '''
+import logging
+
 import numpy as np
+import scipy.interpolate
 import scipy.linalg
 
 
@@ -35,6 +39,99 @@ def cell_walls(node_low, node_high, nodes, default_voxel_width=1000.0):
     else:
         step = (node_high - node_low) / (nodes - 1)
     return np.linspace(node_low - 0.5 * step, node_high + 0.5 * step, nodes + 1)
+
+
+def hypersurface_idx(ary, axis, cell):
+    """Index tuple of the one-cell slab ``cell`` along ``axis``, keeping that axis (grid_tools.py:51-55)."""
+    idx = [slice(None)] * ary.ndim
+    idx[axis] = slice(cell, cell + 1)
+    return tuple(idx)
+
+
+def AddGhostCells(a, nodes, ax):
+    """Prepend one ghost cell along ``ax``: a copy of the first cell, at one node spacing below the first node
+    (grid_tools.py:81-86)."""
+    ghost = a[hypersurface_idx(a, ax, 0)]
+    return np.concatenate((ghost, a), axis=ax), np.concatenate(([2 * nodes[0] - nodes[1]], nodes))
+
+
+def DataFromGrid(w, x, y, wn, xn, yn, the_data):
+    """Values of ``the_data`` (shape (len(wn), len(xn), len(yn))) at scattered points (w, x, y): trilinear on the
+    regular nodes, linear extrapolation outside them (grid_tools.py:88-110). A single y node is widened to the
+    y range of the points, with the data constant in y."""
+    if wn.shape[0] == 1:
+        raise ValueError('Cannot relaunch rays with monochromatic data.')
+    if xn.shape[0] == 1:
+        raise ValueError('Missing spatial dimension x.')
+    if yn.shape[0] == 1:
+        ygrid = np.linspace(np.min(y), np.max(y), 2)
+        data = np.concatenate((the_data, the_data), axis=2)
+    else:
+        ygrid, data = yn, the_data
+    f = scipy.interpolate.RegularGridInterpolator((wn, xn, ygrid), data, bounds_error=False, fill_value=None)
+    return f(np.stack((w, x, y), axis=-1))
+
+
+def _node_array(spec, pts):
+    return np.linspace(np.min(pts), np.max(pts), spec) if type(spec) is int else spec
+
+
+def _bins(walls, v):
+    """Bin of each value between consecutive walls, walls[i] <= v < walls[i+1]; -1 outside."""
+    b = np.searchsorted(walls, v, side='right') - 1
+    return np.where((b >= 0) & (b < walls.shape[0] - 1), b, -1)
+
+
+def _plot_extent(wn, xn, yn):
+    ww = cell_walls(wn[0], wn[-1], wn.shape[0])
+    wx = cell_walls(xn[0], xn[-1], xn.shape[0])
+    wy = cell_walls(yn[0], yn[-1], yn.shape[0])
+    return ww, wx, wy, np.array([ww[0], ww[-1], wx[0], wx[-1], wy[0], wy[-1]])
+
+
+def CylGridFromInterpolation(w, x, y, vals, wn=1, xn=100, yn=100, fill=0.0):
+    """Scattered ray data onto a (w, rho, phi) grid (grid_tools.py:112-146). Frequency and azimuth are binned
+    between cell walls; within each (w, phi) bin the data is interpolated linearly in rho, taking the value of
+    the innermost point below it and zero beyond the outermost. Empty bins are zero. ``wn``, ``xn``, ``yn`` are
+    node arrays, or node counts spanning the points. Returns (data, plot extent)."""
+    wn, xn, yn = _node_array(wn, w), _node_array(xn, x), _node_array(yn, y)
+    ww, wx, wy, ext = _plot_extent(wn, xn, yn)
+    out = np.zeros((wn.shape[0], xn.shape[0], yn.shape[0]))
+    bw, by = _bins(ww, w), _bins(wy, y)
+    ok = np.where((bw >= 0) & (by >= 0))[0]
+    key = bw[ok] * yn.shape[0] + by[ok]
+    order = np.argsort(key, kind='stable')  # keeps the points of one bin in their original order
+    ok, key = ok[order], key[order]
+    starts = np.flatnonzero(np.r_[True, key[1:] != key[:-1]]) if key.shape[0] else np.zeros(0, int)
+    for s0, s1 in zip(starts, np.r_[starts[1:], key.shape[0]]):
+        sel = ok[s0:s1]
+        xs, vs = x[sel], vals[sel]
+        v0 = vs[np.argmin(xs)]
+        f = scipy.interpolate.interp1d(xs, vs, kind='linear', bounds_error=False, fill_value=(v0, 0.0))
+        i, j = divmod(int(key[s0]), yn.shape[0])
+        out[i, :, j] = f(xn)
+    return out, ext
+
+
+def GridFromInterpolation(w, x, y, vals, wn=1, xn=100, yn=100, fill=0.0):
+    """Scattered ray data onto a (w, x, y) grid (grid_tools.py:148-177): frequency binned between cell walls,
+    then a cubic (Clough-Tocher) interpolation in (x, y) per bin at the nodes pulled in by 0.9999, ``fill``
+    outside the points' hull. A frequency bin without points ends the run, as in the reference. Returns
+    (data, plot extent)."""
+    wn, xn, yn = _node_array(wn, w), _node_array(xn, x), _node_array(yn, y)
+    ww, wx, wy, ext = _plot_extent(wn, xn, yn)
+    out = np.zeros((wn.shape[0], xn.shape[0], yn.shape[0]))
+    xg = 0.9999 * np.outer(xn, np.ones(yn.shape[0]))
+    yg = 0.9999 * np.outer(np.ones(xn.shape[0]), yn)
+    bw = _bins(ww, w)
+    for i in range(wn.shape[0]):
+        sel = np.where(bw == i)[0]
+        if sel.shape[0] == 0:
+            logging.critical('no rays found in w-bin, w-dimension should be matched for all objects')
+            raise SystemExit(1)
+        pts = np.stack((x[sel], y[sel]), axis=-1)
+        out[i] = scipy.interpolate.griddata(pts, vals[sel], (xg, yg), method='cubic', fill_value=fill)
+    return out, ext
 
 
 class TransverseModeTool:

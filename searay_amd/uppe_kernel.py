@@ -12,10 +12,10 @@ the C ABI of include/searay_uppe.h. ``cl`` is accepted for compatibility and ign
 dictionary keys select the device path: 'uppe precision' in {'fp64', 'fp32', 'bf16x3', 'fp64-plain'}
 (default 'fp64') and 'uppe device' (HIP ordinal, default 0).
 
-Differences from the reference, by design: the ray boundary (``GetBoundaryFields``) and ray relaunch
-(``RelaunchRays``) are not provided, so 'incoming wave' is required and xp/eikonal/vg are returned unchanged;
-the rotational Raman ``rotator`` (a rotations.Rotator) is loaded into the stepper and added to chi in every
-source evaluation.
+The ray boundary follows the reference: without 'incoming wave' the initial field comes from the primary rays
+(``field_tool.GetBoundaryFields(xp[:, 0, :], eikonal, 1)``), and after the last plane the rays are relaunched
+from the exit field (``RelaunchRays``, or ``RelaunchRays1`` with 'full relaunch'), updating xp, eikonal and vg in
+place. The rotational Raman ``rotator`` is loaded into the stepper and added to chi in every source evaluation.
 '''
 import logging
 
@@ -166,7 +166,7 @@ def propagator(cl, ctool, vg, a, mat, ionizer, rotator, NL_band, subcycles, zi, 
 class RunSetup:
     '''Host-side set-up of one track() call (uppe_kernel.py:341-404): grids, initial field, medium, walls.'''
 
-    def __init__(self, vol_dict):
+    def __init__(self, vol_dict, xp=None, eikonal=None):
         band = vol_dict['frequency band']
         self.NL_band = vol_dict['nonlinear band']
         size = (band[1] - band[0],) + tuple(vol_dict['size'])
@@ -182,6 +182,7 @@ class RunSetup:
             raise ValueError('UPPE propagator requires 2**n x-nodes')
         if N[2] not in powersof2:
             raise ValueError('UPPE propagator requires 2**n y-nodes')
+        self.size = size
         self.window_speed = vol_dict.get('window speed', 1.0)
         chi3 = vol_dict.get('chi3', 0.0)
         self.cartesian = vol_dict['wave coordinates'] == 'cartesian'
@@ -194,8 +195,14 @@ class RunSetup:
         w_nodes, x1_nodes, x2_nodes, dom3d = self.field_tool.GetGridInfo()
         self.w = w_nodes
         if 'incoming wave' not in vol_dict:
-            raise NotImplementedError("the ray boundary is not provided by this build: give 'incoming wave'")
-        self.A0 = np.asarray(vol_dict['incoming wave'](w_nodes, x1_nodes, x2_nodes), dtype=np.complex128)
+            if xp is None or eikonal is None:
+                raise ValueError("no 'incoming wave' and no ray data for the UPPE boundary")
+            logging.info('Start UPPE propagation using ray data.')
+            logging.warning('Polarization information is lost upon entering UPPE region.')
+            self.A0 = np.asarray(self.field_tool.GetBoundaryFields(xp[:, 0, :], eikonal, 1), dtype=np.complex128)
+        else:
+            logging.info('Start UPPE propagation using wave data')
+            self.A0 = np.asarray(vol_dict['incoming wave'](w_nodes, x1_nodes, x2_nodes), dtype=np.complex128)
         chi = np.asarray(vol_dict['dispersion inside'].chi(w_nodes)).astype(np.cdouble)
         self.ionizer = None
         if vol_dict.get('ionizer', None) is not None:
@@ -231,9 +238,10 @@ class RunSetup:
         return st
 
 
-def track_with(vol_dict, stepper_factory, record=None):
-    '''The body of ``track`` with an explicit stepper factory. Returns (A, J, chiNL, ne, dom4d, evals).'''
-    rs = RunSetup(vol_dict)
+def track_with(vol_dict, stepper_factory, record=None, xp=None, eikonal=None, vg=None):
+    '''The body of ``track`` with an explicit stepper factory. Returns (A, J, chiNL, ne, dom4d, evals). With ray
+    data, the rays are relaunched from the exit field as ``track`` does (uppe_kernel.py:419-423).'''
+    rs = RunSetup(vol_dict, xp, eikonal)
     N = rs.N
     A = np.zeros(N, dtype=np.complex128)
     J = np.zeros(N, dtype=np.complex128)
@@ -256,14 +264,18 @@ def track_with(vol_dict, stepper_factory, record=None):
         chiNL[..., k + 1] = chi0
         ne[..., k + 1] = ne0
     ctx.stepper.close()
+    if xp is not None:
+        relaunch = rs.field_tool.RelaunchRays1 if vol_dict.get('full relaunch', False) else rs.field_tool.RelaunchRays
+        relaunch(xp, eikonal, vg, A[..., -1], rs.size[3], vol_dict['dispersion inside'])
     return A, J, chiNL, ne, rs.dom4d, evals
 
 
 def track(cl, xp, eikonal, vg, vol_dict):
     '''Propagate with the UPPE stepper on the GPU (uppe_kernel.py:333-424).
 
-    ``cl``, ``xp``, ``eikonal`` and ``vg`` are accepted for signature compatibility; the incoming field comes
-    from vol_dict['incoming wave'].'''
+    ``cl`` is accepted for signature compatibility. ``xp`` (bundles, rays, 8), ``eikonal`` (bundles, 4) and ``vg``
+    (bundles, rays, 4) are the rays on the entrance plane: they give the initial field when vol_dict has no
+    'incoming wave', and are relaunched from the exit field in place. They may be None with 'incoming wave'.'''
     factory = hip_stepper_factory(vol_dict.get(PRECISION_KEY, 'fp64'), vol_dict.get(DEVICE_KEY, 0))
-    A, J, chiNL, ne, dom4d, _ = track_with(vol_dict, factory)
+    A, J, chiNL, ne, dom4d, _ = track_with(vol_dict, factory, None, xp, eikonal, vg)
     return A, J, chiNL, ne, dom4d

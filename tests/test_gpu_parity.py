@@ -164,3 +164,26 @@ def test_volume_propagate_and_report(tmp_path):
     for suffix, ref in zip(volume.UPPE_OUTPUTS, ret):
         assert np.array_equal(np.load(base + '_' + box.name + '_' + suffix + '.npy'), ref)
     assert np.all(np.isfinite(ret[0]))
+
+
+@pytest.mark.parametrize('precision', ['fp64', 'fp32'])
+def test_ray_boundary_track(oracle_mod, precision):
+    '''track() driven by rays on the entrance plane (no 'incoming wave'), through the HIP stepper: the same run as
+    the oracle's from the wave itself, and the relaunched rays carry the exit field (caustic_tools.py:237-254,
+    309-338).'''
+    from test_ray_coupling import radial_rays
+    vol = configs.radial_kerr_air(scale=8, steps=3)
+    _say('ray boundary radial_kerr scale=8 steps=3 %s' % precision)
+    ref = oracle_mod.oracle_track(vol)
+    xp, eik, vg = radial_rays(vol)
+    xp_o, eik_o, vg_o = xp.copy(), eik.copy(), vg.copy()
+    vol_rays = dict(vol)
+    del vol_rays['incoming wave']
+    vol_rays['uppe precision'] = precision
+    got = uppe_kernel.track(None, xp, eik, vg, vol_rays)
+    oracle_mod.oracle_track(dict(vol_rays), None, xp_o, eik_o, vg_o)
+    tol = FP64_TOL if precision == 'fp64' else FP32_FIELD_TOL
+    assert rel_l2(got[0][..., 1:], ref[0][..., 1:]) < tol
+    live = xp[:, 0, 4] > 0
+    assert rel_l2(eik[live, 1], eik_o[live, 1]) < tol
+    assert np.array_equal(xp[live], xp_o[live])
