@@ -1,15 +1,17 @@
 /*
  * searay_uppe.h -- C ABI of the MI355X UPPE stepper (libsearay_uppe.so).
  *
- * This is the native boundary under the Python drop-in for SeaRay's `modules/uppe_kernel.py`.
- * The Python `track`/`propagator` (searay_amd/uppe_kernel.py) keep the reference signatures; underneath,
- * they call these entry points in place of the pyopencl calls listed against each function.
+ * This is the native boundary under the Python drop-ins for SeaRay's `modules/uppe_kernel.py` and
+ * `modules/paraxial_kernel.py`. The Python `track`/`propagator` (searay_amd/uppe_kernel.py,
+ * searay_amd/paraxial_kernel.py) keep the reference signatures; underneath, they call these entry points in
+ * place of the pyopencl calls listed against each function. One handle steps either equation (uppe_desc.model).
  *
  * Conventions
  *  - Every function returns an int status: 0 = OK, otherwise an UPPE_ERR_* code; uppe_last_error(h) has
  *    the text (a static string for a NULL handle).
  *  - Host arrays use the reference layout: spectral arrays are complex128 interleaved (re, im) with shape
- *    (Nw, N1, N2) in C order, time-domain arrays are float64 (Nt, N1, N2), Nt = 2 (Nw - 1).
+ *    (Nw, N1, N2) in C order, time-domain arrays are float64 (Nt, N1, N2), Nt = 2 (Nw - 1) for UPPE (real
+ *    fields) and Nt = Nw for the paraxial envelope (frequencies in the reference's ifftshift order).
  *    Host pointers are owned by the caller and only read or written during the call.
  *  - The handle owns all device memory. A handle is not thread-safe; separate handles are independent.
  *  - No C++ types or exceptions cross this boundary.
@@ -40,8 +42,13 @@ enum {
 
 enum { UPPE_ION_NONE = 0, UPPE_ION_EXPLICIT = 1, UPPE_ION_SERIES = 2 };
 
-/* Grid description. Replaces the shapes pyopencl arrays carry (uppe_kernel.py:95-119) and the transverse
- * tool choice (uppe_kernel.py:374-377). nw = 2^n + 1; n1, n2 powers of 2; nk <= n1 radial modes (radial only). */
+/* The equation a handle steps: the UPPE of real fields (uppe_kernel.py), or the paraxial envelope equation
+ * (paraxial_kernel.py:29-331) with the same transverse transforms, RK4 and step-size control. */
+enum { UPPE_MODEL_UPPE = 0, UPPE_MODEL_PARAXIAL = 1 };
+
+/* Grid description. Replaces the shapes pyopencl arrays carry (uppe_kernel.py:95-119, paraxial_kernel.py:92-117)
+ * and the transverse tool choice (uppe_kernel.py:374-377). nw = 2^n + 1 (UPPE) or 2^n (paraxial); n1, n2
+ * powers of 2; nk <= n1 radial modes (radial only). */
 typedef struct uppe_desc {
   int geometry;   /* UPPE_RADIAL or UPPE_CARTESIAN */
   int nw;         /* frequency nodes */
@@ -50,7 +57,7 @@ typedef struct uppe_desc {
   int nk;         /* radial modes kept; ignored for Cartesian */
   int precision;  /* UPPE_PREC_* */
   int device;     /* HIP device ordinal */
-  int reserved;
+  int model;      /* UPPE_MODEL_* (0, the UPPE, for descriptors that leave it zero) */
 } uppe_desc;
 
 /* Rate-kernel parameters, exactly the scalars Ionizer.RateCL passes (ionization.py:131-150). */
@@ -75,6 +82,19 @@ typedef struct uppe_medium {
   double dt;                  /* pi / max(w) */
   uppe_ionization ion;
 } uppe_medium;
+
+/* Paraxial medium. Replaces the paraxial Material.UpdateMaterial uploads (paraxial_kernel.py:72-102) and the
+ * Source frequency array (:117-135): the envelope is advanced as q = exp(-i kz z) a with the complex
+ * kz = (w^2 (1 + chi0 - ng^2) - w0^2 dn^2 - 2 w w0 ng dn - kperp^2) / (2 k0), k0 = w0 dn + w ng. */
+typedef struct uppe_paraxial_medium {
+  const double* w;            /* [nw] frequencies in the reference's ifftshift order */
+  const double* kz;           /* complex128 [nw][n1][n2] */
+  const double* k0;           /* [nw] */
+  const double* ngas;         /* [n1][n2] gas density in plasma units */
+  double redchi3;             /* chi3 / n_ref */
+  double dt;                  /* 2 pi / (nw dw) */
+  uppe_ionization ion;        /* rates of the envelope modulus (ionization.cl:46-69, 99-125) */
+} uppe_paraxial_medium;
 
 /* One call runs subcycles sub0 .. sub0+n_sub-1 of length `length`. Subcycle s starts at z0 = zi + s*length
  * and its steps use the phase coordinate (z0 - zi) + z, formed in that order as the reference does.
@@ -112,8 +132,13 @@ int uppe_create(const uppe_desc* desc, uppe_handle** out);
 /* Radial basis: H[nk][nr][nm] eigenvectors and Lambda[nr] root volumes (grid_tools.py:294,263,300-301). */
 int uppe_set_hankel(uppe_handle* h, const double* H, const double* lambda_);
 
-/* Medium tables; may be called again between advance calls (uppe_kernel.py:303). */
+/* Medium tables; may be called again between advance calls (uppe_kernel.py:303). UPPE model only. */
 int uppe_set_medium(uppe_handle* h, const uppe_medium* m);
+
+/* Paraxial medium tables; may be called again between advance calls (paraxial_kernel.py:317). Paraxial model
+ * only. The paraxial stepper has no field conditioner and no nonlinear band filter, and its caller passes
+ * dzmin = 0 (paraxial_kernel.py:209-220 has no minimum step). */
+int uppe_set_paraxial_medium(uppe_handle* h, const uppe_paraxial_medium* m);
 
 /* Rotational (Raman) susceptibility added to chi in every source evaluation (update_current's
  * rotator.AddChi, uppe_kernel.py:174-176; rotations.py:44-85; rotations.cl:80-129): `states` levels with the
@@ -138,9 +163,10 @@ int uppe_set_field(uppe_handle* h, const double* a_r);
  * steps_out (nullable) receives the steps taken by this call. */
 int uppe_advance(uppe_handle* h, const uppe_step_control* c, int* steps_out);
 
-/* finish() semantics (uppe_kernel.py:234-258): propagate to z_total = zf - zi, transform to r-space and
- * evaluate the diagnostics. Outputs (each nullable): a_r, j_r complex128 [nw][n1][n2]; chi_t, ne_t float64
- * [nt][n1][n2] (the caller forms rfft along axis 0 as the reference does on the host). */
+/* finish() semantics (uppe_kernel.py:234-258, paraxial_kernel.py:222-249): propagate to z_total = zf - zi,
+ * transform to r-space and evaluate the diagnostics. Outputs (each nullable): a_r, j_r complex128
+ * [nw][n1][n2]; chi_t, ne_t float64 [nt][n1][n2] (the caller forms rfft (UPPE) or fft (paraxial) along
+ * axis 0 as the reference does on the host). */
 int uppe_finish(uppe_handle* h, double z_total, double* a_r, double* j_r, double* chi_t, double* ne_t);
 
 /* Read the k-space state q[nw][n1][n2] (complex128). No reference counterpart: used for checks. */

@@ -42,8 +42,8 @@ def _p(a):
 
 
 class OracleStepper(_abi.Stepper):
-    def __init__(self, geometry, nw, n1, n2, nk, precision=_abi.PREC_FP64, device=0):
-        super().__init__(library(), 'orc_uppe_', geometry, nw, n1, n2, nk, precision, device)
+    def __init__(self, geometry, nw, n1, n2, nk, precision=_abi.PREC_FP64, device=0, model=_abi.MODEL_UPPE):
+        super().__init__(library(), 'orc_uppe_', geometry, nw, n1, n2, nk, precision, device, model)
 
     def rhs(self, q, zphase):
         '''One right-hand-side evaluation S(zphase, q) (k-space in and out).'''
@@ -53,7 +53,7 @@ class OracleStepper(_abi.Stepper):
         return s
 
     def source(self, e_r):
-        '''update_current on an r-space field: (J_r, chi(t), ne(t)).'''
+        '''update_current on an r-space field (UPPE: E; paraxial: A): (J_r, chi(t), ne(t)).'''
         e_r = np.ascontiguousarray(e_r, dtype=np.complex128)
         j = np.zeros_like(e_r)
         tshape = (self.nt,) + self.shape[1:]
@@ -74,12 +74,16 @@ class OracleStepper(_abi.Stepper):
         return a
 
 
-def oracle_factory(geometry, nw, n1, n2, nk):
-    return OracleStepper(geometry, nw, n1, n2, nk)
+def oracle_factory(geometry, nw, n1, n2, nk, model=_abi.MODEL_UPPE):
+    return OracleStepper(geometry, nw, n1, n2, nk, model=model)
 
 
 def oracle_track(vol_dict, record=None, xp=None, eikonal=None, vg=None):
-    '''The reference's track() on the oracle: (A, J, chiNL, ne, dom4d, evals); ray data as track() takes it.'''
+    '''The reference's track() on the oracle, UPPE or paraxial by vol_dict['propagator']:
+    (A, J, chiNL, ne, dom4d, evals); ray data as track() takes it.'''
+    if vol_dict.get('propagator', 'uppe') == 'paraxial':
+        from searay_amd import paraxial_kernel
+        return paraxial_kernel.track_with(vol_dict, oracle_factory, record, xp, eikonal, vg)
     return uppe_kernel.track_with(vol_dict, oracle_factory, record, xp, eikonal, vg)
 
 

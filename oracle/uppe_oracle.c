@@ -19,6 +19,12 @@
  * plasma currents, the density scan and the rates against closed forms and the host models.
  * Parity unpinned: the full nonlinear run as a whole (the reference stores no nonlinear output to compare
  * with); it is pinned only through the parts above.
+ *
+ * The paraxial envelope equation (modules/paraxial_kernel.py, kernels/paraxial.cl) is the handle's second
+ * model: complex time strips of Nt = Nw samples, the paraxial linear propagator and RHS, envelope rates, the
+ * 0.75 Kerr factor, the envelope rotator drive and the plasma current ne (|A|^2/4 - 1) A. Pinned by the
+ * reference's paraxial Gaussian-beam test (test/gaussian_test.py:63-108), the self-phase modulation of a
+ * monochromatic plane wave, and the same step-control and transform checks as the UPPE model.
  */
 #include "../include/searay_uppe.h"
 
@@ -176,8 +182,11 @@ void orc_irfft_strip(const double* in, double* out, long modes, long s) {
 typedef struct orc_handle {
   uppe_desc d;
   long nw, n1, n2, nk, nm, nt, ntr, npt, ntt;
+  int paraxial;  /* UPPE_MODEL_PARAXIAL */
   /* medium */
   double *w, *kz, *kg, *ng, *nlf, *cond;
+  double* k0;            /* paraxial: k0(w) = w0 dn + w ng */
+  double *Etc, *Atc, *Jtc; /* paraxial: complex time strips of E, A and the current */
   double redchi3, dt;
   uppe_ionization ion;
   /* rotational susceptibility (rotations.py:44-85) */
@@ -436,6 +445,126 @@ static void k_load_step_size(orc_handle* h, double dz0, double dphi, double amin
 }
 
 /* ---------------------------------------------------------------------------------------------------- */
+/* Paraxial envelope kernels (kernels/paraxial.cl, ionization.cl:46-69,99-125, rotations.cl:26-78)       */
+/* ---------------------------------------------------------------------------------------------------- */
+
+/* A = exp(i kz z) q with complex kz: iKz = (-Ki z, Kr z), exp(iKz) = exp(-Ki z)(cos Kr z, sin Kr z)
+ * (paraxial.cl:60-77) */
+static void k_propagate_linear_env(orc_handle* h, double* q, double z) {
+#pragma omp parallel for schedule(static)
+  for (long idx = 0; idx < h->npt; idx++) {
+    double kr = h->kz[2 * idx], ki = h->kz[2 * idx + 1];
+    double ar = -ki * z, ai = kr * z;
+    double er = cos(ai) * exp(ar), ei = sin(ai) * exp(ar);
+    double qr = q[2 * idx], qi = q[2 * idx + 1];
+    q[2 * idx] = er * qr - ei * qi;
+    q[2 * idx + 1] = er * qi + ei * qr;
+  }
+}
+
+/* S = (i/2) exp(-i kz z) J / k0(w) (paraxial.cl:79-97) */
+static void k_current_to_rhs_env(orc_handle* h, double* J, double z) {
+  const long per = h->ntr;
+#pragma omp parallel for schedule(static)
+  for (long idx = 0; idx < h->npt; idx++) {
+    double kr = h->kz[2 * idx], ki = h->kz[2 * idx + 1];
+    double ar = ki * z, ai = -kr * z; /* -iKz */
+    double er = cos(ai) * exp(ar), ei = sin(ai) * exp(ar);
+    double jr = J[2 * idx], ji = J[2 * idx + 1];
+    double tr = er * jr - ei * ji, ti = er * ji + ei * jr;
+    double k0 = h->k0[idx / per];
+    J[2 * idx] = (0.0 * tr - 0.5 * ti) / k0;
+    J[2 * idx + 1] = (0.0 * ti + 0.5 * tr) / k0;
+  }
+}
+
+/* complex strip transforms of every (x, y) strip: FFT (inversion 1) or IFFT with 1/N (fft.cl:199-227) */
+static void strip_fft_all(orc_handle* h, double* a, double inversion) {
+#pragma omp parallel for schedule(static)
+  for (long t = 0; t < h->ntr; t++) orc_complex_fft(a + 2 * t, a + 2 * t + 1, h->nt, 2 * h->ntr, inversion, -1.0);
+}
+
+/* rates of the envelope modulus |E| (EnvelopeRate, EnvelopeRateSeries) */
+static void k_rate_env(orc_handle* h) {
+  const uppe_ionization* p = &h->ion;
+#pragma omp parallel for schedule(static)
+  for (long idx = 0; idx < h->ntt; idx++) {
+    double er = h->Etc[2 * idx], ei = h->Etc[2 * idx + 1];
+    double e = sqrt(er * er + ei * ei);
+    if (p->kind == UPPE_ION_EXPLICIT) {
+      double eau = p->e_conv * e + p->e_cutoff;
+      h->ne[idx] = p->rate_conv * p->c_pre * pow(eau, p->c_pow) * exp(p->c_exp / eau);
+    } else {
+      double lg = log(p->e_conv * e + p->e_cutoff);
+      h->ne[idx] = p->rate_conv * exp(p->c[0] + p->c[1] * lg + p->c[2] * lg * lg + p->c[3] * pow(lg, 3) +
+                                      p->c[4] * pow(lg, 4) + p->c[5] * pow(lg, 5));
+    }
+  }
+}
+
+/* chi += 0.75 ng chi3 |E|^2 (paraxial.cl:112-131) */
+static void k_add_kerr_chi_env(orc_handle* h) {
+  const long per = h->ntr;
+#pragma omp parallel for schedule(static)
+  for (long idx = 0; idx < h->ntt; idx++) {
+    double er = h->Etc[2 * idx], ei = h->Etc[2 * idx + 1];
+    h->chi[idx] += 0.75 * h->ng[idx % per] * h->redchi3 * (er * er + ei * ei);
+  }
+}
+
+/* rotational susceptibility driven by the cycle average |E|^2 / 2 (rotations.cl:26-78, AddChiParaxial) */
+static void k_add_chi_rotations_env(orc_handle* h) {
+  const long per = h->ntr, steps = h->nt;
+  const double dt = h->dt, dalpha = h->rot_dalpha, hbar = h->rot_hbar;
+#pragma omp parallel for schedule(static)
+  for (long xy = 0; xy < per; xy++) {
+    for (int j = 2; j < h->rot_states; j++) {
+      const double wr = h->rot_wdt[2 * j], wi = h->rot_wdt[2 * j + 1];
+      const double Sjj = 2.0 * j * (j - 1) / (15 * (2 * j - 1));
+      double rr = 0.0, ri = 0.0, er = 1.0, ei = 0.0, cum = 0.0;
+      const double hr = cos(0.5 * wr) * exp(-0.5 * wi), hi = sin(0.5 * wr) * exp(-0.5 * wi);
+      for (long k = steps - 1; k >= 0; --k) {
+        const long idx = k * per + xy;
+        const double ar = rr * er + ri * ei, ai = -rr * ei + ri * er;
+        cum += 2 * (h->rot_tjj[j] * Sjj * (ar * wi + ai * wr)) * h->ng[xy] * dalpha;
+        h->chi[idx] += cum;
+        const double Er = h->Etc[2 * idx], Ei = h->Etc[2 * idx + 1];
+        const double F = 0.5 * (dalpha / hbar) * 0.5 * (Er * Er + Ei * Ei);
+        double nr = er * hr - ei * hi, ni = er * hi + ei * hr;
+        er = nr;
+        ei = ni;
+        rr -= dt * F * (-ei);
+        ri -= dt * F * er;
+        nr = er * hr - ei * hi;
+        ni = er * hi + ei * hr;
+        er = nr;
+        ei = ni;
+      }
+    }
+  }
+}
+
+/* P = chi E, complex (paraxial.cl:133-147) */
+static void k_source_polarization_env(orc_handle* h) {
+#pragma omp parallel for schedule(static)
+  for (long idx = 0; idx < h->ntt; idx++) {
+    h->Jtc[2 * idx] = h->chi[idx] * h->Etc[2 * idx];
+    h->Jtc[2 * idx + 1] = h->chi[idx] * h->Etc[2 * idx + 1];
+  }
+}
+
+/* J += ne (|A|^2/4 - 1) A (paraxial.cl:99-110) */
+static void k_plasma_current_env(orc_handle* h) {
+#pragma omp parallel for schedule(static)
+  for (long idx = 0; idx < h->ntt; idx++) {
+    double ar = h->Atc[2 * idx], ai = h->Atc[2 * idx + 1];
+    double f = h->ne[idx] * (0.25 * (ar * ar + ai * ai) - 1.0);
+    h->Jtc[2 * idx] += f * ar;
+    h->Jtc[2 * idx + 1] += f * ai;
+  }
+}
+
+/* ---------------------------------------------------------------------------------------------------- */
 /* Transverse transforms (grid_tools.py:220-229, 308-325; fft.cl:363-477)                               */
 /* ---------------------------------------------------------------------------------------------------- */
 
@@ -575,6 +704,30 @@ static void update_current(orc_handle* h) {
   k_filter(h, h->Jw, h->nlf);
 }
 
+/* paraxial update_current (paraxial_kernel.py:140-184): Ew holds A(w) in r-space on entry; Jw receives J(w) */
+static void update_current_env(orc_handle* h) {
+  memcpy(h->Etc, h->Ew, sizeof(double) * 2 * h->ntt);
+  memcpy(h->Atc, h->Ew, sizeof(double) * 2 * h->ntt);
+  memset(h->ne, 0, sizeof(double) * h->ntt);
+  memset(h->chi, 0, sizeof(double) * h->ntt);
+  k_dt_spectral(h, h->Etc, -1.0);
+  strip_fft_all(h, h->Etc, -1.0);
+  strip_fft_all(h, h->Atc, -1.0);
+  if (h->ion.kind != UPPE_ION_NONE) {
+    k_rate_env(h);
+    k_plasma_density(h);
+  }
+  if (h->rot_states > 2) k_add_chi_rotations_env(h);
+  k_add_kerr_chi_env(h);
+  k_source_polarization_env(h);
+  strip_fft_all(h, h->Jtc, 1.0);
+  k_dt_spectral(h, h->Jtc, 1.0);
+  strip_fft_all(h, h->Jtc, -1.0);
+  k_plasma_current_env(h);
+  strip_fft_all(h, h->Jtc, 1.0);
+  memcpy(h->Jw, h->Jtc, sizeof(double) * 2 * h->npt);
+}
+
 static double reduce_sum(const double* v, long n) {
   double s = 0.0;
 #pragma omp parallel for reduction(+ : s) schedule(static)
@@ -605,6 +758,14 @@ static double load_source(orc_handle* h, double zphase, int return_dz, double zl
       h->Jw[2 * i] = h->lam_re * qr - h->lam_im * qi;
       h->Jw[2 * i + 1] = h->lam_re * qi + h->lam_im * qr;
     }
+  } else if (h->paraxial) {
+    /* paraxial_kernel.py:186-220 */
+    memcpy(h->Ew, h->qw, sizeof(double) * 2 * h->npt);
+    k_propagate_linear_env(h, h->Ew, zphase);
+    rspacex(h, h->Ew);
+    update_current_env(h);
+    kspacex(h, h->Jw);
+    k_current_to_rhs_env(h, h->Jw, zphase);
   } else {
     memcpy(h->Ew, h->qw, sizeof(double) * 2 * h->npt);
     k_propagate_linear(h, h->Ew, zphase);
@@ -626,8 +787,10 @@ static double load_source(orc_handle* h, double zphase, int return_dz, double zl
   return dz;
 }
 
-/* conditioner: zero the lowest bins after every stage (uppe_kernel.py:131-137) */
-static void condition_field(orc_handle* h) { k_filter(h, h->qw, h->cond); }
+/* conditioner: zero the lowest bins after every stage (uppe_kernel.py:131-137); the paraxial stepper has none */
+static void condition_field(orc_handle* h) {
+  if (!h->paraxial) k_filter(h, h->qw, h->cond);
+}
 
 /* ---------------------------------------------------------------------------------------------------- */
 /* Exported ABI                                                                                          */
@@ -636,7 +799,7 @@ static void condition_field(orc_handle* h) { k_filter(h, h->qw, h->cond); }
 static void free_all(orc_handle* h) {
   double** ptrs[] = {&h->w, &h->kz, &h->kg, &h->ng, &h->nlf, &h->cond, &h->H, &h->lam, &h->qw, &h->qi,
                      &h->qf, &h->Ew, &h->Jw, &h->scratchw, &h->hscratch, &h->dzb, &h->Et, &h->ne, &h->chi,
-                     &h->Jt, &h->rot_tjj, &h->rot_wdt};
+                     &h->Jt, &h->rot_tjj, &h->rot_wdt, &h->k0, &h->Etc, &h->Atc, &h->Jtc};
   for (size_t i = 0; i < sizeof(ptrs) / sizeof(ptrs[0]); i++) {
     free(*ptrs[i]);
     *ptrs[i] = NULL;
@@ -646,7 +809,10 @@ static void free_all(orc_handle* h) {
 int orc_uppe_create(const uppe_desc* d, orc_handle** out) {
   if (!d || !out) return UPPE_ERR_ARG;
   *out = NULL;
-  if (d->nw < 3 || !is_pow2(d->nw - 1) || !is_pow2(d->n1) || !is_pow2(d->n2)) return UPPE_ERR_ARG;
+  if (d->model != UPPE_MODEL_UPPE && d->model != UPPE_MODEL_PARAXIAL) return UPPE_ERR_ARG;
+  const int par = d->model == UPPE_MODEL_PARAXIAL;
+  if (par ? (d->nw < 2 || !is_pow2(d->nw)) : (d->nw < 3 || !is_pow2(d->nw - 1))) return UPPE_ERR_ARG;
+  if (!is_pow2(d->n1) || !is_pow2(d->n2)) return UPPE_ERR_ARG;
   if (d->geometry != UPPE_RADIAL && d->geometry != UPPE_CARTESIAN) return UPPE_ERR_ARG;
   if (d->geometry == UPPE_RADIAL && (d->nk < 1 || d->nk > d->n1)) return UPPE_ERR_ARG;
   orc_handle* h = (orc_handle*)calloc(1, sizeof(orc_handle));
@@ -657,7 +823,8 @@ int orc_uppe_create(const uppe_desc* d, orc_handle** out) {
   h->n2 = d->n2;
   h->nk = d->geometry == UPPE_RADIAL ? d->nk : d->n1;
   h->nm = d->n2;
-  h->nt = 2 * (d->nw - 1);
+  h->paraxial = par;
+  h->nt = par ? d->nw : 2 * (d->nw - 1);
   h->ntr = (long)d->n1 * d->n2;
   h->npt = h->nw * h->ntr;
   h->ntt = h->nt * h->ntr;
@@ -682,6 +849,17 @@ int orc_uppe_create(const uppe_desc* d, orc_handle** out) {
   h->ne = calloc(1, ts);
   h->chi = calloc(1, ts);
   h->Jt = calloc(1, ts);
+  if (par) {
+    h->k0 = calloc((size_t)h->nw, sizeof(double));
+    h->Etc = calloc(1, 2 * ts);
+    h->Atc = calloc(1, 2 * ts);
+    h->Jtc = calloc(1, 2 * ts);
+    if (!h->k0 || !h->Etc || !h->Atc || !h->Jtc) {
+      free_all(h);
+      free(h);
+      return UPPE_ERR_DEVICE;
+    }
+  }
   if (!h->w || !h->kz || !h->kg || !h->ng || !h->nlf || !h->cond || !h->H || !h->lam || !h->qw || !h->qi ||
       !h->qf || !h->Ew || !h->Jw || !h->scratchw || !h->hscratch || !h->dzb || !h->Et || !h->ne || !h->chi ||
       !h->Jt) {
@@ -726,6 +904,7 @@ int orc_uppe_set_rotator(orc_handle* h, const uppe_rotator* r) {
 int orc_uppe_set_medium(orc_handle* h, const uppe_medium* m) {
   if (!h || !m || !m->w || !m->kz || !m->kg || !m->ngas || !m->nl_filter || !m->cond_filter)
     return fail(h, UPPE_ERR_ARG, "null argument");
+  if (h->paraxial) return fail(h, UPPE_ERR_STATE, "uppe_set_medium on a paraxial handle");
   if (m->w[0] != 0.0) return fail(h, UPPE_ERR_ARG, "UPPE propagator requires frequency range [0,...]");
   memcpy(h->w, m->w, sizeof(double) * h->nw);
   memcpy(h->kz, m->kz, sizeof(double) * 2 * h->npt);
@@ -733,6 +912,21 @@ int orc_uppe_set_medium(orc_handle* h, const uppe_medium* m) {
   memcpy(h->ng, m->ngas, sizeof(double) * h->ntr);
   memcpy(h->nlf, m->nl_filter, sizeof(double) * h->nw);
   memcpy(h->cond, m->cond_filter, sizeof(double) * h->nw);
+  h->redchi3 = m->redchi3;
+  h->dt = m->dt;
+  h->ion = m->ion;
+  h->have_medium = 1;
+  return UPPE_OK;
+}
+
+/* paraxial Material.UpdateMaterial uploads and the Source frequency table (paraxial_kernel.py:72-135) */
+int orc_uppe_set_paraxial_medium(orc_handle* h, const uppe_paraxial_medium* m) {
+  if (!h || !m || !m->w || !m->kz || !m->k0 || !m->ngas) return fail(h, UPPE_ERR_ARG, "null argument");
+  if (!h->paraxial) return fail(h, UPPE_ERR_STATE, "uppe_set_paraxial_medium on a UPPE handle");
+  memcpy(h->w, m->w, sizeof(double) * h->nw);
+  memcpy(h->kz, m->kz, sizeof(double) * 2 * h->npt);
+  memcpy(h->k0, m->k0, sizeof(double) * h->nw);
+  memcpy(h->ng, m->ngas, sizeof(double) * h->ntr);
   h->redchi3 = m->redchi3;
   h->dt = m->dt;
   h->ion = m->ion;
@@ -798,12 +992,21 @@ int orc_uppe_advance(orc_handle* h, const uppe_step_control* c, int* steps_out) 
 int orc_uppe_finish(orc_handle* h, double z_total, double* a_r, double* j_r, double* chi_t, double* ne_t) {
   if (!h) return UPPE_ERR_ARG;
   if (!h->have_field || !h->have_medium) return fail(h, UPPE_ERR_STATE, "field or medium not set");
-  k_propagate_linear(h, h->qw, z_total);
-  rspacex(h, h->qw);
-  memcpy(h->Ew, h->qw, sizeof(double) * 2 * h->npt);
-  k_vp_to_e(h, h->Ew);
-  update_current(h);
-  if (a_r) memcpy(a_r, h->qw, sizeof(double) * 2 * h->npt);
+  if (h->paraxial) {
+    /* paraxial_kernel.py:222-249: A = rspace(exp(i kz z) q), then update_current on it */
+    memcpy(h->Ew, h->qw, sizeof(double) * 2 * h->npt);
+    k_propagate_linear_env(h, h->Ew, z_total);
+    rspacex(h, h->Ew);
+    update_current_env(h);
+    if (a_r) memcpy(a_r, h->Ew, sizeof(double) * 2 * h->npt);
+  } else {
+    k_propagate_linear(h, h->qw, z_total);
+    rspacex(h, h->qw);
+    memcpy(h->Ew, h->qw, sizeof(double) * 2 * h->npt);
+    k_vp_to_e(h, h->Ew);
+    update_current(h);
+    if (a_r) memcpy(a_r, h->qw, sizeof(double) * 2 * h->npt);
+  }
   if (j_r) memcpy(j_r, h->Jw, sizeof(double) * 2 * h->npt);
   if (chi_t) memcpy(chi_t, h->chi, sizeof(double) * h->ntt);
   if (ne_t) memcpy(ne_t, h->ne, sizeof(double) * h->ntt);
@@ -849,11 +1052,14 @@ int orc_uppe_rhs(orc_handle* h, const double* q, double zphase, double* s) {
   return UPPE_OK;
 }
 
-/* update_current on an r-space field Ew: Jw (r-space spectral), chi(t), ne(t) */
+/* update_current on an r-space field (UPPE: E; paraxial: A): Jw (r-space spectral), chi(t), ne(t) */
 int orc_uppe_source(orc_handle* h, const double* e_r, double* j_r, double* chi_t, double* ne_t) {
   if (!h || !e_r) return UPPE_ERR_ARG;
   memcpy(h->Ew, e_r, sizeof(double) * 2 * h->npt);
-  update_current(h);
+  if (h->paraxial)
+    update_current_env(h);
+  else
+    update_current(h);
   if (j_r) memcpy(j_r, h->Jw, sizeof(double) * 2 * h->npt);
   if (chi_t) memcpy(chi_t, h->chi, sizeof(double) * h->ntt);
   if (ne_t) memcpy(ne_t, h->ne, sizeof(double) * h->ntt);

@@ -14,6 +14,8 @@ PREC_FP64 = 1
 PREC_FP32 = 2
 PREC_BF16X3 = 3
 PRECISIONS = {'fp64-plain': PREC_FP64_PLAIN, 'fp64': PREC_FP64, 'fp32': PREC_FP32, 'bf16x3': PREC_BF16X3}
+MODEL_UPPE = 0
+MODEL_PARAXIAL = 1
 
 KC_HANKEL, KC_SOURCE, KC_POINT, KC_REDUCE, KC_TRANSVERSE_FFT = range(5)
 KERNEL_CLASSES = {'hankel': KC_HANKEL, 'source': KC_SOURCE, 'point': KC_POINT, 'reduce': KC_REDUCE,
@@ -24,7 +26,7 @@ _dp = ct.POINTER(ct.c_double)
 
 class uppe_desc(ct.Structure):
     _fields_ = [('geometry', ct.c_int), ('nw', ct.c_int), ('n1', ct.c_int), ('n2', ct.c_int),
-                ('nk', ct.c_int), ('precision', ct.c_int), ('device', ct.c_int), ('reserved', ct.c_int)]
+                ('nk', ct.c_int), ('precision', ct.c_int), ('device', ct.c_int), ('model', ct.c_int)]
 
 
 class uppe_ionization(ct.Structure):
@@ -36,6 +38,11 @@ class uppe_ionization(ct.Structure):
 class uppe_medium(ct.Structure):
     _fields_ = [('w', _dp), ('kz', _dp), ('kg', _dp), ('ngas', _dp), ('nl_filter', _dp), ('cond_filter', _dp),
                 ('redchi3', ct.c_double), ('dt', ct.c_double), ('ion', uppe_ionization)]
+
+
+class uppe_paraxial_medium(ct.Structure):
+    _fields_ = [('w', _dp), ('kz', _dp), ('k0', _dp), ('ngas', _dp), ('redchi3', ct.c_double), ('dt', ct.c_double),
+                ('ion', uppe_ionization)]
 
 
 class uppe_rotator(ct.Structure):
@@ -55,8 +62,8 @@ class uppe_info(ct.Structure):
                 ('reserved', ct.c_int), ('z_last', ct.c_double)]
 
 
-ENTRY_POINTS = ('create', 'set_hankel', 'set_medium', 'set_rotator', 'set_field', 'advance', 'finish', 'get_state',
-                'get_info', 'destroy', 'last_error')
+ENTRY_POINTS = ('create', 'set_hankel', 'set_medium', 'set_paraxial_medium', 'set_rotator', 'set_field', 'advance',
+                'finish', 'get_state', 'get_info', 'destroy', 'last_error')
 HIP_ONLY_ENTRY_POINTS = ('profile_enable', 'profile_read', 'synchronize')
 
 
@@ -76,18 +83,30 @@ class UppeError(RuntimeError):
     pass
 
 
+def _ion_struct(ion):
+    io = uppe_ionization()
+    io.kind = int(ion['kind'])
+    for key in ('e_conv', 'rate_conv', 'e_cutoff', 'c_pre', 'c_pow', 'c_exp'):
+        setattr(io, key, ion[key])
+    for i in range(6):
+        io.c[i] = ion['c'][i]
+    return io
+
+
 class Stepper:
     '''One stepper handle. The caller keeps numpy arrays in the reference layout:
-    spectral (Nw, N1, N2) complex128, time (Nt, N1, N2) float64.'''
+    spectral (Nw, N1, N2) complex128, time (Nt, N1, N2) float64 with Nt = 2 (Nw - 1) for the UPPE and Nt = Nw
+    for the paraxial envelope (model MODEL_PARAXIAL).'''
 
-    def __init__(self, lib, prefix, geometry, nw, n1, n2, nk, precision, device=0):
+    def __init__(self, lib, prefix, geometry, nw, n1, n2, nk, precision, device=0, model=MODEL_UPPE):
         self.lib = lib
         self.prefix = prefix
         self.shape = (nw, n1, n2)
-        self.nt = 2 * (nw - 1)
+        self.model = model
+        self.nt = nw if model == MODEL_PARAXIAL else 2 * (nw - 1)
         self._bind()
         self._keep = []
-        d = uppe_desc(geometry, nw, n1, n2, nk if nk is not None else n1, precision, device, 0)
+        d = uppe_desc(geometry, nw, n1, n2, nk if nk is not None else n1, precision, device, model)
         h = ct.c_void_p()
         rc = self._f['create'](ct.byref(d), ct.byref(h))
         if rc != UPPE_OK:
@@ -120,19 +139,18 @@ class Stepper:
     def set_medium(self, w, kz, kg, ngas, nl_filter, cond_filter, redchi3, dt, ion):
         arrs = dict(w=_f64(w), kz=_c128(kz).view(np.float64), kg=_f64(kg).reshape(-1), ngas=_f64(ngas),
                     nl=_f64(nl_filter), cond=_f64(cond_filter))
-        io = uppe_ionization()
-        io.kind = int(ion['kind'])
-        io.e_conv = ion['e_conv']
-        io.rate_conv = ion['rate_conv']
-        io.e_cutoff = ion['e_cutoff']
-        io.c_pre = ion['c_pre']
-        io.c_pow = ion['c_pow']
-        io.c_exp = ion['c_exp']
-        for i in range(6):
-            io.c[i] = ion['c'][i]
+        io = _ion_struct(ion)
         m = uppe_medium(_ptr(arrs['w']), _ptr(arrs['kz']), _ptr(arrs['kg']), _ptr(arrs['ngas']),
                         _ptr(arrs['nl']), _ptr(arrs['cond']), float(redchi3), float(dt), io)
         self._check(self._f['set_medium'](self.h, ct.byref(m)), 'set_medium')
+
+    def set_paraxial_medium(self, w, kz, k0, ngas, redchi3, dt, ion):
+        '''Paraxial medium (paraxial_kernel.py:72-102): frequencies in ifftshift order, complex kz (k-space),
+        k0(w), gas density, chi3 / n_ref, time step, envelope rate parameters.'''
+        arrs = dict(w=_f64(w), kz=_c128(kz).view(np.float64), k0=_f64(k0).reshape(-1), ngas=_f64(ngas))
+        m = uppe_paraxial_medium(_ptr(arrs['w']), _ptr(arrs['kz']), _ptr(arrs['k0']), _ptr(arrs['ngas']),
+                                 float(redchi3), float(dt), _ion_struct(ion))
+        self._check(self._f['set_paraxial_medium'](self.h, ct.byref(m)), 'set_paraxial_medium')
 
     def set_rotator(self, rotator, dt):
         '''Load a rotator's tables for source time step dt, or switch the rotator off (None). Any object with the

@@ -28,8 +28,8 @@ def library():
 class HipStepper(_abi.Stepper):
     '''Stepper handle on the HIP library, with the timing entry points.'''
 
-    def __init__(self, geometry, nw, n1, n2, nk, precision, device=0):
-        super().__init__(library(), 'uppe_', geometry, nw, n1, n2, nk, precision, device)
+    def __init__(self, geometry, nw, n1, n2, nk, precision, device=0, model=_abi.MODEL_UPPE):
+        super().__init__(library(), 'uppe_', geometry, nw, n1, n2, nk, precision, device, model)
 
     def profile_enable(self, enable=True):
         self._check(self.lib.uppe_profile_enable(self.h, int(bool(enable))), 'profile_enable')

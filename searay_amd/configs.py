@@ -241,3 +241,110 @@ def gaussian_vacuum(coords='cartesian', n=128, nw=65, precision=None):
     if precision:
         vol['uppe precision'] = precision
     return vol, dict(a00=a00, w00=w00, r00=r00, zR=zR)
+
+
+def gaussian_paraxial(coords='cartesian', n=128, nw=64, precision=None):
+    '''The reference's paraxial Gaussian-beam test (test/gaussian_test.py:63-108): vacuum, a00 = 1, w00 = 1,
+    r00 = 0.01 cm, band (0.95, 1.05) with a 0.01-wide spectrum, size 36 r00 x 36 r00 x 4 zR, three planes; the
+    on-axis amplitude falls to 1/sqrt(5) and 1/sqrt(17) at the second and third planes.'''
+    cm = 100 * MKS_LENGTH
+    a00, w00, r00 = 1.0, 1.0, 0.01 / cm
+    zR = 0.5 * w00 * r00 ** 2
+    vol = {
+        'density reference': 1.0,
+        'density function': '1.0',
+        'density lambda': lambda x, y, z, r2: np.ones(x.shape),
+        'frequency band': (0.95, 1.05),
+        'wave grid': (nw, n, n if coords == 'cartesian' else 1, 3),
+        'wave coordinates': coords,
+        'dispersion inside': dispersion.Vacuum(),
+        'dispersion outside': dispersion.Vacuum(),
+        'size': (36 * r00, 36 * r00, 4 * zR),
+        'propagator': 'paraxial',
+        'subcycles': 1,
+    }
+    if coords != 'cartesian':
+        vol['radial modes'] = n
+
+    def f(x, x0, dx):
+        return np.exp(-(x - x0) ** 2 / dx ** 2)
+
+    def incoming(w, x, y):
+        if coords == 'cartesian':
+            return a00 * np.einsum('i,j,k', f(w, w00, 0.01), f(x, 0, r00), f(y, 0, r00))
+        return a00 * np.einsum('i,j,k', f(w, w00, 0.01), f(x, 0, r00), np.ones(y.shape))
+    vol['incoming wave'] = incoming
+    vol['object'] = _box(vol)
+    if precision:
+        vol['uppe precision'] = precision
+    return vol, dict(a00=a00, w00=w00, r00=r00, zR=zR)
+
+
+def paraxial_air_filament(scale=1, steps=None, seed=1004, precision=None, rate='explicit', power=1.0,
+                          length='40 cm', planes=20, coords='cylindrical', rotator_states=0):
+    '''The paraxial air filament of examples/paraxial/air-fil.py:25-109: 800 nm, 1 mJ in 50 fs, focused by a
+    lens of 40 cm onto the centre of a 40 cm air box (entrance spot r00/2 with 20 cm radius of curvature),
+    envelope band of 25 transform-limited widths either side of w00, Nw = 256, Nr = 256 with 128 Hankel modes
+    (Cartesian: 256 x 256), Kerr (1e-23 m^2/W) and O2 photoionisation (12.1 eV, Z = 0.53, cycle-averaged rate,
+    5.4e18 cm^-3), 20 diagnostic planes, one subcycle per plane. ``power`` scales the pulse energy; 1 %
+    random spectral phase breaks the symmetry of the test inputs.'''
+    helper = input_tools.InputHelper(MKS_LENGTH)
+    um = 1e6 * MKS_LENGTH
+    cm = 100 * MKS_LENGTH
+    mm = 1000 * MKS_LENGTH
+    wavelength = 0.8 / um
+    waist = 150 / um
+    w00 = 2 * np.pi / wavelength
+    t00 = helper.dnum('50 fs')
+    L = helper.dnum(length)
+    time_to_focus = 40 / cm
+    r00 = time_to_focus * wavelength / (np.pi * waist)
+    sigma_w = 2 / t00
+    band = (w00 - 25 * sigma_w, w00 + 25 * sigma_w)
+    rbox = 3 / mm
+    nw = 256 // scale
+    nr = max(8, 256 // scale)
+    air = dispersion.SimpleAir(MKS_LENGTH)
+    vol = {
+        'frequency band': band,
+        'size': (2 * rbox, 2 * rbox, L),
+        'wave grid': (nw, nr, nr if coords == 'cartesian' else 1, planes),
+        'wave coordinates': coords,
+        'subcycles': 1 if steps is None else steps,
+        'density reference': helper.dnum(GAS_DENSITY),
+        'density function': '1.0',
+        'density lambda': lambda x, y, z, r2: np.ones(r2.shape),
+        'propagator': 'paraxial',
+        'dispersion inside': air,
+        'dispersion outside': dispersion.Vacuum(),
+        'chi3': helper.chi3(1.0, '%g m2/W' % N2_AIR),
+        'window speed': air.GroupVelocityMagnitude(1.0),
+    }
+    if coords != 'cartesian':
+        vol['radial modes'] = max(4, 128 // scale)
+    if rate is not None:
+        Uion = helper.dnum('12.1 eV')
+        cls = ionization.PPT_Tunneling if rate == 'explicit' else ionization.PPT
+        vol['ionizer'] = cls(MKS_LENGTH, True, Uion, 0.53, lstar=0, l=0, m=0, w0=w00, terms=80)
+    if rotator_states:
+        vol['rotator'] = n2_rotator(rotator_states)
+    vol['object'] = _box(vol)
+    a00 = helper.a0(power * helper.dnum('1 mJ'), t00, r00, w00)
+    # at the box entrance, halfway from the lens to the focus: half the spot, converging with radius 20 cm
+    w_in, R = 0.5 * r00, 0.5 * time_to_focus
+
+    def incoming(w, x, y):
+        rng = np.random.default_rng(seed)
+        spec = np.exp(-(w - w00) ** 2 * t00 ** 2 / 4) * np.exp(2j * np.pi * 0.01 * (rng.random(w.shape) - 0.5))
+        if coords == 'cartesian':
+            r2 = np.add.outer(x ** 2, y ** 2)
+        else:
+            r2 = np.outer(x ** 2, np.ones(y.shape))
+        env = np.exp(-r2 / w_in ** 2)[None] * np.exp(-1j * w[:, None, None] * r2[None] / (2 * R))
+        # the envelope's time profile is ifft of the ifftshift-ordered spectrum: peak a00
+        amp = a00 / np.max(np.abs(np.fft.ifft(np.fft.ifftshift(spec))))
+        return amp * spec[:, None, None] * env
+    vol['incoming wave'] = incoming
+    if precision:
+        vol['uppe precision'] = precision
+    return vol

@@ -84,7 +84,11 @@ extern "C" {
 int uppe_create(const uppe_desc* d, uppe_handle** out) {
   if (!d || !out) return UPPE_ERR_ARG;
   *out = nullptr;
-  if (d->nw < 3 || !pow2(d->nw - 1) || !pow2(d->n1) || !pow2(d->n2)) return UPPE_ERR_ARG;
+  if (d->model != UPPE_MODEL_UPPE && d->model != UPPE_MODEL_PARAXIAL) return UPPE_ERR_ARG;
+  const bool par = d->model == UPPE_MODEL_PARAXIAL;
+  // 2^n + 1 frequencies for real fields (uppe_kernel.py:354-360), 2^n for the envelope (paraxial_kernel.py:371-377)
+  if (par ? (d->nw < 2 || !pow2(d->nw)) : (d->nw < 3 || !pow2(d->nw - 1))) return UPPE_ERR_ARG;
+  if (!pow2(d->n1) || !pow2(d->n2)) return UPPE_ERR_ARG;
   if (d->geometry != UPPE_RADIAL && d->geometry != UPPE_CARTESIAN) return UPPE_ERR_ARG;
   if (d->geometry == UPPE_RADIAL && (d->nk < 1 || d->nk > d->n1)) return UPPE_ERR_ARG;
   if (d->precision < UPPE_PREC_FP64_PLAIN || d->precision > UPPE_PREC_BF16X3) return UPPE_ERR_ARG;
@@ -95,7 +99,8 @@ int uppe_create(const uppe_desc* d, uppe_handle** out) {
   h->n2 = d->n2;
   h->nk = d->geometry == UPPE_RADIAL ? d->nk : d->n1;
   h->nm = d->n2;
-  h->nt = 2 * (d->nw - 1);
+  h->paraxial = par;
+  h->nt = par ? d->nw : 2 * (d->nw - 1);
   h->P = (long)d->n1 * d->n2;
   h->npt = (long)h->nw * h->P;
   h->ntt = (long)h->nt * h->P;
@@ -149,6 +154,10 @@ int uppe_set_medium(uppe_handle* h, const uppe_medium* m) {
     h->err = "null argument";
     return UPPE_ERR_ARG;
   }
+  if (h->paraxial) {
+    h->err = "uppe_set_medium on a paraxial handle: use uppe_set_paraxial_medium";
+    return UPPE_ERR_STATE;
+  }
   if (m->w[0] != 0.0) {
     h->err = "UPPE propagator requires frequency range [0,...]";
     return UPPE_ERR_ARG;
@@ -164,6 +173,36 @@ int uppe_set_medium(uppe_handle* h, const uppe_medium* m) {
       plain_set_medium(h, m);
     else
       fused_set_medium(h, m);
+    h->have_medium = true;
+  });
+}
+
+int uppe_set_paraxial_medium(uppe_handle* h, const uppe_paraxial_medium* m) {
+  if (!h) return UPPE_ERR_ARG;
+  if (!m || !m->w || !m->kz || !m->k0 || !m->ngas) {
+    h->err = "null argument";
+    return UPPE_ERR_ARG;
+  }
+  if (!h->paraxial) {
+    h->err = "uppe_set_paraxial_medium on a UPPE handle: use uppe_set_medium";
+    return UPPE_ERR_STATE;
+  }
+  if (m->ion.kind < UPPE_ION_NONE || m->ion.kind > UPPE_ION_SERIES) {
+    h->err = "unknown ionization kind";
+    return UPPE_ERR_ARG;
+  }
+  for (int f = 0; f < h->nw; f++)
+    if (!(m->k0[f] != 0.0)) {
+      h->err = "paraxial medium: k0 must be nonzero";
+      return UPPE_ERR_ARG;
+    }
+  return guarded(h, [&] {
+    UPPE_HIP(hipSetDevice(h->d.device));
+    h->w.assign(m->w, m->w + h->nw);
+    if (h->plain)
+      plain_set_paraxial_medium(h, m);
+    else
+      fused_set_paraxial_medium(h, m);
     h->have_medium = true;
   });
 }

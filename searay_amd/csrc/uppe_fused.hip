@@ -95,7 +95,10 @@ struct RotRow {
 
 constexpr int BM = 64, BN = 64, BK = 16;
 
-enum { PRO_NONE = 0, PRO_STAGE = 1, PRO_FINISH = 2 };
+// k2r prologue: none, RK stage input with E = i kz A (UPPE), linear propagation only (finish), RK stage input kept
+// as A (paraxial envelope)
+enum { PRO_NONE = 0, PRO_STAGE = 1, PRO_FINISH = 2, PRO_STAGE_ENV = 3 };
+constexpr bool pro_stage(int pro) { return pro == PRO_STAGE || pro == PRO_STAGE_ENV; }
 enum { EPI_NONE = 0, EPI_STAGE = 1 };
 
 __device__ inline unsigned long long dbits(double x) { return (unsigned long long)__double_as_longlong(x); }
@@ -176,15 +179,15 @@ __global__ __launch_bounds__(256) void f_gemm(const T* __restrict__ A, const cpl
       if (gk < K) {
         long idx = (long)gk * nwp + f;
         v = B[idx];
-        if (PRO == PRO_STAGE) {
+        if (pro_stage(PRO)) {
           if (stage > 1) {
             cplx<T> kp = kprev[idx];
             T c = (T)cu;
             v = cscale(cmk(v.re + c * kp.re, v.im + c * kp.im), cond[f]);
           }
           cplx<T> kz = kzc[idx];
-          cplx<T> a = cmul(linear_factor<T>(phs[idx], kz.im, zph, +1), v);
-          v = cmul(cmk(-kz.im, kz.re), a);  // E = i kz A
+          v = cmul(linear_factor<T>(phs[idx], kz.im, zph, +1), v);
+          if (PRO == PRO_STAGE) v = cmul(cmk(-kz.im, kz.re), v);  // E = i kz A
         } else if (PRO == PRO_FINISH) {
           cplx<T> kz = kzc[idx];
           v = cmul(linear_factor<T>(phs[idx], kz.im, zph, +1), v);
@@ -402,15 +405,15 @@ __device__ inline cplx<T> load_b(const cplx<T>* __restrict__ B, long idx, int f,
                                  const double* __restrict__ phs, const cplx<T>* __restrict__ kzc,
                                  const T* __restrict__ cond, const cplx<T>* __restrict__ kprev) {
   cplx<T> v = B[idx];
-  if (PRO == PRO_STAGE) {
+  if (pro_stage(PRO)) {
     if (stage > 1) {
       cplx<T> kp = kprev[idx];
       T c = (T)cu;
       v = cscale(cmk(v.re + c * kp.re, v.im + c * kp.im), cond[f]);
     }
     cplx<T> kz = kzc[idx];
-    cplx<T> a = cmul(linear_factor<T>(phs[idx], kz.im, zph, +1), v);
-    v = cmul(cmk(-kz.im, kz.re), a);
+    v = cmul(linear_factor<T>(phs[idx], kz.im, zph, +1), v);
+    if (PRO == PRO_STAGE) v = cmul(cmk(-kz.im, kz.re), v);  // E = i kz A
   } else if (PRO == PRO_FINISH) {
     cplx<T> kz = kzc[idx];
     v = cmul(linear_factor<T>(phs[idx], kz.im, zph, +1), v);
@@ -513,7 +516,7 @@ __global__ __launch_bounds__(1024) void f_hankel(
           vz[j] = kzc[idx];
           vp[j] = phs[idx];
         }
-        if (PRO == PRO_STAGE && stage > 1) {
+        if (pro_stage(PRO) && stage > 1) {
           vk[j] = kprev[idx];
           vc[j] = cond[f];
         }
@@ -525,13 +528,13 @@ __global__ __launch_bounds__(1024) void f_hankel(
       if (j >= eb || e >= ne) break;
       const int k = e >> lgh, cf = e & (hcw - 1);
       C v = vb[j];
-      if (PRO == PRO_STAGE) {
+      if (pro_stage(PRO)) {
         if (stage > 1) {
           T c = (T)cu;
           v = cscale(cmk(v.re + c * vk[j].re, v.im + c * vk[j].im), vc[j]);
         }
-        C a = cmul(linear_factor<T>(vp[j], vz[j].im, zph, +1), v);
-        v = cmul(cmk(-vz[j].im, vz[j].re), a);  // E = i kz A
+        v = cmul(linear_factor<T>(vp[j], vz[j].im, zph, +1), v);
+        if (PRO == PRO_STAGE) v = cmul(cmk(-vz[j].im, vz[j].re), v);  // E = i kz A
       } else if (PRO == PRO_FINISH) {
         v = cmul(linear_factor<T>(vp[j], vz[j].im, zph, +1), v);
       }
@@ -1045,6 +1048,95 @@ constexpr int ROT_CHUNK = 64;
 // rotator states per barrier in the source kernel
 constexpr int ROT_G32 = 8, ROT_G64 = 2;
 
+// Rotational (Raman) susceptibility on the SPT samples a thread owns (rotations.cl:26-129, AddChiUPPE and
+// AddChiParaxial). For each state j >= 2 the reference runs the recurrence rho <- rho - i dt F_m E^(2m+1) and
+// cum <- cum + 2 Im(T_j S_j rho conj(E^(2m)) W_j dt) n_g dalpha, adding cum to chi, over m = N-1-n (time from the
+// last sample), E = exp(i W_j dt / 2), F_m = dalpha/(2 hbar) drive_m with drive = E(t)^2 for real fields and
+// |E|^2 / 2 for the envelope. Both running sums are prefix sums over m, and the thread order already follows m:
+// rho_m is a DPP wave scan plus the totals of the earlier waves, one barrier per G states. The sum over states
+// commutes with the outer prefix sum, so the per-state terms g_j(m) are summed first and scanned once.
+template <typename R, int SPT>
+__device__ inline void rot_chi(const R* drive, R ngp, R* dchi, const RotRow<R>* __restrict__ rot_tab, int rot_states,
+                               double rot_fc, double dt, int N, int a0) {
+  const int tid = threadIdx.x, nth = blockDim.x, lane = tid & 63, wave = tid >> 6, nwv = nth >> 6;
+  __shared__ R rot_wsr[16];
+  constexpr int G = sizeof(R) == 4 ? ROT_G32 : ROT_G64;  // states per barrier
+  const R dtr = (R)dt, fc = (R)rot_fc;
+  const int m0 = N - a0 - SPT;  // = SPT tid: m of this thread's last sample; m grows as the sample index falls
+  const int wv_u = __builtin_amdgcn_readfirstlane(wave);
+  R gs[SPT];
+#pragma unroll
+  for (int j = 0; j < SPT; j++) gs[j] = R(0);
+  // per-state constants, staged in LDS for ROT_CHUNK states at a time; the wave totals of G states per
+  // barrier, double-buffered
+  __shared__ RotRow<R> rot_r[ROT_CHUNK];
+  __shared__ cplx<R> rot_ws[2][G][16];
+  int par = 0;
+  for (int s0 = 2; s0 < rot_states; s0 += ROT_CHUNK) {
+    const int nc = min(ROT_CHUNK, rot_states - s0);
+    __syncthreads();  // the previous chunk's constants are no longer read
+    {
+      const unsigned* src = reinterpret_cast<const unsigned*>(rot_tab + (s0 - 2));
+      unsigned* dst = reinterpret_cast<unsigned*>(rot_r);
+      const int words = nc * (int)(sizeof(RotRow<R>) / sizeof(unsigned));
+      for (int i = tid; i < words; i += nth) dst[i] = src[i];
+    }
+    __syncthreads();
+    for (int q0 = 0; q0 < nc; q0 += G) {
+      // pass 1: this thread's sum of F_m E^(2m+1), its exclusive scan over the wave, the wave totals
+      cplx<R> pw0[G], run[G];
+#pragma unroll
+      for (int g = 0; g < G; g++) {
+        const RotRow<R>& rw = rot_r[min(q0 + g, nc - 1)];  // past the chunk end: a repeat, not used
+        const cplx<R> e1 = rw.e[0], e2 = rw.e[1];
+        cplx<R> pw = rot_start(rw, tid, m0), tot = cmk(R(0), R(0));
+        pw0[g] = pw;
+#pragma unroll
+        for (int jj = SPT - 1; jj >= 0; jj--) {
+          tot = cadd(tot, cscale(cmul(pw, e1), fc * drive[jj]));
+          pw = cmul(pw, e2);
+        }
+        run[g] = cmk(wave_excl_prefix(tot.re), wave_excl_prefix(tot.im));
+        if (lane == 63) rot_ws[par][g][wave] = cadd(run[g], tot);
+      }
+      __syncthreads();
+      // pass 2: rho_m = -i dt (sum of F E^(2m+1) over earlier m); g_j(m) = c_j Im(rho conj(E^(2m)) W dt)
+#pragma unroll
+      for (int g = 0; g < G; g++) {
+        if (q0 + g < nc) {
+          const RotRow<R>& rw = rot_r[q0 + g];
+          const R cj = rw.c * ngp, wr = rw.wr, wi = rw.wi;
+          const cplx<R> e1 = rw.e[0], e2 = rw.e[1];
+          const cplx<R> wt = lane < nwv ? rot_ws[par][g][lane] : cmk(R(0), R(0));
+          cplx<R> r = cadd(run[g], cmk(lanes_below(wt.re, wv_u), lanes_below(wt.im, wv_u)));
+          cplx<R> pw = pw0[g];
+#pragma unroll
+          for (int jj = SPT - 1; jj >= 0; jj--) {
+            const cplx<R> rho = cmk(dtr * r.im, -dtr * r.re);
+            const cplx<R> qv = cmul(rho, cconj(pw));
+            gs[jj] += cj * (qv.re * wi + qv.im * wr);
+            r = cadd(r, cscale(cmul(pw, e1), fc * drive[jj]));
+            pw = cmul(pw, e2);
+          }
+        }
+      }
+      par ^= 1;
+    }
+  }
+  R gtot = R(0);
+#pragma unroll
+  for (int jj = 0; jj < SPT; jj++) gtot += gs[jj];
+  R cum = wave_excl_prefix(gtot);
+  if (lane == 63) rot_wsr[wave] = cum + gtot;
+  __syncthreads();
+  for (int w2 = 0; w2 < wave; w2++) cum += rot_wsr[w2];
+#pragma unroll
+  for (int jj = SPT - 1; jj >= 0; jj--) {
+    cum += gs[jj];
+    dchi[jj] = cum;
+  }
+}
+
 // one workgroup per strip, Nt/4 threads (at least min(Nt, 64)); thread t owns SPT = Nt / threads samples
 template <typename T, int SPT, bool DIAG, bool ROT>
 __global__ __launch_bounds__(1024) void f_source(const cplx<T>* __restrict__ er, cplx<T>* __restrict__ jr,
@@ -1058,7 +1150,6 @@ __global__ __launch_bounds__(1024) void f_source(const cplx<T>* __restrict__ er,
   extern __shared__ char smem[];
   cplx<T>* Y = reinterpret_cast<cplx<T>*>(smem);
   __shared__ T wsum[16];
-  __shared__ T rot_wsr[16];
   const int N = nt, M = nt / 2, lgm = lgn - 1;
   const long p = blockIdx.x;
   const int tid = threadIdx.x, nth = blockDim.x, lane = tid & 63, wave = tid >> 6, nwv = nth >> 6;
@@ -1179,91 +1270,15 @@ __global__ __launch_bounds__(1024) void f_source(const cplx<T>* __restrict__ er,
     __syncthreads();  // every thread has read its samples before the strip is overwritten
   }
 
-  // 2b. rotational (Raman) susceptibility (rotations.cl:80-129, AddChiUPPE). For each state j >= 2 the
-  // reference runs the recurrence rho <- rho - i dt F_m E^(2m+1) and cum <- cum + 2 Im(T_j S_j rho conj(E^(2m))
-  // W_j dt) n_g dalpha, adding cum to chi, over m = N-1-n (time from the last sample), E = exp(i W_j dt / 2),
-  // F_m = dalpha/(2 hbar) E(t)^2. Both running sums are prefix sums over m, and the thread order already follows
-  // m: rho_m is a DPP wave scan plus the totals of the earlier waves, one barrier per state. The sum over states
-  // commutes with the outer prefix sum, so the per-state terms g_j(m) are summed first and scanned once.
+  // 2b. rotational (Raman) susceptibility, driven by E(t)^2
   R dchi[SPT];
 #pragma unroll
   for (int j = 0; j < SPT; j++) dchi[j] = R(0);
   if (ROT) {
-    constexpr int G = sizeof(R) == 4 ? ROT_G32 : ROT_G64;  // states per barrier
-    const R dtr = (R)dt, fc = (R)rot_fc;
-    const int m0 = N - a0 - SPT;  // = SPT tid: m of this thread's last sample; m grows as the sample index falls
-    const int wv_u = __builtin_amdgcn_readfirstlane(wave);
-    R gs[SPT];
+    R drive[SPT];
 #pragma unroll
-    for (int j = 0; j < SPT; j++) gs[j] = R(0);
-    // per-state constants, staged in LDS for ROT_CHUNK states at a time; the wave totals of G states per
-    // barrier, double-buffered
-    __shared__ RotRow<R> rot_r[ROT_CHUNK];
-    __shared__ cplx<R> rot_ws[2][G][16];
-    int par = 0;
-    for (int s0 = 2; s0 < rot_states; s0 += ROT_CHUNK) {
-      const int nc = min(ROT_CHUNK, rot_states - s0);
-      __syncthreads();  // the previous chunk's constants are no longer read
-      {
-        const unsigned* src = reinterpret_cast<const unsigned*>(rot_tab + (s0 - 2));
-        unsigned* dst = reinterpret_cast<unsigned*>(rot_r);
-        const int words = nc * (int)(sizeof(RotRow<R>) / sizeof(unsigned));
-        for (int i = tid; i < words; i += nth) dst[i] = src[i];
-      }
-      __syncthreads();
-      for (int q0 = 0; q0 < nc; q0 += G) {
-        // pass 1: this thread's sum of F_m E^(2m+1), its exclusive scan over the wave, the wave totals
-        cplx<R> pw0[G], run[G];
-#pragma unroll
-        for (int g = 0; g < G; g++) {
-          const RotRow<R>& rw = rot_r[min(q0 + g, nc - 1)];  // past the chunk end: a repeat, not used
-          const cplx<R> e1 = rw.e[0], e2 = rw.e[1];
-          cplx<R> pw = rot_start(rw, tid, m0), tot = cmk(R(0), R(0));
-          pw0[g] = pw;
-#pragma unroll
-          for (int jj = SPT - 1; jj >= 0; jj--) {
-            tot = cadd(tot, cscale(cmul(pw, e1), fc * e[jj] * e[jj]));
-            pw = cmul(pw, e2);
-          }
-          run[g] = cmk(wave_excl_prefix(tot.re), wave_excl_prefix(tot.im));
-          if (lane == 63) rot_ws[par][g][wave] = cadd(run[g], tot);
-        }
-        __syncthreads();
-        // pass 2: rho_m = -i dt (sum of F E^(2m+1) over earlier m); g_j(m) = c_j Im(rho conj(E^(2m)) W dt)
-#pragma unroll
-        for (int g = 0; g < G; g++) {
-          if (q0 + g < nc) {
-            const RotRow<R>& rw = rot_r[q0 + g];
-            const R cj = rw.c * ngp, wr = rw.wr, wi = rw.wi;
-            const cplx<R> e1 = rw.e[0], e2 = rw.e[1];
-            const cplx<R> wt = lane < nwv ? rot_ws[par][g][lane] : cmk(R(0), R(0));
-            cplx<R> r = cadd(run[g], cmk(lanes_below(wt.re, wv_u), lanes_below(wt.im, wv_u)));
-            cplx<R> pw = pw0[g];
-#pragma unroll
-            for (int jj = SPT - 1; jj >= 0; jj--) {
-              const cplx<R> rho = cmk(dtr * r.im, -dtr * r.re);
-              const cplx<R> qv = cmul(rho, cconj(pw));
-              gs[jj] += cj * (qv.re * wi + qv.im * wr);
-              r = cadd(r, cscale(cmul(pw, e1), fc * e[jj] * e[jj]));
-              pw = cmul(pw, e2);
-            }
-          }
-        }
-        par ^= 1;
-      }
-    }
-    R gtot = R(0);
-#pragma unroll
-    for (int jj = 0; jj < SPT; jj++) gtot += gs[jj];
-    R cum = wave_excl_prefix(gtot);
-    if (lane == 63) rot_wsr[wave] = cum + gtot;
-    __syncthreads();
-    for (int w2 = 0; w2 < wave; w2++) cum += rot_wsr[w2];
-#pragma unroll
-    for (int jj = SPT - 1; jj >= 0; jj--) {
-      cum += gs[jj];
-      dchi[jj] = cum;
-    }
+    for (int j = 0; j < SPT; j++) drive[j] = e[j] * e[j];
+    rot_chi<R, SPT>(drive, ngp, dchi, rot_tab, rot_states, rot_fc, dt, N, a0);
   }
 
   // 3. Kerr chi = ng chi3/nref E^2 (uppe.cl:132-151), P = chi E, J_p = ne E, packed as P + i J_p
@@ -1305,6 +1320,137 @@ __global__ __launch_bounds__(1024) void f_source(const cplx<T>* __restrict__ er,
       out = cmk(wk * pk.im, -wk * pk.re);
     }
     jr[p * nwp + k] = cscale(out, fv[i]);
+  }
+}
+
+// Paraxial envelope source (paraxial_kernel.py:140-184): one workgroup per strip of N = Nw complex samples,
+// max(min(N, 64), N/4) threads, thread t owning the SPT frequencies t + i nth and the SPT time samples
+// a0 .. a0 + SPT - 1, a0 = (nth - 1 - t) SPT (the density integral runs from the last sample backward). With
+// E(w) = i w A(w) (DtSpectral, sgn -1) and A(w) inverse-transformed side by side in LDS, the rate of |E|, the
+// depletion scan, the rotator driven by |E|^2 / 2 and the Kerr term 0.75 n_g chi3 |E|^2 give P = chi E and
+// Q = n_e (|A|^2/4 - 1) A; the reference's J = FFT(IFFT(-i w FFT(P)) + Q) is formed as -i w FFT(P) + FFT(Q).
+template <typename T, int SPT, bool DIAG, bool ROT>
+__global__ __launch_bounds__(1024) void f_source_env(const cplx<T>* __restrict__ ar, cplx<T>* __restrict__ jr,
+                                                     T* __restrict__ chi_out, T* __restrict__ ne_out,
+                                                     const cplx<T>* __restrict__ tw, const T* __restrict__ w,
+                                                     const double* __restrict__ ng, int nt, int lgn, int nwp,
+                                                     double redchi3, double dt, uppe_ionization ion, double log_pre,
+                                                     const RotRow<T>* __restrict__ rot_tab, int rot_states,
+                                                     double rot_fc) {
+  extern __shared__ char smem[];
+  cplx<T>* Y1 = reinterpret_cast<cplx<T>*>(smem);  // E, then P
+  cplx<T>* Y2 = Y1 + nt;                            // A, then Q
+  __shared__ T wsum[16];
+  const int N = nt;
+  const long p = blockIdx.x;
+  const int tid = threadIdx.x, nth = blockDim.x, lane = tid & 63, wave = tid >> 6;
+  const cplx<T>* X = ar + p * nwp;
+  cplx<T> xa[SPT];
+  T wv[SPT];
+#pragma unroll
+  for (int i = 0; i < SPT; i++) {
+    const int k = tid + i * nth;
+    xa[i] = X[k];
+    wv[i] = w[k];
+  }
+  // the N-point pass tables: in LDS for FP32, read from L2 for FP64
+  const cplx<T>* tws = tw;
+  if constexpr (sizeof(T) == 4) {
+    cplx<T>* tl = Y2 + N;
+    const int tlen = r8_table_len(lgn);
+    for (int k = tid; k < tlen; k += nth) tl[k] = tw[k];
+    tws = tl;
+  }
+#pragma unroll
+  for (int i = 0; i < SPT; i++) {
+    const int k = tid + i * nth;
+    Y1[lds_swz<T>(k)] = cmk(-wv[i] * xa[i].im, wv[i] * xa[i].re);
+    Y2[lds_swz<T>(k)] = xa[i];
+  }
+  __syncthreads();
+  lds_fft_r8(Y1, N, lgn, tws, +1, tid);
+  lds_fft_r8(Y2, N, lgn, tws, +1, tid);
+  const T invn = T(1) / T(N);
+  const int a0 = (nth - 1 - tid) * SPT;
+  cplx<T> e[SPT];
+#pragma unroll
+  for (int j = 0; j < SPT; j++) e[j] = cscale(Y1[lds_swz<T>(a0 + j)], invn);
+
+  // rate of the envelope modulus and the backward trapezoid with depletion, the last rate sample zeroed
+  // (ionization.cl:46-69, 99-155): I_n = dt (r_n / 2 + sum_{m > n} r_m), no time window in this model
+  using R = T;
+  const R ngp = (R)ng[p];
+  R ne[SPT];
+  if (ion.kind != UPPE_ION_NONE) {
+    const R dtr = (R)dt;
+    R r[SPT];
+    R tot = R(0);
+#pragma unroll
+    for (int j = 0; j < SPT; j++) {
+      const int n = a0 + j;
+      const double em = sqrt((double)e[j].re * e[j].re + (double)e[j].im * e[j].im);
+      const R v = (R)ion_rate<T>(em, ion, log_pre);
+      r[j] = (n >= N - 1) ? R(0) : v;
+      tot += r[j];
+    }
+    const R ex = wave_excl_prefix(tot);
+    if (lane == 63) wsum[wave] = ex + tot;
+    __syncthreads();
+    R later = ex;
+    for (int w2 = 0; w2 < wave; w2++) later += wsum[w2];
+#pragma unroll
+    for (int j = SPT - 1; j >= 0; j--) {
+      const R integral = (later + R(0.5) * r[j]) * dtr;
+      later += r[j];
+      if constexpr (sizeof(T) == 4)
+        ne[j] = ngp * -expm1f(-integral);
+      else
+        ne[j] = ngp * (1.0 - exp(-integral));
+    }
+  } else {
+#pragma unroll
+    for (int j = 0; j < SPT; j++) ne[j] = R(0);
+  }
+
+  // rotational susceptibility driven by the cycle average |E|^2 / 2 (rotations.cl:26-78, AddChiParaxial)
+  R dchi[SPT];
+#pragma unroll
+  for (int j = 0; j < SPT; j++) dchi[j] = R(0);
+  if (ROT) {
+    R drive[SPT];
+#pragma unroll
+    for (int j = 0; j < SPT; j++) drive[j] = R(0.5) * (e[j].re * e[j].re + e[j].im * e[j].im);
+    rot_chi<R, SPT>(drive, ngp, dchi, rot_tab, rot_states, rot_fc, dt, N, a0);
+  }
+
+  // chi = rotator + 0.75 n_g chi3 |E|^2 (paraxial.cl:112-131), P = chi E, Q = n_e (|A|^2/4 - 1) A (:99-110);
+  // A(t) is read from LDS only now, which keeps it out of registers through the scans
+  const T kc = (T)(0.75 * (double)ngp * redchi3);
+  cplx<T> a[SPT];
+#pragma unroll
+  for (int j = 0; j < SPT; j++) a[j] = cscale(Y2[lds_swz<T>(a0 + j)], invn);
+  __syncthreads();  // every thread has read its samples of E and A
+#pragma unroll
+  for (int j = 0; j < SPT; j++) {
+    const int n = a0 + j;
+    const T chi = (T)dchi[j] + kc * (e[j].re * e[j].re + e[j].im * e[j].im);
+    const T fq = (T)ne[j] * (T(0.25) * (a[j].re * a[j].re + a[j].im * a[j].im) - T(1));
+    Y1[lds_swz<T>(n)] = cscale(e[j], chi);
+    Y2[lds_swz<T>(n)] = cscale(a[j], fq);
+    if (DIAG) {
+      chi_out[p * N + n] = chi;
+      ne_out[p * N + n] = (T)ne[j];
+    }
+  }
+  __syncthreads();
+  lds_fft_r8(Y1, N, lgn, tws, -1, tid);
+  lds_fft_r8(Y2, N, lgn, tws, -1, tid);
+  // J = -i w FFT(P) + FFT(Q) (DtSpectral, sgn +1)
+#pragma unroll
+  for (int i = 0; i < SPT; i++) {
+    const int k = tid + i * nth;
+    const cplx<T> pk = Y1[lds_swz<T>(k)], qk = Y2[lds_swz<T>(k)];
+    jr[p * nwp + k] = cmk(wv[i] * pk.im + qk.re, -wv[i] * pk.re + qk.im);
   }
 }
 
@@ -1425,15 +1571,21 @@ __global__ void f_time_to_host(const T* src, double* dst, int nt, long P) {
 // medium tables: phs = Re kz + kg (FP64, as uppe.cl:69), kz as T; padding gets kz = 1, phs = 0
 template <typename T>
 __global__ void f_medium(const double* kz_host, const double* kg, double* phs, cplx<T>* kzc, int nw, long P,
-                         int nwp) {
+                         int nwp, const double* k0) {
   long i = blockIdx.x * (long)blockDim.x + threadIdx.x;
   if (i >= P * nwp) return;
   long p = i / nwp;
   int f = (int)(i % nwp);
   if (f < nw) {
     double kr = kz_host[2 * (f * P + p)], ki = kz_host[2 * (f * P + p) + 1];
-    phs[i] = kr + kg[f];
-    kzc[i] = cmk((T)kr, (T)ki);
+    if (k0) {
+      // envelope: phase rate Re kz, and k0(w) in the slot the UPPE right-hand side divides by (Re kz)
+      phs[i] = kr;
+      kzc[i] = cmk((T)k0[f], (T)ki);
+    } else {
+      phs[i] = kr + kg[f];
+      kzc[i] = cmk((T)kr, (T)ki);
+    }
   } else {
     phs[i] = 0.0;
     kzc[i] = cmk(T(1), T(0));
@@ -1672,7 +1824,7 @@ struct Fused {
     UPPE_HIP(hipMemcpy(kg, m->kg, sizeof(double) * h->nw, hipMemcpyHostToDevice));
     UPPE_HIP(hipMemcpy(s->stage, m->kz, sizeof(double) * 2 * h->npt, hipMemcpyHostToDevice));
     launchf(h, UPPE_KC_POINT, f_medium<T>, dim3(grid_for(P * s->nwp)), dim3(kBlock), 0, (const double*)s->stage,
-            (const double*)kg, s->phs, (C*)s->kzc, h->nw, P, s->nwp);
+            (const double*)kg, s->phs, (C*)s->kzc, h->nw, P, s->nwp, (const double*)nullptr);
     UPPE_HIP(hipStreamSynchronize(h->stream));
     UPPE_HIP(hipFree(kg));
     s->redchi3 = m->redchi3;
@@ -1685,6 +1837,43 @@ struct Fused {
     const int f1 = std::min(h->nw, std::max(last_nl + 1, 4));
     if (f1 != s->f1) {
       s->f1 = f1;
+      if (h->have_hankel) s->plan[1] = plan_hankel(h->nk, s->f1, s->n_cu);
+      const size_t cs = sizeof(C) * P * s->nwp;
+      UPPE_HIP(hipMemset(s->k1, 0, cs));
+      UPPE_HIP(hipMemset(s->k2, 0, cs));
+      UPPE_HIP(hipMemset(s->k3, 0, cs));
+      if (h->have_field) static_qmax(h, s);
+    }
+  }
+
+  // paraxial medium (paraxial_kernel.py:72-135): no nonlinear filter or conditioner (both one), Re kz as the
+  // phase rate, k0(w) as the divisor of the right-hand side
+  static void set_paraxial_medium(uppe_handle* h, FusedState* s, const uppe_paraxial_medium* m) {
+    const long P = h->P;
+    std::vector<T> w(s->nwp, T(0)), one(s->nwp, T(0));
+    for (int f = 0; f < h->nw; f++) {
+      w[f] = (T)m->w[f];
+      one[f] = T(1);
+    }
+    UPPE_HIP(hipMemcpy(s->w, w.data(), sizeof(T) * s->nwp, hipMemcpyHostToDevice));
+    UPPE_HIP(hipMemcpy(s->nlf, one.data(), sizeof(T) * s->nwp, hipMemcpyHostToDevice));
+    UPPE_HIP(hipMemcpy(s->cond, one.data(), sizeof(T) * s->nwp, hipMemcpyHostToDevice));
+    UPPE_HIP(hipMemcpy(s->ng, m->ngas, sizeof(double) * P, hipMemcpyHostToDevice));
+    double* k0 = nullptr;
+    UPPE_HIP(hipMalloc(&k0, sizeof(double) * h->nw));
+    UPPE_HIP(hipMemcpy(k0, m->k0, sizeof(double) * h->nw, hipMemcpyHostToDevice));
+    UPPE_HIP(hipMemcpy(s->stage, m->kz, sizeof(double) * 2 * h->npt, hipMemcpyHostToDevice));
+    launchf(h, UPPE_KC_POINT, f_medium<T>, dim3(grid_for(P * s->nwp)), dim3(kBlock), 0, (const double*)s->stage,
+            (const double*)nullptr, s->phs, (C*)s->kzc, h->nw, P, s->nwp, (const double*)k0);
+    UPPE_HIP(hipStreamSynchronize(h->stream));
+    UPPE_HIP(hipFree(k0));
+    s->redchi3 = m->redchi3;
+    s->dt = m->dt;
+    s->ion = m->ion;
+    s->max_ki = 0.0;
+    for (long i = 0; i < h->npt; i++) s->max_ki = std::max(s->max_ki, m->kz[2 * i + 1]);
+    if (h->nw != s->f1) {
+      s->f1 = h->nw;
       if (h->have_hankel) s->plan[1] = plan_hankel(h->nk, s->f1, s->n_cu);
       const size_t cs = sizeof(C) * P * s->nwp;
       UPPE_HIP(hipMemset(s->k1, 0, cs));
@@ -1724,6 +1913,23 @@ struct Fused {
     (void)st;
     const bool rot = h->rot_states > 2;
     const dim3 grid((unsigned)h->P), block(nth);
+    if (h->paraxial) {
+      // two N-point complex strips (E and A), plus the N-point tables in FP32
+      const size_t shm_env = sizeof(C) * (sizeof(T) == 4 ? 3 * N : 2 * N);
+#define UPPE_SRC_ENV(S)                                                                                          \
+  if (spt == S) {                                                                                                \
+    auto kern = rot ? (diag ? f_source_env<T, S, true, true> : f_source_env<T, S, false, true>)                 \
+                    : (diag ? f_source_env<T, S, true, false> : f_source_env<T, S, false, false>);              \
+    launchf(h, UPPE_KC_SOURCE, kern, grid, block, shm_env, er, jr, diag ? (T*)s->chi_t : (T*)nullptr,             \
+            diag ? (T*)s->ne_t : (T*)nullptr, (const C*)s->tw, (const T*)s->w, (const double*)s->ng, N, lgn,      \
+            s->nwp, s->redchi3, s->dt, s->ion, log_pre, (const RotRow<T>*)s->rot_tab, h->rot_states,              \
+            0.5 * h->rot_dalpha / h->rot_hbar);                                                                  \
+    return;                                                                                                      \
+  }
+      UPPE_SRC_ENV(1) UPPE_SRC_ENV(2) UPPE_SRC_ENV(4)
+#undef UPPE_SRC_ENV
+      throw Error{UPPE_ERR_UNSUPPORTED, "fused envelope source kernel: unsupported Nt"};
+    }
 #define UPPE_SRC(S)                                                                                              \
   if (spt == S) {                                                                                                \
     auto kern = rot ? (diag ? f_source<T, S, true, true> : f_source<T, S, false, true>)                         \
@@ -1775,7 +1981,10 @@ struct Fused {
   static void step_mfma(uppe_handle* h, FusedState* s) {
     const C* kp[5] = {nullptr, nullptr, (const C*)s->k1, (const C*)s->k2, (const C*)s->k3};
     for (int stage = 1; stage <= 4; stage++) {
-      hankel_mfma<CM, PRO_STAGE, EPI_NONE>(h, s, 0, (const C*)s->q, (C*)s->er, kp[stage], stage, 0.0, s->st);
+      if (h->paraxial)
+        hankel_mfma<CM, PRO_STAGE_ENV, EPI_NONE>(h, s, 0, (const C*)s->q, (C*)s->er, kp[stage], stage, 0.0, s->st);
+      else
+        hankel_mfma<CM, PRO_STAGE, EPI_NONE>(h, s, 0, (const C*)s->q, (C*)s->er, kp[stage], stage, 0.0, s->st);
       source(h, s, (const C*)s->er, (C*)s->jr, false, s->st);
       hankel_mfma<CM, PRO_NONE, EPI_STAGE>(h, s, 1, (const C*)s->jr, (C*)nullptr, nullptr, stage, 0.0, s->st);
       if (stage == 1)
@@ -1809,7 +2018,10 @@ struct Fused {
     const C* kp[5] = {nullptr, nullptr, (const C*)s->k1, (const C*)s->k2, (const C*)s->k3};
     C *q = (C*)s->q, *er = (C*)s->er, *jr = (C*)s->jr;
     for (int stage = 1; stage <= 4; stage++) {
-      tpass<PRO_STAGE, EPI_NONE>(h, s, 2, q, er, h->nw, +1, kp[stage], stage, 0.0, s->st);
+      if (h->paraxial)
+        tpass<PRO_STAGE_ENV, EPI_NONE>(h, s, 2, q, er, h->nw, +1, kp[stage], stage, 0.0, s->st);
+      else
+        tpass<PRO_STAGE, EPI_NONE>(h, s, 2, q, er, h->nw, +1, kp[stage], stage, 0.0, s->st);
       tpass<PRO_NONE, EPI_NONE>(h, s, 1, er, er, h->nw, +1, nullptr, stage, 0.0, s->st);
       source(h, s, er, jr, false, s->st);
       tpass<PRO_NONE, EPI_NONE>(h, s, 1, jr, jr, s->f1, -1, nullptr, stage, 0.0, s->st);
@@ -1839,7 +2051,8 @@ struct Fused {
     dim3 g_k2r(2 * s->nwp / BN, (h->n1 + BM - 1) / BM), g_r2k((2 * s->f1 + BN - 1) / BN, (h->nk + BM - 1) / BM);
     C* kp[5] = {nullptr, nullptr, (C*)s->k1, (C*)s->k2, (C*)s->k3};
     for (int stage = 1; stage <= 4; stage++) {
-      launchf(h, UPPE_KC_HANKEL, f_gemm<T, PRO_STAGE, EPI_NONE>, g_k2r, dim3(256), 0, (const T*)s->Hb,
+      launchf(h, UPPE_KC_HANKEL, h->paraxial ? f_gemm<T, PRO_STAGE_ENV, EPI_NONE> : f_gemm<T, PRO_STAGE, EPI_NONE>,
+              g_k2r, dim3(256), 0, (const T*)s->Hb,
               (const C*)s->q, (C*)s->er, h->n1, h->nk, s->nwp, (const double*)s->phs, (const C*)s->kzc,
               (const T*)s->cond, (C*)nullptr, (const C*)kp[stage], (C*)nullptr, (C*)nullptr, (C*)nullptr, s->st,
               s->qmax, s->qmax_next, stage, 0.0);
@@ -1957,9 +2170,14 @@ struct Fused {
               (const T*)s->cond, (C*)nullptr, (const C*)nullptr, (C*)nullptr, (C*)nullptr, (C*)nullptr,
               (StepState*)nullptr, (unsigned long long*)nullptr, (unsigned long long*)nullptr, 0, z_total);
     }
-    UPPE_HIP(hipMemcpyAsync(s->k1, s->er, sizeof(C) * n, hipMemcpyDeviceToDevice, h->stream));
-    launchf(h, UPPE_KC_POINT, f_vp_to_e<T>, dim3(grid_for(n)), dim3(kBlock), 0, (C*)s->k1, (const C*)s->kzc, n);
-    source(h, s, (const C*)s->k1, (C*)s->jr, true, nullptr);
+    if (h->paraxial) {
+      // the envelope source takes A itself (paraxial_kernel.py:243-245)
+      source(h, s, (const C*)s->er, (C*)s->jr, true, nullptr);
+    } else {
+      UPPE_HIP(hipMemcpyAsync(s->k1, s->er, sizeof(C) * n, hipMemcpyDeviceToDevice, h->stream));
+      launchf(h, UPPE_KC_POINT, f_vp_to_e<T>, dim3(grid_for(n)), dim3(kBlock), 0, (C*)s->k1, (const C*)s->kzc, n);
+      source(h, s, (const C*)s->k1, (C*)s->jr, true, nullptr);
+    }
     auto out_c = [&](const void* src, double* dst) {
       launchf(h, UPPE_KC_POINT, f_to_host<T>, dim3(grid_for(P * h->nw)), dim3(kBlock), 0, (const C*)src, s->stage,
               h->nw, P, s->nwp);
@@ -2082,6 +2300,13 @@ void fused_set_medium(uppe_handle* h, const uppe_medium* m) {
     Fused<double>::set_medium(h, h->fused, m);
   else
     Fused<float>::set_medium(h, h->fused, m);
+}
+
+void fused_set_paraxial_medium(uppe_handle* h, const uppe_paraxial_medium* m) {
+  if (is_fp64(h))
+    Fused<double>::set_paraxial_medium(h, h->fused, m);
+  else
+    Fused<float>::set_paraxial_medium(h, h->fused, m);
 }
 
 void fused_set_field(uppe_handle* h, const double* a_r) {

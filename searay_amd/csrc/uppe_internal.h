@@ -273,6 +273,7 @@ struct FusedState;
 struct uppe_handle {
   uppe_desc d;
   int nw = 0, n1 = 0, n2 = 0, nk = 0, nm = 0, nt = 0;
+  bool paraxial = false;  // UPPE_MODEL_PARAXIAL: complex envelope strips, Nt = Nw
   long P = 0;     // transverse points n1*n2
   long npt = 0;   // nw * P
   long ntt = 0;   // nt * P
@@ -321,6 +322,7 @@ void plain_create(uppe_handle* h);
 void plain_destroy(uppe_handle* h);
 void plain_set_hankel(uppe_handle* h);
 void plain_set_medium(uppe_handle* h, const uppe_medium* m);
+void plain_set_paraxial_medium(uppe_handle* h, const uppe_paraxial_medium* m);
 void plain_set_rotator(uppe_handle* h);
 void plain_set_field(uppe_handle* h, const double* a_r);
 int plain_advance(uppe_handle* h, const uppe_step_control* c);
@@ -333,6 +335,7 @@ void fused_destroy(uppe_handle* h);
 void fused_set_hankel(uppe_handle* h);
 void fused_set_rotator(uppe_handle* h);
 void fused_set_medium(uppe_handle* h, const uppe_medium* m);
+void fused_set_paraxial_medium(uppe_handle* h, const uppe_paraxial_medium* m);
 void fused_set_field(uppe_handle* h, const double* a_r);
 int fused_advance(uppe_handle* h, const uppe_step_control* c);
 void fused_finish(uppe_handle* h, double z_total, double* a_r, double* j_r, double* chi_t, double* ne_t);

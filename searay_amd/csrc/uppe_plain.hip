@@ -4,7 +4,8 @@
 // (Nw, N1, N2), time (Nt, N1, N2)), FP64 throughout, with the step size reduced on the host every step
 // (modules/uppe_kernel.py:188-232). The strip FFTs are simple LDS radix-2 transforms (one workgroup per
 // strip) in place of the reference's serial strip FFT; the radial transforms are the naive contraction
-// (kernels/fft.cl:443-477). This path is the baseline the fused path is measured against.
+// (kernels/fft.cl:443-477). This path is the baseline the fused path is measured against. The paraxial
+// envelope model (modules/paraxial_kernel.py, kernels/paraxial.cl) runs the same way on complex strips.
 #include "uppe_internal.h"
 
 #include <algorithm>
@@ -24,6 +25,8 @@ struct PlainState {
          *hscratch = nullptr, *dzb = nullptr;
   double *Et = nullptr, *ne = nullptr, *chi = nullptr, *Jt = nullptr;
   double *rot_tjj = nullptr, *rot_wdt = nullptr;  // rotator tables (device), when h->rot_states > 0
+  // paraxial model: k0(w) and complex time strips of E, A and the current
+  double *k0 = nullptr, *Etc = nullptr, *Atc = nullptr, *Jtc = nullptr;
   double* part = nullptr;       // per-block reduction partials (device)
   double* part_host = nullptr;  // pinned host copy
   int nblk_red = 0;
@@ -60,6 +63,120 @@ __global__ void plain_add_chi_rotations(double* chi, const double* Et, const dou
       ei = ni;
     }
   }
+}
+
+// ---------------------------------------------------------------------------------------------------
+// paraxial envelope kernels (kernels/paraxial.cl, ionization.cl:46-69,99-125, rotations.cl:26-78)
+// ---------------------------------------------------------------------------------------------------
+
+// A = exp(i kz z) q with complex kz (paraxial.cl:60-77)
+__global__ void plain_propagate_linear_env(double* q, const double* kz, double z, long npt) {
+  long idx = blockIdx.x * (long)blockDim.x + threadIdx.x;
+  if (idx >= npt) return;
+  double kr = kz[2 * idx], ki = kz[2 * idx + 1];
+  double ar = -ki * z, ai = kr * z;
+  double er = cos(ai) * exp(ar), ei = sin(ai) * exp(ar);
+  double qr = q[2 * idx], qi = q[2 * idx + 1];
+  q[2 * idx] = er * qr - ei * qi;
+  q[2 * idx + 1] = er * qi + ei * qr;
+}
+
+// S = (i/2) exp(-i kz z) J / k0(w) (paraxial.cl:79-97)
+__global__ void plain_current_to_rhs_env(double* J, const double* kz, const double* k0, double z, long npt, long P) {
+  long idx = blockIdx.x * (long)blockDim.x + threadIdx.x;
+  if (idx >= npt) return;
+  double kr = kz[2 * idx], ki = kz[2 * idx + 1];
+  double ar = ki * z, ai = -kr * z;
+  double er = cos(ai) * exp(ar), ei = sin(ai) * exp(ar);
+  double jr = J[2 * idx], ji = J[2 * idx + 1];
+  double tr = er * jr - ei * ji, ti = er * ji + ei * jr;
+  double k = k0[idx / P];
+  J[2 * idx] = (0.0 * tr - 0.5 * ti) / k;
+  J[2 * idx + 1] = (0.0 * ti + 0.5 * tr) / k;
+}
+
+// DtSpectral with its sign: A <- -sgn i w A (fft.cl:282-299)
+__global__ void plain_dt_spectral_sgn(double* A, const double* w, double sgn, long npt, long P) {
+  long idx = blockIdx.x * (long)blockDim.x + threadIdx.x;
+  if (idx >= npt) return;
+  double w0 = w[idx / P];
+  double ar = A[2 * idx], ai = A[2 * idx + 1];
+  A[2 * idx] = sgn * w0 * ai;
+  A[2 * idx + 1] = -sgn * w0 * ar;
+}
+
+// rates of the envelope modulus (EnvelopeRate, EnvelopeRateSeries)
+__global__ void plain_rate_env(double* rate, const double* E, uppe_ionization p, long ntt) {
+  long idx = blockIdx.x * (long)blockDim.x + threadIdx.x;
+  if (idx >= ntt) return;
+  double er = E[2 * idx], ei = E[2 * idx + 1];
+  double e = sqrt(er * er + ei * ei);
+  if (p.kind == UPPE_ION_EXPLICIT) {
+    double eau = p.e_conv * e + p.e_cutoff;
+    rate[idx] = p.rate_conv * p.c_pre * pow(eau, p.c_pow) * exp(p.c_exp / eau);
+  } else {
+    double lg = log(p.e_conv * e + p.e_cutoff);
+    rate[idx] = p.rate_conv * exp(p.c[0] + p.c[1] * lg + p.c[2] * lg * lg + p.c[3] * pow(lg, 3.0) +
+                                  p.c[4] * pow(lg, 4.0) + p.c[5] * pow(lg, 5.0));
+  }
+}
+
+// chi += 0.75 ng chi3 |E|^2 (paraxial.cl:112-131)
+__global__ void plain_kerr_chi_env(double* chi, const double* E, const double* ng, double chi3, long ntt, long P) {
+  long idx = blockIdx.x * (long)blockDim.x + threadIdx.x;
+  if (idx >= ntt) return;
+  double er = E[2 * idx], ei = E[2 * idx + 1];
+  chi[idx] += 0.75 * ng[idx % P] * chi3 * (er * er + ei * ei);
+}
+
+// chi += rotational susceptibility driven by |E|^2 / 2 (rotations.cl:26-78, AddChiParaxial): one thread per
+// strip, serially over the states and over time from the last sample
+__global__ void plain_add_chi_rotations_env(double* chi, const double* E, const double* ng, const double* tjj,
+                                            const double* wdt, double dt, double dalpha, double hbar, int steps,
+                                            int states, long P) {
+  const long xy = blockIdx.x * (long)blockDim.x + threadIdx.x;
+  if (xy >= P) return;
+  for (int j = 2; j < states; j++) {
+    const double wr = wdt[2 * j], wi = wdt[2 * j + 1];
+    const double Sjj = 2.0 * j * (j - 1) / (15 * (2 * j - 1));
+    double rr = 0.0, ri = 0.0, er = 1.0, ei = 0.0, cum = 0.0;
+    const double hr = cos(0.5 * wr) * exp(-0.5 * wi), hi = sin(0.5 * wr) * exp(-0.5 * wi);
+    for (int k = steps - 1; k >= 0; --k) {
+      const long idx = (long)k * P + xy;
+      const double ar = rr * er + ri * ei, ai = -rr * ei + ri * er;
+      cum += 2 * (tjj[j] * Sjj * (ar * wi + ai * wr)) * ng[xy] * dalpha;
+      chi[idx] += cum;
+      const double Er = E[2 * idx], Ei = E[2 * idx + 1];
+      const double F = 0.5 * (dalpha / hbar) * 0.5 * (Er * Er + Ei * Ei);
+      double nr = er * hr - ei * hi, ni = er * hi + ei * hr;
+      er = nr;
+      ei = ni;
+      rr -= dt * F * (-ei);
+      ri -= dt * F * er;
+      nr = er * hr - ei * hi;
+      ni = er * hi + ei * hr;
+      er = nr;
+      ei = ni;
+    }
+  }
+}
+
+// P = chi E, complex (paraxial.cl:133-147)
+__global__ void plain_polarization_env(double* J, const double* chi, const double* E, long ntt) {
+  long idx = blockIdx.x * (long)blockDim.x + threadIdx.x;
+  if (idx >= ntt) return;
+  J[2 * idx] = chi[idx] * E[2 * idx];
+  J[2 * idx + 1] = chi[idx] * E[2 * idx + 1];
+}
+
+// J += ne (|A|^2/4 - 1) A (paraxial.cl:99-110)
+__global__ void plain_plasma_current_env(double* J, const double* A, const double* ne, long ntt) {
+  long idx = blockIdx.x * (long)blockDim.x + threadIdx.x;
+  if (idx >= ntt) return;
+  double ar = A[2 * idx], ai = A[2 * idx + 1];
+  double f = ne[idx] * (0.25 * (ar * ar + ai * ai) - 1.0);
+  J[2 * idx] += f * ar;
+  J[2 * idx + 1] += f * ai;
 }
 
 // ---------------------------------------------------------------------------------------------------
@@ -399,6 +516,20 @@ __global__ void plain_axis_fft(double* a, int n, int lg, int axis, int n1, int n
   }
 }
 
+// complex FFT of every time strip of (Nt, P): numpy sign, inverse scaled by 1/n (FFT, IFFT: fft.cl:199-227)
+__global__ void plain_strip_fft(double* a, int n, int lg, long P, const c64* tw, int inverse) {
+  extern __shared__ c64 z[];
+  const long t = blockIdx.x;
+  for (int i = threadIdx.x; i < n; i += blockDim.x) z[i] = cmk(a[2 * (i * P + t)], a[2 * (i * P + t) + 1]);
+  __syncthreads();
+  lds_fft_radix2(z, n, lg, tw, n, inverse ? +1 : -1);
+  const double s = inverse ? 1.0 / (double)n : 1.0;
+  for (int i = threadIdx.x; i < n; i += blockDim.x) {
+    a[2 * (i * P + t)] = z[i].re * s;
+    a[2 * (i * P + t) + 1] = z[i].im * s;
+  }
+}
+
 // ---------------------------------------------------------------------------------------------------
 // host driver
 // ---------------------------------------------------------------------------------------------------
@@ -439,6 +570,8 @@ void plain_set_rotator(uppe_handle* h) {
 
 void plain_create(uppe_handle* h) {
   if (h->nt > kMaxNtPlain) throw Error{UPPE_ERR_UNSUPPORTED, "plain path supports Nt <= 16384"};
+  if (h->paraxial && h->nt > kMaxNtPlain / 2)
+    throw Error{UPPE_ERR_UNSUPPORTED, "plain path supports Nw <= 8192 for the paraxial envelope"};
   auto* s = new PlainState();
   h->plain = s;
   const size_t cs = sizeof(double) * 2 * h->npt, ts = sizeof(double) * h->ntt;
@@ -462,6 +595,14 @@ void plain_create(uppe_handle* h) {
   s->nblk_red = (int)std::min<long>(1024, grid_for(h->npt));
   UPPE_HIP(hipMalloc(&s->part, sizeof(double) * 3 * s->nblk_red));
   UPPE_HIP(hipHostMalloc(&s->part_host, sizeof(double) * 3 * s->nblk_red, hipHostMallocDefault));
+  if (h->paraxial) {
+    UPPE_HIP(hipMalloc(&s->k0, sizeof(double) * h->nw));
+    double** ebufs[] = {&s->Etc, &s->Atc, &s->Jtc};
+    for (auto p : ebufs) {
+      UPPE_HIP(hipMalloc(p, 2 * ts));
+      UPPE_HIP(hipMemset(*p, 0, 2 * ts));
+    }
+  }
   s->tw_t = make_twiddles(h->nt);
   s->tw1 = make_twiddles(h->n1);
   s->tw2 = make_twiddles(h->n2);
@@ -476,7 +617,7 @@ void plain_destroy(uppe_handle* h) {
   if (!s) return;
   void* ptrs[] = {s->w, s->kz, s->kg, s->ng, s->nlf, s->cond, s->H, s->lam, s->tw_t, s->tw1, s->tw2, s->qw,
                   s->qi, s->qf, s->Ew, s->Jw, s->scratchw, s->hscratch, s->dzb, s->Et, s->ne, s->chi, s->Jt,
-                  s->part, s->rot_tjj, s->rot_wdt};
+                  s->part, s->rot_tjj, s->rot_wdt, s->k0, s->Etc, s->Atc, s->Jtc};
   for (void* p : ptrs)
     if (p) (void)hipFree(p);
   if (s->part_host) (void)hipHostFree(s->part_host);
@@ -498,6 +639,17 @@ void plain_set_medium(uppe_handle* h, const uppe_medium* m) {
   UPPE_HIP(hipMemcpy(s->ng, m->ngas, sizeof(double) * h->P, hipMemcpyHostToDevice));
   UPPE_HIP(hipMemcpy(s->nlf, m->nl_filter, sizeof(double) * h->nw, hipMemcpyHostToDevice));
   UPPE_HIP(hipMemcpy(s->cond, m->cond_filter, sizeof(double) * h->nw, hipMemcpyHostToDevice));
+  s->redchi3 = m->redchi3;
+  s->dt = m->dt;
+  s->ion = m->ion;
+}
+
+void plain_set_paraxial_medium(uppe_handle* h, const uppe_paraxial_medium* m) {
+  PlainState* s = h->plain;
+  UPPE_HIP(hipMemcpy(s->w, m->w, sizeof(double) * h->nw, hipMemcpyHostToDevice));
+  UPPE_HIP(hipMemcpy(s->kz, m->kz, sizeof(double) * 2 * h->npt, hipMemcpyHostToDevice));
+  UPPE_HIP(hipMemcpy(s->k0, m->k0, sizeof(double) * h->nw, hipMemcpyHostToDevice));
+  UPPE_HIP(hipMemcpy(s->ng, m->ngas, sizeof(double) * h->P, hipMemcpyHostToDevice));
   s->redchi3 = m->redchi3;
   s->dt = m->dt;
   s->ion = m->ion;
@@ -587,6 +739,45 @@ static void update_current(uppe_handle* h) {
   launch(h, UPPE_KC_POINT, plain_filter, grid_for(npt), kBlock, 0, s->Jw, (const double*)s->nlf, npt, P);
 }
 
+// paraxial update_current (paraxial_kernel.py:140-184): Ew holds A(w) in r-space; Jw receives J(w)
+static void update_current_env(uppe_handle* h) {
+  PlainState* s = h->plain;
+  const long ntt = h->ntt, P = h->P;
+  const int lg = ilog2(h->nt);
+  const size_t shm = sizeof(c64) * h->nt;
+  auto strips = [&](double* a, int inverse) {
+    launch(h, UPPE_KC_SOURCE, plain_strip_fft, (unsigned)P, kBlock, shm, a, h->nt, lg, P, (const c64*)s->tw_t,
+           inverse);
+  };
+  UPPE_HIP(hipMemcpyAsync(s->Etc, s->Ew, sizeof(double) * 2 * ntt, hipMemcpyDeviceToDevice, h->stream));
+  UPPE_HIP(hipMemcpyAsync(s->Atc, s->Ew, sizeof(double) * 2 * ntt, hipMemcpyDeviceToDevice, h->stream));
+  UPPE_HIP(hipMemsetAsync(s->ne, 0, sizeof(double) * ntt, h->stream));
+  UPPE_HIP(hipMemsetAsync(s->chi, 0, sizeof(double) * ntt, h->stream));
+  launch(h, UPPE_KC_POINT, plain_dt_spectral_sgn, grid_for(ntt), kBlock, 0, s->Etc, (const double*)s->w, -1.0, ntt, P);
+  strips(s->Etc, 1);
+  strips(s->Atc, 1);
+  if (s->ion.kind != UPPE_ION_NONE) {
+    launch(h, UPPE_KC_POINT, plain_rate_env, grid_for(ntt), kBlock, 0, s->ne, (const double*)s->Etc, s->ion, ntt);
+    launch(h, UPPE_KC_SOURCE, plain_plasma_density, grid_for(P, 64), 64, 0, s->ne, (const double*)s->ng, s->dt,
+           h->nt, P);
+  }
+  if (h->rot_states > 2)
+    launch(h, UPPE_KC_SOURCE, plain_add_chi_rotations_env, grid_for(P, 64), 64, 0, s->chi, (const double*)s->Etc,
+           (const double*)s->ng, (const double*)s->rot_tjj, (const double*)s->rot_wdt, s->dt, h->rot_dalpha,
+           h->rot_hbar, h->nt, h->rot_states, P);
+  launch(h, UPPE_KC_POINT, plain_kerr_chi_env, grid_for(ntt), kBlock, 0, s->chi, (const double*)s->Etc,
+         (const double*)s->ng, s->redchi3, ntt, P);
+  launch(h, UPPE_KC_POINT, plain_polarization_env, grid_for(ntt), kBlock, 0, s->Jtc, (const double*)s->chi,
+         (const double*)s->Etc, ntt);
+  strips(s->Jtc, 0);
+  launch(h, UPPE_KC_POINT, plain_dt_spectral_sgn, grid_for(ntt), kBlock, 0, s->Jtc, (const double*)s->w, 1.0, ntt, P);
+  strips(s->Jtc, 1);
+  launch(h, UPPE_KC_POINT, plain_plasma_current_env, grid_for(ntt), kBlock, 0, s->Jtc, (const double*)s->Atc,
+         (const double*)s->ne, ntt);
+  strips(s->Jtc, 0);
+  UPPE_HIP(hipMemcpyAsync(s->Jw, s->Jtc, sizeof(double) * 2 * ntt, hipMemcpyDeviceToDevice, h->stream));
+}
+
 // host-side final stage of the reductions: sum, max, min
 static void reduce_host(uppe_handle* h, const double* v, double* sum, double* mx, double* mn) {
   PlainState* s = h->plain;
@@ -614,14 +805,25 @@ static double load_source(uppe_handle* h, double zphase, bool return_dz, double 
   PlainState* s = h->plain;
   const long npt = h->npt, P = h->P;
   UPPE_HIP(hipMemcpyAsync(s->Ew, s->qw, sizeof(double) * 2 * npt, hipMemcpyDeviceToDevice, h->stream));
-  launch(h, UPPE_KC_POINT, plain_propagate_linear, grid_for(npt), kBlock, 0, s->Ew, (const double*)s->kz,
-         (const double*)s->kg, zphase, npt, P);
-  launch(h, UPPE_KC_POINT, plain_vp_to_e, grid_for(npt), kBlock, 0, s->Ew, (const double*)s->kz, npt);
-  rspacex(h, s->Ew);
-  update_current(h);
-  kspacex(h, s->Jw);
-  launch(h, UPPE_KC_POINT, plain_current_to_rhs, grid_for(npt), kBlock, 0, s->Jw, (const double*)s->kz,
-         (const double*)s->kg, zphase, npt, P);
+  if (h->paraxial) {
+    // paraxial_kernel.py:186-208
+    launch(h, UPPE_KC_POINT, plain_propagate_linear_env, grid_for(npt), kBlock, 0, s->Ew, (const double*)s->kz,
+           zphase, npt);
+    rspacex(h, s->Ew);
+    update_current_env(h);
+    kspacex(h, s->Jw);
+    launch(h, UPPE_KC_POINT, plain_current_to_rhs_env, grid_for(npt), kBlock, 0, s->Jw, (const double*)s->kz,
+           (const double*)s->k0, zphase, npt, P);
+  } else {
+    launch(h, UPPE_KC_POINT, plain_propagate_linear, grid_for(npt), kBlock, 0, s->Ew, (const double*)s->kz,
+           (const double*)s->kg, zphase, npt, P);
+    launch(h, UPPE_KC_POINT, plain_vp_to_e, grid_for(npt), kBlock, 0, s->Ew, (const double*)s->kz, npt);
+    rspacex(h, s->Ew);
+    update_current(h);
+    kspacex(h, s->Jw);
+    launch(h, UPPE_KC_POINT, plain_current_to_rhs, grid_for(npt), kBlock, 0, s->Jw, (const double*)s->kz,
+           (const double*)s->kg, zphase, npt, P);
+  }
   h->evals++;
   if (!return_dz) return 0.0;
   launch(h, UPPE_KC_POINT, plain_modulus, grid_for(npt), kBlock, 0, (const double*)s->qw, s->dzb, npt);
@@ -639,6 +841,7 @@ static double load_source(uppe_handle* h, double zphase, bool return_dz, double 
 }
 
 static void condition_field(uppe_handle* h) {
+  if (h->paraxial) return;  // the paraxial stepper has no conditioner
   PlainState* s = h->plain;
   launch(h, UPPE_KC_POINT, plain_filter, grid_for(h->npt), kBlock, 0, s->qw, (const double*)s->cond, h->npt, h->P);
 }
@@ -699,6 +902,20 @@ int plain_advance(uppe_handle* h, const uppe_step_control* c) {
 void plain_finish(uppe_handle* h, double z_total, double* a_r, double* j_r, double* chi_t, double* ne_t) {
   PlainState* s = h->plain;
   const long npt = h->npt;
+  if (h->paraxial) {
+    // paraxial_kernel.py:222-249
+    UPPE_HIP(hipMemcpyAsync(s->Ew, s->qw, sizeof(double) * 2 * npt, hipMemcpyDeviceToDevice, h->stream));
+    launch(h, UPPE_KC_POINT, plain_propagate_linear_env, grid_for(npt), kBlock, 0, s->Ew, (const double*)s->kz,
+           z_total, npt);
+    rspacex(h, s->Ew);
+    update_current_env(h);
+    UPPE_HIP(hipStreamSynchronize(h->stream));
+    if (a_r) UPPE_HIP(hipMemcpy(a_r, s->Ew, sizeof(double) * 2 * npt, hipMemcpyDeviceToHost));
+    if (j_r) UPPE_HIP(hipMemcpy(j_r, s->Jw, sizeof(double) * 2 * npt, hipMemcpyDeviceToHost));
+    if (chi_t) UPPE_HIP(hipMemcpy(chi_t, s->chi, sizeof(double) * h->ntt, hipMemcpyDeviceToHost));
+    if (ne_t) UPPE_HIP(hipMemcpy(ne_t, s->ne, sizeof(double) * h->ntt, hipMemcpyDeviceToHost));
+    return;
+  }
   launch(h, UPPE_KC_POINT, plain_propagate_linear, grid_for(npt), kBlock, 0, s->qw, (const double*)s->kz,
          (const double*)s->kg, z_total, npt, h->P);
   rspacex(h, s->qw);

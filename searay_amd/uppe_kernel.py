@@ -95,8 +95,8 @@ def hip_stepper_factory(precision='fp64', device=0):
     from . import _native
     prec = _abi.PRECISIONS[precision] if isinstance(precision, str) else int(precision)
 
-    def make(geometry, nw, n1, n2, nk):
-        return _native.HipStepper(geometry, nw, n1, n2, nk, prec, device)
+    def make(geometry, nw, n1, n2, nk, model=_abi.MODEL_UPPE):
+        return _native.HipStepper(geometry, nw, n1, n2, nk, prec, device, model)
     return make
 
 

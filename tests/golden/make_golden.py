@@ -4,7 +4,7 @@ Run from the repository root after building the oracle (python oracle/build.py):
     python tests/golden/make_golden.py
 
 Each .npz holds, for the last diagnostic plane of the case, the r-space field A and the listed diagnostics
-(complex128, the reference layout (Nw, N1, N2)), the dz of every step taken (record), and the RHS evaluation
+(complex128, the reference layout (Nw, N1, N2); the paraxial cases are envelopes, frequencies sorted), the dz of every step taken (record), and the RHS evaluation
 count per plane. The fixtures freeze the oracle's results: the CPU tests check that the oracle still
 reproduces them, the GPU tests compare the HIP library with them.
 '''
@@ -34,6 +34,16 @@ CASES = {
     # 3-D Cartesian filamentation (super-Gaussian with amplitude noise, Kerr + O2), Nw = 33, 16 x 16, 3 fixed
     # steps over 1 mm: A
     'cartesian_fil_33x16x16': (lambda: configs.cartesian_filamentation(scale=16, steps=3, length='1 mm'), ()),
+    # paraxial envelope: the focused air filament (Kerr + cycle-averaged O2 rate), Nw = 64, Nr = 64 with 32
+    # modes, 50 mJ, one plane of 4 mm with adaptive steps: A, J, chiNL, ne
+    'paraxial_fil_64x64': (lambda: configs.paraxial_air_filament(scale=4, planes=2, length='4 mm', power=50.0),
+                           ('J', 'chi', 'ne')),
+    # the same with the 32-state N2 rotator: A, chiNL
+    'paraxial_raman_64x64': (lambda: configs.paraxial_air_filament(scale=4, planes=2, length='4 mm', power=50.0,
+                                                                   rotator_states=32), ('chi',)),
+    # paraxial envelope in 3-D Cartesian, Nw = 16, 16 x 16, 200 mJ: A, ne
+    'paraxial_cart_16x16x16': (lambda: configs.paraxial_air_filament(scale=16, planes=2, length='4 mm', power=200.0,
+                                                                     coords='cartesian'), ('ne',)),
 }
 
 

@@ -10,7 +10,7 @@ import numpy as np
 import pytest
 
 from conftest import rel_l2
-from searay_amd import configs, uppe_kernel
+from searay_amd import configs, paraxial_kernel, uppe_kernel
 
 pytestmark = pytest.mark.gpu
 
@@ -24,8 +24,12 @@ def _say(msg):
     sys.stdout.flush()
 
 
+def _module(vol):
+    return paraxial_kernel if vol.get('propagator', 'uppe') == 'paraxial' else uppe_kernel
+
+
 def _run(vol, precision, record=None):
-    return uppe_kernel.track_with(vol, uppe_kernel.hip_stepper_factory(precision), record)
+    return _module(vol).track_with(vol, uppe_kernel.hip_stepper_factory(precision), record)
 
 
 @pytest.mark.parametrize('coords', ['cylindrical', 'cartesian'])
@@ -138,18 +142,25 @@ def test_golden_fp32_replayed(name, precision):
     '''FP32 / BF16x3 on the fixture's dz schedule: field within 1e-5, electron density within 1e-4.'''
     make, fx = golden_cases.load(name)
     _say('golden replay %s %s' % (name, precision))
-    rs = uppe_kernel.RunSetup(make())
+    vol = make()
+    paraxial = vol.get('propagator', 'uppe') == 'paraxial'
+    rs = _module(vol).RunSetup(vol)
     assert rs.diagnostic_steps == 1
     st = rs.make_stepper(uppe_kernel.hip_stepper_factory(precision))
     zi, zf, L = rs.plane(0)
-    st.set_field(rs.A0)
-    n, _ = st.advance(zi, L, 0, rs.subcycles, rs.dzmin, dz_schedule=fx['dz'])
+    st.set_field(rs.shifted(rs.A0) if paraxial else rs.A0)
+    n, _ = st.advance(zi, L, 0, rs.subcycles, 0.0 if paraxial else rs.dzmin, dz_schedule=fx['dz'])
     a, J, chi_t, ne_t = st.finish(zf - zi)
     st.close()
     assert n == len(fx['dz'])
+    if paraxial:
+        a = rs.sorted(a)
+        ne_w = rs.sorted(np.fft.fft(ne_t, axis=0))
+    else:
+        ne_w = np.fft.rfft(ne_t, axis=0)
     assert rel_l2(a, fx['A']) < FP32_FIELD_TOL
     if 'ne' in fx:
-        assert rel_l2(np.fft.rfft(ne_t, axis=0), fx['ne']) < FP32_NE_TOL
+        assert rel_l2(ne_w, fx['ne']) < FP32_NE_TOL
 
 
 def test_volume_propagate_and_report(tmp_path):
@@ -187,3 +198,54 @@ def test_ray_boundary_track(oracle_mod, precision):
     live = xp[:, 0, 4] > 0
     assert rel_l2(eik[live, 1], eik_o[live, 1]) < tol
     assert np.array_equal(xp[live], xp_o[live])
+
+
+# ---- paraxial envelope (paraxial_kernel.py, paraxial.cl) ----------------------------------------------------
+
+@pytest.mark.parametrize('coords', ['cartesian', 'cylindrical'])
+@pytest.mark.parametrize('precision', ['fp64-plain', 'fp64', 'fp32', 'bf16x3'])
+def test_paraxial_gaussian_reference(coords, precision):
+    """The reference's paraxial Gaussian-beam test (test/gaussian_test.py:63-108) through the HIP library: the
+    lineout at the central frequency against a00 exp(-x^2 / (f r00^2)) / sqrt(f), f = 5 and 17, with the
+    reference's tolerances (rtol 0.02, atol 0.01)."""
+    vol, p = configs.gaussian_paraxial(coords, n=128, nw=16)
+    _say('paraxial gaussian %s %s' % (coords, precision))
+    A, _, _, _, dom4d, _ = _run(vol, precision)
+    c = A.shape[0] // 2
+    dx = (dom4d[3] - dom4d[2]) / A.shape[1]
+    x = dom4d[2] + dx * (np.arange(A.shape[1]) + 0.5)
+    if coords == 'cartesian':
+        dy = (dom4d[5] - dom4d[4]) / A.shape[2]
+        y = dom4d[4] + dy * 64.5
+        lineout = lambda k: np.abs(A[c, :, 64, k])
+    else:
+        y = 0.0
+        lineout = lambda k: np.abs(A[c, :, 0, k])
+    for plane, f in ((1, 5), (2, 17)):
+        expected = p['a00'] * np.exp(-y ** 2 / p['r00'] ** 2 / f) * np.exp(-x ** 2 / p['r00'] ** 2 / f) / f ** 0.5
+        assert np.allclose(lineout(plane), expected, .02, .01)
+
+
+@pytest.mark.parametrize('coords', ['cylindrical', 'cartesian'])
+@pytest.mark.parametrize('precision', ['fp64-plain', 'fp64', 'fp32', 'bf16x3'])
+def test_paraxial_filament(oracle_mod, coords, precision):
+    """The focused air filament (Kerr, cycle-averaged O2 rate, the 32-state rotator) against the oracle: FP64
+    paths 1e-10 with the same step schedule; FP32 / BF16x3 on their own adaptive schedule 1e-4 (field) and
+    1e-3 (electron density)."""
+    scale = 4 if coords == 'cylindrical' else 16
+    vol = configs.paraxial_air_filament(scale=scale, planes=2, length='4 mm', power=50.0 if scale == 4 else 200.0,
+                                        coords=coords, rotator_states=32)
+    _say('paraxial filament %s %s' % (coords, precision))
+    rec_ref = []
+    ref = oracle_mod.oracle_track(vol, record=rec_ref)
+    rec = []
+    got = _run(vol, precision, rec)
+    if precision.startswith('fp64'):
+        assert len(rec) == len(rec_ref)
+        assert np.allclose(rec, rec_ref, rtol=1e-8, atol=0)
+        tol, tol_ne = FP64_TOL, FP64_TOL
+    else:
+        tol, tol_ne = 1e-4, 1e-3
+    assert rel_l2(got[0][..., 1], ref[0][..., 1]) < tol
+    assert rel_l2(got[2][..., 1], ref[2][..., 1]) < tol_ne
+    assert rel_l2(got[3][..., 1], ref[3][..., 1]) < tol_ne
