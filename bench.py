@@ -19,6 +19,10 @@ Printed JSON line (one):
   cpu_baseline  the FP64 oracle (reference algorithm and pass structure, OpenMP) on a bounded sample
   extras        speedup over the plain FP64 GPU stepper, relative L2 error of the state against the fused FP64
                 stepper on the same step schedule
+
+``--workload paraxial`` measures the paraxial envelope stepper the same way on the focused air filament of the
+reference's paraxial example (Nw = 256 envelope frequencies, Nr = 256 cells with 128 Hankel modes, Kerr + O2
+photoionisation with the cycle-averaged rate) with 400 subcycles in the first 2 cm plane.
 '''
 import argparse
 import json
@@ -32,9 +36,10 @@ ROOT = os.path.dirname(os.path.abspath(__file__))
 if ROOT not in sys.path:
     sys.path.insert(0, ROOT)
 
-from searay_amd import configs, uppe_kernel  # noqa: E402
+from searay_amd import configs, paraxial_kernel, uppe_kernel  # noqa: E402
 
 METRIC = 'UPPE grid-pt z-steps/s (1 GPU) vs FP64 plain at 1e-5 rel err; adjoint/fwd cost'
+METRIC_PARAXIAL = 'paraxial envelope grid-pt z-steps/s (1 GPU)'
 HBM_PEAK_GBS = 8000.0
 PEAK_TFLOPS = {'fp64': 78.6, 'fp32': 157.3, 'bf16x3': 2500.0}
 DTYPE = {'fp64': 'f64', 'fp32': 'f32', 'bf16x3': 'f32 (Hankel: bf16x3 MFMA, f32 accumulate)', 'fp64-plain': 'f64'}
@@ -46,7 +51,7 @@ def log(msg):
 
 def run_steps(rs, st, K, schedule=None, record=False):
     zi, zf, L = rs.plane(0)
-    st.set_field(rs.A0)
+    st.set_field(rs.stepper_field())
     st.synchronize()
     t0 = time.perf_counter()
     n, rec = st.advance(zi, L, 0, rs.subcycles, rs.dzmin, max_steps=K, dz_schedule=schedule,
@@ -67,9 +72,10 @@ def step_bytes(rs, el):
     return 384.0 * (el / 4) * rs.N[0] * rs.N[1]
 
 
-def measured_traffic(prec, kernel):
+def measured_traffic(prec, kernel, workload='uppe'):
     '''HBM bytes per launch from the committed rocprofv3 counter summary of this workload, if present.'''
-    path = os.path.join(ROOT, 'profiles', 'traffic_radial_o2_2049x256.json')
+    name = 'traffic_radial_o2_2049x256.json' if workload == 'uppe' else 'traffic_paraxial_fil_256x256.json'
+    path = os.path.join(ROOT, 'profiles', name)
     try:
         with open(path) as f:
             tab = json.load(f)
@@ -97,7 +103,7 @@ def pcie_inclusive(rs, prec, K):
     run_steps(rs, st, 2)
     st.synchronize()
     t0 = time.perf_counter()
-    st.set_field(rs.A0)
+    st.set_field(rs.stepper_field())
     n, _ = st.advance(zi, L, 0, rs.subcycles, rs.dzmin, max_steps=K)
     st.finish(zf - zi)
     t1 = time.perf_counter()
@@ -127,14 +133,14 @@ def config1_speedup(prec, steps_plain):
                 speedup_vs_fp64_plain=out[prec] / out['fp64-plain'], rel_l2_err_vs_fp64=err)
 
 
-def cpu_baseline(vol, steps):
+def cpu_baseline(module, vol, steps):
     from oracle import oracle
-    rs = uppe_kernel.RunSetup(vol)
+    rs = module.RunSetup(vol)
     st = rs.make_stepper(oracle.oracle_factory)
     zi, zf, L = rs.plane(0)
-    st.set_field(rs.A0)
+    st.set_field(rs.stepper_field())
     st.advance(zi, L, 0, rs.subcycles, rs.dzmin, max_steps=1)
-    st.set_field(rs.A0)
+    st.set_field(rs.stepper_field())
     t0 = time.perf_counter()
     n, _ = st.advance(zi, L, 0, rs.subcycles, rs.dzmin, max_steps=steps)
     dt = time.perf_counter() - t0
@@ -153,6 +159,7 @@ def main():
     ap.add_argument('--cpu-steps', type=int, default=60)
     ap.add_argument('--no-cpu', action='store_true')
     ap.add_argument('--no-compare', action='store_true')
+    ap.add_argument('--workload', default='uppe', choices=['uppe', 'paraxial'])
     ap.add_argument('--calibrate-copy', type=int, default=0, metavar='MIB',
                     help='only run a plain device-to-device copy of MIB MiB (for scaling HBM byte counters)')
     args = ap.parse_args()
@@ -172,8 +179,14 @@ def main():
         print('bench.py runs on one GPU only: this stepper has no multi-GPU path (use --gpus 1)', file=sys.stderr)
         sys.exit(2)
 
-    vol = configs.radial_o2_ionization(rate=args.rate)
-    rs = uppe_kernel.RunSetup(vol)
+    paraxial = args.workload == 'paraxial'
+    if paraxial:
+        vol = configs.paraxial_air_filament(steps=400, rate=args.rate)
+        module = paraxial_kernel
+    else:
+        vol = configs.radial_o2_ionization(rate=args.rate)
+        module = uppe_kernel
+    rs = module.RunSetup(vol)
     pts = rs.N[0] * rs.N[1]
     K, W = args.steps, args.warmup
     prec = args.precision
@@ -218,7 +231,7 @@ def main():
         extra = {}
     achieved = B / avg_s / 1e9
     roofline = dict(bound='hbm', achieved=achieved, peak=HBM_PEAK_GBS, unit='GB/s', frac=achieved / HBM_PEAK_GBS,
-                    traffic=measured_traffic(prec, dom), kernel=kname, avg_launch_us=avg_s * 1e6, launches=cnt,
+                    traffic=measured_traffic(prec, dom, args.workload), kernel=kname, avg_launch_us=avg_s * 1e6, launches=cnt,
                     bytes_per_launch=B, **extra)
     roofline['step_hbm_frac'] = step_bytes(rs, el) / (ms_per_step * 1e-3) / 1e9 / HBM_PEAK_GBS
     share = {k: round(v[0] / max(1e-12, sum(c[0] for c in classes.values())), 4) for k, v in classes.items()
@@ -248,23 +261,30 @@ def main():
         extras['error_note'] = 'k-space state after %d steps on the FP64 step schedule' % K
         log('host-inclusive rate (set_field + %d steps + finish)' % K)
         extras['pcie_inclusive_pt_steps_per_s'] = pcie_inclusive(rs, prec, K)
-        log('radial Kerr workload (Nw=513, Nr=128): %s vs plain FP64' % prec)
-        extras['config1'] = config1_speedup(prec, args.plain_steps)
+        if not paraxial:
+            log('radial Kerr workload (Nw=513, Nr=128): %s vs plain FP64' % prec)
+            extras['config1'] = config1_speedup(prec, args.plain_steps)
 
     cpu = None
     if not args.no_cpu:
         log('CPU baseline: FP64 oracle, %d steps' % args.cpu_steps)
-        cv, cn, ct_, threads = cpu_baseline(vol, args.cpu_steps)
+        cv, cn, ct_, threads = cpu_baseline(module, vol, args.cpu_steps)
         cpu = dict(value=cv, unit='grid-pt z-steps/s', cores=threads, kind='port',
-                   sample='%d RK4 steps of this workload (2049 x 256) from the initial field, FP64, %.1f s'
-                   % (cn, ct_))
+                   sample='%d RK4 steps of this workload (%d x %d) from the initial field, FP64, %.1f s'
+                   % (cn, rs.N[0], rs.N[1], ct_))
 
+    if paraxial:
+        workload = ('radial paraxial envelope, focused air filament: Kerr + O2 photoionisation (cycle-averaged rate) '
+                    '+ plasma current, Nw=256, Nr=256 with 128 Hankel modes, 400 adaptive subcycles in the first '
+                    '2 cm plane; first %d RK4 steps' % K)
+    else:
+        workload = ('radial UPPE, Kerr + O2 PPT photoionisation + plasma current, Nw=2049, Nr=256, '
+                    '400 adaptive subcycles over 20 cm; first %d RK4 steps' % K)
     out = {
-        'metric': METRIC, 'value': value, 'unit': 'grid-pt z-steps/s', 'n_gpus': 1, 'steps': n, 'warmup': W,
+        'metric': METRIC_PARAXIAL if paraxial else METRIC, 'value': value, 'unit': 'grid-pt z-steps/s', 'n_gpus': 1, 'steps': n, 'warmup': W,
         'ms_per_step': ms_per_step, 'higher_is_better': True, 'scaling': 'weak', 'vs_baseline': None,
         'dtype': DTYPE[prec], 'data': 'synthetic (seeded Gaussian pulse with 1% spectral-phase noise)',
-        'config': {'workload': 'radial UPPE, Kerr + O2 PPT photoionisation + plasma current, Nw=2049, Nr=256, '
-                               '400 adaptive subcycles over 20 cm; first %d RK4 steps' % K,
+        'config': {'workload': workload,
                    'precision': prec, 'rate_model': args.rate, 'grid': [rs.N[0], rs.N[1]]},
         'roofline': roofline,
         'cpu_baseline': cpu,

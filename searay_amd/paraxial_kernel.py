@@ -206,6 +206,7 @@ class RunSetup:
         field_walls = grid_tools.cell_walls(dens_nodes[0], dens_nodes[-1], steps)
         self.diagnostic_walls = np.linspace(-size[3] / 2, size[3] / 2, N[3])
         self.dom4d = np.concatenate((dom3d, [field_walls[0], field_walls[-1]]))
+        self.dzmin = 0.0  # no minimum step in the paraxial step control (paraxial_kernel.py:209-220)
 
     def plane(self, k):
         '''(zi, zf, subcycle length) of diagnostic interval k.'''
@@ -229,6 +230,10 @@ class RunSetup:
         st.set_paraxial_medium(w, self.mat.kz, self.mat.k0, self.mat.ngas, self.mat.redchi3, dt, ion)
         st.set_rotator(self.rotator, dt)
         return st
+
+    def stepper_field(self):
+        '''The initial field in the stepper's frequency order.'''
+        return self.shifted(self.A0)
 
     @staticmethod
     def shifted(a):

@@ -218,6 +218,10 @@ class RunSetup:
         self.dt, self.nl_filter, self.cond_filter = source_tables(w_nodes, self.NL_band)
         self.ion = self.ionizer.kernel_params() if self.ionizer is not None else ionization.no_ionization_params()
 
+    def stepper_field(self):
+        '''The initial field in the stepper's frequency order.'''
+        return self.A0
+
     def plane(self, k):
         '''(zi, zf, subcycle length) of diagnostic interval k.'''
         zi = self.diagnostic_walls[k]

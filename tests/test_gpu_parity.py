@@ -163,16 +163,23 @@ def test_golden_fp32_replayed(name, precision):
         assert rel_l2(ne_w, fx['ne']) < FP32_NE_TOL
 
 
-def test_volume_propagate_and_report(tmp_path):
-    '''The volume's UPPE branch (volume.py:606-613, 635-641) through the HIP path: the files hold track's arrays.'''
+@pytest.mark.parametrize('propagator', ['uppe', 'paraxial'])
+def test_volume_propagate_and_report(tmp_path, propagator):
+    '''The volume's wave branches (volume.py:598-613, 627-641) through the HIP path: the files hold track's
+    arrays.'''
     from searay_amd import volume
-    vol = configs.radial_kerr_air(scale=8, steps=2, precision='fp32')
-    _say('volume propagate + report (radial Kerr, fp32)')
+    if propagator == 'uppe':
+        vol = configs.radial_kerr_air(scale=8, steps=2, precision='fp32')
+        names = volume.UPPE_OUTPUTS
+    else:
+        vol = configs.paraxial_air_filament(scale=8, planes=3, length='2 mm', precision='fp32')
+        names = volume.PARAXIAL_OUTPUTS
+    _say('volume propagate + report (%s, fp32)' % propagator)
     box = vol['object']
     ret = box.Propagate(None, None, None)
     base = str(tmp_path / 'run')
     box.Report(base, configs.MKS_LENGTH)
-    for suffix, ref in zip(volume.UPPE_OUTPUTS, ret):
+    for suffix, ref in zip(names, ret):
         assert np.array_equal(np.load(base + '_' + box.name + '_' + suffix + '.npy'), ref)
     assert np.all(np.isfinite(ret[0]))
 

@@ -274,6 +274,20 @@ class TestReportFiles:
             got = np.load(base + '_air_' + suffix + '.npy')
             assert got.dtype == ref.dtype and np.array_equal(got, ref)
 
+    def test_paraxial_report_names(self, tmp_path):
+        from searay_amd import volume
+        box = volume.AnalyticBox('air')
+        box.Initialize({'propagator': 'paraxial'})
+        rng = np.random.default_rng(6)
+        shape = (8, 4, 1, 2)
+        arrs = [rng.standard_normal(shape) + 1j * rng.standard_normal(shape) for _ in range(4)]
+        box.paraxial_wave, box.paraxial_j, box.paraxial_chi, box.paraxial_plasma = arrs
+        box.dom4d = np.arange(8.0)
+        base = str(tmp_path / 'run')
+        box.Report(base, 1.0)
+        for suffix, ref in zip(volume.PARAXIAL_OUTPUTS, arrs + [box.dom4d]):
+            assert np.array_equal(np.load(base + '_air_' + suffix + '.npy'), ref)
+
     def test_sim_json(self, tmp_path):
         import json
         from searay_amd import volume
