@@ -169,6 +169,20 @@ int uppe_advance(uppe_handle* h, const uppe_step_control* c, int* steps_out);
  * axis 0 as the reference does on the host). */
 int uppe_finish(uppe_handle* h, double z_total, double* a_r, double* j_r, double* chi_t, double* ne_t);
 
+/* Gradients (no reference counterpart; the adjoint the input spectral-phase objectives need). Kerr-only UPPE
+ * media on the radial matrix-core paths (FP64, FP32, BF16x3).
+ *
+ * uppe_advance_checkpointed: uppe_advance that also keeps what the adjoint needs: the state every
+ * checkpoint_every steps and the phase origin and dz of every step. The step sizes are treated as constants by
+ * the adjoint (fixed or replayed schedules give exact gradients).
+ *
+ * uppe_adjoint: the reverse sweep over the recorded run. g_a_r is the cotangent dL/dRe a + i dL/dIm a of the
+ * r-space field uppe_finish(z_total) returns (complex128 [nw][n1][n2]); g_a0_r receives the cotangent of the
+ * r-space field given to uppe_set_field. Each segment is replayed from its checkpoint. The medium must be the
+ * one of the recorded run; afterwards the stepper needs uppe_set_field again. */
+int uppe_advance_checkpointed(uppe_handle* h, const uppe_step_control* c, int checkpoint_every, int* steps_out);
+int uppe_adjoint(uppe_handle* h, double z_total, const double* g_a_r, double* g_a0_r);
+
 /* Read the k-space state q[nw][n1][n2] (complex128). No reference counterpart: used for checks. */
 int uppe_get_state(uppe_handle* h, double* q_k);
 

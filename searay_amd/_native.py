@@ -5,6 +5,8 @@ There is no fallback: if the library is missing or does not load, an ImportError
 import ctypes as ct
 import os
 
+import numpy as np
+
 from . import _abi
 
 LIB_PATH = os.path.join(os.path.dirname(os.path.abspath(__file__)), '_lib', 'libsearay_uppe.so')
@@ -43,3 +45,26 @@ class HipStepper(_abi.Stepper):
 
     def synchronize(self):
         self._check(self.lib.uppe_synchronize(self.h), 'synchronize')
+
+    def advance_checkpointed(self, zi, length, sub0, n_sub, dzmin, every, dphi=0.4, rel_amin=1e-2, dz_schedule=None,
+                             record_cap=0):
+        '''advance() keeping what adjoint() needs: the state every `every` steps and each step's phase origin and
+        dz. Returns (steps, recorded dz or None).'''
+        sched = None if dz_schedule is None else _abi._f64(dz_schedule)
+        rec = np.zeros(max(record_cap, 0)) if record_cap > 0 else None
+        c = _abi.uppe_step_control(float(zi), float(length), int(sub0), 0, float(dzmin), float(dphi),
+                                   float(rel_amin), int(n_sub), 0 if sched is None else len(sched), _abi._ptr(sched),
+                                   _abi._ptr(rec), record_cap if rec is not None else 0, 0)
+        n = ct.c_int(0)
+        self._check(self.lib.uppe_advance_checkpointed(self.h, ct.byref(c), int(every), ct.byref(n)),
+                    'advance_checkpointed')
+        return n.value, (rec[:min(n.value, record_cap)].copy() if rec is not None else None)
+
+    def adjoint(self, z_total, g_a):
+        '''Cotangent of the r-space field given to set_field, from the cotangent g_a = dL/dRe a + i dL/dIm a of the
+        field finish(z_total) returns, over the last checkpointed run.'''
+        g = _abi._c128(g_a).reshape(self.shape)
+        out = np.zeros(self.shape, dtype=np.complex128)
+        self._check(self.lib.uppe_adjoint(self.h, ct.c_double(z_total), _abi._ptr(g.view(np.float64)),
+                                          _abi._ptr(out.view(np.float64))), 'adjoint')
+        return out

@@ -71,6 +71,12 @@ def _base(helper, band, size, grid, nl_band, subcycles, dzmin, coords, modes=Non
     return vol
 
 
+def kerr_pulse_sigma():
+    '''Spectral 1/e half-width sigma = 2/t0 of the radial Kerr workload's pulse (50 fs FWHM), in units of w00.'''
+    helper = input_tools.InputHelper(MKS_LENGTH)
+    return 2.0 / (helper.dnum('50 fs') / np.sqrt(2 * np.log(2)))
+
+
 def radial_kerr_air(scale=1, steps=None, seed=1000, precision=None):
     helper = input_tools.InputHelper(MKS_LENGTH)
     nw = (512 // scale) + 1

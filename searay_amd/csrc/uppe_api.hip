@@ -276,6 +276,50 @@ int uppe_advance(uppe_handle* h, const uppe_step_control* c, int* steps_out) {
   return rc;
 }
 
+int uppe_advance_checkpointed(uppe_handle* h, const uppe_step_control* c, int checkpoint_every, int* steps_out) {
+  if (!h) return UPPE_ERR_ARG;
+  if (!c || checkpoint_every < 1) {
+    h->err = "null step control or checkpoint interval below 1";
+    return UPPE_ERR_ARG;
+  }
+  if (!h->have_field || !h->have_medium) {
+    h->err = "field or medium not set";
+    return UPPE_ERR_STATE;
+  }
+  if (!h->fused) {
+    h->err = "adjoint: the fused paths only (not fp64-plain)";
+    return UPPE_ERR_UNSUPPORTED;
+  }
+  if (c->n_sub < 1 || !(c->length > 0.0) || (c->n_schedule > 0 && !c->dz_schedule) || c->n_schedule < 0) {
+    h->err = "bad step control";
+    return UPPE_ERR_ARG;
+  }
+  int taken = 0;
+  int rc = guarded(h, [&] {
+    UPPE_HIP(hipSetDevice(h->d.device));
+    taken = fused_advance_checkpointed(h, c, checkpoint_every);
+  });
+  if (steps_out) *steps_out = taken;
+  return rc;
+}
+
+int uppe_adjoint(uppe_handle* h, double z_total, const double* g_a_r, double* g_a0_r) {
+  if (!h) return UPPE_ERR_ARG;
+  if (!g_a_r || !g_a0_r) {
+    h->err = "null argument";
+    return UPPE_ERR_ARG;
+  }
+  if (!h->fused) {
+    h->err = "adjoint: the fused paths only (not fp64-plain)";
+    return UPPE_ERR_UNSUPPORTED;
+  }
+  return guarded(h, [&] {
+    UPPE_HIP(hipSetDevice(h->d.device));
+    fused_adjoint(h, z_total, g_a_r, g_a0_r);
+    h->have_field = false;  // the replays leave the stepper mid-run
+  });
+}
+
 int uppe_finish(uppe_handle* h, double z_total, double* a_r, double* j_r, double* chi_t, double* ne_t) {
   if (!h) return UPPE_ERR_ARG;
   if (!h->have_field || !h->have_medium) {

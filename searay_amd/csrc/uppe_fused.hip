@@ -80,6 +80,22 @@ struct FusedState {
   void *tw_x = nullptr, *tw_y = nullptr;
   int twlen_x = 0, twlen_y = 0;
   void* rot_tab = nullptr;  // rotator rows (device, RotRow<T> per state j >= 2), when the handle has a rotator
+  // adjoint (Kerr-only radial MFMA path): fragments of Hf^T (the shape of Hb) and Hb^T (the shape of Hf)
+  void *fab_hi = nullptr, *fab_lo = nullptr, *faf_hi = nullptr, *faf_lo = nullptr;
+  // checkpointed forward record: q, the step state and qmax every ck_every steps; (zoff, z, dz) of every step;
+  // the stage fields E_r of one segment (4 per step); cotangent work arrays
+  int ck_every = 0, ck_n = 0, rec_steps = 0, er_cap = 0;
+  int ckq_cap = 0, ckst_cap = 0, ckqm_cap = 0;  // segments
+  int zrec_cap = 0;                             // doubles (3 per step)
+  bool recorded = false, er_last_valid = false;
+  void* ck_q = nullptr;
+  StepState* ck_st = nullptr;
+  unsigned long long* ck_qmax = nullptr;
+  double* zrec = nullptr;
+  double* zrec_active = nullptr;
+  std::vector<double> zrec_host;
+  void* er_store = nullptr;
+  void *gs = nullptr, *gq = nullptr, *gk = nullptr, *g2 = nullptr, *g3 = nullptr;
 };
 
 // per-state constants of the rotator for the source kernel: the coefficient T_j S_j 2 dalpha (times n_g in the
@@ -96,10 +112,13 @@ struct RotRow {
 constexpr int BM = 64, BN = 64, BK = 16;
 
 // k2r prologue: none, RK stage input with E = i kz A (UPPE), linear propagation only (finish), RK stage input kept
-// as A (paraxial envelope)
-enum { PRO_NONE = 0, PRO_STAGE = 1, PRO_FINISH = 2, PRO_STAGE_ENV = 3 };
+// as A (paraxial envelope), and the adjoint's k2r input conj((i/2) e^{-i kz z} / Re kz) g (the transposed RHS
+// factor)
+enum { PRO_NONE = 0, PRO_STAGE = 1, PRO_FINISH = 2, PRO_STAGE_ENV = 3, PRO_ADJ = 4 };
 constexpr bool pro_stage(int pro) { return pro == PRO_STAGE || pro == PRO_STAGE_ENV; }
-enum { EPI_NONE = 0, EPI_STAGE = 1 };
+// r2k epilogue: none, the RK stage (forward), the adjoint RK stage (conj(i kz e^{i kz z}) times the contraction,
+// then the reverse RK4 recurrence), the adjoint of finish's linear propagation
+enum { EPI_NONE = 0, EPI_STAGE = 1, EPI_ADJ = 2, EPI_ADJ_FIN = 3 };
 
 __device__ inline unsigned long long dbits(double x) { return (unsigned long long)__double_as_longlong(x); }
 __device__ inline double bitsd(unsigned long long b) { return __longlong_as_double((long long)b); }
@@ -399,6 +418,53 @@ __device__ inline void stage_epilogue(cplx<T> J, long idx, int f, int stage, dou
                  nan);
 }
 
+// one point of the adjoint r2k epilogue. x = conj(i kz e^{i kz z}) times the contraction is the cotangent of the
+// stage input; y = C x for stages 2-4 (their inputs are conditioned), x for stage 1. With gs the conditioned
+// cotangent of the step's output and gq the running cotangent of its input, the reverse of
+// q' = C(q + dz/6 k1 + dz/3 k2 + dz/3 k3 + dz/6 k4), k_s = S(C(q + c_s dz k_{s-1})) is
+//   stage 4: gq = gs + y, gk3 = dz/3 gs + dz y;   stage 3: gq += y, gk2 = dz/3 gs + dz/2 y;
+//   stage 2: gq += y, gk1 = dz/6 gs + dz/2 y;     stage 1: gq += y, the cotangent of the step's input,
+// which for an earlier step (dzprev > 0) becomes its gs = C gq and gk4 = dzprev/6 gs. EPI_ADJ_FIN takes the
+// adjoint of finish's exp(i kz z): gs = C conj(e^{i kz z}) x, gk4 = dzprev/6 gs.
+template <typename T, int EPI>
+__device__ inline void adj_point(cplx<T> acc, cplx<T> kz, double ph, T cf, long idx, int stage, double zph, double dz,
+                                 double dzprev, cplx<T>* __restrict__ gq, cplx<T>* __restrict__ gs,
+                                 cplx<T>* __restrict__ gk) {
+  const cplx<T> back = cmul(linear_factor<T>(ph, -kz.im, zph, -1), acc);  // conj(e^{i (Re kz + kg) z} e^{-Im kz z})
+  if (EPI == EPI_ADJ_FIN) {
+    const cplx<T> g = cscale(back, cf);
+    gs[idx] = g;
+    gk[idx] = cscale(g, (T)(dzprev / 6));
+    return;
+  }
+  const cplx<T> x = cmul(cmk(-kz.im, -kz.re), back);  // conj(i kz) = -Im kz - i Re kz
+  cplx<T> y = x;
+  if (stage > 1) y = cscale(x, cf);
+  const T d = (T)dz;
+  if (stage == 4) {
+    const cplx<T> g = gs[idx];
+    gq[idx] = cadd(g, y);
+    gk[idx] = cadd(cscale(g, d / T(3)), cscale(y, d));
+  } else if (stage == 3) {
+    const cplx<T> g = gs[idx];
+    gq[idx] = cadd(gq[idx], y);
+    gk[idx] = cadd(cscale(g, d / T(3)), cscale(y, d / T(2)));
+  } else if (stage == 2) {
+    const cplx<T> g = gs[idx];
+    gq[idx] = cadd(gq[idx], y);
+    gk[idx] = cadd(cscale(g, d / T(6)), cscale(y, d / T(2)));
+  } else {
+    const cplx<T> g = cadd(gq[idx], y);
+    if (dzprev > 0.0) {
+      const cplx<T> gn = cscale(g, cf);
+      gs[idx] = gn;
+      gk[idx] = cscale(gn, (T)(dzprev / 6));
+    } else {
+      gq[idx] = g;
+    }
+  }
+}
+
 // B-operand element (k, f) with the prologue of the k2r direction applied
 template <typename T, int PRO>
 __device__ inline cplx<T> load_b(const cplx<T>* __restrict__ B, long idx, int f, int stage, double zph, double cu,
@@ -440,7 +506,8 @@ __global__ __launch_bounds__(1024) void f_hankel(
     cplx<typename MfmaTraits<CM>::T>* __restrict__ q, const cplx<typename MfmaTraits<CM>::T>* __restrict__ kprev,
     cplx<typename MfmaTraits<CM>::T>* __restrict__ k1, cplx<typename MfmaTraits<CM>::T>* __restrict__ k2,
     cplx<typename MfmaTraits<CM>::T>* __restrict__ k3, StepState* __restrict__ st,
-    unsigned long long* __restrict__ qmax, unsigned long long* __restrict__ qmax_next, int stage, double zfix) {
+    unsigned long long* __restrict__ qmax, unsigned long long* __restrict__ qmax_next, int stage, double zfix,
+    double dzfix, double dzprev) {
   using T = typename MfmaTraits<CM>::T;
   using C = cplx<T>;
   using acc_t = typename MfmaTraits<CM>::acc_t;
@@ -460,7 +527,7 @@ __global__ __launch_bounds__(1024) void f_hankel(
   const int ct_n = (tile == (int)gridDim.x - 1 && fend - fc0 > HC) ? 2 : 1;
   const int hcw = ct_n * HC;       // complex columns staged (columns >= fend are zero)
   constexpr int BW = 2 * BN;       // LDS row width in real columns (room for two MFMA column tiles)
-  double zph = zfix, cu = 0.0, dz = 0.0;
+  double zph = zfix, cu = 0.0, dz = dzfix;
   if (st) {
     dz = st->dz;
     double cs = stage == 1 ? 0.0 : (stage == 4 ? 1.0 : 0.5);
@@ -537,6 +604,10 @@ __global__ __launch_bounds__(1024) void f_hankel(
         if (PRO == PRO_STAGE) v = cmul(cmk(-vz[j].im, vz[j].re), v);  // E = i kz A
       } else if (PRO == PRO_FINISH) {
         v = cmul(linear_factor<T>(vp[j], vz[j].im, zph, +1), v);
+      } else if (PRO == PRO_ADJ) {
+        // conj of S = (i/2) e^{-i kz z} J / Re kz: (-i/2) e^{i (Re kz + kg) z} e^{Im kz z} g / Re kz
+        const C a = cmul(linear_factor<T>(vp[j], -vz[j].im, zph, +1), v);
+        v = cmk((T(0.5) * a.im) / vz[j].re, (T(-0.5) * a.re) / vz[j].re);
       }
       if constexpr (CM == CM_BF16X3) {
         __bf16* bhi = reinterpret_cast<__bf16*>(smem);
@@ -646,6 +717,19 @@ __global__ __launch_bounds__(1024) void f_hankel(
 
   // ---- epilogue over complex points of the tile (the r2k direction leaves the state alone once the run
   // has finished)
+  if (EPI == EPI_ADJ || EPI == EPI_ADJ_FIN) {
+    // adjoint: q = gq (cotangent of the step's input state, accumulated), k1 = gs (conditioned cotangent of
+    // the step's output), k2 = gk (cotangent of the stage derivative the next adjoint stage takes)
+    const int npts = bm * hcw;
+    for (int e = tid; e < npts; e += nth) {
+      const int row = e >> lgh, cf = e & (hcw - 1), r = row0 + row, f = fc0 + cf;
+      if (r >= M || f >= fend) continue;
+      const long idx = (long)r * nwp + f;
+      adj_point<T, EPI>(cmk(os[row * BW + 2 * cf], os[row * BW + 2 * cf + 1]), kzc[idx], phs[idx], cond[f], idx,
+                        stage, zph, dz, dzprev, q, k1, k2);
+    }
+    return;
+  }
   if (EPI == EPI_STAGE && !st->active) return;
   const double amin = (EPI == EPI_STAGE && stage == 1 && !st->fixed) ? st->rel_amin * bitsd(*qmax) : 0.0;
   double loc = (EPI == EPI_STAGE && stage == 4) ? 0.0 : INFINITY;
@@ -1454,12 +1538,81 @@ __global__ __launch_bounds__(1024) void f_source_env(const cplx<T>* __restrict__
   }
 }
 
+// Adjoint of the Kerr source on one strip: the vector-Jacobian product of J = F_NL (-i w) rfft(P), P = n_g chi3 e^3,
+// e = irfft(E) (uppe.cl:132-165, fft.cl:229-299). Given the cotangent G of J (G = dL/dRe J + i dL/dIm J), with
+// R = (i w F_NL) G:
+//   g_P(t) = Re sum_{k=0}^{Nt/2} R_k e^{2 pi i k t / Nt}          (adjoint of rfft)
+//   g_e(t) = 3 n_g chi3 e(t)^2 g_P(t)
+//   G_E(k) = c_k rfft(g_e)_k / Nt, c = 2 inside, 1 at DC and Nyquist, whose imaginary parts carry no cotangent
+//                                                                  (adjoint of irfft)
+// e and g_P come from one unnormalised inverse transform of Z = E^ext + i Y^ext (Hermitian extensions, Y = R with
+// its inner bins halved), g_e from one forward transform. One workgroup per strip, max(min(Nt, 64), Nt/4)
+// threads.
+template <typename T, int SPT>
+__global__ __launch_bounds__(1024) void f_source_adj(const cplx<T>* __restrict__ er, const cplx<T>* __restrict__ gj,
+                                                     cplx<T>* __restrict__ ge, const cplx<T>* __restrict__ tw,
+                                                     const T* __restrict__ w, const T* __restrict__ nlf,
+                                                     const double* __restrict__ ng, int nt, int lgn, int nwp,
+                                                     double redchi3) {
+  extern __shared__ char smem[];
+  cplx<T>* Z = reinterpret_cast<cplx<T>*>(smem);
+  const int N = nt, M = nt / 2;
+  const long p = blockIdx.x;
+  const int tid = threadIdx.x, nth = blockDim.x;
+  const cplx<T>* X = er + p * nwp;
+  const cplx<T>* G = gj + p * nwp;
+  const cplx<T>* tws = tw;
+  if constexpr (sizeof(T) == 4) {
+    cplx<T>* tl = Z + N;
+    const int tlen = r8_table_len(lgn);
+    for (int k = tid; k < tlen; k += nth) tl[k] = tw[k];
+    tws = tl;
+  }
+  for (int k = tid; k < N; k += nth) {
+    const int kk = k <= M ? k : N - k;
+    cplx<T> x = X[kk];
+    const cplx<T> g = G[kk];
+    const T wf = w[kk] * nlf[kk];
+    cplx<T> y = cmk(-wf * g.im, wf * g.re);  // i w F_NL G
+    if (kk == 0 || kk == M) {
+      x.im = T(0);
+      y.im = T(0);
+    } else {
+      y = cscale(y, T(0.5));
+    }
+    if (k > M) {
+      x = cconj(x);
+      y = cconj(y);
+    }
+    Z[lds_swz<T>(k)] = cmk(x.re - y.im, x.im + y.re);
+  }
+  __syncthreads();
+  lds_fft_r8(Z, N, lgn, tws, +1, tid);
+  const T kc3 = (T)(3.0 * ng[p] * redchi3), invn = T(1) / T(N);
+  T gev[SPT];
+#pragma unroll
+  for (int i = 0; i < SPT; i++) {
+    const cplx<T> z = Z[lds_swz<T>(tid + i * nth)];
+    const T e = z.re * invn;
+    gev[i] = kc3 * e * e * z.im;
+  }
+  __syncthreads();
+#pragma unroll
+  for (int i = 0; i < SPT; i++) Z[lds_swz<T>(tid + i * nth)] = cmk(gev[i], T(0));
+  __syncthreads();
+  lds_fft_r8(Z, N, lgn, tws, -1, tid);
+  for (int k = tid; k <= M; k += nth) {
+    const cplx<T> f = Z[lds_swz<T>(k)];
+    ge[p * nwp + k] = (k == 0 || k == M) ? cmk(f.re * invn, T(0)) : cscale(f, T(2) * invn);
+  }
+}
+
 // ---------------------------------------------------------------------------------------------------
 // control, layout and reduction kernels
 // ---------------------------------------------------------------------------------------------------
 
 // step size after the first RHS (uppe_kernel.py:221-227)
-__global__ void f_ctl_dz(StepState* st, const double* sched, double* rec) {
+__global__ void f_ctl_dz(StepState* st, const double* sched, double* rec, double* zrec, int zrec_cap) {
   if (!st->active) return;
   double dz;
   if (st->sched_n > 0) {
@@ -1477,6 +1630,12 @@ __global__ void f_ctl_dz(StepState* st, const double* sched, double* rec) {
     if (st->z + dz > st->L) dz = st->L - st->z;
   }
   if (rec && st->steps < st->rec_cap) rec[st->steps] = dz;
+  if (zrec && st->steps < zrec_cap) {
+    // the phase origin of the step, for the adjoint's replay of the stage phases
+    zrec[3 * st->steps] = st->zoff;
+    zrec[3 * st->steps + 1] = st->z;
+    zrec[3 * st->steps + 2] = dz;
+  }
   st->dz = dz;
 }
 
@@ -1781,11 +1940,20 @@ struct Fused {
     std::vector<T> hbt(hb.begin(), hb.end()), hft(hf.begin(), hf.end());
     UPPE_HIP(hipMemcpy(s->Hb, hbt.data(), sizeof(T) * hbt.size(), hipMemcpyHostToDevice));
     UPPE_HIP(hipMemcpy(s->Hf, hft.data(), sizeof(T) * hft.size(), hipMemcpyHostToDevice));
-    void* old[] = {s->frb_hi, s->frb_lo, s->frf_hi, s->frf_lo};
+    void* old[] = {s->frb_hi, s->frb_lo, s->frf_hi, s->frf_lo, s->fab_hi, s->fab_lo, s->faf_hi, s->faf_lo};
     for (void* p : old)
       if (p) UPPE_HIP(hipFree(p));
     s->frb_hi = s->frb_lo = s->frf_hi = s->frf_lo = nullptr;
+    s->fab_hi = s->fab_lo = s->faf_hi = s->faf_lo = nullptr;
     if (s->cm < 0) return;
+    // the adjoint's transposes: Hf^T has the shape of Hb, Hb^T that of Hf
+    std::vector<double> hft_((size_t)nr * nk), hbt_((size_t)nk * nr);
+    for (int k = 0; k < nk; k++)
+      for (int j = 0; j < nr; j++) {
+        hft_[(size_t)j * nk + k] = hf[(size_t)k * nr + j];
+        hbt_[(size_t)k * nr + j] = hb[(size_t)j * nk + k];
+      }
+    int kpa = 0;
     if (s->cm == CM_BF16X3) {
       std::vector<uint16_t> hi, lo;
       pack_bf16x3(hb, nr, nk, hi, lo, &s->kp_b);
@@ -1794,12 +1962,22 @@ struct Fused {
       pack_bf16x3(hf, nk, nr, hi, lo, &s->kp_f);
       s->frf_hi = upload(hi);
       s->frf_lo = upload(lo);
+      pack_bf16x3(hft_, nr, nk, hi, lo, &kpa);
+      s->fab_hi = upload(hi);
+      s->fab_lo = upload(lo);
+      pack_bf16x3(hbt_, nk, nr, hi, lo, &kpa);
+      s->faf_hi = upload(hi);
+      s->faf_lo = upload(lo);
     } else {
       std::vector<T> fr;
       pack_k4<T>(hb, nr, nk, fr, &s->kp_b);
       s->frb_hi = upload(fr);
       pack_k4<T>(hf, nk, nr, fr, &s->kp_f);
       s->frf_hi = upload(fr);
+      pack_k4<T>(hft_, nr, nk, fr, &kpa);
+      s->fab_hi = upload(fr);
+      pack_k4<T>(hbt_, nk, nr, fr, &kpa);
+      s->faf_hi = upload(fr);
     }
     s->plan[0] = plan_hankel(nr, h->nw, s->n_cu);
     s->plan[1] = plan_hankel(nk, s->f1, s->n_cu);
@@ -1945,16 +2123,31 @@ struct Fused {
     throw Error{UPPE_ERR_UNSUPPORTED, "fused source kernel: unsupported Nt"};
   }
 
-  // matrix-core contraction of one RHS stage; dir 0 = k2r (prologue), 1 = r2k (epilogue)
+  // adjoint of the Kerr source on every strip (f_source_adj): er = the stage's E_r, gj = cotangent of J_r
+  static void source_adj(uppe_handle* h, FusedState* s, const C* er, const C* gj, C* ge) {
+    const int N = h->nt;
+    const int nth = std::max(std::min(N, 64), N / 4);
+    const int spt = N / nth;
+    const size_t shm = sizeof(C) * (sizeof(T) == 4 ? 2 * N : N);
+    const int lgn = ilog2i(N);
+    const dim3 grid((unsigned)h->P), block(nth);
+#define UPPE_SRC_ADJ(S)                                                                                          \
+  if (spt == S) {                                                                                                \
+    launchf(h, UPPE_KC_SOURCE, f_source_adj<T, S>, grid, block, shm, er, gj, ge, (const C*)s->tw, (const T*)s->w, \
+            (const T*)s->nlf, (const double*)s->ng, N, lgn, s->nwp, s->redchi3);                                  \
+    return;                                                                                                      \
+  }
+    UPPE_SRC_ADJ(1) UPPE_SRC_ADJ(2) UPPE_SRC_ADJ(4)
+#undef UPPE_SRC_ADJ
+    throw Error{UPPE_ERR_UNSUPPORTED, "adjoint source kernel: unsupported Nt"};
+  }
+
+  // one launch of the matrix-core contraction out[M][ncols] = A[M][K] B[K][ncols] with fragments (ahi, alo) of A,
+  // its launch plan, prologue PRO and epilogue EPI; q, k1..k3 are the state arrays the epilogue updates
   template <int CM, int PRO, int EPI>
-  static void hankel_mfma(uppe_handle* h, FusedState* s, int dir, const C* B, C* out, const C* kprev, int stage,
-                          double zfix, StepState* st) {
-    const int M = dir == 0 ? h->n1 : h->nk, K = dir == 0 ? h->nk : h->n1;
-    const int kp = dir == 0 ? s->kp_b : s->kp_f;
-    const FusedState::HkPlan& pl = s->plan[dir];
-    const int ncols = dir == 0 ? h->nw : s->f1;  // complex columns covered
-    const void* ahi = dir == 0 ? s->frb_hi : s->frf_hi;
-    const void* alo = dir == 0 ? s->frb_lo : s->frf_lo;
+  static void hk_launch(uppe_handle* h, FusedState* s, const void* ahi, const void* alo, int kp, int M, int K,
+                        int ncols, const FusedState::HkPlan& pl, const C* B, C* out, const C* kprev, C* q, C* k1,
+                        C* k2, C* k3, StepState* st, int stage, double zfix, double dzfix, double dzprev) {
     using F = typename MfmaTraits<CM>::frag_t;
     constexpr int BNV = 16, BW = 2 * BNV;
     const int rw = 16 * pl.rt;
@@ -1968,8 +2161,7 @@ struct Fused {
 #define UPPE_HK(RTV)                                                                                             \
   launchf(h, UPPE_KC_HANKEL, f_hankel<CM, PRO, EPI, BNV, RTV>, grid, dim3(64 * pl.nwv), shm, (const F*)ahi,    \
           (const F*)alo, B, out, M, K, kp, rw, s->nwp, 0, ncols, (const double*)s->phs, (const C*)s->kzc,         \
-          (const T*)s->cond, (C*)s->q, kprev, (C*)s->k1, (C*)s->k2, (C*)s->k3, st, s->qmax, s->qmax_next, stage, \
-          zfix)
+          (const T*)s->cond, q, kprev, k1, k2, k3, st, s->qmax, s->qmax_next, stage, zfix, dzfix, dzprev)
     if (pl.rt == 1)
       UPPE_HK(1);
     else
@@ -1977,18 +2169,34 @@ struct Fused {
 #undef UPPE_HK
   }
 
+  // matrix-core contraction of one RHS stage; dir 0 = k2r (prologue), 1 = r2k (epilogue)
+  template <int CM, int PRO, int EPI>
+  static void hankel_mfma(uppe_handle* h, FusedState* s, int dir, const C* B, C* out, const C* kprev, int stage,
+                          double zfix, StepState* st) {
+    const int M = dir == 0 ? h->n1 : h->nk, K = dir == 0 ? h->nk : h->n1;
+    const int kp = dir == 0 ? s->kp_b : s->kp_f;
+    const int ncols = dir == 0 ? h->nw : s->f1;  // complex columns covered
+    hk_launch<CM, PRO, EPI>(h, s, dir == 0 ? s->frb_hi : s->frf_hi, dir == 0 ? s->frb_lo : s->frf_lo, kp, M, K,
+                            ncols, s->plan[dir], B, out, kprev, (C*)s->q, (C*)s->k1, (C*)s->k2, (C*)s->k3, st,
+                            stage, zfix, 0.0, 0.0);
+  }
+
+  // one RK4 step; with erslot, stage s leaves its E_r in erslot[s - 1] (slots of P nwp points) for the adjoint
   template <int CM>
-  static void step_mfma(uppe_handle* h, FusedState* s) {
+  static void step_mfma(uppe_handle* h, FusedState* s, C* erslot = nullptr) {
     const C* kp[5] = {nullptr, nullptr, (const C*)s->k1, (const C*)s->k2, (const C*)s->k3};
+    const long stride = h->P * (long)s->nwp;
     for (int stage = 1; stage <= 4; stage++) {
+      C* er = erslot ? erslot + (stage - 1) * stride : (C*)s->er;
       if (h->paraxial)
-        hankel_mfma<CM, PRO_STAGE_ENV, EPI_NONE>(h, s, 0, (const C*)s->q, (C*)s->er, kp[stage], stage, 0.0, s->st);
+        hankel_mfma<CM, PRO_STAGE_ENV, EPI_NONE>(h, s, 0, (const C*)s->q, er, kp[stage], stage, 0.0, s->st);
       else
-        hankel_mfma<CM, PRO_STAGE, EPI_NONE>(h, s, 0, (const C*)s->q, (C*)s->er, kp[stage], stage, 0.0, s->st);
-      source(h, s, (const C*)s->er, (C*)s->jr, false, s->st);
+        hankel_mfma<CM, PRO_STAGE, EPI_NONE>(h, s, 0, (const C*)s->q, er, kp[stage], stage, 0.0, s->st);
+      source(h, s, (const C*)er, (C*)s->jr, false, s->st);
       hankel_mfma<CM, PRO_NONE, EPI_STAGE>(h, s, 1, (const C*)s->jr, (C*)nullptr, nullptr, stage, 0.0, s->st);
       if (stage == 1)
-        launchf(h, UPPE_KC_REDUCE, f_ctl_dz, dim3(1), dim3(1), 0, s->st, (const double*)s->sched, s->rec_active);
+        launchf(h, UPPE_KC_REDUCE, f_ctl_dz, dim3(1), dim3(1), 0, s->st, (const double*)s->sched, s->rec_active,
+                s->zrec_active, s->zrec_cap / 3);
     }
     launchf(h, UPPE_KC_REDUCE, f_ctl_end, dim3(1), dim3(1), 0, s->st, s->qmax, s->qmax_next);
   }
@@ -2027,7 +2235,8 @@ struct Fused {
       tpass<PRO_NONE, EPI_NONE>(h, s, 1, jr, jr, s->f1, -1, nullptr, stage, 0.0, s->st);
       tpass<PRO_NONE, EPI_STAGE>(h, s, 2, jr, (C*)nullptr, s->f1, -1, nullptr, stage, 0.0, s->st);
       if (stage == 1)
-        launchf(h, UPPE_KC_REDUCE, f_ctl_dz, dim3(1), dim3(1), 0, s->st, (const double*)s->sched, s->rec_active);
+        launchf(h, UPPE_KC_REDUCE, f_ctl_dz, dim3(1), dim3(1), 0, s->st, (const double*)s->sched, s->rec_active,
+                s->zrec_active, s->zrec_cap / 3);
     }
     launchf(h, UPPE_KC_REDUCE, f_ctl_end, dim3(1), dim3(1), 0, s->st, s->qmax, s->qmax_next);
   }
@@ -2062,7 +2271,8 @@ struct Fused {
               (const T*)s->cond, (C*)s->q, (const C*)nullptr, (C*)s->k1, (C*)s->k2, (C*)s->k3, s->st, s->qmax,
               s->qmax_next, stage, 0.0);
       if (stage == 1)
-        launchf(h, UPPE_KC_REDUCE, f_ctl_dz, dim3(1), dim3(1), 0, s->st, (const double*)s->sched, s->rec_active);
+        launchf(h, UPPE_KC_REDUCE, f_ctl_dz, dim3(1), dim3(1), 0, s->st, (const double*)s->sched, s->rec_active,
+                s->zrec_active, s->zrec_cap / 3);
     }
     launchf(h, UPPE_KC_REDUCE, f_ctl_end, dim3(1), dim3(1), 0, s->st, s->qmax, s->qmax_next);
   }
@@ -2085,6 +2295,200 @@ struct Fused {
     launchf(h, UPPE_KC_REDUCE, f_qmax_cols<T>, dim3(256), dim3(kBlock), 0, (const C*)s->q, P, s->nwp, s->f1,
             s->qmax + 2);
     UPPE_HIP(hipStreamSynchronize(h->stream));
+  }
+
+  // the device step state of a new advance call (uppe_kernel.py:299-325), uploaded on the handle's stream
+  static StepState* init_state(uppe_handle* h, FusedState* s, const uppe_step_control* c) {
+    if (sizeof(T) == 4) {
+      const double zmax = ((c->zi + (double)(c->sub0 + c->n_sub) * c->length) - c->zi);
+      if (s->max_ki * zmax > 80.0)
+        throw Error{UPPE_ERR_UNSUPPORTED,
+                    "FP32 range: exp(Im kz z) would exceed 3e38 in this interval (max Im kz * z = " +
+                        std::to_string(s->max_ki * zmax) + "); use FP64 or more diagnostic planes"};
+    }
+    StepState* sh = s->st_host;
+    std::memset(sh, 0, sizeof(StepState));
+    sh->zi = c->zi;
+    sh->L = c->length;
+    sh->dzmin = c->dzmin;
+    sh->dphi = c->dphi;
+    sh->rel_amin = c->rel_amin;
+    sh->sub = c->sub0;
+    sh->sub_end = c->sub0 + c->n_sub;
+    sh->z = 0.0;
+    sh->zoff = (c->zi + (double)c->sub0 * c->length) - c->zi;
+    sh->active = 1;
+    sh->sched_n = c->n_schedule;
+    sh->fixed = (c->n_schedule == 0 && c->dzmin >= c->length) ? 1 : 0;
+    sh->dzraw = 0x7ff0000000000000ull;  // +inf
+    sh->rec_cap = 0;
+    sh->max_steps = c->max_steps > 0 ? c->max_steps : 0;
+    if (c->n_schedule > s->sched_cap) {
+      if (s->sched) UPPE_HIP(hipFree(s->sched));
+      UPPE_HIP(hipMalloc(&s->sched, sizeof(double) * c->n_schedule));
+      s->sched_cap = c->n_schedule;
+    }
+    if (c->n_schedule > 0)
+      UPPE_HIP(hipMemcpy(s->sched, c->dz_schedule, sizeof(double) * c->n_schedule, hipMemcpyHostToDevice));
+    return sh;
+  }
+
+  template <typename V>
+  static void grow(V** p, int* cap, int need, size_t per, hipStream_t stream) {
+    if (need <= *cap) return;
+    int n = std::max(need, 2 * *cap);
+    V* q = nullptr;
+    UPPE_HIP(hipMalloc(&q, per * n));
+    if (*p) {
+      UPPE_HIP(hipMemcpyAsync(q, *p, per * *cap, hipMemcpyDeviceToDevice, stream));
+      UPPE_HIP(hipStreamSynchronize(stream));
+      UPPE_HIP(hipFree(*p));
+    }
+    *p = q;
+    *cap = n;
+  }
+
+  // forward run that keeps what the adjoint needs (Kerr-only radial MFMA path): q, the step state and qmax at the
+  // start of every segment of `every` steps, (zoff, z, dz) of every step, and the stage fields E_r of the last
+  // segment
+  template <int CM>
+  static int advance_ck(uppe_handle* h, FusedState* s, const uppe_step_control* c, int every) {
+    const long P = h->P, stride = P * (long)s->nwp;
+    StepState* sh = init_state(h, s, c);
+    s->rec_active = nullptr;
+    UPPE_HIP(hipMemcpyAsync(s->st, sh, sizeof(StepState), hipMemcpyHostToDevice, h->stream));
+    if (s->er_cap < 4 * every) {
+      if (s->er_store) UPPE_HIP(hipFree(s->er_store));
+      UPPE_HIP(hipMalloc(&s->er_store, sizeof(C) * stride * 4 * every));
+      s->er_cap = 4 * every;
+    }
+    s->recorded = false;
+    int seg = 0;
+    while (true) {
+      grow(&s->zrec, &s->zrec_cap, 3 * (seg + 1) * every, sizeof(double), h->stream);
+      grow((char**)&s->ck_q, &s->ckq_cap, seg + 1, sizeof(C) * stride, h->stream);
+      grow(&s->ck_st, &s->ckst_cap, seg + 1, sizeof(StepState), h->stream);
+      grow(&s->ck_qmax, &s->ckqm_cap, 3 * (seg + 1), sizeof(unsigned long long), h->stream);
+      UPPE_HIP(hipMemcpyAsync((C*)s->ck_q + seg * stride, s->q, sizeof(C) * stride, hipMemcpyDeviceToDevice, h->stream));
+      UPPE_HIP(hipMemcpyAsync(s->ck_st + seg, s->st, sizeof(StepState), hipMemcpyDeviceToDevice, h->stream));
+      UPPE_HIP(hipMemcpyAsync(s->ck_qmax + 3 * seg, s->qmax, 3 * sizeof(unsigned long long), hipMemcpyDeviceToDevice,
+                              h->stream));
+      s->zrec_active = s->zrec;
+      for (int i = 0; i < every; i++) step_mfma<CM>(h, s, (C*)s->er_store + (long)i * 4 * stride);
+      s->zrec_active = nullptr;
+      UPPE_HIP(hipMemcpyAsync(sh, s->st, sizeof(StepState), hipMemcpyDeviceToHost, h->stream));
+      UPPE_HIP(hipStreamSynchronize(h->stream));
+      seg++;
+      if (!sh->active) break;
+      if ((long long)seg * every > 100000000LL) throw Error{UPPE_ERR_STATE, "step loop did not terminate"};
+    }
+    if (sh->error) throw Error{UPPE_ERR_ARG, "dz schedule exhausted"};
+    const int steps = sh->steps;
+    s->ck_every = every;
+    s->ck_n = (steps + every - 1) / every;
+    s->rec_steps = steps;
+    s->zrec_host.resize((size_t)3 * steps);
+    if (steps > 0)
+      UPPE_HIP(hipMemcpy(s->zrec_host.data(), s->zrec, sizeof(double) * 3 * steps, hipMemcpyDeviceToHost));
+    if (c->dz_record)
+      for (int n = 0; n < std::min(steps, c->record_cap); n++) c->dz_record[n] = s->zrec_host[3 * n + 2];
+    // the stage fields in store belong to the last segment when it started at a segment boundary
+    s->er_last_valid = steps > 0 && (seg == s->ck_n);
+    s->recorded = true;
+    h->steps += steps;
+    h->evals += 4LL * steps;
+    h->nan_steps += sh->nan_steps;
+    h->z_last = sh->z_last;
+    return steps;
+  }
+
+  // reverse sweep: g_a_r is the cotangent of finish's r-space field (dL/dRe a + i dL/dIm a), g_a0_r receives the
+  // cotangent of the field given to set_field
+  template <int CM>
+  static void adjoint_cm(uppe_handle* h, FusedState* s, double z_total, const double* g_a_r, double* g_a0_r) {
+    const long P = h->P, stride = P * (long)s->nwp;
+    const int nr = h->n1, nk = h->nk, every = s->ck_every, steps = s->rec_steps;
+    const std::vector<double>& zr = s->zrec_host;
+    void** work[] = {&s->gs, &s->gq, &s->gk, &s->g2, &s->g3};
+    for (void** w : work)
+      if (!*w) UPPE_HIP(hipMalloc(w, sizeof(C) * stride));
+    const FusedState::HkPlan pl_ak = plan_hankel(nr, s->f1, s->n_cu), pl_ar = plan_hankel(nk, h->nw, s->n_cu),
+                             pl_a0 = plan_hankel(nr, h->nw, s->n_cu);
+    C *gs = (C*)s->gs, *gq = (C*)s->gq, *gk = (C*)s->gk, *g2 = (C*)s->g2, *g3 = (C*)s->g3;
+    UPPE_HIP(hipMemsetAsync(g2, 0, sizeof(C) * stride, h->stream));  // columns >= f1 stay zero
+    UPPE_HIP(hipMemsetAsync(g3, 0, sizeof(C) * stride, h->stream));
+    UPPE_HIP(hipMemcpyAsync(s->stage, g_a_r, sizeof(double) * 2 * h->npt, hipMemcpyHostToDevice, h->stream));
+    launchf(h, UPPE_KC_POINT, f_from_host<T>, dim3(grid_for(P * s->nwp)), dim3(kBlock), 0, (const double*)s->stage,
+            g3, h->nw, P, s->nwp);
+    // finish: a = Hb (e^{i kz z} q)
+    hk_launch<CM, PRO_NONE, EPI_ADJ_FIN>(h, s, s->faf_hi, s->faf_lo, s->kp_f, nk, nr, h->nw, pl_ar, g3, nullptr,
+                                         nullptr, gq, gs, gk, nullptr, nullptr, 0, z_total, 0.0,
+                                         zr[3 * (steps - 1) + 2]);
+    for (int seg = s->ck_n - 1; seg >= 0; seg--) {
+      const int n0 = seg * every, n1 = std::min(steps, n0 + every);
+      if (seg != s->ck_n - 1 || !s->er_last_valid) {
+        // replay the segment from its checkpoint, keeping the stage fields
+        UPPE_HIP(hipMemcpyAsync(s->q, (C*)s->ck_q + seg * stride, sizeof(C) * stride, hipMemcpyDeviceToDevice,
+                                h->stream));
+        UPPE_HIP(hipMemcpyAsync(s->st, s->ck_st + seg, sizeof(StepState), hipMemcpyDeviceToDevice, h->stream));
+        UPPE_HIP(hipMemcpyAsync(s->qmax, s->ck_qmax + 3 * seg, 3 * sizeof(unsigned long long),
+                                hipMemcpyDeviceToDevice, h->stream));
+        s->rec_active = nullptr;
+        s->zrec_active = nullptr;
+        for (int i = 0; i < n1 - n0; i++) step_mfma<CM>(h, s, (C*)s->er_store + (long)i * 4 * stride);
+      }
+      for (int n = n1 - 1; n >= n0; n--) {
+        const double zoff = zr[3 * n], z = zr[3 * n + 1], dz = zr[3 * n + 2];
+        for (int stage = 4; stage >= 1; stage--) {
+          const double cs = stage == 1 ? 0.0 : (stage == 4 ? 1.0 : 0.5);
+          const double zph = zoff + (z + cs * dz);
+          const C* ers = (const C*)s->er_store + ((long)(n - n0) * 4 + (stage - 1)) * stride;
+          hk_launch<CM, PRO_ADJ, EPI_NONE>(h, s, s->fab_hi, s->fab_lo, s->kp_b, nr, nk, s->f1, pl_ak, gk, g2, nullptr,
+                                           nullptr, nullptr, nullptr, nullptr, nullptr, stage, zph, 0.0, 0.0);
+          source_adj(h, s, ers, g2, g3);
+          const double dzprev = (stage == 1 && n > 0) ? zr[3 * (n - 1) + 2] : 0.0;
+          hk_launch<CM, PRO_NONE, EPI_ADJ>(h, s, s->faf_hi, s->faf_lo, s->kp_f, nk, nr, h->nw, pl_ar, g3, nullptr,
+                                           nullptr, gq, gs, gk, nullptr, nullptr, stage, zph, dz, dzprev);
+        }
+      }
+    }
+    // set_field: q0 = Hf a0
+    hk_launch<CM, PRO_NONE, EPI_NONE>(h, s, s->fab_hi, s->fab_lo, s->kp_b, nr, nk, h->nw, pl_a0, gq, g2, nullptr,
+                                      nullptr, nullptr, nullptr, nullptr, nullptr, 0, 0.0, 0.0, 0.0);
+    launchf(h, UPPE_KC_POINT, f_to_host<T>, dim3(grid_for(P * h->nw)), dim3(kBlock), 0, (const C*)g2, s->stage,
+            h->nw, P, s->nwp);
+    UPPE_HIP(hipMemcpyAsync(g_a0_r, s->stage, sizeof(double) * 2 * h->npt, hipMemcpyDeviceToHost, h->stream));
+    UPPE_HIP(hipStreamSynchronize(h->stream));
+  }
+
+  static void check_adjoint(uppe_handle* h, FusedState* s) {
+    if (h->d.geometry != UPPE_RADIAL || s->cm < 0)
+      throw Error{UPPE_ERR_UNSUPPORTED, "adjoint: radial geometry on the matrix-core path only"};
+    if (h->paraxial || s->ion.kind != UPPE_ION_NONE || h->rot_states > 2)
+      throw Error{UPPE_ERR_UNSUPPORTED, "adjoint: Kerr-only UPPE media (no ionizer, no rotator)"};
+  }
+
+  static int advance_checkpointed(uppe_handle* h, FusedState* s, const uppe_step_control* c, int every) {
+    check_adjoint(h, s);
+    if constexpr (sizeof(T) == 8) {
+      return advance_ck<CM_F64>(h, s, c, every);
+    } else {
+      if (s->cm == CM_BF16X3) return advance_ck<CM_BF16X3>(h, s, c, every);
+      return advance_ck<CM_F32>(h, s, c, every);
+    }
+  }
+
+  static void adjoint(uppe_handle* h, FusedState* s, double z_total, const double* g_a_r, double* g_a0_r) {
+    check_adjoint(h, s);
+    if (!s->recorded || s->rec_steps < 1) throw Error{UPPE_ERR_STATE, "adjoint: no checkpointed forward run"};
+    if constexpr (sizeof(T) == 8) {
+      adjoint_cm<CM_F64>(h, s, z_total, g_a_r, g_a0_r);
+    } else {
+      if (s->cm == CM_BF16X3)
+        adjoint_cm<CM_BF16X3>(h, s, z_total, g_a_r, g_a0_r);
+      else
+        adjoint_cm<CM_F32>(h, s, z_total, g_a_r, g_a0_r);
+    }
   }
 
   static int advance(uppe_handle* h, FusedState* s, const uppe_step_control* c) {
@@ -2240,7 +2644,8 @@ void fused_destroy(uppe_handle* h) {
   if (!s) return;
   void* ptrs[] = {s->phs, s->kzc, s->w, s->nlf, s->cond, s->ng, s->Hb, s->Hf, s->tw, s->q, s->k1, s->k2, s->k3,
                   s->er, s->jr, s->chi_t, s->ne_t, s->stage, s->qmax, s->st, s->sched, s->rec, s->frb_hi,
-                  s->frb_lo, s->frf_hi, s->frf_lo, s->tw_x, s->tw_y, s->rot_tab};
+                  s->frb_lo, s->frf_hi, s->frf_lo, s->tw_x, s->tw_y, s->rot_tab, s->fab_hi, s->fab_lo, s->faf_hi,
+                  s->faf_lo, s->ck_q, s->ck_st, s->ck_qmax, s->zrec, s->er_store, s->gs, s->gq, s->gk, s->g2, s->g3};
   for (void* p : ptrs)
     if (p) (void)hipFree(p);
   if (s->st_host) (void)hipHostFree(s->st_host);
@@ -2300,6 +2705,18 @@ void fused_set_medium(uppe_handle* h, const uppe_medium* m) {
     Fused<double>::set_medium(h, h->fused, m);
   else
     Fused<float>::set_medium(h, h->fused, m);
+}
+
+int fused_advance_checkpointed(uppe_handle* h, const uppe_step_control* c, int every) {
+  return is_fp64(h) ? Fused<double>::advance_checkpointed(h, h->fused, c, every)
+                    : Fused<float>::advance_checkpointed(h, h->fused, c, every);
+}
+
+void fused_adjoint(uppe_handle* h, double z_total, const double* g_a_r, double* g_a0_r) {
+  if (is_fp64(h))
+    Fused<double>::adjoint(h, h->fused, z_total, g_a_r, g_a0_r);
+  else
+    Fused<float>::adjoint(h, h->fused, z_total, g_a_r, g_a0_r);
 }
 
 void fused_set_paraxial_medium(uppe_handle* h, const uppe_paraxial_medium* m) {

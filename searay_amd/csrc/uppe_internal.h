@@ -336,6 +336,8 @@ void fused_set_hankel(uppe_handle* h);
 void fused_set_rotator(uppe_handle* h);
 void fused_set_medium(uppe_handle* h, const uppe_medium* m);
 void fused_set_paraxial_medium(uppe_handle* h, const uppe_paraxial_medium* m);
+int fused_advance_checkpointed(uppe_handle* h, const uppe_step_control* c, int every);
+void fused_adjoint(uppe_handle* h, double z_total, const double* g_a_r, double* g_a0_r);
 void fused_set_field(uppe_handle* h, const double* a_r);
 int fused_advance(uppe_handle* h, const uppe_step_control* c);
 void fused_finish(uppe_handle* h, double z_total, double* a_r, double* j_r, double* chi_t, double* ne_t);
