@@ -33,12 +33,12 @@ class SpectralPhaseProblem:
     '''The spectral-phase objective on one diagonal interval of a radial UPPE workload (vol_dict as for track).'''
 
     def __init__(self, vol, sigma, precision='fp32', checkpoint_every=10, n_phase=32, w0=1.0, target_seed=1005,
-                 target=None):
+                 target=None, stepper_factory=None):
         self.rs = uppe_kernel.RunSetup(vol)
         if self.rs.diagnostic_steps != 1:
             raise ValueError('the objective is defined on one diagonal interval (wave grid with two planes)')
         self.every = int(checkpoint_every)
-        self.st = self.rs.make_stepper(uppe_kernel.hip_stepper_factory(precision))
+        self.st = self.rs.make_stepper(stepper_factory or uppe_kernel.hip_stepper_factory(precision))
         self.basis = legendre_basis(self.rs.w, w0, sigma, n_phase)
         self.zi, self.zf, self.L = self.rs.plane(0)
         self._a0 = None

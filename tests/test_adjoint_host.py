@@ -1,0 +1,63 @@
+'''Host side of the spectral-phase gradient (searay_amd.adjoint), on the CPU: the Legendre basis, and the
+assembly dL/dc_n = -sum P_n Im(conj(G_0) A_0) from the cotangents, checked by central differences with a linear
+stand-in for the stepper (finish returns M a0 per radial cell, the adjoint M^H g).'''
+import numpy as np
+
+from conftest import rel_l2
+from searay_amd import adjoint, configs
+
+
+def test_legendre_basis():
+    w = np.linspace(0.0, 2.0, 201)
+    B = adjoint.legendre_basis(w, 1.0, 0.125, 4)  # 3 sigma = 0.375 exactly
+    x = (w - 1.0) / 0.375
+    inside = np.abs(x) <= 1
+    assert np.all(B[:, ~inside] == 0.0)
+    assert np.allclose(B[0, inside], 1.0) and np.allclose(B[1, inside], x[inside])
+    assert np.allclose(B[2, inside], 0.5 * (3 * x[inside] ** 2 - 1))
+
+
+class _LinearStepper:
+    def __init__(self, nw, seed=9):
+        rng = np.random.default_rng(seed)
+        self.M = (rng.standard_normal((nw, nw)) + 1j * rng.standard_normal((nw, nw))) / np.sqrt(nw)
+        self.a0 = None
+
+    def set_hankel(self, *a):
+        pass
+
+    def set_medium(self, *a):
+        pass
+
+    def set_rotator(self, *a):
+        pass
+
+    def set_field(self, a0):
+        self.a0 = np.array(a0)
+
+    def advance(self, *a, **k):
+        return 1, None
+
+    def advance_checkpointed(self, *a, **k):
+        return 1, None
+
+    def finish(self, z, want_diagnostics=True):
+        return np.einsum('fg,grm->frm', self.M, self.a0), None, None, None
+
+    def adjoint(self, z, g):
+        return np.einsum('gf,grm->frm', np.conj(self.M), g)
+
+    def close(self):
+        pass
+
+
+def test_gradient_assembly_against_central_differences():
+    vol = configs.radial_kerr_air(scale=16, steps=2)
+    nw = vol['wave grid'][0]
+    p = adjoint.SpectralPhaseProblem(vol, configs.kerr_pulse_sigma(), n_phase=8,
+                                     stepper_factory=lambda *a, **k: _LinearStepper(nw))
+    c = 0.3 * np.random.default_rng(2).standard_normal(8)
+    loss, g = p.value_and_grad(c)
+    fd = np.array([(p.loss(c + 1e-6 * e) - p.loss(c - 1e-6 * e)) / 2e-6 for e in np.eye(8)])
+    assert loss > 0 and np.linalg.norm(g) > 0
+    assert rel_l2(g, fd) < 1e-6

@@ -1,0 +1,104 @@
+'''Adjoint gradients of the spectral-phase objective (searay_amd.adjoint) through the HIP library.
+
+The FP64 adjoint is exact for the discrete forward map, so it must match central finite differences of the FP64
+forward to their truncation error (h = 1e-4: ~1e-8 relative, tested at 1e-6). FP32 and BF16x3 gradients are
+compared with the FP64 adjoint (1e-4 and 1e-3 relative L2). Replays from checkpoints are deterministic, so the
+gradient does not depend on the checkpoint interval.
+'''
+import sys
+
+import numpy as np
+import pytest
+
+from conftest import rel_l2
+from searay_amd import _abi, adjoint, configs
+
+pytestmark = pytest.mark.gpu
+
+
+def _say(msg):
+    print(msg, flush=True)
+    sys.stdout.flush()
+
+
+def _problem(scale, steps, precision, every, target=None):
+    vol = configs.radial_kerr_air(scale=scale, steps=steps)
+    return adjoint.SpectralPhaseProblem(vol, configs.kerr_pulse_sigma(), precision=precision,
+                                        checkpoint_every=every, target=target)
+
+
+def _coeffs(seed=3):
+    return 0.05 * np.random.default_rng(seed).standard_normal(32)
+
+
+def test_fp64_gradient_against_central_differences():
+    _say('adjoint fp64 vs central FD, radial Kerr scale=8 steps=4')
+    p = _problem(8, 4, 'fp64', 2)
+    c = _coeffs()
+    loss, g = p.value_and_grad(c)
+    fd = np.array([(p.loss(c + 1e-4 * e) - p.loss(c - 1e-4 * e)) / 2e-4 for e in np.eye(32)])
+    p.close()
+    assert loss > 0 and np.linalg.norm(g) > 0
+    assert rel_l2(g, fd) < 1e-6
+
+
+def test_fp64_directional_derivative_with_segment_replay():
+    '''20 steps in segments of 3 (the last one partial): every segment but the last is replayed.'''
+    _say('adjoint fp64 directional derivative, scale=4 steps=20 every=3')
+    p = _problem(4, 20, 'fp64', 3)
+    c = _coeffs(4)
+    _, g = p.value_and_grad(c)
+    d = np.random.default_rng(5).standard_normal(32)
+    d /= np.linalg.norm(d)
+    fd = (p.loss(c + 1e-4 * d) - p.loss(c - 1e-4 * d)) / 2e-4
+    p.close()
+    assert abs(fd - g @ d) < 1e-6 * abs(fd)
+
+
+def test_gradient_independent_of_checkpoint_interval():
+    _say('adjoint fp32, checkpoint intervals 1, 7, 20')
+    c = _coeffs(6)
+    grads = []
+    target = None
+    for every in (1, 7, 20):
+        p = _problem(4, 20, 'fp32', every, target)
+        target = p.target
+        grads.append(p.value_and_grad(c)[1])
+        p.close()
+    assert np.array_equal(grads[0], grads[1]) and np.array_equal(grads[0], grads[2])
+
+
+@pytest.mark.parametrize('precision,tol', [('fp32', 1e-4), ('bf16x3', 1e-3)])
+def test_reduced_precision_gradient_at_full_size(precision, tol):
+    '''The radial Kerr workload at full size (Nw = 513, Nr = 128, 100 steps), checkpoints every 10 steps.'''
+    _say('adjoint %s vs fp64 adjoint, radial Kerr 513 x 128, 100 steps' % precision)
+    c = _coeffs(7)
+    p64 = _problem(1, None, 'fp64', 10)
+    _, g64 = p64.value_and_grad(c)
+    p = _problem(1, None, precision, 10, p64.target)
+    _, g = p.value_and_grad(c)
+    p64.close()
+    p.close()
+    assert rel_l2(g, g64) < tol
+
+
+def test_autograd_function():
+    import torch
+    _say('adjoint through torch.autograd.Function')
+    p = _problem(8, 4, 'fp32', 2)
+    c = _coeffs(8)
+    loss_ref, g_ref = p.value_and_grad(c)
+    fn = adjoint.spectral_phase_loss_function()
+    ct = torch.tensor(c, dtype=torch.float64, requires_grad=True)
+    loss = fn.apply(ct, p)
+    (3.0 * loss).backward()
+    p.close()
+    assert abs(float(loss) - loss_ref) <= 1e-12 * abs(loss_ref)
+    assert np.allclose(ct.grad.numpy(), 3.0 * g_ref, rtol=1e-12, atol=0)
+
+
+def test_adjoint_rejects_ionization():
+    _say('adjoint rejects an ionizing medium')
+    vol = configs.radial_o2_ionization(scale=8, steps=2, length='1 mm')
+    with pytest.raises(_abi.UppeError):
+        adjoint.SpectralPhaseProblem(vol, configs.kerr_pulse_sigma(), precision='fp32').value_and_grad(np.zeros(32))
