@@ -133,6 +133,40 @@ def config1_speedup(prec, steps_plain):
                 speedup_vs_fp64_plain=out[prec] / out['fp64-plain'], rel_l2_err_vs_fp64=err)
 
 
+def adjoint_cost(prec='fp32', reps=3):
+    '''Config 5: the gradient of the on-axis output-spectrum objective with respect to 32 Legendre spectral-phase
+    coefficients over the radial Kerr workload (Nw = 513, Nr = 128, 100 steps), by the adjoint with checkpoints
+    every 10 steps (and with every stage kept, 100), against a plain forward run of the same objective; and the
+    gradient's relative L2 error against the FP64 adjoint.'''
+    from searay_amd import adjoint
+    vol = configs.radial_kerr_air()
+    sig = configs.kerr_pulse_sigma()
+    c = 0.05 * np.random.default_rng(3).standard_normal(32)
+    p64 = adjoint.SpectralPhaseProblem(vol, sig, precision='fp64', checkpoint_every=10)
+    _, g64 = p64.value_and_grad(c)
+    p64.close()
+    out = dict(workload='gradient of the on-axis output spectrum objective, 32 Legendre phase coefficients, '
+                        'radial Kerr Nw=513 Nr=128, 100 steps', precision=prec)
+    for every in (10, 100):
+        p = adjoint.SpectralPhaseProblem(vol, sig, precision=prec, checkpoint_every=every, target=p64.target)
+        p.loss(c)
+        p.value_and_grad(c)
+        t0 = time.perf_counter()
+        for _ in range(reps):
+            p.loss(c)
+        tf = (time.perf_counter() - t0) / reps
+        t0 = time.perf_counter()
+        for _ in range(reps):
+            _, g = p.value_and_grad(c)
+        tg = (time.perf_counter() - t0) / reps
+        p.close()
+        out['checkpoint_every_%d' % every] = dict(forward_ms=tf * 1e3, forward_and_gradient_ms=tg * 1e3,
+                                                  gradient_over_forward=tg / tf,
+                                                  grad_rel_l2_err_vs_fp64=float(np.linalg.norm(g - g64) /
+                                                                                np.linalg.norm(g64)))
+    return out
+
+
 def cpu_baseline(module, vol, steps):
     from oracle import oracle
     rs = module.RunSetup(vol)
@@ -264,6 +298,8 @@ def main():
         if not paraxial:
             log('radial Kerr workload (Nw=513, Nr=128): %s vs plain FP64' % prec)
             extras['config1'] = config1_speedup(prec, args.plain_steps)
+            log('adjoint gradient cost (config 5)')
+            extras['adjoint'] = adjoint_cost()
 
     cpu = None
     if not args.no_cpu:
