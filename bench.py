@@ -18,7 +18,13 @@ Printed JSON line (one):
                 summary of the same kernel and workload, or null
   cpu_baseline  the FP64 oracle (reference algorithm and pass structure, OpenMP) on a bounded sample
   extras        speedup over the plain FP64 GPU stepper, relative L2 error of the state against the fused FP64
-                stepper on the same step schedule
+                stepper on the same step schedule; the same on the radial Kerr workload (Nw = 513, Nr = 128,
+                100 fixed steps); the cost of the adjoint gradient of an output-spectrum objective on that workload
+                against a forward run, with checkpoints every 10 steps and with every stage kept, and its error
+                against the FP64 adjoint
+
+``--workload adjoint`` runs only that adjoint measurement (forward and gradient sweeps, FP32 by default), for
+kernel traces of the adjoint.
 
 ``--workload paraxial`` measures the paraxial envelope stepper the same way on the focused air filament of the
 reference's paraxial example (Nw = 256 envelope frequencies, Nr = 256 cells with 128 Hankel modes, Kerr + O2
@@ -111,7 +117,7 @@ def pcie_inclusive(rs, prec, K):
     return rs.N[0] * rs.N[1] * n / (t1 - t0)
 
 
-def config1_speedup(prec, steps_plain):
+def kerr_513x128_speedup(prec, steps_plain):
     '''Radial Kerr workload (Nw = 513, Nr = 128, 100 fixed steps over 1 m): the fused stepper against the plain
     FP64 stepper, and its field error against the fused FP64 stepper (fixed steps: identical schedules).'''
     vol = configs.radial_kerr_air()
@@ -134,7 +140,7 @@ def config1_speedup(prec, steps_plain):
 
 
 def adjoint_cost(prec='fp32', reps=3):
-    '''Config 5: the gradient of the on-axis output-spectrum objective with respect to 32 Legendre spectral-phase
+    '''The gradient of the on-axis output-spectrum objective with respect to 32 Legendre spectral-phase
     coefficients over the radial Kerr workload (Nw = 513, Nr = 128, 100 steps), by the adjoint with checkpoints
     every 10 steps (and with every stage kept, 100), against a plain forward run of the same objective; and the
     gradient's relative L2 error against the FP64 adjoint.'''
@@ -187,13 +193,14 @@ def main():
     ap.add_argument('--gpus', type=int, default=1)
     ap.add_argument('--steps', type=int, default=400)
     ap.add_argument('--warmup', type=int, default=20)
-    ap.add_argument('--precision', default='bf16x3', choices=['bf16x3', 'fp32', 'fp64', 'fp64-plain'])
+    ap.add_argument('--precision', default=None, choices=['bf16x3', 'fp32', 'fp64', 'fp64-plain'],
+                    help='default: bf16x3 (fp32 for --workload adjoint)')
     ap.add_argument('--rate', default='explicit', choices=['explicit', 'series'])
     ap.add_argument('--plain-steps', type=int, default=20)
     ap.add_argument('--cpu-steps', type=int, default=60)
     ap.add_argument('--no-cpu', action='store_true')
     ap.add_argument('--no-compare', action='store_true')
-    ap.add_argument('--workload', default='uppe', choices=['uppe', 'paraxial'])
+    ap.add_argument('--workload', default='uppe', choices=['uppe', 'paraxial', 'adjoint'])
     ap.add_argument('--calibrate-copy', type=int, default=0, metavar='MIB',
                     help='only run a plain device-to-device copy of MIB MiB (for scaling HBM byte counters)')
     args = ap.parse_args()
@@ -213,6 +220,14 @@ def main():
         print('bench.py runs on one GPU only: this stepper has no multi-GPU path (use --gpus 1)', file=sys.stderr)
         sys.exit(2)
 
+    if args.workload == 'adjoint':
+        prec = args.precision or 'fp32'
+        if prec == 'fp64-plain':
+            sys.exit('the adjoint runs on the fused paths only')
+        log('adjoint gradient cost (%s), radial Kerr 513 x 128' % prec)
+        print(json.dumps(adjoint_cost(prec)))
+        return
+
     paraxial = args.workload == 'paraxial'
     if paraxial:
         vol = configs.paraxial_air_filament(steps=400, rate=args.rate)
@@ -223,7 +238,7 @@ def main():
     rs = module.RunSetup(vol)
     pts = rs.N[0] * rs.N[1]
     K, W = args.steps, args.warmup
-    prec = args.precision
+    prec = args.precision or 'bf16x3'
 
     st = rs.make_stepper(uppe_kernel.hip_stepper_factory(prec))
     log('warmup %d steps (%s)' % (W, prec))
@@ -297,8 +312,8 @@ def main():
         extras['pcie_inclusive_pt_steps_per_s'] = pcie_inclusive(rs, prec, K)
         if not paraxial:
             log('radial Kerr workload (Nw=513, Nr=128): %s vs plain FP64' % prec)
-            extras['config1'] = config1_speedup(prec, args.plain_steps)
-            log('adjoint gradient cost (config 5)')
+            extras['radial_kerr_513x128'] = kerr_513x128_speedup(prec, args.plain_steps)
+            log('adjoint gradient cost, radial Kerr 513 x 128')
             extras['adjoint'] = adjoint_cost()
 
     cpu = None
