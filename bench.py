@@ -173,6 +173,47 @@ def adjoint_cost(prec='fp32', reps=3):
     return out
 
 
+def batch_workload(prec, K, W, seq_steps):
+    '''The 64-instance batched workload (configs.batched_radial): every instance stepped K fixed steps in one
+    batch (one launch per kernel for each run of equally shaped instances, autotuned launch shapes), against the
+    same instances stepped one after another as standalone handles (seq_steps steps each).'''
+    from searay_amd import batch
+    vols, drawn = configs.batched_radial(n=64, steps=200)
+    pts = sum(v['wave grid'][0] * v['wave grid'][1] for v in vols)
+    b = batch.Batch(vols, precision=prec)
+    b.set_fields()
+    log('batch warmup %d steps (autotune)' % W)
+    b.advance(max_steps=max(W, 1))
+    b.set_fields()
+    b.synchronize()
+    log('batch timed region: %d steps of 64 instances' % K)
+    t0 = time.perf_counter()
+    steps = b.advance(max_steps=K)
+    b.synchronize()
+    secs = time.perf_counter() - t0
+    plans = {}
+    for i, d in enumerate(drawn):
+        plans.setdefault(d[0], b.plan(i))
+    nan = sum(m.info()['nan_steps'] for m in b.members)
+    finite = all(bool(np.all(np.isfinite(m.get_state()))) for m in b.members)
+    b.close()
+    n = int(steps.min())
+    log('sequential standalone handles, %d steps each' % seq_steps)
+    tseq = 0.0
+    for v in vols:
+        rs = uppe_kernel.RunSetup(v)
+        st = rs.make_stepper(uppe_kernel.hip_stepper_factory(prec))
+        run_steps(rs, st, 1)
+        _, dt, _ = run_steps(rs, st, seq_steps)
+        tseq += dt
+        st.close()
+    seq_value = pts * seq_steps / tseq
+    return dict(value=pts * n / secs, steps=n, secs=secs, pts=pts, nan_steps=nan, finite=finite,
+                grids=sorted({d[0] for d in drawn}), plans={str(k): v for k, v in sorted(plans.items())},
+                sequential_pt_steps_per_s=seq_value, speedup_vs_sequential=pts * n / secs / seq_value,
+                instances=[dict(nr=d[0], power_pcr=round(d[1], 4), gas=d[2]) for d in drawn])
+
+
 def cpu_baseline(module, vol, steps):
     from oracle import oracle
     rs = module.RunSetup(vol)
@@ -200,7 +241,7 @@ def main():
     ap.add_argument('--cpu-steps', type=int, default=60)
     ap.add_argument('--no-cpu', action='store_true')
     ap.add_argument('--no-compare', action='store_true')
-    ap.add_argument('--workload', default='uppe', choices=['uppe', 'paraxial', 'adjoint'])
+    ap.add_argument('--workload', default='uppe', choices=['uppe', 'paraxial', 'adjoint', 'batch'])
     ap.add_argument('--calibrate-copy', type=int, default=0, metavar='MIB',
                     help='only run a plain device-to-device copy of MIB MiB (for scaling HBM byte counters)')
     args = ap.parse_args()
@@ -226,6 +267,23 @@ def main():
             sys.exit('the adjoint runs on the fused paths only')
         log('adjoint gradient cost (%s), radial Kerr 513 x 128' % prec)
         print(json.dumps(adjoint_cost(prec)))
+        return
+
+    if args.workload == 'batch':
+        prec = args.precision or 'fp32'
+        if prec == 'fp64-plain':
+            sys.exit('batches run on the fused paths only')
+        K = min(args.steps, 200)
+        r = batch_workload(prec, K, args.warmup, seq_steps=args.plain_steps)
+        log('%d steps in %.3f s: %.4e pt-steps/s (%.2fx the sequential handles)' % (r['steps'], r['secs'], r['value'],
+                                                                                r['speedup_vs_sequential']))
+        print(json.dumps(dict(metric='batched radial UPPE grid-pt z-steps/s (1 GPU)', value=r['value'],
+                              unit='grid-pt z-steps/s', n_gpus=1, steps=r['steps'], warmup=args.warmup,
+                              ms_per_step=1e3 * r['secs'] / r['steps'], higher_is_better=True, scaling='weak',
+                              vs_baseline=None, dtype=DTYPE[prec], data='synthetic (seeded instances)',
+                              config=dict(workload='64 radial instances, Nw=1025, Nr in {64,128,256,512}, O2/Ar, '
+                                                   '0.1-5 P_cr, 200 fixed steps over 5 mm', precision=prec),
+                              batch=r)))
         return
 
     paraxial = args.workload == 'paraxial'

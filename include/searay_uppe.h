@@ -198,6 +198,27 @@ enum { UPPE_KC_HANKEL = 0, UPPE_KC_SOURCE = 1, UPPE_KC_POINT = 2, UPPE_KC_REDUCE
 int uppe_profile_enable(uppe_handle* h, int enable);
 int uppe_profile_read(uppe_handle* h, int kernel_class, double* total_ms, long long* launches);
 
+/* Batches (no reference counterpart: the reference steps one grid per propagator call). Many radial UPPE
+ * instances stepped together: uppe_batch_create makes one member handle per descriptor (radial, UPPE model,
+ * FP64 / FP32 / BF16X3), whose per-instance arrays the batch keeps in one slab per (Nw, Nr, Nk, precision)
+ * group. Members are loaded like any handle (uppe_set_hankel, uppe_set_medium, uppe_set_field) and read back
+ * with uppe_finish / uppe_get_state / uppe_get_info; uppe_destroy refuses them (uppe_batch_destroy releases
+ * them). uppe_batch_advance runs uppe_advance on every member at once, c[i] being member i's step control
+ * (adaptive or fixed steps: no schedule, no record; members may differ in subcycles and step counts) and
+ * steps_out[i] (nullable) its step count: consecutive members of a group with the same nonlinear band and
+ * frequency tables share every launch. Rotators are not taken. With autotune on (the default) the first
+ * advance of each run times the candidate contraction tilings and source launch shapes on the run and keeps
+ * the fastest; uppe_batch_plan reports member i's run as {k2r tiles, waves, row tiles per wave, r2k tiles,
+ * waves, row tiles per wave, source samples per thread (0 = default), run length, first slot, group}. */
+typedef struct uppe_batch uppe_batch;
+int uppe_batch_create(const uppe_desc* descs, int n, uppe_batch** out);
+int uppe_batch_member(uppe_batch* b, int i, uppe_handle** member);
+int uppe_batch_advance(uppe_batch* b, const uppe_step_control* c, int* steps_out);
+int uppe_batch_set_autotune(uppe_batch* b, int enable);
+int uppe_batch_plan(const uppe_batch* b, int member, int* plan /* [10] */);
+int uppe_batch_destroy(uppe_batch* b);
+const char* uppe_batch_last_error(const uppe_batch* b);
+
 /* Block until all work on the handle's stream is done. */
 int uppe_synchronize(uppe_handle* h);
 

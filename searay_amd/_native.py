@@ -21,8 +21,9 @@ def library():
             raise ImportError('libsearay_uppe.so not found at %s; build it with `python -m searay_amd.build`'
                               % LIB_PATH)
         lib = ct.CDLL(LIB_PATH)
-        for name in _abi.HIP_ONLY_ENTRY_POINTS:
+        for name in _abi.HIP_ONLY_ENTRY_POINTS + _abi.BATCH_ENTRY_POINTS:
             getattr(lib, 'uppe_' + name).restype = ct.c_int
+        lib.uppe_batch_last_error.restype = ct.c_char_p
         _lib = lib
     return _lib
 
@@ -68,3 +69,21 @@ class HipStepper(_abi.Stepper):
         self._check(self.lib.uppe_adjoint(self.h, ct.c_double(z_total), _abi._ptr(g.view(np.float64)),
                                           _abi._ptr(out.view(np.float64))), 'adjoint')
         return out
+
+
+class BatchMember(HipStepper):
+    '''A member handle of a batch (searay_amd.batch): a stepper like any other, owned by its batch.'''
+
+    def __init__(self, handle, shape, model=_abi.MODEL_UPPE):
+        self.lib = library()
+        self.prefix = 'uppe_'
+        self.shape = tuple(shape)
+        self.model = model
+        self.nt = 2 * (self.shape[0] - 1)
+        self._bind()
+        self._keep = []
+        self.h = handle
+
+    def close(self):
+        # the batch releases its members
+        self.h = None

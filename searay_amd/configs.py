@@ -354,3 +354,58 @@ def paraxial_air_filament(scale=1, steps=None, seed=1004, precision=None, rate='
     if precision:
         vol['uppe precision'] = precision
     return vol
+
+
+def argon_ionizer():
+    '''Ar: Ip = 15.76 eV (photoionization_base.py:68), Z = 1, PPT tunnelling limit.'''
+    helper = input_tools.InputHelper(MKS_LENGTH)
+    return ionization.PPT_Tunneling(MKS_LENGTH, False, helper.dnum('15.76 eV'), 1.0, lstar=0, l=0, m=0, w0=1.0,
+                                    terms=80)
+
+
+def batched_radial(n=64, seed=1040, scale=1, steps=200, length='5 mm', precision=None):
+    '''Independent radial UPPE instances for a batched run: Nr drawn uniformly from {64, 128, 256, 512}
+    (each divided by `scale`, at least 8), Nw = 1025 over band (0, 4) (Nw - 1 and the band divided by `scale`),
+    800 nm 40 fs pulses with w0 = 100 um and peak power log-uniform over 0.1-5 P_cr with 1 % spectral-phase
+    noise, the gas O2 (12.07 eV, Z = 0.53) or Ar (15.76 eV, Z = 1) at 5.4e18 cm^-3 with Kerr, photoionisation
+    (PPT tunnelling) and the plasma current; a 2 mm window and `steps` fixed steps over `length` (5 mm: the
+    most powerful pulses self-focus within about 1 cm, where fixed 100 um steps no longer resolve the collapse
+    and the field overflows, in FP64 as in FP32). Returns the list of vol_dicts and the drawn (Nr, P/P_cr, gas)
+    of each.'''
+    helper = input_tools.InputHelper(MKS_LENGTH)
+    rng = np.random.default_rng(seed)
+    nw = 1024 // scale + 1
+    band = (0.0, 4.0 / scale)
+    L = helper.dnum(length)
+    t0 = helper.dnum('40 fs') / np.sqrt(2 * np.log(2))
+    w0 = helper.dnum('100 um')
+    vols, drawn = [], []
+    for i in range(n):
+        nr = max(8, int(rng.choice([64, 128, 256, 512])) // scale)
+        power = 10 ** rng.uniform(np.log10(0.1), np.log10(5.0))
+        gas = 'O2' if rng.integers(2) == 0 else 'Ar'
+        air = dispersion.SimpleAir(MKS_LENGTH)
+        vol = _base(helper, band, (helper.dnum('2 mm'), helper.dnum('2 mm'), L), (nw, nr, 1, 2), band, steps,
+                    L / steps, 'cylindrical', nr)
+        vol['dispersion inside'] = air
+        vol['dispersion outside'] = dispersion.Vacuum()
+        vol['window speed'] = air.GroupVelocityMagnitude(1.0)
+        vol['chi3'] = helper.chi3(1.0, '%g m2/W' % N2_AIR)
+        vol['ionizer'] = o2_ionizer('explicit') if gas == 'O2' else argon_ionizer()
+        P0 = power * critical_power()
+        energy = P0 * (t0 * helper.t1) / np.sqrt(2 / np.pi) / helper.u1
+        a0 = helper.a0(energy, t0, w0, 1.0)
+        phase_seed = seed * 1000 + i
+
+        def incoming(w, r, phi, a0=a0, phase_seed=phase_seed):
+            prng = np.random.default_rng(phase_seed)
+            phase = 2 * np.pi * 0.01 * (prng.random(w.shape) - 0.5)
+            spec = _gaussian_spectrum(w, t0)
+            amp = _normalise(spec, a0)
+            return amp * np.einsum('i,j,k', spec * np.exp(1j * phase), np.exp(-r ** 2 / w0 ** 2), np.ones(phi.shape))
+        vol['incoming wave'] = incoming
+        if precision:
+            vol['uppe precision'] = precision
+        vols.append(vol)
+        drawn.append((nr, power, gas))
+    return vols, drawn

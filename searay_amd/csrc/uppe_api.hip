@@ -79,9 +79,8 @@ static void destroy_impl(uppe_handle* h) {
   if (h->stream) (void)hipStreamDestroy(h->stream);
 }
 
-extern "C" {
-
-int uppe_create(const uppe_desc* d, uppe_handle** out) {
+// a new handle for descriptor d; a batch member (slots non-null) keeps its per-instance arrays in the batch's slabs
+static int create_handle(const uppe_desc* d, const MemberSlots* slots, uppe_handle** out) {
   if (!d || !out) return UPPE_ERR_ARG;
   *out = nullptr;
   if (d->model != UPPE_MODEL_UPPE && d->model != UPPE_MODEL_PARAXIAL) return UPPE_ERR_ARG;
@@ -110,10 +109,12 @@ int uppe_create(const uppe_desc* d, uppe_handle** out) {
     if (d->device < 0 || d->device >= ndev) throw Error{UPPE_ERR_DEVICE, "no such HIP device"};
     UPPE_HIP(hipSetDevice(d->device));
     UPPE_HIP(hipStreamCreateWithFlags(&h->stream, hipStreamNonBlocking));
-    if (d->precision == UPPE_PREC_FP64_PLAIN)
+    if (d->precision == UPPE_PREC_FP64_PLAIN) {
+      if (slots) throw Error{UPPE_ERR_UNSUPPORTED, "batches run on the fused paths"};
       plain_create(h);
-    else
-      fused_create(h);
+    } else {
+      fused_create(h, slots);
+    }
   });
   if (rc != UPPE_OK) {
     destroy_impl(h);
@@ -123,6 +124,10 @@ int uppe_create(const uppe_desc* d, uppe_handle** out) {
   *out = h;
   return UPPE_OK;
 }
+
+extern "C" {
+
+int uppe_create(const uppe_desc* d, uppe_handle** out) { return create_handle(d, nullptr, out); }
 
 int uppe_set_hankel(uppe_handle* h, const double* H, const double* lam) {
   if (!h) return UPPE_ERR_ARG;
@@ -397,6 +402,10 @@ int uppe_synchronize(uppe_handle* h) {
 
 int uppe_destroy(uppe_handle* h) {
   if (!h) return UPPE_OK;
+  if (h->batch) {
+    h->err = "handle belongs to a batch: uppe_batch_destroy releases it";
+    return UPPE_ERR_STATE;
+  }
   (void)hipSetDevice(h->d.device);
   if (h->stream) (void)hipStreamSynchronize(h->stream);
   destroy_impl(h);
@@ -405,5 +414,85 @@ int uppe_destroy(uppe_handle* h) {
 }
 
 const char* uppe_last_error(const uppe_handle* h) { return h ? h->err.c_str() : "null handle"; }
+
+}  // extern "C"
+
+// ---- batches ------------------------------------------------------------------------------------------
+
+template <typename F>
+static int batch_guarded(uppe_batch* b, F&& f) {
+  try {
+    f();
+    return UPPE_OK;
+  } catch (const Error& e) {
+    if (b) b->err = e.msg;
+    return e.code;
+  } catch (const std::exception& e) {
+    if (b) b->err = e.what();
+    return UPPE_ERR_DEVICE;
+  }
+}
+
+extern "C" {
+
+int uppe_batch_create(const uppe_desc* descs, int n, uppe_batch** out) {
+  if (!descs || !out || n < 1) return UPPE_ERR_ARG;
+  *out = nullptr;
+  auto* b = new uppe_batch();
+  b->device = descs[0].device;
+  int rc = batch_guarded(b, [&] {
+    UPPE_HIP(hipSetDevice(b->device));
+    fused_batch_create(b, descs, n, create_handle);
+  });
+  if (rc != UPPE_OK) {
+    fused_batch_destroy(b);
+    delete b;
+    return rc;
+  }
+  *out = b;
+  return UPPE_OK;
+}
+
+int uppe_batch_member(uppe_batch* b, int i, uppe_handle** h) {
+  if (!b || !h) return UPPE_ERR_ARG;
+  if (i < 0 || i >= (int)b->members.size()) {
+    b->err = "member index out of range";
+    return UPPE_ERR_ARG;
+  }
+  *h = b->members[(size_t)i];
+  return UPPE_OK;
+}
+
+int uppe_batch_advance(uppe_batch* b, const uppe_step_control* c, int* steps_out) {
+  if (!b) return UPPE_ERR_ARG;
+  if (!c) {
+    b->err = "null step controls";
+    return UPPE_ERR_ARG;
+  }
+  return batch_guarded(b, [&] {
+    UPPE_HIP(hipSetDevice(b->device));
+    fused_batch_advance(b, c, steps_out);
+  });
+}
+
+int uppe_batch_set_autotune(uppe_batch* b, int enable) {
+  if (!b) return UPPE_ERR_ARG;
+  return batch_guarded(b, [&] { fused_batch_set_autotune(b, enable); });
+}
+
+int uppe_batch_plan(const uppe_batch* b, int member, int* plan) {
+  if (!b || !plan || member < 0 || member >= (int)b->members.size()) return UPPE_ERR_ARG;
+  return batch_guarded(const_cast<uppe_batch*>(b), [&] { fused_batch_plan(b, member, plan); });
+}
+
+int uppe_batch_destroy(uppe_batch* b) {
+  if (!b) return UPPE_OK;
+  (void)hipSetDevice(b->device);
+  fused_batch_destroy(b);
+  delete b;
+  return UPPE_OK;
+}
+
+const char* uppe_batch_last_error(const uppe_batch* b) { return b ? b->err.c_str() : "null batch"; }
 
 }  // extern "C"

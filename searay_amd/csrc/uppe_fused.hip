@@ -37,7 +37,27 @@ struct StepState {
   int nanflag, nan_steps, steps, sched_n, sched_pos, rec_cap, error, max_steps;
 };
 
+// per-instance scalars of the source kernel in a batched launch (chi3/n_ref, rate model and its prefactor)
+struct SrcInst {
+  double redchi3, log_pre;
+  uppe_ionization ion;
+};
+
+// arrays a batch member keeps inside its batch's slabs (one slab per kind, instances a fixed stride apart)
+enum {
+  SL_PHS, SL_KZC, SL_NG, SL_Q, SL_K1, SL_K2, SL_K3, SL_ER, SL_JR, SL_ST, SL_QMAX,
+  SL_FRB_HI, SL_FRB_LO, SL_FRF_HI, SL_FRF_LO, SL_COUNT
+};
+struct MemberSlots {
+  void* p[SL_COUNT] = {};
+};
+
 struct FusedState {
+  // batch member: device addresses of the per-instance arrays inside the batch's slabs (not owned); null for
+  // a standalone handle
+  const MemberSlots* slots = nullptr;
+  // host copies of the frequency tables and source time step (a batched launch shares them between instances)
+  std::vector<double> w_h, nl_h, cond_h;
   int prec = 0;
   int nwp = 0;
   size_t el = 0;  // sizeof(T)
@@ -489,6 +509,12 @@ __device__ inline cplx<T> load_b(const cplx<T>* __restrict__ B, long idx, int f,
 
 // Workgroups are dealt to the 8 XCDs round-robin; renumber them so that each XCD owns a contiguous run of
 // column tiles, whose partial 128-byte lines then meet in that XCD's L2.
+// pointer of instance z of a batched launch (null stays null)
+template <typename P>
+__device__ inline P* zshift(P* p, long o) {
+  return p ? p + o : p;
+}
+
 __device__ inline int xcd_tile(int b, int G) {
   const int q = G >> 3, r = G & 7, x = b & 7, slot = b >> 3;
   return x < r ? x * (q + 1) + slot : r * (q + 1) + (x - r) * q + slot;
@@ -507,10 +533,30 @@ __global__ __launch_bounds__(1024) void f_hankel(
     cplx<typename MfmaTraits<CM>::T>* __restrict__ k1, cplx<typename MfmaTraits<CM>::T>* __restrict__ k2,
     cplx<typename MfmaTraits<CM>::T>* __restrict__ k3, StepState* __restrict__ st,
     unsigned long long* __restrict__ qmax, unsigned long long* __restrict__ qmax_next, int stage, double zfix,
-    double dzfix, double dzprev) {
+    double dzfix, double dzprev, long zs) {
   using T = typename MfmaTraits<CM>::T;
   using C = cplx<T>;
   using acc_t = typename MfmaTraits<CM>::acc_t;
+  if (zs) {
+    // batched launch: instance blockIdx.z of equally shaped instances whose arrays lie zs elements apart
+    // (state, tables), one fragment set apart (operator), one step state and three maxima apart
+    const long z = blockIdx.z, zo = z * zs;
+    const long fs = (long)((M + 15) / 16) * (CM == CM_BF16X3 ? Kp / 32 : Kp / 16) * 64;
+    Ahi = zshift(Ahi, z * fs);
+    Alo = zshift(Alo, z * fs);
+    B = zshift(B, zo);
+    out = zshift(out, zo);
+    phs = zshift(phs, zo);
+    kzc = zshift(kzc, zo);
+    q = zshift(q, zo);
+    kprev = zshift(kprev, zo);
+    k1 = zshift(k1, zo);
+    k2 = zshift(k2, zo);
+    k3 = zshift(k3, zo);
+    st = zshift(st, z);
+    qmax = zshift(qmax, 3 * z);
+    qmax_next = zshift(qmax_next, 3 * z);
+  }
   // no early exit for a finished run: the contraction only writes scratch (E_r) or stops before its
   // epilogue, so its loads need not wait for the step state
   extern __shared__ char smem[];
@@ -1229,13 +1275,24 @@ __global__ __launch_bounds__(1024) void f_source(const cplx<T>* __restrict__ er,
                                                  const T* __restrict__ nlf, const double* __restrict__ ng, int nt,
                                                  int lgn, int nwp, double redchi3, double dt, uppe_ionization ion,
                                                  double log_pre, const RotRow<T>* __restrict__ rot_tab,
-                                                 int rot_states, double rot_fc) {
+                                                 int rot_states, double rot_fc, const SrcInst* __restrict__ inst,
+                                                 long zs) {
   // no early exit for a finished run: the kernel only writes scratch (J_r, diagnostics)
   extern __shared__ char smem[];
   cplx<T>* Y = reinterpret_cast<cplx<T>*>(smem);
   __shared__ T wsum[16];
   const int N = nt, M = nt / 2, lgm = lgn - 1;
   const long p = blockIdx.x;
+  if (inst) {
+    // batched launch: instance blockIdx.y, strips zs elements apart, its own medium scalars
+    const int z = blockIdx.y;
+    er += z * zs;
+    jr += z * zs;
+    ng += (long)z * gridDim.x;
+    redchi3 = inst[z].redchi3;
+    log_pre = inst[z].log_pre;
+    ion = inst[z].ion;
+  }
   const int tid = threadIdx.x, nth = blockDim.x, lane = tid & 63, wave = tid >> 6, nwv = nth >> 6;
   const cplx<T>* X = er + p * nwp;
   // every global load the kernel makes up front, issued together: this strip's spectrum, the frequency and
@@ -1612,7 +1669,9 @@ __global__ __launch_bounds__(1024) void f_source_adj(const cplx<T>* __restrict__
 // ---------------------------------------------------------------------------------------------------
 
 // step size after the first RHS (uppe_kernel.py:221-227)
+// (one block per instance in a batched launch, which records nothing)
 __global__ void f_ctl_dz(StepState* st, const double* sched, double* rec, double* zrec, int zrec_cap) {
+  st += blockIdx.x;
   if (!st->active) return;
   double dz;
   if (st->sched_n > 0) {
@@ -1642,6 +1701,9 @@ __global__ void f_ctl_dz(StepState* st, const double* sched, double* rec, double
 // end of step: advance z, subcycle bookkeeping (uppe_kernel.py:299-325)
 // qmax[2] holds max|q| over the columns the r2k direction never updates (the current is zero there)
 __global__ void f_ctl_end(StepState* st, unsigned long long* qmax, unsigned long long* qmax_next) {
+  st += blockIdx.x;
+  qmax += 3 * blockIdx.x;
+  qmax_next += 3 * blockIdx.x;
   if (!st->active) return;
   st->z = st->z + st->dz;
   st->steps++;
@@ -1875,6 +1937,20 @@ static FusedState::HkPlan plan_hankel(int M, int ncols, int n_cu) {
   return p;
 }
 
+// whether p is one of a batch member's slab addresses (owned by the batch, never freed by the member)
+static bool slotted(const FusedState* s, const void* p) {
+  if (!s->slots) return false;
+  for (void* q : s->slots->p)
+    if (q == p) return true;
+  return false;
+}
+
+// fragments of a packed m x k operator (pack_k4 / pack_bf16x3): 16-row tiles x contraction chunks x 64 lanes
+static long frag_count(int cm, int m, int k) {
+  const long m16 = (m + 15) / 16;
+  return cm == CM_BF16X3 ? m16 * ((k + 31) / 32) * 64 : m16 * ((k + 15) / 16) * 64;
+}
+
 template <typename T>
 struct Fused {
   using C = cplx<T>;
@@ -1889,24 +1965,32 @@ struct Fused {
     UPPE_HIP(hipDeviceGetAttribute(&ncu, hipDeviceAttributeMultiprocessorCount, dev));
     s->n_cu = std::max(1, ncu);
     const size_t cs = sizeof(C) * P * s->nwp;
-    UPPE_HIP(hipMalloc(&s->phs, sizeof(double) * P * s->nwp));
-    UPPE_HIP(hipMalloc(&s->kzc, cs));
+    // a batch member's per-instance arrays live in its batch's slabs
+    auto take = [&](void** dst, int kind, size_t bytes) {
+      if (s->slots)
+        *dst = s->slots->p[kind];
+      else
+        UPPE_HIP(hipMalloc(dst, bytes));
+    };
+    take((void**)&s->phs, SL_PHS, sizeof(double) * P * s->nwp);
+    take(&s->kzc, SL_KZC, cs);
     UPPE_HIP(hipMalloc(&s->w, sizeof(T) * s->nwp));
     UPPE_HIP(hipMalloc(&s->nlf, sizeof(T) * s->nwp));
     UPPE_HIP(hipMalloc(&s->cond, sizeof(T) * s->nwp));
-    UPPE_HIP(hipMalloc(&s->ng, sizeof(double) * P));
+    take((void**)&s->ng, SL_NG, sizeof(double) * P);
     void** bufs[] = {&s->q, &s->k1, &s->k2, &s->k3, &s->er, &s->jr};
-    for (auto b : bufs) {
-      UPPE_HIP(hipMalloc(b, cs));
-      UPPE_HIP(hipMemset(*b, 0, cs));
+    const int kinds[] = {SL_Q, SL_K1, SL_K2, SL_K3, SL_ER, SL_JR};
+    for (int i = 0; i < 6; i++) {
+      take(bufs[i], kinds[i], cs);
+      UPPE_HIP(hipMemset(*bufs[i], 0, cs));
     }
     UPPE_HIP(hipMalloc(&s->chi_t, sizeof(T) * h->ntt));
     UPPE_HIP(hipMalloc(&s->ne_t, sizeof(T) * h->ntt));
     UPPE_HIP(hipMalloc(&s->stage, sizeof(double) * std::max(2 * h->npt, h->ntt)));
-    UPPE_HIP(hipMalloc(&s->qmax, sizeof(unsigned long long) * 3));
+    take((void**)&s->qmax, SL_QMAX, sizeof(unsigned long long) * 3);
     s->qmax_next = s->qmax + 1;
     UPPE_HIP(hipMemset(s->qmax, 0, sizeof(unsigned long long) * 3));
-    UPPE_HIP(hipMalloc(&s->st, sizeof(StepState)));
+    take((void**)&s->st, SL_ST, sizeof(StepState));
     UPPE_HIP(hipMemset(s->st, 0, sizeof(StepState)));
     UPPE_HIP(hipHostMalloc(&s->st_host, sizeof(StepState), hipHostMallocDefault));
     std::memset(s->st_host, 0, sizeof(StepState));
@@ -1942,7 +2026,7 @@ struct Fused {
     UPPE_HIP(hipMemcpy(s->Hf, hft.data(), sizeof(T) * hft.size(), hipMemcpyHostToDevice));
     void* old[] = {s->frb_hi, s->frb_lo, s->frf_hi, s->frf_lo, s->fab_hi, s->fab_lo, s->faf_hi, s->faf_lo};
     for (void* p : old)
-      if (p) UPPE_HIP(hipFree(p));
+      if (p && !slotted(s, p)) UPPE_HIP(hipFree(p));
     s->frb_hi = s->frb_lo = s->frf_hi = s->frf_lo = nullptr;
     s->fab_hi = s->fab_lo = s->faf_hi = s->faf_lo = nullptr;
     if (s->cm < 0) return;
@@ -1954,14 +2038,21 @@ struct Fused {
         hbt_[(size_t)k * nr + j] = hb[(size_t)j * nk + k];
       }
     int kpa = 0;
+    // the forward operators of a batch member go to its slots (sizes fixed by the shape: frag_count)
+    auto place = [&](const auto& v, int kind) -> void* {
+      if (!s->slots) return upload(v);
+      void* d = s->slots->p[kind];
+      UPPE_HIP(hipMemcpy(d, v.data(), sizeof(v[0]) * v.size(), hipMemcpyHostToDevice));
+      return d;
+    };
     if (s->cm == CM_BF16X3) {
       std::vector<uint16_t> hi, lo;
       pack_bf16x3(hb, nr, nk, hi, lo, &s->kp_b);
-      s->frb_hi = upload(hi);
-      s->frb_lo = upload(lo);
+      s->frb_hi = place(hi, SL_FRB_HI);
+      s->frb_lo = place(lo, SL_FRB_LO);
       pack_bf16x3(hf, nk, nr, hi, lo, &s->kp_f);
-      s->frf_hi = upload(hi);
-      s->frf_lo = upload(lo);
+      s->frf_hi = place(hi, SL_FRF_HI);
+      s->frf_lo = place(lo, SL_FRF_LO);
       pack_bf16x3(hft_, nr, nk, hi, lo, &kpa);
       s->fab_hi = upload(hi);
       s->fab_lo = upload(lo);
@@ -1971,9 +2062,9 @@ struct Fused {
     } else {
       std::vector<T> fr;
       pack_k4<T>(hb, nr, nk, fr, &s->kp_b);
-      s->frb_hi = upload(fr);
+      s->frb_hi = place(fr, SL_FRB_HI);
       pack_k4<T>(hf, nk, nr, fr, &s->kp_f);
-      s->frf_hi = upload(fr);
+      s->frf_hi = place(fr, SL_FRF_HI);
       pack_k4<T>(hft_, nr, nk, fr, &kpa);
       s->fab_hi = upload(fr);
       pack_k4<T>(hbt_, nk, nr, fr, &kpa);
@@ -1997,6 +2088,9 @@ struct Fused {
     UPPE_HIP(hipMemcpy(s->nlf, nl.data(), sizeof(T) * s->nwp, hipMemcpyHostToDevice));
     UPPE_HIP(hipMemcpy(s->cond, cd.data(), sizeof(T) * s->nwp, hipMemcpyHostToDevice));
     UPPE_HIP(hipMemcpy(s->ng, m->ngas, sizeof(double) * P, hipMemcpyHostToDevice));
+    s->w_h.assign(m->w, m->w + h->nw);
+    s->nl_h.assign(m->nl_filter, m->nl_filter + h->nw);
+    s->cond_h.assign(m->cond_filter, m->cond_filter + h->nw);
     double* kg = nullptr;
     UPPE_HIP(hipMalloc(&kg, sizeof(double) * h->nw));
     UPPE_HIP(hipMemcpy(kg, m->kg, sizeof(double) * h->nw, hipMemcpyHostToDevice));
@@ -2077,11 +2171,16 @@ struct Fused {
             (unsigned long long*)nullptr, (unsigned long long*)nullptr, 0, 0.0);
   }
 
-  static void source(uppe_handle* h, FusedState* s, const C* er, C* jr, bool diag, StepState* st) {
+  // the source kernel over every strip; a batched launch (nz instances zs elements apart, per-instance scalars
+  // inst) may ask for spt_req samples per thread
+  static void source(uppe_handle* h, FusedState* s, const C* er, C* jr, bool diag, StepState* st, int nz = 1,
+                     const SrcInst* inst = nullptr, long zs = 0, int spt_req = 0) {
     const int N = h->nt;
     // four samples per thread (Nt >= 256): short strips then get short workgroups, several per CU
-    const int nth = std::max(std::min(N, 64), N / 4);
+    const int nth = spt_req > 0 ? N / spt_req : std::max(std::min(N, 64), N / 4);
     const int spt = N / nth;
+    if (spt_req > 0 && (h->rot_states > 2 || h->paraxial || nth > 1024 || nth < 64))
+      throw Error{UPPE_ERR_ARG, "source kernel: unsupported samples per thread"};
     const size_t shm = sizeof(C) * (sizeof(T) == 4 ? N + N + N / 2 : N);
     const int lgn = ilog2i(N);
     // the explicit-rate prefactor, in the base of the exponential each precision uses
@@ -2090,7 +2189,7 @@ struct Fused {
                                                              : std::log(s->ion.rate_conv * s->ion.c_pre);
     (void)st;
     const bool rot = h->rot_states > 2;
-    const dim3 grid((unsigned)h->P), block(nth);
+    const dim3 grid((unsigned)h->P, (unsigned)nz), block(nth);
     if (h->paraxial) {
       // two N-point complex strips (E and A), plus the N-point tables in FP32
       const size_t shm_env = sizeof(C) * (sizeof(T) == 4 ? 3 * N : 2 * N);
@@ -2115,7 +2214,7 @@ struct Fused {
     launchf(h, UPPE_KC_SOURCE, kern, grid, block, shm, er, jr, diag ? (T*)s->chi_t : (T*)nullptr,                 \
             diag ? (T*)s->ne_t : (T*)nullptr, (const C*)s->tw, (const T*)s->w, (const T*)s->nlf,                  \
             (const double*)s->ng, N, lgn, s->nwp, s->redchi3, s->dt, s->ion, log_pre,                             \
-            (const RotRow<T>*)s->rot_tab, h->rot_states, 0.5 * h->rot_dalpha / h->rot_hbar);                    \
+            (const RotRow<T>*)s->rot_tab, h->rot_states, 0.5 * h->rot_dalpha / h->rot_hbar, inst, zs);          \
     return;                                                                                                      \
   }
     UPPE_SRC(1) UPPE_SRC(2) UPPE_SRC(4)
@@ -2147,21 +2246,22 @@ struct Fused {
   template <int CM, int PRO, int EPI>
   static void hk_launch(uppe_handle* h, FusedState* s, const void* ahi, const void* alo, int kp, int M, int K,
                         int ncols, const FusedState::HkPlan& pl, const C* B, C* out, const C* kprev, C* q, C* k1,
-                        C* k2, C* k3, StepState* st, int stage, double zfix, double dzfix, double dzprev) {
+                        C* k2, C* k3, StepState* st, int stage, double zfix, double dzfix, double dzprev,
+                        int nz = 1, long zs = 0) {
     using F = typename MfmaTraits<CM>::frag_t;
     constexpr int BNV = 16, BW = 2 * BNV;
     const int rw = 16 * pl.rt;
     const int bm = pl.nwv * rw;
     if ((long)pl.tiles * 8 + 8 <= ncols || (long)pl.tiles * 8 >= ncols + 8)
       throw Error{UPPE_ERR_STATE, "radial contraction plan out of date"};
-    dim3 grid((unsigned)pl.tiles, (unsigned)((M + bm - 1) / bm));
+    dim3 grid((unsigned)pl.tiles, (unsigned)((M + bm - 1) / bm), (unsigned)nz);
     size_t slab = CM == CM_BF16X3 ? 2ull * BW * (kp + 8) * 2 : (size_t)kp * BW * sizeof(T);
     size_t tile = (size_t)bm * BW * sizeof(T);
     size_t shm = std::max(slab, tile);
 #define UPPE_HK(RTV)                                                                                             \
   launchf(h, UPPE_KC_HANKEL, f_hankel<CM, PRO, EPI, BNV, RTV>, grid, dim3(64 * pl.nwv), shm, (const F*)ahi,    \
           (const F*)alo, B, out, M, K, kp, rw, s->nwp, 0, ncols, (const double*)s->phs, (const C*)s->kzc,         \
-          (const T*)s->cond, q, kprev, k1, k2, k3, st, s->qmax, s->qmax_next, stage, zfix, dzfix, dzprev)
+          (const T*)s->cond, q, kprev, k1, k2, k3, st, s->qmax, s->qmax_next, stage, zfix, dzfix, dzprev, zs)
     if (pl.rt == 1)
       UPPE_HK(1);
     else
@@ -2612,9 +2712,423 @@ struct Fused {
   }
 };
 
+// ---------------------------------------------------------------------------------------------------
+// Batches: many radial instances stepped together. Members with the same (Nw, Nr, Nk, precision) form a group
+// whose per-instance arrays lie in one slab per kind, a fixed stride apart, so that one launch of each step
+// kernel covers a run of consecutive members (grid z for the contractions, grid y for the source). A run
+// also needs the same nonlinear band and frequency tables; members whose tables differ start a new run. The
+// contraction plans and the source kernel's samples per thread of every run are chosen by timing the
+// candidates on the run itself (autotune), or taken from the single-instance heuristic.
+// ---------------------------------------------------------------------------------------------------
+struct BatchGroup {
+  int nw = 0, n1 = 0, nk = 0, prec = 0, cm = -1;
+  std::vector<int> members;         // member index of each slot
+  void* slab[SL_COUNT] = {};
+  size_t stride[SL_COUNT] = {};     // bytes between consecutive slots
+  SrcInst* src = nullptr;           // device, one per slot
+  StepState* st_host = nullptr;     // pinned, one per slot
+};
+
+struct BatchRun {
+  int group = 0, first = 0, count = 0;  // slots [first, first + count) of the group
+  FusedState::HkPlan plan[2];           // k2r, r2k
+  int spt = 0;                          // source samples per thread (0: the single-instance choice)
+};
+
+struct FusedBatch {
+  std::vector<BatchGroup> groups;
+  std::vector<MemberSlots> slots;  // per member (addresses handed to the member states)
+  std::vector<int> group_of, slot_of;
+  std::vector<BatchRun> runs;      // runs of the last advance
+  // tuned launch shapes by run: (group, first, count, f1) -> plans and samples per thread
+  std::vector<std::pair<std::vector<long>, BatchRun>> tuned;
+  bool autotune = true;
+};
+
+static int cm_of(int prec) {
+  return prec == UPPE_PREC_FP64 ? CM_F64 : (prec == UPPE_PREC_BF16X3 ? CM_BF16X3 : CM_F32);
+}
+
+void fused_batch_create(uppe_batch* b, const uppe_desc* descs, int n, MemberFactory make_member) {
+  auto* fb = new FusedBatch();
+  b->fb = fb;
+  fb->slots.resize((size_t)n);
+  fb->group_of.assign((size_t)n, -1);
+  fb->slot_of.assign((size_t)n, -1);
+  for (int i = 0; i < n; i++) {
+    const uppe_desc& d = descs[i];
+    if (d.geometry != UPPE_RADIAL || d.model != UPPE_MODEL_UPPE || d.n2 != 1)
+      throw Error{UPPE_ERR_UNSUPPORTED, "batch member " + std::to_string(i) + ": radial UPPE grids only"};
+    if (d.precision < UPPE_PREC_FP64 || d.precision > UPPE_PREC_BF16X3)
+      throw Error{UPPE_ERR_UNSUPPORTED, "batch member " + std::to_string(i) + ": fused precisions only"};
+    if (d.device != descs[0].device) throw Error{UPPE_ERR_ARG, "batch members must share one device"};
+    if (d.nk < 1 || d.nk > d.n1 || d.n1 < 1 || d.nw < 3) throw Error{UPPE_ERR_ARG, "batch member grid"};
+    int g = 0;
+    for (; g < (int)fb->groups.size(); g++) {
+      const BatchGroup& G = fb->groups[(size_t)g];
+      if (G.nw == d.nw && G.n1 == d.n1 && G.nk == d.nk && G.prec == d.precision) break;
+    }
+    if (g == (int)fb->groups.size()) {
+      BatchGroup G;
+      G.nw = d.nw;
+      G.n1 = d.n1;
+      G.nk = d.nk;
+      G.prec = d.precision;
+      G.cm = cm_of(d.precision);
+      fb->groups.push_back(G);
+    }
+    fb->group_of[(size_t)i] = g;
+    fb->slot_of[(size_t)i] = (int)fb->groups[(size_t)g].members.size();
+    fb->groups[(size_t)g].members.push_back(i);
+  }
+  for (BatchGroup& G : fb->groups) {
+    const size_t el = G.prec == UPPE_PREC_FP64 ? 8 : 4, nwp = (size_t)((G.nw + 31) / 32) * 32, P = (size_t)G.n1;
+    const size_t cs = 2 * el * P * nwp;
+    const size_t fbytes = G.cm == CM_F64 ? 32 : 16;  // one fragment: 4 doubles, 4 floats or 8 bf16
+    G.stride[SL_PHS] = 8 * P * nwp;
+    G.stride[SL_NG] = 8 * P;
+    for (int k : {SL_KZC, SL_Q, SL_K1, SL_K2, SL_K3, SL_ER, SL_JR}) G.stride[k] = cs;
+    G.stride[SL_ST] = sizeof(StepState);
+    G.stride[SL_QMAX] = 3 * sizeof(unsigned long long);
+    G.stride[SL_FRB_HI] = fbytes * (size_t)frag_count(G.cm, G.n1, G.nk);
+    G.stride[SL_FRF_HI] = fbytes * (size_t)frag_count(G.cm, G.nk, G.n1);
+    G.stride[SL_FRB_LO] = G.cm == CM_BF16X3 ? G.stride[SL_FRB_HI] : 0;
+    G.stride[SL_FRF_LO] = G.cm == CM_BF16X3 ? G.stride[SL_FRF_HI] : 0;
+    const size_t cnt = G.members.size();
+    for (int k = 0; k < SL_COUNT; k++) {
+      if (!G.stride[k]) continue;
+      UPPE_HIP(hipMalloc(&G.slab[k], G.stride[k] * cnt));
+      UPPE_HIP(hipMemset(G.slab[k], 0, G.stride[k] * cnt));
+    }
+    UPPE_HIP(hipMalloc(&G.src, sizeof(SrcInst) * cnt));
+    UPPE_HIP(hipHostMalloc(&G.st_host, sizeof(StepState) * cnt, hipHostMallocDefault));
+    for (size_t j = 0; j < cnt; j++) {
+      MemberSlots& ms = fb->slots[(size_t)G.members[j]];
+      for (int k = 0; k < SL_COUNT; k++)
+        ms.p[k] = G.stride[k] ? static_cast<char*>(G.slab[k]) + j * G.stride[k] : nullptr;
+    }
+  }
+  b->members.assign((size_t)n, nullptr);
+  for (int i = 0; i < n; i++) {
+    uppe_handle* h = nullptr;
+    const int rc = make_member(&descs[i], &fb->slots[(size_t)i], &h);
+    if (rc != UPPE_OK) throw Error{rc, "batch member " + std::to_string(i) + ": create failed"};
+    h->batch = b;
+    b->members[(size_t)i] = h;
+    if (h->fused->cm != fb->groups[(size_t)fb->group_of[(size_t)i]].cm)
+      throw Error{UPPE_ERR_UNSUPPORTED, "batches run on the matrix-core contraction"};
+  }
+}
+
+void fused_batch_destroy(uppe_batch* b) {
+  for (uppe_handle*& h : b->members) {
+    if (!h) continue;
+    h->batch = nullptr;
+    (void)uppe_destroy(h);
+    h = nullptr;
+  }
+  FusedBatch* fb = b->fb;
+  if (!fb) return;
+  for (BatchGroup& G : fb->groups) {
+    for (void* p : G.slab)
+      if (p) (void)hipFree(p);
+    if (G.src) (void)hipFree(G.src);
+    if (G.st_host) (void)hipHostFree(G.st_host);
+  }
+  delete fb;
+  b->fb = nullptr;
+}
+
+void fused_batch_set_autotune(uppe_batch* b, int enable) { b->fb->autotune = enable != 0; }
+
+void fused_batch_plan(const uppe_batch* b, int member, int* out) {
+  const FusedBatch* fb = b->fb;
+  const int g = fb->group_of[(size_t)member], j = fb->slot_of[(size_t)member];
+  for (const BatchRun& r : fb->runs) {
+    if (r.group != g || j < r.first || j >= r.first + r.count) continue;
+    const int v[10] = {r.plan[0].tiles, r.plan[0].nwv, r.plan[0].rt, r.plan[1].tiles, r.plan[1].nwv,
+                       r.plan[1].rt,    r.spt,         r.count,      r.first,         g};
+    std::memcpy(out, v, sizeof(v));
+    return;
+  }
+  throw Error{UPPE_ERR_STATE, "member has not been advanced in this batch"};
+}
+
+template <typename T>
+struct BatchOps {
+  using F = Fused<T>;
+  using C = cplx<T>;
+
+  // one RK4 step of a run: step_mfma with every launch covering the run's instances
+  template <int CM>
+  static void step_run(uppe_batch* b, const BatchRun& r) {
+    const BatchGroup& G = b->fb->groups[(size_t)r.group];
+    uppe_handle* h = b->members[(size_t)G.members[(size_t)r.first]];
+    FusedState* s = h->fused;
+    const long zs = h->P * (long)s->nwp;
+    const C* kp[5] = {nullptr, nullptr, (const C*)s->k1, (const C*)s->k2, (const C*)s->k3};
+    C *q = (C*)s->q, *k1 = (C*)s->k1, *k2 = (C*)s->k2, *k3 = (C*)s->k3;
+    for (int stage = 1; stage <= 4; stage++) {
+      F::template hk_launch<CM, PRO_STAGE, EPI_NONE>(h, s, s->frb_hi, s->frb_lo, s->kp_b, h->n1, h->nk, h->nw,
+                                                     r.plan[0], (const C*)q, (C*)s->er, kp[stage], q, k1, k2, k3,
+                                                     s->st, stage, 0.0, 0.0, 0.0, r.count, zs);
+      F::source(h, s, (const C*)s->er, (C*)s->jr, false, s->st, r.count, G.src + r.first, zs, r.spt);
+      F::template hk_launch<CM, PRO_NONE, EPI_STAGE>(h, s, s->frf_hi, s->frf_lo, s->kp_f, h->nk, h->n1, s->f1,
+                                                     r.plan[1], (const C*)s->jr, (C*)nullptr, nullptr, q, k1, k2,
+                                                     k3, s->st, stage, 0.0, 0.0, 0.0, r.count, zs);
+      if (stage == 1)
+        launchf(h, UPPE_KC_REDUCE, f_ctl_dz, dim3((unsigned)r.count), dim3(1), 0, s->st, (const double*)nullptr,
+                (double*)nullptr, (double*)nullptr, 0);
+    }
+    launchf(h, UPPE_KC_REDUCE, f_ctl_end, dim3((unsigned)r.count), dim3(1), 0, s->st, s->qmax, s->qmax_next);
+  }
+
+  static void step_run(uppe_batch* b, const BatchRun& r) {
+    const int cm = b->fb->groups[(size_t)r.group].cm;
+    if constexpr (sizeof(T) == 8) {
+      step_run<CM_F64>(b, r);
+    } else {
+      if (cm == CM_BF16X3)
+        step_run<CM_BF16X3>(b, r);
+      else
+        step_run<CM_F32>(b, r);
+    }
+  }
+
+  // candidate plans of an m-row contraction over ncols complex columns: both column tilings, 1 or 2 row
+  // tiles per wave, 1 to 16 waves per workgroup (no wave without rows)
+  static std::vector<FusedState::HkPlan> candidates(int m, int ncols) {
+    std::vector<FusedState::HkPlan> out;
+    const int m16 = (m + 15) / 16, narrow = (ncols + 7) / 8, merged = std::max(1, ncols / 8);
+    for (int tiles : {narrow, merged}) {
+      if (tiles == merged && merged == narrow && !out.empty()) break;
+      for (int rt = 1; rt <= 2; rt++)
+        for (int nwv = 1; nwv <= 16; nwv *= 2) {
+          if (rt * nwv > m16 && !(rt == 1 && nwv == 1)) continue;
+          if ((size_t)16 * rt * nwv * 32 * sizeof(T) > 65536) continue;  // output tile within 64 KiB of LDS
+          FusedState::HkPlan p;
+          p.tiles = tiles;
+          p.nwv = nwv;
+          p.rt = rt;
+          p.split = (m16 + rt * nwv - 1) / (rt * nwv);
+          out.push_back(p);
+        }
+    }
+    return out;
+  }
+
+  // average time of a launch sequence on stream st (one untimed call first)
+  template <typename L>
+  static float time_launch(hipStream_t st, hipEvent_t e0, hipEvent_t e1, L&& launch, int reps = 3) {
+    launch();
+    UPPE_HIP(hipEventRecord(e0, st));
+    for (int i = 0; i < reps; i++) launch();
+    UPPE_HIP(hipEventRecord(e1, st));
+    UPPE_HIP(hipEventSynchronize(e1));
+    float ms = 0.f;
+    UPPE_HIP(hipEventElapsedTime(&ms, e0, e1));
+    return ms / reps;
+  }
+
+  // choose the launch shapes of run r by timing: the k2r contraction with its stage prologue (writes E_r,
+  // scratch), the r2k contraction without its epilogue into E_r, and the source kernel (writes J_r, scratch);
+  // the state is not touched
+  template <int CM>
+  static void tune(uppe_batch* b, BatchRun& r) {
+    const BatchGroup& G = b->fb->groups[(size_t)r.group];
+    uppe_handle* h = b->members[(size_t)G.members[(size_t)r.first]];
+    FusedState* s = h->fused;
+    const long zs = h->P * (long)s->nwp;
+    C *q = (C*)s->q, *k1 = (C*)s->k1, *k2 = (C*)s->k2, *k3 = (C*)s->k3;
+    hipEvent_t e0, e1;
+    UPPE_HIP(hipEventCreate(&e0));
+    UPPE_HIP(hipEventCreate(&e1));
+    try {
+      float best = 1e30f;
+      for (const auto& p : candidates(h->n1, h->nw)) {
+        const float t = time_launch(h->stream, e0, e1, [&] {
+          F::template hk_launch<CM, PRO_STAGE, EPI_NONE>(h, s, s->frb_hi, s->frb_lo, s->kp_b, h->n1, h->nk, h->nw, p,
+                                                         (const C*)q, (C*)s->er, nullptr, q, k1, k2, k3, s->st, 1,
+                                                         0.0, 0.0, 0.0, r.count, zs);
+        });
+        if (t < best) {
+          best = t;
+          r.plan[0] = p;
+        }
+      }
+      best = 1e30f;
+      for (const auto& p : candidates(h->nk, s->f1)) {
+        const float t = time_launch(h->stream, e0, e1, [&] {
+          F::template hk_launch<CM, PRO_NONE, EPI_NONE>(h, s, s->frf_hi, s->frf_lo, s->kp_f, h->nk, h->n1, s->f1, p,
+                                                        (const C*)s->jr, (C*)s->er, nullptr, q, k1, k2, k3, s->st,
+                                                        1, 0.0, 0.0, 0.0, r.count, zs);
+        });
+        if (t < best) {
+          best = t;
+          r.plan[1] = p;
+        }
+      }
+      best = 1e30f;
+      for (int spt = 1; spt <= 4; spt *= 2) {
+        const int nth = h->nt / spt;
+        if (nth > 1024 || nth < 64) continue;
+        const float t = time_launch(h->stream, e0, e1, [&] {
+          F::source(h, s, (const C*)s->er, (C*)s->jr, false, s->st, r.count, G.src + r.first, zs, spt);
+        });
+        if (t < best) {
+          best = t;
+          r.spt = spt;
+        }
+      }
+    } catch (...) {
+      (void)hipEventDestroy(e0);
+      (void)hipEventDestroy(e1);
+      throw;
+    }
+    UPPE_HIP(hipEventDestroy(e0));
+    UPPE_HIP(hipEventDestroy(e1));
+  }
+
+  static void tune(uppe_batch* b, BatchRun& r) {
+    const int cm = b->fb->groups[(size_t)r.group].cm;
+    if constexpr (sizeof(T) == 8) {
+      tune<CM_F64>(b, r);
+    } else {
+      if (cm == CM_BF16X3)
+        tune<CM_BF16X3>(b, r);
+      else
+        tune<CM_F32>(b, r);
+    }
+  }
+};
+
+// whether members a and b can share one launch: the same nonlinear band, frequency tables and source step
+static bool same_run(const uppe_handle* a, const uppe_handle* b) {
+  const FusedState *x = a->fused, *y = b->fused;
+  return x->f1 == y->f1 && x->w_h == y->w_h && x->nl_h == y->nl_h && x->cond_h == y->cond_h && x->dt == y->dt;
+}
+
+void fused_batch_advance(uppe_batch* b, const uppe_step_control* c, int* steps_out) {
+  FusedBatch* fb = b->fb;
+  const int n = (int)b->members.size();
+  // member checks and the initial step state of every member (uploaded per group below)
+  for (int i = 0; i < n; i++) {
+    uppe_handle* h = b->members[(size_t)i];
+    FusedState* s = h->fused;
+    const std::string who = "batch member " + std::to_string(i) + ": ";
+    if (!h->have_hankel || !h->have_medium || !h->have_field)
+      throw Error{UPPE_ERR_STATE, who + "needs the Hankel basis, the medium and the field first"};
+    if (h->rot_states > 2) throw Error{UPPE_ERR_UNSUPPORTED, who + "batches take no rotator"};
+    if (c[i].n_schedule != 0 || c[i].dz_record)
+      throw Error{UPPE_ERR_UNSUPPORTED, who + "batches run adaptive or fixed steps (no schedule, no record)"};
+    if (c[i].n_sub < 1 || !(c[i].length > 0.0)) throw Error{UPPE_ERR_ARG, who + "bad step control"};
+    const BatchGroup& G = fb->groups[(size_t)fb->group_of[(size_t)i]];
+    StepState* sh = s->prec == UPPE_PREC_FP64 ? Fused<double>::init_state(h, s, &c[i])
+                                              : Fused<float>::init_state(h, s, &c[i]);
+    sh->rec_cap = 0;
+    G.st_host[fb->slot_of[(size_t)i]] = *sh;
+  }
+  for (BatchGroup& G : fb->groups) {
+    std::vector<SrcInst> src(G.members.size());
+    for (size_t j = 0; j < G.members.size(); j++) {
+      const FusedState* s = b->members[(size_t)G.members[j]]->fused;
+      src[j].redchi3 = s->redchi3;
+      src[j].ion = s->ion;
+      src[j].log_pre = s->ion.kind != UPPE_ION_EXPLICIT ? 0.0
+                       : G.prec == UPPE_PREC_FP64       ? std::log(s->ion.rate_conv * s->ion.c_pre)
+                                                        : std::log2(s->ion.rate_conv * s->ion.c_pre);
+    }
+    UPPE_HIP(hipMemcpy(G.src, src.data(), sizeof(SrcInst) * src.size(), hipMemcpyHostToDevice));
+    UPPE_HIP(hipMemcpy(G.slab[SL_ST], G.st_host, sizeof(StepState) * G.members.size(), hipMemcpyHostToDevice));
+  }
+  // runs of consecutive slots that share their tables, with their launch shapes
+  fb->runs.clear();
+  for (int g = 0; g < (int)fb->groups.size(); g++) {
+    const BatchGroup& G = fb->groups[(size_t)g];
+    for (int j = 0; j < (int)G.members.size();) {
+      BatchRun r;
+      r.group = g;
+      r.first = j;
+      const uppe_handle* h0 = b->members[(size_t)G.members[(size_t)j]];
+      while (j < (int)G.members.size() && same_run(h0, b->members[(size_t)G.members[(size_t)j]])) j++;
+      r.count = j - r.first;
+      r.plan[0] = h0->fused->plan[0];
+      r.plan[1] = h0->fused->plan[1];
+      if (fb->autotune) {
+        const std::vector<long> key = {g, r.first, r.count, h0->fused->f1};
+        auto it = std::find_if(fb->tuned.begin(), fb->tuned.end(), [&](const auto& e) { return e.first == key; });
+        if (it != fb->tuned.end()) {
+          r = it->second;
+        } else {
+          if (G.prec == UPPE_PREC_FP64)
+            BatchOps<double>::tune(b, r);
+          else
+            BatchOps<float>::tune(b, r);
+          fb->tuned.emplace_back(key, r);
+        }
+      }
+      fb->runs.push_back(r);
+    }
+  }
+  // step every run on its first member's stream (runs of different groups overlap) until no member is
+  // active; the states come back once per chunk of steps
+  int chunk = 1 << 30;
+  for (int i = 0; i < n; i++) {
+    int m = c[i].n_sub;
+    if (c[i].max_steps > 0) m = std::min(m, c[i].max_steps);
+    chunk = std::min(chunk, m);
+  }
+  chunk = std::max(1, chunk);
+  long long guard = 0;
+  while (true) {
+    for (int k = 0; k < chunk; k++)
+      for (const BatchRun& r : fb->runs) {
+        if (fb->groups[(size_t)r.group].prec == UPPE_PREC_FP64)
+          BatchOps<double>::step_run(b, r);
+        else
+          BatchOps<float>::step_run(b, r);
+      }
+    for (const BatchRun& r : fb->runs) {
+      const BatchGroup& G = fb->groups[(size_t)r.group];
+      uppe_handle* h = b->members[(size_t)G.members[(size_t)r.first]];
+      UPPE_HIP(hipMemcpyAsync(G.st_host + r.first, static_cast<StepState*>(G.slab[SL_ST]) + r.first,
+                              sizeof(StepState) * r.count, hipMemcpyDeviceToHost, h->stream));
+    }
+    for (const BatchRun& r : fb->runs)
+      UPPE_HIP(hipStreamSynchronize(b->members[(size_t)fb->groups[(size_t)r.group].members[(size_t)r.first]]->stream));
+    guard += chunk;
+    // remaining steps: per member, its remaining subcycles times its observed steps per subcycle
+    double left = 0.0;
+    bool active = false;
+    for (int i = 0; i < n; i++) {
+      const StepState& sh = fb->groups[(size_t)fb->group_of[(size_t)i]].st_host[fb->slot_of[(size_t)i]];
+      if (!sh.active) continue;
+      active = true;
+      const int done_sub = sh.sub - c[i].sub0;
+      const double per = done_sub > 0 ? (double)sh.steps / done_sub : (double)sh.steps + 1.0;
+      double rem = per * (sh.sub_end - sh.sub) * 0.5;
+      if (sh.max_steps > 0) rem = std::min(rem, (double)(sh.max_steps - sh.steps));
+      left = std::max(left, rem);
+    }
+    if (!active) break;
+    if (guard > 100000000LL) throw Error{UPPE_ERR_STATE, "batch step loop did not terminate"};
+    chunk = std::max(1, (int)std::min(1e6, left));
+  }
+  for (int i = 0; i < n; i++) {
+    uppe_handle* h = b->members[(size_t)i];
+    const StepState& sh = fb->groups[(size_t)fb->group_of[(size_t)i]].st_host[fb->slot_of[(size_t)i]];
+    if (sh.error) throw Error{UPPE_ERR_STATE, "batch member " + std::to_string(i) + ": step control error"};
+    h->steps += sh.steps;
+    h->evals += 4LL * sh.steps;
+    h->nan_steps += sh.nan_steps;
+    h->z_last = sh.z_last;
+    if (steps_out) steps_out[i] = sh.steps;
+  }
+}
+
 static bool is_fp64(uppe_handle* h) { return h->fused->prec == UPPE_PREC_FP64; }
 
-void fused_create(uppe_handle* h) {
+void fused_create(uppe_handle* h, const MemberSlots* slots) {
   if (h->d.geometry == UPPE_RADIAL && h->n2 != 1)
     throw Error{UPPE_ERR_UNSUPPORTED, "fused path: radial geometry with one azimuthal mode only"};
   if (h->d.geometry == UPPE_CARTESIAN && (h->n1 < 2 || h->n2 < 2 || h->n1 > 512 || h->n2 > 512))
@@ -2624,6 +3138,7 @@ void fused_create(uppe_handle* h) {
     throw Error{UPPE_ERR_UNSUPPORTED, "fused path supports up to 512 radial cells"};
   auto* s = new FusedState();
   h->fused = s;
+  s->slots = slots;
   s->prec = h->d.precision;
   const char* env = getenv("SEARAY_UPPE_SIMT_HANKEL");
   const bool simt = env && env[0] == '1';
@@ -2647,7 +3162,7 @@ void fused_destroy(uppe_handle* h) {
                   s->frb_lo, s->frf_hi, s->frf_lo, s->tw_x, s->tw_y, s->rot_tab, s->fab_hi, s->fab_lo, s->faf_hi,
                   s->faf_lo, s->ck_q, s->ck_st, s->ck_qmax, s->zrec, s->er_store, s->gs, s->gq, s->gk, s->g2, s->g3};
   for (void* p : ptrs)
-    if (p) (void)hipFree(p);
+    if (p && !slotted(s, p)) (void)hipFree(p);
   if (s->st_host) (void)hipHostFree(s->st_host);
   delete s;
   h->fused = nullptr;

@@ -267,6 +267,8 @@ struct ProfRecord {
 
 struct PlainState;
 struct FusedState;
+struct MemberSlots;
+struct FusedBatch;
 
 }  // namespace uppe
 
@@ -298,6 +300,16 @@ struct uppe_handle {
   // implementation state (exactly one is non-null)
   uppe::PlainState* plain = nullptr;
   uppe::FusedState* fused = nullptr;
+  // the batch that owns this handle (its arrays live in the batch's slabs), or null
+  struct uppe_batch* batch = nullptr;
+};
+
+// a set of radial UPPE instances stepped together (one launch per kernel for each run of equally shaped ones)
+struct uppe_batch {
+  std::vector<uppe_handle*> members;
+  uppe::FusedBatch* fb = nullptr;
+  int device = 0;
+  std::string err;
 };
 
 namespace uppe {
@@ -329,8 +341,9 @@ int plain_advance(uppe_handle* h, const uppe_step_control* c);
 void plain_finish(uppe_handle* h, double z_total, double* a_r, double* j_r, double* chi_t, double* ne_t);
 void plain_get_state(uppe_handle* h, double* q);
 
-// fused path (strip-major layout, FP64 / FP32 / BF16x3 Hankel)
-void fused_create(uppe_handle* h);
+// fused path (strip-major layout, FP64 / FP32 / BF16x3 Hankel); a batch member takes its per-instance arrays
+// from slots
+void fused_create(uppe_handle* h, const MemberSlots* slots = nullptr);
 void fused_destroy(uppe_handle* h);
 void fused_set_hankel(uppe_handle* h);
 void fused_set_rotator(uppe_handle* h);
@@ -342,5 +355,14 @@ void fused_set_field(uppe_handle* h, const double* a_r);
 int fused_advance(uppe_handle* h, const uppe_step_control* c);
 void fused_finish(uppe_handle* h, double z_total, double* a_r, double* j_r, double* chi_t, double* ne_t);
 void fused_get_state(uppe_handle* h, double* q);
+
+// batches of radial instances (uppe_batch_*): create allocates the slabs and the member handles through
+// make_member; advance steps every member; plan reports the launch shapes chosen for a member's run
+using MemberFactory = int (*)(const uppe_desc* d, const MemberSlots* slots, uppe_handle** out);
+void fused_batch_create(uppe_batch* b, const uppe_desc* descs, int n, MemberFactory make_member);
+void fused_batch_destroy(uppe_batch* b);
+void fused_batch_advance(uppe_batch* b, const uppe_step_control* c, int* steps_out);
+void fused_batch_plan(const uppe_batch* b, int member, int* out);
+void fused_batch_set_autotune(uppe_batch* b, int enable);
 
 }  // namespace uppe

@@ -1,0 +1,143 @@
+'''Batched radial stepping through uppe_batch_* against standalone handles and the FP64 oracle.
+
+A batch member runs the same kernels as a standalone handle on the same inputs, so with the source kernel's
+default launch shape (autotune off) the two agree to rounding of identical operation orders: 1e-12 relative
+L2 in FP64 and FP32 alike (in practice bit for bit). With autotune on, the source kernel may take another
+number of samples per thread, which reorders the FP32 depletion sums: 1e-5 then, the FP32 field tolerance. Each
+member is also checked against the oracle's track of its own vol_dict: 1e-10 (FP64) and 1e-5 (FP32) on the
+field; 1e-10 / 1e-4 relative on the electron density, or 1e-12 of the gas density where the gas is barely
+ionised (a weak field's rate is a high power of the field, which multiplies the field's rounding error).
+'''
+import sys
+
+import numpy as np
+import pytest
+
+from conftest import rel_l2
+from searay_amd import batch, configs, uppe_kernel
+
+pytestmark = pytest.mark.gpu
+
+
+def _say(msg):
+    print(msg, flush=True)
+    sys.stdout.flush()
+
+
+def _standalone(vol, precision, max_steps=0):
+    rs = uppe_kernel.RunSetup(vol)
+    st = rs.make_stepper(uppe_kernel.hip_stepper_factory(precision))
+    st.set_field(rs.stepper_field())
+    zi, zf, L = rs.plane(0)
+    n, _ = st.advance(zi, L, 0, rs.subcycles, rs.dzmin, max_steps=max_steps)
+    q = st.get_state()
+    st.close()
+    return n, q
+
+
+def _mixed_vols(adaptive, n=6, steps=4):
+    vols, drawn = configs.batched_radial(n=n, scale=8, steps=steps)
+    if adaptive:
+        for v in vols:
+            v['minimum step'] = v['minimum step'] / 50.0
+    return vols, drawn
+
+
+@pytest.mark.parametrize('adaptive', [False, True])
+@pytest.mark.parametrize('precision', ['fp64', 'fp32', 'bf16x3'])
+def test_batch_matches_standalone(precision, adaptive):
+    vols, drawn = _mixed_vols(adaptive)
+    _say('batch vs standalone %s adaptive=%s grids %s' % (precision, adaptive, [d[0] for d in drawn]))
+    with batch.Batch(vols, precision=precision, autotune=False) as b:
+        b.set_fields()
+        steps = b.advance()
+        qs = [m.get_state() for m in b.members]
+        infos = [m.info() for m in b.members]
+    for i, vol in enumerate(vols):
+        n, q = _standalone(vol, precision)
+        assert steps[i] == n, (i, steps[i], n)
+        assert infos[i]['steps'] == n
+        assert np.all(np.isfinite(q))
+        assert rel_l2(qs[i], q) < 1e-12, (i, rel_l2(qs[i], q))
+
+
+@pytest.mark.parametrize('precision', ['fp32', 'bf16x3'])
+def test_batch_autotuned_matches_standalone(precision):
+    vols, drawn = _mixed_vols(False)
+    _say('autotuned batch vs standalone %s' % precision)
+    with batch.Batch(vols, precision=precision, autotune=True) as b:
+        b.set_fields()
+        b.advance()
+        qs = [m.get_state() for m in b.members]
+        plans = [b.plan(i) for i in range(len(vols))]
+        # a second advance reuses the tuned shapes
+        b.set_fields()
+        b.advance()
+        qs2 = [m.get_state() for m in b.members]
+        assert [b.plan(i) for i in range(len(vols))] == plans
+    for i, vol in enumerate(vols):
+        _, q = _standalone(vol, precision)
+        assert rel_l2(qs[i], q) < 1e-5, (i, rel_l2(qs[i], q))
+        assert rel_l2(qs2[i], qs[i]) == 0.0
+    # members of one shape and one table set share a run
+    by_grid = {}
+    for i, d in enumerate(drawn):
+        by_grid.setdefault(d[0], set()).add(plans[i]['group'])
+    assert all(len(g) == 1 for g in by_grid.values())
+
+
+@pytest.mark.parametrize('precision', ['fp64', 'fp32'])
+def test_batch_members_against_oracle(oracle_mod, precision):
+    vols, drawn = _mixed_vols(True, n=4, steps=3)
+    _say('batch members vs oracle %s' % precision)
+    with batch.Batch(vols, precision=precision) as b:
+        b.set_fields()
+        b.advance()
+        outs = []
+        for rs, m in zip(b.setups, b.members):
+            zi, zf, _ = rs.plane(0)
+            a, j, chi, ne = m.finish(zf - zi)
+            outs.append((a, ne))
+    tol = 1e-10 if precision == 'fp64' else 1e-5
+    ne_tol = 1e-10 if precision == 'fp64' else 1e-4
+    for i, vol in enumerate(vols):
+        ref = oracle_mod.oracle_track(vol)
+        a, ne = outs[i]
+        assert rel_l2(a, ref[0][..., 1]) < tol, (i, rel_l2(a, ref[0][..., 1]))
+        # the electron density to ne_tol relative, or, where the gas is barely ionised, to 1e-12 of the gas
+        # density (the rate's steep field dependence multiplies the field's rounding error there)
+        ne_ref = np.fft.irfft(ref[3][..., 1], axis=0)
+        ngas = vol['density reference']  # uniform gas: n_g = n_ref
+        err, dev = rel_l2(ne, ne_ref), np.max(np.abs(ne - ne_ref)) / ngas
+        _say('member %d %s: field %.2e, ne %.2e relative, %.2e of the gas density (peak ne %.2e)'
+             % (i, drawn[i], rel_l2(a, ref[0][..., 1]), err, dev, np.max(np.abs(ne_ref)) / ngas))
+        assert err < ne_tol or dev < 1e-12, (i, err, dev)
+
+
+def test_batch_full_size_instances():
+    '''The 64-instance workload (Nw = 1025, Nr in {64, ..., 512}, O2 / Ar, 0.1-5 P_cr), two steps: every member
+    stays finite, and a member of each grid size agrees with its standalone run (FP32, autotuned).'''
+    vols, drawn = configs.batched_radial(n=64, steps=200)
+    _say('full-size batch: 64 instances, 2 steps')
+    with batch.Batch(vols, precision='fp32') as b:
+        b.set_fields()
+        steps = b.advance(max_steps=2)
+        assert np.all(steps == 2)
+        qs = {}
+        for i, m in enumerate(b.members):
+            q = m.get_state()
+            assert np.all(np.isfinite(q)), i
+            if drawn[i][0] not in qs:
+                qs[drawn[i][0]] = (i, q)
+    for nr, (i, q) in sorted(qs.items()):
+        _, qref = _standalone(vols[i], 'fp32', max_steps=2)
+        assert rel_l2(q, qref) < 1e-5, (nr, i, rel_l2(q, qref))
+
+
+def test_member_handles_belong_to_batch():
+    vols, _ = _mixed_vols(False, n=2)
+    with batch.Batch(vols, precision='fp32') as b:
+        m = b.members[0]
+        assert b.lib.uppe_destroy(m.h) != 0  # a member is released by its batch only
+        b.set_fields()
+        assert np.all(b.advance(max_steps=1) == 1)
