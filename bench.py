@@ -173,14 +173,14 @@ def adjoint_cost(prec='fp32', reps=3):
     return out
 
 
-def batch_workload(prec, K, W, seq_steps):
+def batch_workload(prec, K, W, seq_steps, graph_steps=8):
     '''The 64-instance batched workload (configs.batched_radial): every instance stepped K fixed steps in one
     batch (one launch per kernel for each run of equally shaped instances, autotuned launch shapes), against the
     same instances stepped one after another as standalone handles (seq_steps steps each).'''
     from searay_amd import batch
     vols, drawn = configs.batched_radial(n=64, steps=200)
     pts = sum(v['wave grid'][0] * v['wave grid'][1] for v in vols)
-    b = batch.Batch(vols, precision=prec)
+    b = batch.Batch(vols, precision=prec, graph_steps=graph_steps)
     b.set_fields()
     log('batch warmup %d steps (autotune)' % W)
     b.advance(max_steps=max(W, 1))
@@ -208,7 +208,7 @@ def batch_workload(prec, K, W, seq_steps):
         tseq += dt
         st.close()
     seq_value = pts * seq_steps / tseq
-    return dict(value=pts * n / secs, steps=n, secs=secs, pts=pts, nan_steps=nan, finite=finite,
+    return dict(value=pts * n / secs, steps=n, secs=secs, pts=pts, nan_steps=nan, finite=finite, graph_steps=graph_steps,
                 grids=sorted({d[0] for d in drawn}), plans={str(k): v for k, v in sorted(plans.items())},
                 sequential_pt_steps_per_s=seq_value, speedup_vs_sequential=pts * n / secs / seq_value,
                 instances=[dict(nr=d[0], power_pcr=round(d[1], 4), gas=d[2]) for d in drawn])

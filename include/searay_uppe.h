@@ -209,12 +209,17 @@ int uppe_profile_read(uppe_handle* h, int kernel_class, double* total_ms, long l
  * frequency tables share every launch. Rotators are not taken. With autotune on (the default) the first
  * advance of each run times the candidate contraction tilings and source launch shapes on the run and keeps
  * the fastest; uppe_batch_plan reports member i's run as {k2r tiles, waves, row tiles per wave, r2k tiles,
- * waves, row tiles per wave, source samples per thread (0 = default), run length, first slot, group}. */
+ * waves, row tiles per wave, source samples per thread (0 = default), run length, first slot, group}.
+ * uppe_batch_set_graph sets how many steps of all runs one HIP graph replays (default 8; 0 launches kernel by
+ * kernel); the graph is captured at every advance, whose step count is then rounded up to a multiple of it
+ * (steps after a member's end leave it untouched). Runs with event timing on (uppe_profile_enable) launch
+ * kernel by kernel. */
 typedef struct uppe_batch uppe_batch;
 int uppe_batch_create(const uppe_desc* descs, int n, uppe_batch** out);
 int uppe_batch_member(uppe_batch* b, int i, uppe_handle** member);
 int uppe_batch_advance(uppe_batch* b, const uppe_step_control* c, int* steps_out);
 int uppe_batch_set_autotune(uppe_batch* b, int enable);
+int uppe_batch_set_graph(uppe_batch* b, int steps_per_graph);
 int uppe_batch_plan(const uppe_batch* b, int member, int* plan /* [10] */);
 int uppe_batch_destroy(uppe_batch* b);
 const char* uppe_batch_last_error(const uppe_batch* b);

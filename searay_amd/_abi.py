@@ -65,8 +65,8 @@ class uppe_info(ct.Structure):
 ENTRY_POINTS = ('create', 'set_hankel', 'set_medium', 'set_paraxial_medium', 'set_rotator', 'set_field', 'advance',
                 'finish', 'get_state', 'get_info', 'destroy', 'last_error')
 HIP_ONLY_ENTRY_POINTS = ('profile_enable', 'profile_read', 'synchronize', 'advance_checkpointed', 'adjoint')
-BATCH_ENTRY_POINTS = ('batch_create', 'batch_member', 'batch_advance', 'batch_set_autotune', 'batch_plan',
-                      'batch_destroy')
+BATCH_ENTRY_POINTS = ('batch_create', 'batch_member', 'batch_advance', 'batch_set_autotune', 'batch_set_graph',
+                      'batch_plan', 'batch_destroy')
 
 
 def _ptr(a):

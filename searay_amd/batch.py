@@ -26,7 +26,7 @@ class BatchError(RuntimeError):
 
 
 class Batch:
-    def __init__(self, vols, precision='fp32', device=0, autotune=True):
+    def __init__(self, vols, precision='fp32', device=0, autotune=True, graph_steps=8):
         if len(vols) < 1:
             raise ValueError('a batch needs at least one instance')
         self.lib = _native.library()
@@ -46,6 +46,7 @@ class Batch:
                              % (rc, (self.lib.uppe_batch_last_error(b) or b'').decode()))
         self.b = b
         self._check(self.lib.uppe_batch_set_autotune(self.b, int(bool(autotune))), 'set_autotune')
+        self.set_graph_steps(graph_steps)
         self.members = []
         for i, rs in enumerate(self.setups):
             h = ct.c_void_p()
@@ -53,6 +54,10 @@ class Batch:
             m = _native.BatchMember(h, (rs.N[0], rs.N[1], 1))
             rs.make_stepper(lambda *args, m=m, **kw: m)  # Hankel basis, medium of the first subcycle, rotator
             self.members.append(m)
+
+    def set_graph_steps(self, steps):
+        '''Steps of every run that one HIP graph replays (0: launch kernel by kernel).'''
+        self._check(self.lib.uppe_batch_set_graph(self.b, int(steps)), 'set_graph')
 
     def _check(self, rc, what):
         if rc != _abi.UPPE_OK:

@@ -480,6 +480,11 @@ int uppe_batch_set_autotune(uppe_batch* b, int enable) {
   return batch_guarded(b, [&] { fused_batch_set_autotune(b, enable); });
 }
 
+int uppe_batch_set_graph(uppe_batch* b, int steps_per_graph) {
+  if (!b) return UPPE_ERR_ARG;
+  return batch_guarded(b, [&] { fused_batch_set_graph(b, steps_per_graph); });
+}
+
 int uppe_batch_plan(const uppe_batch* b, int member, int* plan) {
   if (!b || !plan || member < 0 || member >= (int)b->members.size()) return UPPE_ERR_ARG;
   return batch_guarded(const_cast<uppe_batch*>(b), [&] { fused_batch_plan(b, member, plan); });

@@ -2743,6 +2743,14 @@ struct FusedBatch {
   // tuned launch shapes by run: (group, first, count, f1) -> plans and samples per thread
   std::vector<std::pair<std::vector<long>, BatchRun>> tuned;
   bool autotune = true;
+  // HIP graph of graph_steps steps of every run (0: launch step by step), launched on `main`; the runs'
+  // streams fork from it and join it again inside the graph
+  int graph_steps = 8;
+  hipStream_t main = nullptr;
+  hipEvent_t fork = nullptr;
+  std::vector<hipEvent_t> joins;
+  hipGraph_t graph = nullptr;
+  hipGraphExec_t exec = nullptr;
 };
 
 static int cm_of(int prec) {
@@ -2835,11 +2843,21 @@ void fused_batch_destroy(uppe_batch* b) {
     if (G.src) (void)hipFree(G.src);
     if (G.st_host) (void)hipHostFree(G.st_host);
   }
+  if (fb->exec) (void)hipGraphExecDestroy(fb->exec);
+  if (fb->graph) (void)hipGraphDestroy(fb->graph);
+  for (hipEvent_t e : fb->joins) (void)hipEventDestroy(e);
+  if (fb->fork) (void)hipEventDestroy(fb->fork);
+  if (fb->main) (void)hipStreamDestroy(fb->main);
   delete fb;
   b->fb = nullptr;
 }
 
 void fused_batch_set_autotune(uppe_batch* b, int enable) { b->fb->autotune = enable != 0; }
+
+void fused_batch_set_graph(uppe_batch* b, int steps) {
+  if (steps < 0 || steps > 4096) throw Error{UPPE_ERR_ARG, "graph steps must be in [0, 4096]"};
+  b->fb->graph_steps = steps;
+}
 
 void fused_batch_plan(const uppe_batch* b, int member, int* out) {
   const FusedBatch* fb = b->fb;
@@ -3002,6 +3020,53 @@ struct BatchOps {
   }
 };
 
+static hipStream_t run_stream(const uppe_batch* b, const BatchRun& r) {
+  return b->members[(size_t)b->fb->groups[(size_t)r.group].members[(size_t)r.first]]->stream;
+}
+
+static void step_all(uppe_batch* b) {
+  for (const BatchRun& r : b->fb->runs) {
+    if (b->fb->groups[(size_t)r.group].prec == UPPE_PREC_FP64)
+      BatchOps<double>::step_run(b, r);
+    else
+      BatchOps<float>::step_run(b, r);
+  }
+}
+
+// capture `steps` steps of every run into one executable graph: the runs' streams fork from the main stream and
+// join it at the end, so runs of different groups still overlap inside the graph
+static void build_graph(uppe_batch* b, int steps) {
+  FusedBatch* fb = b->fb;
+  if (fb->exec) UPPE_HIP(hipGraphExecDestroy(fb->exec));
+  fb->exec = nullptr;
+  if (fb->graph) UPPE_HIP(hipGraphDestroy(fb->graph));
+  fb->graph = nullptr;
+  if (!fb->main) UPPE_HIP(hipStreamCreateWithFlags(&fb->main, hipStreamNonBlocking));
+  if (!fb->fork) UPPE_HIP(hipEventCreateWithFlags(&fb->fork, hipEventDisableTiming));
+  while (fb->joins.size() < fb->runs.size()) {
+    hipEvent_t e;
+    UPPE_HIP(hipEventCreateWithFlags(&e, hipEventDisableTiming));
+    fb->joins.push_back(e);
+  }
+  UPPE_HIP(hipStreamBeginCapture(fb->main, hipStreamCaptureModeThreadLocal));
+  try {
+    UPPE_HIP(hipEventRecord(fb->fork, fb->main));
+    for (const BatchRun& r : fb->runs) UPPE_HIP(hipStreamWaitEvent(run_stream(b, r), fb->fork, 0));
+    for (int k = 0; k < steps; k++) step_all(b);
+    for (size_t i = 0; i < fb->runs.size(); i++) {
+      UPPE_HIP(hipEventRecord(fb->joins[i], run_stream(b, fb->runs[i])));
+      UPPE_HIP(hipStreamWaitEvent(fb->main, fb->joins[i], 0));
+    }
+  } catch (...) {
+    hipGraph_t g = nullptr;
+    (void)hipStreamEndCapture(fb->main, &g);
+    if (g) (void)hipGraphDestroy(g);
+    throw;
+  }
+  UPPE_HIP(hipStreamEndCapture(fb->main, &fb->graph));
+  UPPE_HIP(hipGraphInstantiate(&fb->exec, fb->graph, nullptr, nullptr, 0));
+}
+
 // whether members a and b can share one launch: the same nonlinear band, frequency tables and source step
 static bool same_run(const uppe_handle* a, const uppe_handle* b) {
   const FusedState *x = a->fused, *y = b->fused;
@@ -3079,24 +3144,30 @@ void fused_batch_advance(uppe_batch* b, const uppe_step_control* c, int* steps_o
     chunk = std::min(chunk, m);
   }
   chunk = std::max(1, chunk);
+  // graphs replay graph_steps steps per launch (steps past a member's end are no-ops on its state); event
+  // timing of the launches (profiling) takes the step-by-step path
+  bool prof = false;
+  for (const uppe_handle* h : b->members) prof = prof || h->prof;
+  const int gs = prof ? 0 : fb->graph_steps;
+  if (gs > 0) build_graph(b, gs);
   long long guard = 0;
   while (true) {
-    for (int k = 0; k < chunk; k++)
-      for (const BatchRun& r : fb->runs) {
-        if (fb->groups[(size_t)r.group].prec == UPPE_PREC_FP64)
-          BatchOps<double>::step_run(b, r);
-        else
-          BatchOps<float>::step_run(b, r);
-      }
+    int ran = 0;
+    if (gs > 0) {
+      for (; ran < chunk; ran += gs) UPPE_HIP(hipGraphLaunch(fb->exec, fb->main));
+    } else {
+      for (; ran < chunk; ran++) step_all(b);
+    }
     for (const BatchRun& r : fb->runs) {
       const BatchGroup& G = fb->groups[(size_t)r.group];
-      uppe_handle* h = b->members[(size_t)G.members[(size_t)r.first]];
       UPPE_HIP(hipMemcpyAsync(G.st_host + r.first, static_cast<StepState*>(G.slab[SL_ST]) + r.first,
-                              sizeof(StepState) * r.count, hipMemcpyDeviceToHost, h->stream));
+                              sizeof(StepState) * r.count, hipMemcpyDeviceToHost, gs > 0 ? fb->main : run_stream(b, r)));
     }
-    for (const BatchRun& r : fb->runs)
-      UPPE_HIP(hipStreamSynchronize(b->members[(size_t)fb->groups[(size_t)r.group].members[(size_t)r.first]]->stream));
-    guard += chunk;
+    if (gs > 0)
+      UPPE_HIP(hipStreamSynchronize(fb->main));
+    else
+      for (const BatchRun& r : fb->runs) UPPE_HIP(hipStreamSynchronize(run_stream(b, r)));
+    guard += ran;
     // remaining steps: per member, its remaining subcycles times its observed steps per subcycle
     double left = 0.0;
     bool active = false;

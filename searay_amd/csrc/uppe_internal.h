@@ -364,5 +364,6 @@ void fused_batch_destroy(uppe_batch* b);
 void fused_batch_advance(uppe_batch* b, const uppe_step_control* c, int* steps_out);
 void fused_batch_plan(const uppe_batch* b, int member, int* out);
 void fused_batch_set_autotune(uppe_batch* b, int enable);
+void fused_batch_set_graph(uppe_batch* b, int steps);
 
 }  // namespace uppe

@@ -141,3 +141,23 @@ def test_member_handles_belong_to_batch():
         assert b.lib.uppe_destroy(m.h) != 0  # a member is released by its batch only
         b.set_fields()
         assert np.all(b.advance(max_steps=1) == 1)
+
+
+@pytest.mark.parametrize('graph_steps', [1, 3])
+def test_batch_graph_replays_match_kernel_launches(graph_steps):
+    '''Graph replays (graph_steps steps each, several per advance, a new capture per advance, host reads in
+    between) leave every member exactly where kernel-by-kernel launches leave it.'''
+    vols, _ = _mixed_vols(True, n=5, steps=4)
+    states = {}
+    for gs in (0, graph_steps):
+        with batch.Batch(vols, precision='fp32', autotune=False, graph_steps=gs) as b:
+            b.set_fields()
+            seq = []
+            for max_steps in (2, 3, 0):
+                steps = b.advance(max_steps=max_steps)
+                seq.append((steps.copy(), [m.get_state() for m in b.members]))
+            states[gs] = seq
+    for (s0, q0), (s1, q1) in zip(states[0], states[graph_steps]):
+        assert np.array_equal(s0, s1)
+        for a, c in zip(q0, q1):
+            assert np.array_equal(a, c)
