@@ -95,6 +95,8 @@ struct FusedState {
     int tiles = 1, split = 1, nwv = 1, rt = 1;
   } plan[2];
   int n_cu = 256;
+  int spt = 0;          // source samples per thread of the step (0: the default shape)
+  int tuned_f1 = -1;    // nonlinear band the contraction plans were timed for (-1: not yet)
   int f1 = 0;  // the r2k direction covers complex columns [0, f1): where the nonlinear filter is nonzero
   // Cartesian: radix-8 pass tables of the x and y transforms (complex T)
   void *tw_x = nullptr, *tw_y = nullptr;
@@ -1952,6 +1954,9 @@ static long frag_count(int cm, int m, int k) {
 }
 
 template <typename T>
+void tune_single(uppe_handle* h, FusedState* s);
+
+template <typename T>
 struct Fused {
   using C = cplx<T>;
 
@@ -2072,6 +2077,7 @@ struct Fused {
     }
     s->plan[0] = plan_hankel(nr, h->nw, s->n_cu);
     s->plan[1] = plan_hankel(nk, s->f1, s->n_cu);
+    s->tuned_f1 = -1;
   }
 
   static void set_medium(uppe_handle* h, FusedState* s, const uppe_medium* m) {
@@ -2292,7 +2298,7 @@ struct Fused {
         hankel_mfma<CM, PRO_STAGE_ENV, EPI_NONE>(h, s, 0, (const C*)s->q, er, kp[stage], stage, 0.0, s->st);
       else
         hankel_mfma<CM, PRO_STAGE, EPI_NONE>(h, s, 0, (const C*)s->q, er, kp[stage], stage, 0.0, s->st);
-      source(h, s, (const C*)er, (C*)s->jr, false, s->st);
+      source(h, s, (const C*)er, (C*)s->jr, false, s->st, 1, nullptr, 0, s->spt);
       hankel_mfma<CM, PRO_NONE, EPI_STAGE>(h, s, 1, (const C*)s->jr, (C*)nullptr, nullptr, stage, 0.0, s->st);
       if (stage == 1)
         launchf(h, UPPE_KC_REDUCE, f_ctl_dz, dim3(1), dim3(1), 0, s->st, (const double*)s->sched, s->rec_active,
@@ -2457,6 +2463,7 @@ struct Fused {
     StepState* sh = init_state(h, s, c);
     s->rec_active = nullptr;
     UPPE_HIP(hipMemcpyAsync(s->st, sh, sizeof(StepState), hipMemcpyHostToDevice, h->stream));
+    tune_single<T>(h, s);
     if (s->er_cap < 4 * every) {
       if (s->er_store) UPPE_HIP(hipFree(s->er_store));
       UPPE_HIP(hipMalloc(&s->er_store, sizeof(C) * stride * 4 * every));
@@ -2632,6 +2639,7 @@ struct Fused {
     double* rec_dev = sh->rec_cap > 0 ? s->rec : nullptr;
     s->rec_active = rec_dev;
     UPPE_HIP(hipMemcpyAsync(s->st, sh, sizeof(StepState), hipMemcpyHostToDevice, h->stream));
+    tune_single<T>(h, s);
     long long guard = 0;
     int chunk = c->n_sub;
     if (sh->max_steps > 0) chunk = std::min(chunk, sh->max_steps);
@@ -2948,15 +2956,14 @@ struct BatchOps {
     return ms / reps;
   }
 
-  // choose the launch shapes of run r by timing: the k2r contraction with its stage prologue (writes E_r,
-  // scratch), the r2k contraction without its epilogue into E_r, and the source kernel (writes J_r, scratch);
-  // the state is not touched
+  // choose launch shapes by timing on nz instances zs elements apart starting at member (h, s): the k2r
+  // contraction with its stage prologue (writes E_r, scratch) and the r2k contraction without its epilogue
+  // into E_r give the two plans; with src, the source kernel's samples per thread are timed too (it writes
+  // J_r, scratch). The state is not touched, and the plans do not change any result (every output sums its
+  // contraction in the same order).
   template <int CM>
-  static void tune(uppe_batch* b, BatchRun& r) {
-    const BatchGroup& G = b->fb->groups[(size_t)r.group];
-    uppe_handle* h = b->members[(size_t)G.members[(size_t)r.first]];
-    FusedState* s = h->fused;
-    const long zs = h->P * (long)s->nwp;
+  static void tune_shapes(uppe_handle* h, FusedState* s, int nz, long zs, const SrcInst* src,
+                          FusedState::HkPlan plan[2], int* spt_out) {
     C *q = (C*)s->q, *k1 = (C*)s->k1, *k2 = (C*)s->k2, *k3 = (C*)s->k3;
     hipEvent_t e0, e1;
     UPPE_HIP(hipEventCreate(&e0));
@@ -2965,13 +2972,18 @@ struct BatchOps {
       float best = 1e30f;
       for (const auto& p : candidates(h->n1, h->nw)) {
         const float t = time_launch(h->stream, e0, e1, [&] {
-          F::template hk_launch<CM, PRO_STAGE, EPI_NONE>(h, s, s->frb_hi, s->frb_lo, s->kp_b, h->n1, h->nk, h->nw, p,
-                                                         (const C*)q, (C*)s->er, nullptr, q, k1, k2, k3, s->st, 1,
-                                                         0.0, 0.0, 0.0, r.count, zs);
+          if (h->paraxial)
+            F::template hk_launch<CM, PRO_STAGE_ENV, EPI_NONE>(h, s, s->frb_hi, s->frb_lo, s->kp_b, h->n1, h->nk,
+                                                               h->nw, p, (const C*)q, (C*)s->er, nullptr, q, k1, k2,
+                                                               k3, s->st, 1, 0.0, 0.0, 0.0, nz, zs);
+          else
+            F::template hk_launch<CM, PRO_STAGE, EPI_NONE>(h, s, s->frb_hi, s->frb_lo, s->kp_b, h->n1, h->nk, h->nw,
+                                                           p, (const C*)q, (C*)s->er, nullptr, q, k1, k2, k3, s->st,
+                                                           1, 0.0, 0.0, 0.0, nz, zs);
         });
         if (t < best) {
           best = t;
-          r.plan[0] = p;
+          plan[0] = p;
         }
       }
       best = 1e30f;
@@ -2979,23 +2991,23 @@ struct BatchOps {
         const float t = time_launch(h->stream, e0, e1, [&] {
           F::template hk_launch<CM, PRO_NONE, EPI_NONE>(h, s, s->frf_hi, s->frf_lo, s->kp_f, h->nk, h->n1, s->f1, p,
                                                         (const C*)s->jr, (C*)s->er, nullptr, q, k1, k2, k3, s->st,
-                                                        1, 0.0, 0.0, 0.0, r.count, zs);
+                                                        1, 0.0, 0.0, 0.0, nz, zs);
         });
         if (t < best) {
           best = t;
-          r.plan[1] = p;
+          plan[1] = p;
         }
       }
       best = 1e30f;
-      for (int spt = 1; spt <= 4; spt *= 2) {
+      for (int spt = 1; src && spt <= 4; spt *= 2) {
         const int nth = h->nt / spt;
         if (nth > 1024 || nth < 64) continue;
         const float t = time_launch(h->stream, e0, e1, [&] {
-          F::source(h, s, (const C*)s->er, (C*)s->jr, false, s->st, r.count, G.src + r.first, zs, spt);
+          F::source(h, s, (const C*)s->er, (C*)s->jr, false, s->st, nz, src, zs, spt);
         });
         if (t < best) {
           best = t;
-          r.spt = spt;
+          *spt_out = spt;
         }
       }
     } catch (...) {
@@ -3007,18 +3019,48 @@ struct BatchOps {
     UPPE_HIP(hipEventDestroy(e1));
   }
 
-  static void tune(uppe_batch* b, BatchRun& r) {
-    const int cm = b->fb->groups[(size_t)r.group].cm;
+  static void tune_shapes(uppe_handle* h, FusedState* s, int nz, long zs, const SrcInst* src,
+                          FusedState::HkPlan plan[2], int* spt_out) {
     if constexpr (sizeof(T) == 8) {
-      tune<CM_F64>(b, r);
+      tune_shapes<CM_F64>(h, s, nz, zs, src, plan, spt_out);
     } else {
-      if (cm == CM_BF16X3)
-        tune<CM_BF16X3>(b, r);
+      if (s->cm == CM_BF16X3)
+        tune_shapes<CM_BF16X3>(h, s, nz, zs, src, plan, spt_out);
       else
-        tune<CM_F32>(b, r);
+        tune_shapes<CM_F32>(h, s, nz, zs, src, plan, spt_out);
     }
   }
+
+  // the launch shapes of batch run r
+  static void tune(uppe_batch* b, BatchRun& r) {
+    const BatchGroup& G = b->fb->groups[(size_t)r.group];
+    uppe_handle* h = b->members[(size_t)G.members[(size_t)r.first]];
+    tune_shapes(h, h->fused, r.count, h->P * (long)h->fused->nwp, G.src + r.first, r.plan, &r.spt);
+  }
 };
+
+// the source kernel's samples per thread without timing, for a launch over `strips` strips: the widest
+// workgroup (Nt/1024 samples per thread, at least one) when there are fewer strips than the device has CUs,
+// where a strip's latency rather than the device's throughput sets the time, else the default shape (0).
+// SEARAY_UPPE_SOURCE_SPT overrides; rotator and envelope sources keep their shape.
+static int default_spt(const uppe_handle* h, const FusedState* s, long strips) {
+  if (h->rot_states > 2 || h->paraxial) return 0;
+  const char* env = getenv("SEARAY_UPPE_SOURCE_SPT");
+  const int want = env ? std::atoi(env) : (strips < s->n_cu ? std::max(1, h->nt / 1024) : 0);
+  if (want > 0 && (want & (want - 1)) == 0 && want <= 4 && h->nt / want <= 1024 && h->nt / want >= 64) return want;
+  return 0;
+}
+
+// a standalone radial handle on the matrix cores: contraction plans timed on the handle once per nonlinear band
+// (the r2k plan depends on it), the source shape by default_spt
+template <typename T>
+void tune_single(uppe_handle* h, FusedState* s) {
+  if (s->cm < 0 || h->d.geometry != UPPE_RADIAL || s->tuned_f1 == s->f1) return;
+  int unused = 0;
+  BatchOps<T>::tune_shapes(h, s, 1, 0, nullptr, s->plan, &unused);
+  s->spt = default_spt(h, s, h->P);
+  s->tuned_f1 = s->f1;
+}
 
 static hipStream_t run_stream(const uppe_batch* b, const BatchRun& r) {
   return b->members[(size_t)b->fb->groups[(size_t)r.group].members[(size_t)r.first]]->stream;
@@ -3119,6 +3161,7 @@ void fused_batch_advance(uppe_batch* b, const uppe_step_control* c, int* steps_o
       r.count = j - r.first;
       r.plan[0] = h0->fused->plan[0];
       r.plan[1] = h0->fused->plan[1];
+      r.spt = default_spt(h0, h0->fused, (long)r.count * h0->P);
       if (fb->autotune) {
         const std::vector<long> key = {g, r.first, r.count, h0->fused->f1};
         auto it = std::find_if(fb->tuned.begin(), fb->tuned.end(), [&](const auto& e) { return e.first == key; });

@@ -1,8 +1,8 @@
 '''Batched radial stepping through uppe_batch_* against standalone handles and the FP64 oracle.
 
-A batch member runs the same kernels as a standalone handle on the same inputs, so with the source kernel's
-default launch shape (autotune off) the two agree to rounding of identical operation orders: 1e-12 relative
-L2 in FP64 and FP32 alike (in practice bit for bit). With autotune on, the source kernel may take another
+A batch member runs the same kernels as a standalone handle on the same inputs, so with one source launch shape
+for both (autotune off, samples per thread pinned) the two agree to rounding of identical operation orders:
+1e-12 relative L2 in FP64 and FP32 alike (in practice bit for bit). With autotune on, the source kernel may take another
 number of samples per thread, which reorders the FP32 depletion sums: 1e-5 then, the FP32 field tolerance. Each
 member is also checked against the oracle's track of its own vol_dict: 1e-10 (FP64) and 1e-5 (FP32) on the
 field; 1e-10 / 1e-4 relative on the electron density, or 1e-12 of the gas density where the gas is barely
@@ -43,9 +43,12 @@ def _mixed_vols(adaptive, n=6, steps=4):
     return vols, drawn
 
 
+@pytest.mark.parametrize('spt', ['1', '4'])
 @pytest.mark.parametrize('adaptive', [False, True])
 @pytest.mark.parametrize('precision', ['fp64', 'fp32', 'bf16x3'])
-def test_batch_matches_standalone(precision, adaptive):
+def test_batch_matches_standalone(precision, adaptive, spt, monkeypatch):
+    # the same source launch shape (samples per thread) in the batch and in the standalone handles
+    monkeypatch.setenv('SEARAY_UPPE_SOURCE_SPT', spt)
     vols, drawn = _mixed_vols(adaptive)
     _say('batch vs standalone %s adaptive=%s grids %s' % (precision, adaptive, [d[0] for d in drawn]))
     with batch.Batch(vols, precision=precision, autotune=False) as b:
