@@ -21,7 +21,11 @@ Printed JSON line (one):
                 stepper on the same step schedule; the same on the radial Kerr workload (Nw = 513, Nr = 128,
                 100 fixed steps); the cost of the adjoint gradient of an output-spectrum objective on that workload
                 against a forward run, with checkpoints every 10 steps and with every stage kept, and its error
-                against the FP64 adjoint
+                against the FP64 adjoint; the 64-instance batched workload (Nw = 1025, Nr in {64, 128, 256, 512},
+                O2 / Ar, 0.1-5 P_cr, 200 fixed steps; one launch per kernel for each group of equally shaped
+                instances, autotuned launch shapes, HIP graphs) against the same instances run one after another
+
+``--workload batch`` runs only the batched workload (its JSON line carries the drawn instances).
 
 ``--workload adjoint`` runs only that adjoint measurement (forward and gradient sweeps, FP32 by default), for
 kernel traces of the adjoint.
@@ -373,6 +377,10 @@ def main():
             extras['radial_kerr_513x128'] = kerr_513x128_speedup(prec, args.plain_steps)
             log('adjoint gradient cost, radial Kerr 513 x 128')
             extras['adjoint'] = adjoint_cost()
+            log('batched workload: 64 radial instances (%s)' % prec)
+            bw = batch_workload(prec, 200, 2, seq_steps=10)
+            bw.pop('instances')
+            extras['batched_64'] = bw
 
     cpu = None
     if not args.no_cpu:

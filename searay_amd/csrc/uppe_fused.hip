@@ -92,7 +92,7 @@ struct FusedState {
   // launch plan of each contraction direction (0 = k2r, 1 = r2k): column tiles of 8 complex columns (the
   // last one also takes the remainder), row blocks per tile, waves per workgroup, 16-row tiles per wave
   struct HkPlan {
-    int tiles = 1, split = 1, nwv = 1, rt = 1;
+    int tiles = 1, split = 1, nwv = 1, rt = 1, wide = 0;
   } plan[2];
   int n_cu = 256;
   int spt = 0;          // source samples per thread of the step (0: the default shape)
@@ -535,7 +535,7 @@ __global__ __launch_bounds__(1024) void f_hankel(
     cplx<typename MfmaTraits<CM>::T>* __restrict__ k1, cplx<typename MfmaTraits<CM>::T>* __restrict__ k2,
     cplx<typename MfmaTraits<CM>::T>* __restrict__ k3, StepState* __restrict__ st,
     unsigned long long* __restrict__ qmax, unsigned long long* __restrict__ qmax_next, int stage, double zfix,
-    double dzfix, double dzprev, long zs) {
+    double dzfix, double dzprev, int wide, long zs) {
   using T = typename MfmaTraits<CM>::T;
   using C = cplx<T>;
   using acc_t = typename MfmaTraits<CM>::acc_t;
@@ -568,12 +568,15 @@ __global__ __launch_bounds__(1024) void f_hankel(
   const int bm = nwv * rw;
   const int row0 = blockIdx.y * bm;
   // column tiles of HC complex columns; the last tile of the grid also takes the remainder (fewer than HC
-  // columns), so that the grid never needs a workgroup for a sliver. ct_n = 16-column MFMA tiles in use.
+  // columns), so that the grid never needs a workgroup for a sliver. Wide tiles (wide != 0) take 2 HC columns
+  // each, the last one what is left: every A fragment then feeds two MFMA column tiles, which halves the
+  // operator reads. nmt = 16-column MFMA tiles in use.
   constexpr int HC = BN / 2;
   const int tile = xcd_tile(blockIdx.x, gridDim.x);
-  const int fc0 = f0 + tile * HC;  // first complex column of this tile
-  const int ct_n = (tile == (int)gridDim.x - 1 && fend - fc0 > HC) ? 2 : 1;
+  const int fc0 = f0 + tile * (wide ? 2 * HC : HC);  // first complex column of this tile
+  const int ct_n = wide ? 2 : ((tile == (int)gridDim.x - 1 && fend - fc0 > HC) ? 2 : 1);
   const int hcw = ct_n * HC;       // complex columns staged (columns >= fend are zero)
+  const int nmt = wide ? min(2, (fend - fc0 + HC - 1) / HC) : ct_n;
   constexpr int BW = 2 * BN;       // LDS row width in real columns (room for two MFMA column tiles)
   double zph = zfix, cu = 0.0, dz = dzfix;
   if (st) {
@@ -608,7 +611,7 @@ __global__ __launch_bounds__(1024) void f_hankel(
   // ---- stage the B slab (Kp x hcw) in LDS with the prologue applied; loads of a batch are issued together
   constexpr int EB = hankel_batch(CM, RT);  // register room: elements per thread per batch
   const int ldb = Kp + 8;     // bf16 row stride (k contiguous), padded
-  const int lgh = ct_n == 2 ? __builtin_ctz(HC) + 1 : __builtin_ctz(HC);
+  const int lgh = __builtin_ctz(hcw);
   const int ne = Kp * hcw;
   const int eb = min(EB, (ne + nth - 1) / nth);  // elements per thread per batch: as few as cover the slab
   for (int base = 0; base < ne; base += eb * nth) {
@@ -675,7 +678,7 @@ __global__ __launch_bounds__(1024) void f_hankel(
   __syncthreads();
 
   // ---- MFMA main loop: wave w owns rows [row0 + w rw, row0 + (w+1) rw), rw <= 32
-  constexpr int CT = BW / 16;  // column tiles the accumulators can hold; ct_n of them are in use
+  constexpr int CT = BW / 16;  // column tiles the accumulators can hold; nmt of them are in use
   acc_t acc[RT][CT];
 #pragma unroll
   for (int r = 0; r < RT; r++)
@@ -694,7 +697,7 @@ __global__ __launch_bounds__(1024) void f_hankel(
         bf16x8 bh[CT], bl[CT];
 #pragma unroll
         for (int c = 0; c < CT; c++) {
-          if (c >= ct_n) break;
+          if (c >= nmt) break;
           const int col = c * 16 + (lane & 15), ko = kc * 32 + 8 * (lane >> 4);
           bh[c] = *reinterpret_cast<const bf16x8*>(bhi + col * ldb + ko);
           bl[c] = *reinterpret_cast<const bf16x8*>(blo + col * ldb + ko);
@@ -704,7 +707,7 @@ __global__ __launch_bounds__(1024) void f_hankel(
           if (mt0 + r >= M16) break;
 #pragma unroll
           for (int c = 0; c < CT; c++) {
-            if (c >= ct_n) break;
+            if (c >= nmt) break;
             acc[r][c] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(ring_h[d][r], bh[c], acc[r][c], 0, 0, 0);
             acc[r][c] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(ring_h[d][r], bl[c], acc[r][c], 0, 0, 0);
             acc[r][c] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(ring_l[d][r], bh[c], acc[r][c], 0, 0, 0);
@@ -726,13 +729,13 @@ __global__ __launch_bounds__(1024) void f_hankel(
           const int krow = kq * 16 + 4 * j + (lane >> 4);
           T b[CT];
 #pragma unroll
-          for (int c = 0; c < CT; c++) b[c] = c < ct_n ? bs[krow * BW + c * 16 + (lane & 15)] : T(0);
+          for (int c = 0; c < CT; c++) b[c] = c < nmt ? bs[krow * BW + c * 16 + (lane & 15)] : T(0);
 #pragma unroll
           for (int r = 0; r < 2; r++) {
             if (r >= rt_n || mt0 + r >= M16) break;
 #pragma unroll
             for (int c = 0; c < CT; c++) {
-              if (c >= ct_n) break;
+              if (c >= nmt) break;
               if constexpr (CM == CM_F64)
                 acc[r][c] = __builtin_amdgcn_mfma_f64_16x16x4f64(ring_h[d][r][j], b[c], acc[r][c], 0, 0, 0);
               else
@@ -752,7 +755,7 @@ __global__ __launch_bounds__(1024) void f_hankel(
   for (int r = 0; r < RT; r++) {
 #pragma unroll
     for (int c = 0; c < CT; c++) {
-      if (c >= ct_n) break;
+      if (c >= nmt) break;
 #pragma unroll
       for (int i = 0; i < 4; i++) {
         const int rin = CM == CM_F64 ? 4 * i + (lane >> 4) : 4 * (lane >> 4) + i;
@@ -2258,8 +2261,10 @@ struct Fused {
     constexpr int BNV = 16, BW = 2 * BNV;
     const int rw = 16 * pl.rt;
     const int bm = pl.nwv * rw;
-    if ((long)pl.tiles * 8 + 8 <= ncols || (long)pl.tiles * 8 >= ncols + 8)
-      throw Error{UPPE_ERR_STATE, "radial contraction plan out of date"};
+    // narrow tiles: 8 columns each, the last one up to 15; wide tiles: 16 columns each, the last one what is left
+    const long t8 = (long)pl.tiles * 8, t16 = (long)pl.tiles * 16;
+    const bool fits = pl.wide ? (t16 >= ncols && t16 < ncols + 16) : (t8 + 8 > ncols && t8 < ncols + 8);
+    if (!fits) throw Error{UPPE_ERR_STATE, "radial contraction plan out of date"};
     dim3 grid((unsigned)pl.tiles, (unsigned)((M + bm - 1) / bm), (unsigned)nz);
     size_t slab = CM == CM_BF16X3 ? 2ull * BW * (kp + 8) * 2 : (size_t)kp * BW * sizeof(T);
     size_t tile = (size_t)bm * BW * sizeof(T);
@@ -2267,7 +2272,8 @@ struct Fused {
 #define UPPE_HK(RTV)                                                                                             \
   launchf(h, UPPE_KC_HANKEL, f_hankel<CM, PRO, EPI, BNV, RTV>, grid, dim3(64 * pl.nwv), shm, (const F*)ahi,    \
           (const F*)alo, B, out, M, K, kp, rw, s->nwp, 0, ncols, (const double*)s->phs, (const C*)s->kzc,         \
-          (const T*)s->cond, q, kprev, k1, k2, k3, st, s->qmax, s->qmax_next, stage, zfix, dzfix, dzprev, zs)
+          (const T*)s->cond, q, kprev, k1, k2, k3, st, s->qmax, s->qmax_next, stage, zfix, dzfix, dzprev, pl.wide, \
+          zs)
     if (pl.rt == 1)
       UPPE_HK(1);
     else
@@ -2925,9 +2931,10 @@ struct BatchOps {
   // tiles per wave, 1 to 16 waves per workgroup (no wave without rows)
   static std::vector<FusedState::HkPlan> candidates(int m, int ncols) {
     std::vector<FusedState::HkPlan> out;
-    const int m16 = (m + 15) / 16, narrow = (ncols + 7) / 8, merged = std::max(1, ncols / 8);
-    for (int tiles : {narrow, merged}) {
-      if (tiles == merged && merged == narrow && !out.empty()) break;
+    const int m16 = (m + 15) / 16, narrow = (ncols + 7) / 8, merged = std::max(1, ncols / 8), wide = (ncols + 15) / 16;
+    for (int t = 0; t < 3; t++) {
+      const int tiles = t == 0 ? narrow : (t == 1 ? merged : wide);
+      if (t == 1 && merged == narrow) continue;
       for (int rt = 1; rt <= 2; rt++)
         for (int nwv = 1; nwv <= 16; nwv *= 2) {
           if (rt * nwv > m16 && !(rt == 1 && nwv == 1)) continue;
@@ -2936,6 +2943,7 @@ struct BatchOps {
           p.tiles = tiles;
           p.nwv = nwv;
           p.rt = rt;
+          p.wide = t == 2;
           p.split = (m16 + rt * nwv - 1) / (rt * nwv);
           out.push_back(p);
         }
