@@ -202,19 +202,22 @@ def batch_workload(prec, K, W, seq_steps, graph_steps=8):
     finite = all(bool(np.all(np.isfinite(m.get_state()))) for m in b.members)
     b.close()
     n = int(steps.min())
-    log('sequential standalone handles, %d steps each' % seq_steps)
-    tseq = 0.0
-    for v in vols:
-        rs = uppe_kernel.RunSetup(v)
-        st = rs.make_stepper(uppe_kernel.hip_stepper_factory(prec))
-        run_steps(rs, st, 1)
-        _, dt, _ = run_steps(rs, st, seq_steps)
-        tseq += dt
-        st.close()
-    seq_value = pts * seq_steps / tseq
+    seq_value = None
+    if seq_steps > 0:
+        log('sequential standalone handles, %d steps each' % seq_steps)
+        tseq = 0.0
+        for v in vols:
+            rs = uppe_kernel.RunSetup(v)
+            st = rs.make_stepper(uppe_kernel.hip_stepper_factory(prec))
+            run_steps(rs, st, 1)
+            _, dt, _ = run_steps(rs, st, seq_steps)
+            tseq += dt
+            st.close()
+        seq_value = pts * seq_steps / tseq
     return dict(value=pts * n / secs, steps=n, secs=secs, pts=pts, nan_steps=nan, finite=finite, graph_steps=graph_steps,
                 grids=sorted({d[0] for d in drawn}), plans={str(k): v for k, v in sorted(plans.items())},
-                sequential_pt_steps_per_s=seq_value, speedup_vs_sequential=pts * n / secs / seq_value,
+                sequential_pt_steps_per_s=seq_value,
+                speedup_vs_sequential=(pts * n / secs / seq_value) if seq_value else None,
                 instances=[dict(nr=d[0], power_pcr=round(d[1], 4), gas=d[2]) for d in drawn])
 
 
@@ -274,13 +277,13 @@ def main():
         return
 
     if args.workload == 'batch':
-        prec = args.precision or 'fp32'
+        prec = args.precision or 'bf16x3'
         if prec == 'fp64-plain':
             sys.exit('batches run on the fused paths only')
         K = min(args.steps, 200)
-        r = batch_workload(prec, K, args.warmup, seq_steps=args.plain_steps)
-        log('%d steps in %.3f s: %.4e pt-steps/s (%.2fx the sequential handles)' % (r['steps'], r['secs'], r['value'],
-                                                                                r['speedup_vs_sequential']))
+        r = batch_workload(prec, K, args.warmup, seq_steps=0 if args.no_compare else args.plain_steps)
+        log('%d steps in %.3f s: %.4e pt-steps/s (%sx the sequential handles)'
+            % (r['steps'], r['secs'], r['value'], r['speedup_vs_sequential']))
         print(json.dumps(dict(metric='batched radial UPPE grid-pt z-steps/s (1 GPU)', value=r['value'],
                               unit='grid-pt z-steps/s', n_gpus=1, steps=r['steps'], warmup=args.warmup,
                               ms_per_step=1e3 * r['secs'] / r['steps'], higher_is_better=True, scaling='weak',
