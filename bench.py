@@ -21,7 +21,8 @@ Printed JSON line (one):
                 stepper on the same step schedule; the same on the radial Kerr workload (Nw = 513, Nr = 128,
                 100 fixed steps); the cost of the adjoint gradient of an output-spectrum objective on that workload
                 against a forward run, with checkpoints every 10 steps and with every stage kept, and its error
-                against the FP64 adjoint; the 64-instance batched workload (Nw = 1025, Nr in {64, 128, 256, 512},
+                against the FP64 adjoint, also on the photoionisation workload over its first 4 mm in 100
+                fixed steps (adjoint_ionizing); the 64-instance batched workload (Nw = 1025, Nr in {64, 128, 256, 512},
                 O2 / Ar, 0.1-5 P_cr, 200 fixed steps; one launch per kernel for each group of equally shaped
                 instances, autotuned launch shapes, HIP graphs) against the same instances run one after another
 
@@ -143,20 +144,34 @@ def kerr_513x128_speedup(prec, steps_plain):
                 speedup_vs_fp64_plain=out[prec] / out['fp64-plain'], rel_l2_err_vs_fp64=err)
 
 
-def adjoint_cost(prec='fp32', reps=3):
+def ionizing_adjoint_workload():
+    '''The radial O2 photoionisation workload (Nw = 2049, Nr = 256, Kerr + PPT rate + plasma current) over its
+    first 4 mm in 100 fixed steps of 40 um (the adjoint holds the step sizes constant; the adaptive control starts
+    this workload at 46 um, and fixed 100 um steps overflow the field).'''
+    vol = configs.radial_o2_ionization(steps=100, length='4 mm')
+    vol['minimum step'] = vol['size'][2] / 100
+    return vol
+
+
+def adjoint_cost(prec='fp32', reps=3, workload='kerr'):
     '''The gradient of the on-axis output-spectrum objective with respect to 32 Legendre spectral-phase
-    coefficients over the radial Kerr workload (Nw = 513, Nr = 128, 100 steps), by the adjoint with checkpoints
-    every 10 steps (and with every stage kept, 100), against a plain forward run of the same objective; and the
-    gradient's relative L2 error against the FP64 adjoint.'''
+    coefficients, by the adjoint with checkpoints every 10 steps (and with every stage kept, 100), against a plain
+    forward run of the same objective; and the gradient's relative L2 error against the FP64 adjoint. Workloads:
+    the radial Kerr workload (Nw = 513, Nr = 128, 100 steps), or the photoionisation workload
+    (ionizing_adjoint_workload).'''
     from searay_amd import adjoint
-    vol = configs.radial_kerr_air()
-    sig = configs.kerr_pulse_sigma()
+    if workload == 'kerr':
+        vol, sig = configs.radial_kerr_air(), configs.kerr_pulse_sigma()
+        name = 'radial Kerr Nw=513 Nr=128, 100 steps'
+    else:
+        vol, sig = ionizing_adjoint_workload(), configs.ionizing_pulse_sigma()
+        name = 'radial Kerr + O2 photoionisation + plasma current, Nw=2049 Nr=256, 100 fixed steps over 4 mm'
     c = 0.05 * np.random.default_rng(3).standard_normal(32)
     p64 = adjoint.SpectralPhaseProblem(vol, sig, precision='fp64', checkpoint_every=10)
     _, g64 = p64.value_and_grad(c)
     p64.close()
-    out = dict(workload='gradient of the on-axis output spectrum objective, 32 Legendre phase coefficients, '
-                        'radial Kerr Nw=513 Nr=128, 100 steps', precision=prec)
+    out = dict(workload='gradient of the on-axis output spectrum objective, 32 Legendre phase coefficients, ' + name,
+               precision=prec)
     for every in (10, 100):
         p = adjoint.SpectralPhaseProblem(vol, sig, precision=prec, checkpoint_every=every, target=p64.target)
         p.loss(c)
@@ -272,8 +287,8 @@ def main():
         prec = args.precision or 'fp32'
         if prec == 'fp64-plain':
             sys.exit('the adjoint runs on the fused paths only')
-        log('adjoint gradient cost (%s), radial Kerr 513 x 128' % prec)
-        print(json.dumps(adjoint_cost(prec)))
+        log('adjoint gradient cost (%s), radial Kerr 513 x 128 and radial O2 2049 x 256' % prec)
+        print(json.dumps(dict(kerr=adjoint_cost(prec), ionizing=adjoint_cost(prec, workload='ionizing'))))
         return
 
     if args.workload == 'batch':
@@ -380,6 +395,8 @@ def main():
             extras['radial_kerr_513x128'] = kerr_513x128_speedup(prec, args.plain_steps)
             log('adjoint gradient cost, radial Kerr 513 x 128')
             extras['adjoint'] = adjoint_cost()
+            log('adjoint gradient cost, radial O2 photoionisation 2049 x 256')
+            extras['adjoint_ionizing'] = adjoint_cost(workload='ionizing')
             log('batched workload: 64 radial instances (%s)' % prec)
             bw = batch_workload(prec, 200, 2, seq_steps=10)
             bw.pop('instances')

@@ -1,17 +1,21 @@
 '''Gradients of a UPPE output objective with respect to the input spectral phase, by the stepper's adjoint.
 
-The objective is the on-axis output spectrum's squared distance to a target,
+Two objectives, each the squared distance of an output observable to a target:
 
-    L(c) = sum_w (|A(w, r_0, z_f)|^2 - S_tgt(w))^2,   A(w, r, 0) = A_base(w, r) exp(i sum_n c_n P_n(x(w))),
+    spectrum:  L(c) = sum_w (|A(w, r_0, z_f)|^2 - S_tgt(w))^2
+    fluence:   L(c) = sum_r (F(r) - F_tgt(r))^2,   F(r) = sum_w w^2 |A(w, r, z_f)|^2  (the time-integrated |E|^2 of
+               cell r up to a constant, by Parseval, E = i w A)
 
-with Legendre polynomials P_n over x = (w - w_0) / (3 sigma) on |x| <= 1 (zero outside), r_0 the innermost radial
-cell (the grid has no node at r = 0, caustic_tools.py:223), and S_tgt the unperturbed output spectrum times
-(1 + 0.1 xi), xi uniform in [-1, 1) from a seeded generator.
+with the input A(w, r, 0) = A_base(w, r) exp(i sum_n c_n P_n(x(w))), Legendre polynomials P_n over
+x = (w - w_0) / (3 sigma) on |x| <= 1 (zero outside), r_0 the innermost radial cell (the grid has no node at r = 0,
+caustic_tools.py:223), and the target the unperturbed observable times (1 + 0.1 xi), xi uniform in [-1, 1) from a
+seeded generator.
 
 The forward run goes through uppe_advance_checkpointed (the state every `checkpoint_every` steps), finish gives
-A(z_f), and uppe_adjoint carries the cotangent G = dL/dRe A + i dL/dIm A = 4 (|A|^2 - S) A back to the initial
-field; then dL/dc_n = -sum_{w,r} P_n(x(w)) Im(conj(G_0) A_0). ``SpectralPhaseLoss`` puts the pair behind a
-torch.autograd.Function. Kerr-only media on the radial matrix-core paths.
+A(z_f), and uppe_adjoint carries the cotangent G = dL/dRe A + i dL/dIm A (4 (|A|^2 - S) A on axis, or
+4 (F - F_tgt) w^2 A) back to the initial field; then dL/dc_n = -sum_{w,r} P_n(x(w)) Im(conj(G_0) A_0).
+``spectral_phase_loss_function`` puts the pair behind a torch.autograd.Function. Radial UPPE media with Kerr,
+photoionisation and the plasma current (no rotator) on the matrix-core paths.
 '''
 import numpy as np
 from scipy.special import eval_legendre
@@ -30,10 +34,14 @@ def legendre_basis(w, w0, sigma, n):
 
 
 class SpectralPhaseProblem:
-    '''The spectral-phase objective on one diagonal interval of a radial UPPE workload (vol_dict as for track).'''
+    '''An output objective (objective='spectrum' or 'fluence') of the input spectral phase on one diagonal
+    interval of a radial UPPE workload (vol_dict as for track).'''
 
     def __init__(self, vol, sigma, precision='fp32', checkpoint_every=10, n_phase=32, w0=1.0, target_seed=1005,
-                 target=None, stepper_factory=None):
+                 target=None, stepper_factory=None, objective='spectrum'):
+        if objective not in ('spectrum', 'fluence'):
+            raise ValueError("objective must be 'spectrum' or 'fluence'")
+        self.objective = objective
         self.rs = uppe_kernel.RunSetup(vol)
         if self.rs.diagnostic_steps != 1:
             raise ValueError('the objective is defined on one diagonal interval (wave grid with two planes)')
@@ -63,10 +71,25 @@ class SpectralPhaseProblem:
         a_r, _, _, _ = self.st.finish(self.zf - self.zi, want_diagnostics=False)
         return a0, a_r
 
+    def _observable(self, a_r):
+        if self.objective == 'spectrum':
+            return np.abs(a_r[:, 0, 0]) ** 2
+        return np.sum((self.rs.w ** 2)[:, None] * np.abs(a_r[:, :, 0]) ** 2, axis=0)
+
+    def _cotangent(self, a_r, d):
+        '''dL/dRe A + i dL/dIm A of L = sum d^2 at the output field a_r.'''
+        g = np.zeros_like(a_r)
+        if self.objective == 'spectrum':
+            g[:, 0, 0] = 4 * d * a_r[:, 0, 0]
+        else:
+            g[:, :, 0] = 4 * d[None, :] * (self.rs.w ** 2)[:, None] * a_r[:, :, 0]
+        return g
+
     def spectrum(self, c):
-        '''|A(w, r_0, z_f)|^2 of the run with phase coefficients c.'''
+        '''The observable of the run with phase coefficients c: |A(w, r_0, z_f)|^2 ('spectrum') or F(r)
+        ('fluence').'''
         _, a_r = self._run(c, False)
-        return np.abs(a_r[:, 0, 0]) ** 2
+        return self._observable(a_r)
 
     def loss(self, c):
         '''L(c) by a plain forward run.'''
@@ -75,11 +98,8 @@ class SpectralPhaseProblem:
     def forward(self, c):
         '''L(c) by a checkpointed forward run, keeping what backward() needs.'''
         a0, a_r = self._run(c, True)
-        a = a_r[:, 0, 0]
-        d = np.abs(a) ** 2 - self.target
-        g = np.zeros_like(a_r)
-        g[:, 0, 0] = 4 * d * a
-        self._a0, self._g = a0, g
+        d = self._observable(a_r) - self.target
+        self._a0, self._g = a0, self._cotangent(a_r, d)
         return float(np.sum(d ** 2))
 
     def backward(self):

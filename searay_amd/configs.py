@@ -131,6 +131,12 @@ def radial_raman_air(scale=1, steps=None, seed=1003, precision=None, states=32):
     return vol
 
 
+def ionizing_pulse_sigma():
+    '''Spectral 1/e half-width sigma = 2/t0 of the photoionisation workloads' pulse (40 fs FWHM), in units of w00.'''
+    helper = input_tools.InputHelper(MKS_LENGTH)
+    return 2.0 / (helper.dnum('40 fs') / np.sqrt(2 * np.log(2)))
+
+
 def o2_ionizer(kind='explicit'):
     helper = input_tools.InputHelper(MKS_LENGTH)
     Uion = helper.dnum('12.07 eV')

@@ -1669,6 +1669,179 @@ __global__ __launch_bounds__(1024) void f_source_adj(const cplx<T>* __restrict__
   }
 }
 
+// d rate / d|e| of ion_rate at field modulus ea (sim units) with rate its value there: the explicit form
+// rate_conv C_pre E^C_pow exp(C_exp / E) or the series rate_conv exp(sum_n c_n log(E)^n), E = e_conv ea + e_cutoff
+// (ionization.cl:22-44, 71-97)
+__device__ inline double ion_rate_slope(double ea, const uppe_ionization& ion, double rate) {
+  const double emag = ion.e_conv * ea + ion.e_cutoff;
+  if (ion.kind == UPPE_ION_EXPLICIT) return rate * (ion.c_pow / emag - ion.c_exp / (emag * emag)) * ion.e_conv;
+  const double l = log(emag);
+  const double dp = ion.c[1] + l * (2.0 * ion.c[2] + l * (3.0 * ion.c[3] + l * (4.0 * ion.c[4] + l * 5.0 * ion.c[5])));
+  return rate * dp / emag * ion.e_conv;
+}
+
+// the soft time window applied to the electron density, SoftTimeWindow(4, 64) (fft.cl:331-355)
+template <typename T>
+__device__ inline T soft_window(int n, int N) {
+  T x = (n < 4 || n > N - 5) ? T(0) : T(1);
+  if (n >= 4 && n < 68) {
+    const T q = (T)(n - 4) / T(64);
+    x *= T(10) * q * q * q - T(15) * q * q * q * q + T(6) * q * q * q * q * q;
+  }
+  if (n <= N - 5 && n > N - 69) {
+    const T q = (T)(N - 5 - n) / T(64);
+    x *= T(10) * q * q * q - T(15) * q * q * q * q + T(6) * q * q * q * q * q;
+  }
+  return x;
+}
+
+// exclusive prefix sum of v over the threads of the block in thread order, and the block total; ws holds 16
+// values of LDS scratch
+template <typename T>
+__device__ inline T block_excl_prefix(T v, T* ws, T* total) {
+  const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6, nwv = (blockDim.x + 63) >> 6;
+  const T ex = wave_excl_prefix(v);
+  if (lane == min(63, (int)blockDim.x - 1 - wave * 64)) ws[wave] = ex + v;
+  __syncthreads();
+  T before = T(0), tot = T(0);
+  for (int w2 = 0; w2 < nwv; w2++) {
+    const T t = ws[w2];
+    if (w2 < wave) before += t;
+    tot += t;
+  }
+  __syncthreads();
+  *total = tot;
+  return before + ex;
+}
+
+// Adjoint of the full source on one strip (Kerr, photoionisation rate, depletion integral, soft window, plasma
+// current): the vector-Jacobian product of
+//   J = F_NL [ (-i w) rfft(P) + rfft(J_p) / (0.1 - i w) ],   P = n_g chi3 e^3,   J_p = n_e e,
+//   n_e(n) = W(n) n_g (1 - exp(-I_n)),   I_n = dt (r_n / 2 + sum_{m > n} r_m),   r_n = rate(|e_n|), r_{N-1} = 0,
+// e = irfft(E) (uppe.cl:132-165, ionization.cl:22-44,71-97,127-155, fft.cl:282-355, uppe_kernel.py:139-186).
+// Given the cotangent G of J, with R = (i w F_NL) G and R' = F_NL G / (0.1 + i w):
+//   g_P = Re sum_k R_k e^{2 pi i k t/N},   g_Jp = Re sum_k R'_k e^{2 pi i k t/N}          (adjoints of rfft)
+//   g_I(n) = e_n g_Jp(n) W(n) n_g exp(-I_n)
+//   g_r(m) = dt (g_I(m) / 2 + sum_{n < m} g_I(n)) for m < N - 1, 0 at N - 1             (transposed scan)
+//   g_e = 3 n_g chi3 e^2 g_P + n_e g_Jp + g_r rate'(|e|) sgn(e)
+//   G_E(k) = c_k rfft(g_e)_k / N                                                         (adjoint of irfft)
+// e and g_P come from one inverse transform of E^ext + i R^ext, g_Jp from a second of R'^ext (Hermitian extensions,
+// inner bins halved). Thread t owns samples t SPT .. t SPT + SPT - 1, so the forward suffix sum and the transposed
+// prefix sum are block scans in thread order. One workgroup per strip, max(min(Nt, 64), Nt/4) threads.
+template <typename T, int SPT>
+__global__ __launch_bounds__(1024) void f_source_adj_ion(const cplx<T>* __restrict__ er, const cplx<T>* __restrict__ gj,
+                                                         cplx<T>* __restrict__ ge, const cplx<T>* __restrict__ tw,
+                                                         const T* __restrict__ w, const T* __restrict__ nlf,
+                                                         const double* __restrict__ ng, int nt, int lgn, int nwp,
+                                                         double redchi3, double dt, uppe_ionization ion,
+                                                         double log_pre) {
+  extern __shared__ char smem[];
+  cplx<T>* Z = reinterpret_cast<cplx<T>*>(smem);
+  __shared__ T ws[16];
+  const int N = nt, M = nt / 2;
+  cplx<T>* Z2 = Z + N;
+  const long p = blockIdx.x;
+  const int tid = threadIdx.x, nth = blockDim.x;
+  const cplx<T>* X = er + p * nwp;
+  const cplx<T>* G = gj + p * nwp;
+  const cplx<T>* tws = tw;
+  if constexpr (sizeof(T) == 4) {
+    cplx<T>* tl = Z2 + N;
+    const int tlen = r8_table_len(lgn);
+    for (int k = tid; k < tlen; k += nth) tl[k] = tw[k];
+    tws = tl;
+  }
+  for (int k = tid; k < N; k += nth) {
+    const int kk = k <= M ? k : N - k;
+    cplx<T> x = X[kk];
+    const cplx<T> g = G[kk];
+    const T f = nlf[kk], wk = w[kk];
+    const T wf = wk * f;
+    cplx<T> y = cmk(-wf * g.im, wf * g.re);  // i w F_NL G
+    const T d = f / (T(0.01) + wk * wk);      // F_NL G (0.1 - i w) / (0.01 + w^2)
+    cplx<T> y2 = cmk(d * (T(0.1) * g.re + wk * g.im), d * (T(0.1) * g.im - wk * g.re));
+    if (kk == 0 || kk == M) {
+      x.im = T(0);
+      y.im = T(0);
+      y2.im = T(0);
+    } else {
+      y = cscale(y, T(0.5));
+      y2 = cscale(y2, T(0.5));
+    }
+    if (k > M) {
+      x = cconj(x);
+      y = cconj(y);
+      y2 = cconj(y2);
+    }
+    Z[lds_swz<T>(k)] = cmk(x.re - y.im, x.im + y.re);
+    Z2[lds_swz<T>(k)] = y2;
+  }
+  __syncthreads();
+  lds_fft_r8(Z, N, lgn, tws, +1, tid);
+  lds_fft_r8(Z2, N, lgn, tws, +1, tid);
+  const double ngp = ng[p];
+  const T kc3 = (T)(3.0 * ngp * redchi3), invn = T(1) / T(N), ngt = (T)ngp, dtt = (T)dt;
+  const int a0 = tid * SPT;
+  T e[SPT], gp[SPT], gjp[SPT], r[SPT], dr[SPT];
+  T rtot = T(0);
+#pragma unroll
+  for (int j = 0; j < SPT; j++) {
+    const int n = a0 + j;
+    const cplx<T> z = Z[lds_swz<T>(n)];
+    e[j] = z.re * invn;
+    gp[j] = z.im;
+    gjp[j] = Z2[lds_swz<T>(n)].re;
+    const double ea = fabs((double)e[j]);
+    const double rate = ion_rate<T>(ea, ion, log_pre);
+    r[j] = n >= N - 1 ? T(0) : (T)rate;
+    dr[j] = n >= N - 1 ? T(0) : (T)(ion_rate_slope(ea, ion, rate) * (e[j] > T(0) ? 1.0 : (e[j] < T(0) ? -1.0 : 0.0)));
+    rtot += r[j];
+  }
+  // forward depletion integral: samples after this thread's own hold the rest of the suffix sum
+  T rall;
+  const T rbefore = block_excl_prefix(rtot, ws, &rall);
+  T later = rall - rbefore - rtot;
+  T ne[SPT], gI[SPT];
+  T gtot = T(0);
+#pragma unroll
+  for (int j = SPT - 1; j >= 0; j--) {
+    const int n = a0 + j;
+    const T I = (later + T(0.5) * r[j]) * dtt;
+    later += r[j];
+    const T win = soft_window<T>(n, N);
+    T ex, dep;
+    if constexpr (sizeof(T) == 4) {
+      ex = expf(-I);
+      dep = -expm1f(-I);
+    } else {
+      ex = exp(-I);
+      dep = 1.0 - exp(-I);
+    }
+    ne[j] = win * ngt * dep;
+    gI[j] = e[j] * gjp[j] * win * ngt * ex;
+    gtot += gI[j];
+  }
+  // transposed scan: g_r(m) = dt (g_I(m)/2 + sum of g_I over earlier samples)
+  T gall;
+  T earlier = block_excl_prefix(gtot, ws, &gall);
+  T gev[SPT];
+#pragma unroll
+  for (int j = 0; j < SPT; j++) {
+    const T gr = (earlier + T(0.5) * gI[j]) * dtt;
+    earlier += gI[j];
+    gev[j] = kc3 * e[j] * e[j] * gp[j] + ne[j] * gjp[j] + gr * dr[j];
+  }
+  __syncthreads();
+#pragma unroll
+  for (int j = 0; j < SPT; j++) Z[lds_swz<T>(a0 + j)] = cmk(gev[j], T(0));
+  __syncthreads();
+  lds_fft_r8(Z, N, lgn, tws, -1, tid);
+  for (int k = tid; k <= M; k += nth) {
+    const cplx<T> f = Z[lds_swz<T>(k)];
+    ge[p * nwp + k] = (k == 0 || k == M) ? cmk(f.re * invn, T(0)) : cscale(f, T(2) * invn);
+  }
+}
+
 // ---------------------------------------------------------------------------------------------------
 // control, layout and reduction kernels
 // ---------------------------------------------------------------------------------------------------
@@ -2245,6 +2418,23 @@ struct Fused {
             (const T*)s->nlf, (const double*)s->ng, N, lgn, s->nwp, s->redchi3);                                  \
     return;                                                                                                      \
   }
+    if (s->ion.kind != UPPE_ION_NONE) {
+      // E, G and R' strips, and the FP32 tables
+      const size_t shm_ion = sizeof(C) * (sizeof(T) == 4 ? 3 * N : 2 * N);
+      const double log_pre = s->ion.kind != UPPE_ION_EXPLICIT ? 0.0
+                             : sizeof(T) == 4                  ? std::log2(s->ion.rate_conv * s->ion.c_pre)
+                                                               : std::log(s->ion.rate_conv * s->ion.c_pre);
+#define UPPE_SRC_ADJ_ION(S)                                                                                      \
+  if (spt == S) {                                                                                                \
+    launchf(h, UPPE_KC_SOURCE, f_source_adj_ion<T, S>, grid, block, shm_ion, er, gj, ge, (const C*)s->tw,         \
+            (const T*)s->w, (const T*)s->nlf, (const double*)s->ng, N, lgn, s->nwp, s->redchi3, s->dt, s->ion,    \
+            log_pre);                                                                                            \
+    return;                                                                                                      \
+  }
+      UPPE_SRC_ADJ_ION(1) UPPE_SRC_ADJ_ION(2) UPPE_SRC_ADJ_ION(4)
+#undef UPPE_SRC_ADJ_ION
+      throw Error{UPPE_ERR_UNSUPPORTED, "adjoint source kernel: unsupported Nt"};
+    }
     UPPE_SRC_ADJ(1) UPPE_SRC_ADJ(2) UPPE_SRC_ADJ(4)
 #undef UPPE_SRC_ADJ
     throw Error{UPPE_ERR_UNSUPPORTED, "adjoint source kernel: unsupported Nt"};
@@ -2577,8 +2767,8 @@ struct Fused {
   static void check_adjoint(uppe_handle* h, FusedState* s) {
     if (h->d.geometry != UPPE_RADIAL || s->cm < 0)
       throw Error{UPPE_ERR_UNSUPPORTED, "adjoint: radial geometry on the matrix-core path only"};
-    if (h->paraxial || s->ion.kind != UPPE_ION_NONE || h->rot_states > 2)
-      throw Error{UPPE_ERR_UNSUPPORTED, "adjoint: Kerr-only UPPE media (no ionizer, no rotator)"};
+    if (h->paraxial || h->rot_states > 2)
+      throw Error{UPPE_ERR_UNSUPPORTED, "adjoint: UPPE media without a rotator"};
   }
 
   static int advance_checkpointed(uppe_handle* h, FusedState* s, const uppe_step_control* c, int every) {

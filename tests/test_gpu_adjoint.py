@@ -1,7 +1,9 @@
 '''Adjoint gradients of the spectral-phase objective (searay_amd.adjoint) through the HIP library.
 
 The FP64 adjoint is exact for the discrete forward map, so it must match central finite differences of the FP64
-forward to their truncation error (h = 1e-4: ~1e-8 relative, tested at 1e-6). FP32 and BF16x3 gradients are
+forward to their truncation error (h = 1e-4: ~1e-8 relative, tested at 1e-6), with Kerr only and through the
+photoionisation rate (explicit and series), depletion integral, soft window and plasma current, for the on-axis
+spectrum and the radial fluence objectives. FP32 and BF16x3 gradients are
 compared with the FP64 adjoint (1e-4 and 1e-3 relative L2). Replays from checkpoints are deterministic, so the
 gradient does not depend on the checkpoint interval.
 '''
@@ -97,8 +99,60 @@ def test_autograd_function():
     assert np.allclose(ct.grad.numpy(), 3.0 * g_ref, rtol=1e-12, atol=0)
 
 
-def test_adjoint_rejects_ionization():
-    _say('adjoint rejects an ionizing medium')
-    vol = configs.radial_o2_ionization(scale=8, steps=2, length='1 mm')
+def _ionizing(precision, every, steps=4, scale=4, nr=16, rate='explicit', objective='spectrum', target=None,
+              length='0.5 mm'):
+    '''The radial O2 photoionisation workload (Kerr, PPT rate, depletion, plasma current) at reduced size with fixed
+    steps (the adjoint holds the step sizes constant).'''
+    vol = configs.radial_o2_ionization(scale=scale, steps=steps, rate=rate, length=length, nr=nr)
+    vol['frequency band'] = (0.0, 2.0)
+    vol['nonlinear band'] = (0.0, 2.0)
+    L = vol['size'][2]
+    vol['minimum step'] = L / steps
+    return adjoint.SpectralPhaseProblem(vol, configs.ionizing_pulse_sigma(), precision=precision,
+                                        checkpoint_every=every, objective=objective, target=target)
+
+
+@pytest.mark.parametrize('rate', ['explicit', 'series'])
+@pytest.mark.parametrize('objective', ['spectrum', 'fluence'])
+def test_fp64_ionizing_gradient_against_central_differences(rate, objective):
+    '''Through the photoionisation rate, the depletion integral, the soft window and the plasma current: the
+    FP64 adjoint against central differences of the FP64 forward (h = 1e-4) on 8 random coefficients.'''
+    _say('adjoint fp64 with ionization (%s rate, %s objective) vs central FD' % (rate, objective))
+    p = _ionizing('fp64', 2, rate=rate, objective=objective)
+    c = _coeffs(9)
+    loss, g = p.value_and_grad(c)
+    idx = np.random.default_rng(10).choice(32, 8, replace=False)
+    fd = np.array([(p.loss(c + 1e-4 * e) - p.loss(c - 1e-4 * e)) / 2e-4 for e in np.eye(32)[idx]])
+    # the plasma term contributes: the gradient of the Kerr-only problem differs
+    ne_free = _ionizing('fp64', 2, rate=rate, objective=objective, target=p.target)
+    ne_free.st.set_medium(ne_free.rs.w, ne_free.rs.mat.kz, ne_free.rs.mat.kg, ne_free.rs.mat.ngas,
+                          ne_free.rs.nl_filter, ne_free.rs.cond_filter, ne_free.rs.mat.redchi3, ne_free.rs.dt,
+                          dict(ne_free.rs.ion, kind=0))
+    _, g_kerr = ne_free.value_and_grad(c)
+    ne_free.close()
+    p.close()
+    _say('  gradient vs FD %.2e, Kerr-only gradient vs full %.2e' % (rel_l2(g[idx], fd), rel_l2(g_kerr, g)))
+    assert loss > 0 and np.linalg.norm(g) > 0
+    assert rel_l2(g[idx], fd) < 1e-6, rel_l2(g[idx], fd)
+    assert rel_l2(g_kerr, g) > 1e-3, rel_l2(g_kerr, g)
+
+
+@pytest.mark.parametrize('precision,tol', [('fp32', 1e-4), ('bf16x3', 1e-3)])
+def test_reduced_precision_ionizing_gradient(precision, tol):
+    _say('adjoint %s vs fp64 adjoint with ionization, 1025 x 32, 20 steps' % precision)
+    c = _coeffs(11)
+    p64 = _ionizing('fp64', 5, steps=20, scale=2, nr=32)
+    _, g64 = p64.value_and_grad(c)
+    p = _ionizing(precision, 5, steps=20, scale=2, nr=32, target=p64.target)
+    _, g = p.value_and_grad(c)
+    p64.close()
+    p.close()
+    _say('  %s gradient vs fp64 adjoint %.2e' % (precision, rel_l2(g, g64)))
+    assert rel_l2(g, g64) < tol, rel_l2(g, g64)
+
+
+def test_adjoint_rejects_rotator():
+    _say('adjoint rejects a medium with a rotator')
+    vol = configs.radial_raman_air(scale=8, steps=2)
     with pytest.raises(_abi.UppeError):
         adjoint.SpectralPhaseProblem(vol, configs.kerr_pulse_sigma(), precision='fp32').value_and_grad(np.zeros(32))
