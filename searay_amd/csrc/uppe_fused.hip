@@ -3154,11 +3154,13 @@ struct BatchOps {
     return ms / reps;
   }
 
-  // choose launch shapes by timing on nz instances zs elements apart starting at member (h, s): the k2r
-  // contraction with its stage prologue (writes E_r, scratch) and the r2k contraction without its epilogue
-  // into E_r give the two plans; with src, the source kernel's samples per thread are timed too (it writes
-  // J_r, scratch). The state is not touched, and the plans do not change any result (every output sums its
-  // contraction in the same order).
+  // choose launch shapes by timing on nz instances zs elements apart starting at member (h, s), as the second
+  // RK stage runs them: the k2r contraction with its stage prologue (writes E_r, scratch) and the r2k
+  // contraction with its stage epilogue (writes the second slope, which every step writes before it reads it)
+  // give the two plans; with src, the source kernel's samples per thread are timed too (it writes J_r,
+  // scratch). The state is not touched, and the plans do not change any result (every output sums its
+  // contraction in the same order). The incoming plans are the defaults: another candidate replaces one only
+  // when it is 3 % faster, so that timing noise does not pick between equals.
   template <int CM>
   static void tune_shapes(uppe_handle* h, FusedState* s, int nz, long zs, const SrcInst* src,
                           FusedState::HkPlan plan[2], int* spt_out) {
@@ -3166,47 +3168,63 @@ struct BatchOps {
     hipEvent_t e0, e1;
     UPPE_HIP(hipEventCreate(&e0));
     UPPE_HIP(hipEventCreate(&e1));
+    constexpr float margin = 0.97f;
+    constexpr int reps = 5;
     try {
-      float best = 1e30f;
-      for (const auto& p : candidates(h->n1, h->nw)) {
-        const float t = time_launch(h->stream, e0, e1, [&] {
+      auto k2r = [&](const FusedState::HkPlan& p) {
+        return time_launch(h->stream, e0, e1, [&] {
           if (h->paraxial)
             F::template hk_launch<CM, PRO_STAGE_ENV, EPI_NONE>(h, s, s->frb_hi, s->frb_lo, s->kp_b, h->n1, h->nk,
-                                                               h->nw, p, (const C*)q, (C*)s->er, nullptr, q, k1, k2,
-                                                               k3, s->st, 1, 0.0, 0.0, 0.0, nz, zs);
+                                                               h->nw, p, (const C*)q, (C*)s->er, (const C*)k1, q, k1,
+                                                               k2, k3, s->st, 2, 0.0, 0.0, 0.0, nz, zs);
           else
             F::template hk_launch<CM, PRO_STAGE, EPI_NONE>(h, s, s->frb_hi, s->frb_lo, s->kp_b, h->n1, h->nk, h->nw,
-                                                           p, (const C*)q, (C*)s->er, nullptr, q, k1, k2, k3, s->st,
-                                                           1, 0.0, 0.0, 0.0, nz, zs);
-        });
-        if (t < best) {
+                                                           p, (const C*)q, (C*)s->er, (const C*)k1, q, k1, k2, k3,
+                                                           s->st, 2, 0.0, 0.0, 0.0, nz, zs);
+        }, reps);
+      };
+      auto r2k = [&](const FusedState::HkPlan& p) {
+        return time_launch(h->stream, e0, e1, [&] {
+          F::template hk_launch<CM, PRO_NONE, EPI_STAGE>(h, s, s->frf_hi, s->frf_lo, s->kp_f, h->nk, h->n1, s->f1, p,
+                                                         (const C*)s->jr, (C*)nullptr, nullptr, q, k1, k2, k3, s->st,
+                                                         2, 0.0, 0.0, 0.0, nz, zs);
+        }, reps);
+      };
+      float best = k2r(plan[0]);
+      for (const auto& p : candidates(h->n1, h->nw)) {
+        const float t = k2r(p);
+        if (t < margin * best) {
           best = t;
           plan[0] = p;
         }
       }
-      best = 1e30f;
+      best = r2k(plan[1]);
       for (const auto& p : candidates(h->nk, s->f1)) {
-        const float t = time_launch(h->stream, e0, e1, [&] {
-          F::template hk_launch<CM, PRO_NONE, EPI_NONE>(h, s, s->frf_hi, s->frf_lo, s->kp_f, h->nk, h->n1, s->f1, p,
-                                                        (const C*)s->jr, (C*)s->er, nullptr, q, k1, k2, k3, s->st,
-                                                        1, 0.0, 0.0, 0.0, nz, zs);
-        });
-        if (t < best) {
+        const float t = r2k(p);
+        if (t < margin * best) {
           best = t;
           plan[1] = p;
         }
       }
-      best = 1e30f;
-      for (int spt = 1; src && spt <= 4; spt *= 2) {
-        const int nth = h->nt / spt;
-        if (nth > 1024 || nth < 64) continue;
-        const float t = time_launch(h->stream, e0, e1, [&] {
-          F::source(h, s, (const C*)s->er, (C*)s->jr, false, s->st, nz, src, zs, spt);
-        });
-        if (t < best) {
-          best = t;
-          *spt_out = spt;
+      if (src) {
+        auto source = [&](int spt) {
+          return time_launch(h->stream, e0, e1, [&] {
+            F::source(h, s, (const C*)s->er, (C*)s->jr, false, s->st, nz, src, zs, spt);
+          }, reps);
+        };
+        // the incoming choice (0: the default shape, max(min(Nt, 64), Nt/4) threads)
+        int cur = *spt_out > 0 ? *spt_out : h->nt / std::max(std::min(h->nt, 64), h->nt / 4);
+        best = source(cur);
+        for (int spt = 1; spt <= 4; spt *= 2) {
+          const int nth = h->nt / spt;
+          if (spt == cur || nth > 1024 || nth < 64) continue;
+          const float t = source(spt);
+          if (t < margin * best) {
+            best = t;
+            cur = spt;
+          }
         }
+        *spt_out = cur;
       }
     } catch (...) {
       (void)hipEventDestroy(e0);
