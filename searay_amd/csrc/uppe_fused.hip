@@ -3267,13 +3267,19 @@ static int default_spt(const uppe_handle* h, const FusedState* s, long strips) {
   return 0;
 }
 
-// a standalone radial handle on the matrix cores: contraction plans timed on the handle once per nonlinear band
-// (the r2k plan depends on it), the source shape by default_spt
+// a standalone radial handle on the matrix cores: the source shape by default_spt; with SEARAY_UPPE_TUNE=1 the
+// contraction plans are also timed on the handle once per nonlinear band (the r2k plan depends on it). By
+// default a standalone handle keeps plan_hankel's plans, which fit one workgroup per CU: timing them against
+// the candidates on the single-instance workloads found no plan faster by more than the run-to-run spread, and
+// fixed plans keep every run's launch shapes (and the measured traffic per launch) the same.
 template <typename T>
 void tune_single(uppe_handle* h, FusedState* s) {
   if (s->cm < 0 || h->d.geometry != UPPE_RADIAL || s->tuned_f1 == s->f1) return;
-  int unused = 0;
-  BatchOps<T>::tune_shapes(h, s, 1, 0, nullptr, s->plan, &unused);
+  const char* env = getenv("SEARAY_UPPE_TUNE");
+  if (env && env[0] == '1') {
+    int unused = 0;
+    BatchOps<T>::tune_shapes(h, s, 1, 0, nullptr, s->plan, &unused);
+  }
   s->spt = default_spt(h, s, h->P);
   s->tuned_f1 = s->f1;
 }
