@@ -297,8 +297,9 @@ def main():
             sys.exit('batches run on the fused paths only')
         K = min(args.steps, 200)
         r = batch_workload(prec, K, args.warmup, seq_steps=0 if args.no_compare else args.plain_steps)
-        log('%d steps in %.3f s: %.4e pt-steps/s (%sx the sequential handles)'
-            % (r['steps'], r['secs'], r['value'], r['speedup_vs_sequential']))
+        log('%d steps in %.3f s: %.4e pt-steps/s' % (r['steps'], r['secs'], r['value']) +
+            ('' if r['speedup_vs_sequential'] is None else
+             ' (%.2fx the sequential handles)' % r['speedup_vs_sequential']))
         print(json.dumps(dict(metric='batched radial UPPE grid-pt z-steps/s (1 GPU)', value=r['value'],
                               unit='grid-pt z-steps/s', n_gpus=1, steps=r['steps'], warmup=args.warmup,
                               ms_per_step=1e3 * r['secs'] / r['steps'], higher_is_better=True, scaling='weak',
