@@ -526,7 +526,7 @@ __device__ inline int xcd_tile(int b, int G) {
 constexpr int hankel_batch(int cm, int rt) { return (cm == CM_F64 || rt == 2) ? 2 : 4; }
 
 template <int CM, int PRO, int EPI, int BN, int RT>
-__global__ __launch_bounds__(1024) void f_hankel(
+__global__ __launch_bounds__(BN > 16 ? 512 : 1024) void f_hankel(
     const typename MfmaTraits<CM>::frag_t* __restrict__ Ahi, const typename MfmaTraits<CM>::frag_t* __restrict__ Alo,
     const cplx<typename MfmaTraits<CM>::T>* __restrict__ B, cplx<typename MfmaTraits<CM>::T>* __restrict__ out,
     int M, int K, int Kp, int rw, int nwp, int f0, int fend, const double* __restrict__ phs,
@@ -567,17 +567,18 @@ __global__ __launch_bounds__(1024) void f_hankel(
   const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
   const int bm = nwv * rw;
   const int row0 = blockIdx.y * bm;
-  // column tiles of HC complex columns; the last tile of the grid also takes the remainder (fewer than HC
-  // columns), so that the grid never needs a workgroup for a sliver. Wide tiles (wide != 0) take 2 HC columns
-  // each, the last one what is left: every A fragment then feeds two MFMA column tiles, which halves the
-  // operator reads. nmt = 16-column MFMA tiles in use.
-  constexpr int HC = BN / 2;
+  // column tiles of HC complex columns (one 16-column MFMA tile); the last tile of the grid also takes the
+  // remainder (fewer than HC columns), so that the grid never needs a workgroup for a sliver. Wide tiles take
+  // HC << wide columns each (wide = 1: two MFMA column tiles; wide = 2 with BN = 32: four), the last one what is
+  // left: every A fragment then feeds that many MFMA column tiles, which divides the operator reads.
+  // nmt = MFMA column tiles holding columns below fend.
+  constexpr int HC = 8;
+  constexpr int BW = 2 * BN;       // LDS row width in real columns: BN / 8 MFMA column tiles
   const int tile = xcd_tile(blockIdx.x, gridDim.x);
-  const int fc0 = f0 + tile * (wide ? 2 * HC : HC);  // first complex column of this tile
-  const int ct_n = wide ? 2 : ((tile == (int)gridDim.x - 1 && fend - fc0 > HC) ? 2 : 1);
-  const int hcw = ct_n * HC;       // complex columns staged (columns >= fend are zero)
-  const int nmt = wide ? min(2, (fend - fc0 + HC - 1) / HC) : ct_n;
-  constexpr int BW = 2 * BN;       // LDS row width in real columns (room for two MFMA column tiles)
+  const int tcw = HC << wide;      // complex columns per tile
+  const int fc0 = f0 + tile * tcw; // first complex column of this tile
+  const int hcw = wide ? tcw : ((tile == (int)gridDim.x - 1 && fend - fc0 > HC) ? 2 * HC : HC);  // staged columns
+  const int nmt = min(BW / 16, (min(fend - fc0, hcw) + HC - 1) / HC);
   double zph = zfix, cu = 0.0, dz = dzfix;
   if (st) {
     dz = st->dz;
@@ -2129,6 +2130,15 @@ static long frag_count(int cm, int m, int k) {
   return cm == CM_BF16X3 ? m16 * ((k + 31) / 32) * 64 : m16 * ((k + 15) / 16) * 64;
 }
 
+// dynamic LDS of one contraction launch: the B slab (Kp rows of 2 BN real columns; BF16x3 keeps hi and lo
+// planes with padded rows) or the output tile (bm rows), whichever is larger
+static size_t hk_lds_bytes(int cm, int kp, int bm, int bn) {
+  const size_t bw = 2 * (size_t)bn, el = cm == CM_F64 ? 8 : 4;
+  const size_t slab = cm == CM_BF16X3 ? 2 * bw * (kp + 8) * 2 : (size_t)kp * bw * el;
+  return std::max(slab, (size_t)bm * bw * el);
+}
+constexpr size_t kMaxLds = 160 * 1024;  // LDS of one CU (gfx950)
+
 template <typename T>
 void tune_single(uppe_handle* h, FusedState* s);
 
@@ -2448,26 +2458,32 @@ struct Fused {
                         C* k2, C* k3, StepState* st, int stage, double zfix, double dzfix, double dzprev,
                         int nz = 1, long zs = 0) {
     using F = typename MfmaTraits<CM>::frag_t;
-    constexpr int BNV = 16, BW = 2 * BNV;
     const int rw = 16 * pl.rt;
     const int bm = pl.nwv * rw;
-    // narrow tiles: 8 columns each, the last one up to 15; wide tiles: 16 columns each, the last one what is left
-    const long t8 = (long)pl.tiles * 8, t16 = (long)pl.tiles * 16;
-    const bool fits = pl.wide ? (t16 >= ncols && t16 < ncols + 16) : (t8 + 8 > ncols && t8 < ncols + 8);
-    if (!fits) throw Error{UPPE_ERR_STATE, "radial contraction plan out of date"};
+    // narrow tiles: 8 columns each, the last one up to 15; wide tiles: 8 << wide columns each, the last one what
+    // is left
+    const long t8 = (long)pl.tiles * 8, tw = (long)pl.tiles * (8 << pl.wide);
+    const bool fits = pl.wide ? (tw >= ncols && tw < ncols + (8 << pl.wide)) : (t8 + 8 > ncols && t8 < ncols + 8);
+    if (!fits || pl.wide < 0 || pl.wide > 2 || (pl.wide == 2 && pl.nwv > 8))
+      throw Error{UPPE_ERR_STATE, "radial contraction plan out of date"};
     dim3 grid((unsigned)pl.tiles, (unsigned)((M + bm - 1) / bm), (unsigned)nz);
-    size_t slab = CM == CM_BF16X3 ? 2ull * BW * (kp + 8) * 2 : (size_t)kp * BW * sizeof(T);
-    size_t tile = (size_t)bm * BW * sizeof(T);
-    size_t shm = std::max(slab, tile);
-#define UPPE_HK(RTV)                                                                                             \
+    const size_t shm = hk_lds_bytes(CM, kp, bm, pl.wide == 2 ? 32 : 16);
+#define UPPE_HK(BNV, RTV)                                                                                        \
   launchf(h, UPPE_KC_HANKEL, f_hankel<CM, PRO, EPI, BNV, RTV>, grid, dim3(64 * pl.nwv), shm, (const F*)ahi,    \
           (const F*)alo, B, out, M, K, kp, rw, s->nwp, 0, ncols, (const double*)s->phs, (const C*)s->kzc,         \
           (const T*)s->cond, q, kprev, k1, k2, k3, st, s->qmax, s->qmax_next, stage, zfix, dzfix, dzprev, pl.wide, \
           zs)
-    if (pl.rt == 1)
-      UPPE_HK(1);
-    else
-      UPPE_HK(2);
+    if (pl.wide == 2) {
+      if (pl.rt == 1)
+        UPPE_HK(32, 1);
+      else
+        UPPE_HK(32, 2);
+    } else {
+      if (pl.rt == 1)
+        UPPE_HK(16, 1);
+      else
+        UPPE_HK(16, 2);
+    }
 #undef UPPE_HK
   }
 
@@ -3117,23 +3133,30 @@ struct BatchOps {
     }
   }
 
-  // candidate plans of an m-row contraction over ncols complex columns: both column tilings, 1 or 2 row
-  // tiles per wave, 1 to 16 waves per workgroup (no wave without rows)
-  static std::vector<FusedState::HkPlan> candidates(int m, int ncols) {
+  // candidate plans of an m-row contraction over ncols complex columns with contraction length kp: narrow,
+  // merged, wide (16 columns) and widest (32 columns, at most 8 waves) column tilings, 1 or 2 row tiles per
+  // wave, 1 to 16 waves per workgroup (no wave without rows), within the LDS of one CU.
+  // SEARAY_UPPE_TILE_WIDTH (8, 16 or 32 complex columns) keeps only the tilings of that width.
+  static std::vector<FusedState::HkPlan> candidates(int cm, int m, int ncols, int kp) {
     std::vector<FusedState::HkPlan> out;
-    const int m16 = (m + 15) / 16, narrow = (ncols + 7) / 8, merged = std::max(1, ncols / 8), wide = (ncols + 15) / 16;
-    for (int t = 0; t < 3; t++) {
-      const int tiles = t == 0 ? narrow : (t == 1 ? merged : wide);
-      if (t == 1 && merged == narrow) continue;
+    const int m16 = (m + 15) / 16;
+    const char* env = getenv("SEARAY_UPPE_TILE_WIDTH");
+    const int only = env ? std::atoi(env) : 0;
+    for (int t = 0; t < 4; t++) {
+      const int wide = t < 2 ? 0 : t - 1, width = 8 << wide;
+      const int tiles = t == 0 ? (ncols + 7) / 8 : (t == 1 ? std::max(1, ncols / 8) : (ncols + width - 1) / width);
+      if (t == 1 && tiles == (ncols + 7) / 8) continue;
+      if (only && only != width) continue;
       for (int rt = 1; rt <= 2; rt++)
-        for (int nwv = 1; nwv <= 16; nwv *= 2) {
+        for (int nwv = 1; nwv <= (wide == 2 ? 8 : 16); nwv *= 2) {
           if (rt * nwv > m16 && !(rt == 1 && nwv == 1)) continue;
-          if ((size_t)16 * rt * nwv * 32 * sizeof(T) > 65536) continue;  // output tile within 64 KiB of LDS
+          if (hk_lds_bytes(cm, kp, 16 * rt * nwv, wide == 2 ? 32 : 16) > kMaxLds) continue;
+          if (wide < 2 && (size_t)16 * rt * nwv * 32 * sizeof(T) > 65536) continue;
           FusedState::HkPlan p;
           p.tiles = tiles;
           p.nwv = nwv;
           p.rt = rt;
-          p.wide = t == 2;
+          p.wide = wide;
           p.split = (m16 + rt * nwv - 1) / (rt * nwv);
           out.push_back(p);
         }
@@ -3190,23 +3213,25 @@ struct BatchOps {
                                                          2, 0.0, 0.0, 0.0, nz, zs);
         }, reps);
       };
-      float best = k2r(plan[0]);
-      for (const auto& p : candidates(h->n1, h->nw)) {
+      // with the tile width pinned, the default plan is not a candidate: the first candidate is taken
+      const bool pinned = getenv("SEARAY_UPPE_TILE_WIDTH") != nullptr;
+      float best = pinned ? 1e30f : k2r(plan[0]);
+      for (const auto& p : candidates(s->cm, h->n1, h->nw, s->kp_b)) {
         const float t = k2r(p);
         if (t < margin * best) {
           best = t;
           plan[0] = p;
         }
       }
-      best = r2k(plan[1]);
-      for (const auto& p : candidates(h->nk, s->f1)) {
+      best = pinned ? 1e30f : r2k(plan[1]);
+      for (const auto& p : candidates(s->cm, h->nk, s->f1, s->kp_f)) {
         const float t = r2k(p);
         if (t < margin * best) {
           best = t;
           plan[1] = p;
         }
       }
-      if (src) {
+      if (src && !getenv("SEARAY_UPPE_SOURCE_SPT")) {  // a pinned shape is kept
         auto source = [&](int spt) {
           return time_launch(h->stream, e0, e1, [&] {
             F::source(h, s, (const C*)s->er, (C*)s->jr, false, s->st, nz, src, zs, spt);

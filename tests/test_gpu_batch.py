@@ -164,3 +164,24 @@ def test_batch_graph_replays_match_kernel_launches(graph_steps):
         assert np.array_equal(s0, s1)
         for a, c in zip(q0, q1):
             assert np.array_equal(a, c)
+
+
+@pytest.mark.parametrize('width', [16, 32])
+@pytest.mark.parametrize('precision', ['fp64', 'fp32', 'bf16x3'])
+def test_batch_tile_widths_match_standalone(precision, width, monkeypatch):
+    '''Contraction tiles of 16 and 32 complex columns (every operator fragment feeding two or four MFMA column
+    tiles) give the standalone handles' results exactly: the column tiling does not reorder any sum.'''
+    monkeypatch.setenv('SEARAY_UPPE_TILE_WIDTH', str(width))
+    monkeypatch.setenv('SEARAY_UPPE_SOURCE_SPT', '4')
+    vols, drawn = _mixed_vols(True)
+    _say('batch with %d-column tiles vs standalone %s' % (width, precision))
+    with batch.Batch(vols, precision=precision, autotune=True) as b:
+        b.set_fields()
+        steps = b.advance()
+        qs = [m.get_state() for m in b.members]
+        for i, v in enumerate(vols):
+            assert b.plan(i)['k2r']['tiles'] == -(-v['wave grid'][0] // width)
+    for i, vol in enumerate(vols):
+        n, q = _standalone(vol, precision)
+        assert steps[i] == n
+        assert rel_l2(qs[i], q) < 1e-12, (i, rel_l2(qs[i], q))
