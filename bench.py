@@ -27,6 +27,9 @@ Printed JSON line (one):
                 instances, autotuned launch shapes, HIP graphs) against the same instances run one after another
 
 ``--workload batch`` runs only the batched workload (its JSON line carries the drawn instances).
+``--workload cartesian`` measures the 3-D Cartesian filamentation workload (Nw = 513, 256 x 256, Kerr + O2) the
+same way, its roofline on the transverse FFT pass (28 B per grid point per launch in FP32: 40 B on the passes
+with the stage prologue or epilogue, 16 B on the plain ones).
 
 ``--workload adjoint`` runs only that adjoint measurement (forward and gradient sweeps, FP32 by default), for
 kernel traces of the adjoint.
@@ -236,6 +239,64 @@ def batch_workload(prec, K, W, seq_steps, graph_steps=8):
                 instances=[dict(nr=d[0], power_pcr=round(d[1], 4), gas=d[2]) for d in drawn])
 
 
+def cartesian_workload(args):
+    '''The 3-D Cartesian filamentation workload (Nw = 513, 256 x 256 over 4 mm x 4 mm, 20 P_cr super-Gaussian
+    beam with 10 % amplitude noise, Kerr + O2 photoionisation, 200 fixed steps over 10 cm): K timed steps on one
+    handle, per-kernel-class averages with HIP events, the plain FP64 stepper and the fused FP64 stepper (whose
+    state after 3 steps gives the error), and the FP64 oracle on one step of the host cores.'''
+    prec = args.precision or 'fp32'
+    vol = configs.cartesian_filamentation()
+    rs = uppe_kernel.RunSetup(vol)
+    pts = rs.N[0] * rs.N[1] * rs.N[2]
+    K, W = min(args.steps, 40), min(args.warmup, 2)
+    st = rs.make_stepper(uppe_kernel.hip_stepper_factory(prec))
+    if W > 0:
+        run_steps(rs, st, W)
+    log('cartesian timed region: %d steps (%s)' % (K, prec))
+    n, secs, _ = run_steps(rs, st, K)
+    st.profile_enable(True)
+    run_steps(rs, st, min(K, 10))
+    classes = {kc: st.profile_read(kc) for kc in ('transverse_fft', 'source', 'reduce')}
+    st.profile_enable(False)
+    st.close()
+    el = 8 if prec.startswith('fp64') else 4
+    ms, cnt = classes['transverse_fft']
+    avg_s = ms / max(cnt, 1) * 1e-3
+    # per grid point: 40 B on the passes with the stage prologue or epilogue, 16 B on the plain passes; averaged
+    B = 28.0 * (el / 4) * pts
+    out = dict(metric='UPPE grid-pt z-steps/s (1 GPU), 3-D Cartesian', value=pts * n / secs, unit='grid-pt z-steps/s',
+               n_gpus=1, steps=n, warmup=W, ms_per_step=1e3 * secs / n, higher_is_better=True, scaling='weak',
+               vs_baseline=None, dtype=DTYPE[prec], data='synthetic (seeded super-Gaussian beam, 10% amplitude noise)',
+               config=dict(workload='3-D Cartesian filamentation, Nw=513, 256x256 over 4 mm, Kerr + O2, 200 fixed '
+                                    'steps over 10 cm; first %d steps' % n, precision=prec),
+               roofline=dict(bound='hbm', kernel='transverse FFT pass (f_tfft)', achieved=B / avg_s / 1e9,
+                             peak=HBM_PEAK_GBS, unit='GB/s', frac=B / avg_s / 1e9 / HBM_PEAK_GBS, traffic=None,
+                             avg_launch_us=avg_s * 1e6, launches=cnt, bytes_per_launch=B),
+               class_avg_us={k: (v[0] / v[1] * 1e3 if v[1] else None) for k, v in classes.items()})
+    if not args.no_compare and prec != 'fp64-plain':
+        for p, k in (('fp64-plain', 3), ('fp64', 10)):
+            sx = rs.make_stepper(uppe_kernel.hip_stepper_factory(p))
+            run_steps(rs, sx, 1)
+            nx, tx, _ = run_steps(rs, sx, k)
+            sx.close()
+            out['%s_pt_steps_per_s' % p.replace('-', '_')] = pts * nx / tx
+        out['speedup_vs_fp64_plain'] = out['value'] / out['fp64_plain_pt_steps_per_s']
+        qs = {}
+        for p in ('fp64', prec):
+            sx = rs.make_stepper(uppe_kernel.hip_stepper_factory(p))
+            run_steps(rs, sx, 3)
+            qs[p] = sx.get_state()
+            sx.close()
+        out['rel_l2_err_vs_fp64'] = float(np.linalg.norm(qs[prec] - qs['fp64']) / np.linalg.norm(qs['fp64']))
+        out['error_note'] = 'k-space state after 3 fixed steps'
+    if not args.no_cpu:
+        log('CPU baseline: FP64 oracle, 1 step')
+        cv, cn, ct_, threads = cpu_baseline(uppe_kernel, vol, 1)
+        out['cpu_baseline'] = dict(value=cv, unit='grid-pt z-steps/s', cores=threads, kind='port',
+                                   sample='%d RK4 step of this workload (513 x 256 x 256), FP64, %.1f s' % (cn, ct_))
+    return out
+
+
 def cpu_baseline(module, vol, steps):
     from oracle import oracle
     rs = module.RunSetup(vol)
@@ -248,7 +309,7 @@ def cpu_baseline(module, vol, steps):
     n, _ = st.advance(zi, L, 0, rs.subcycles, rs.dzmin, max_steps=steps)
     dt = time.perf_counter() - t0
     st.close()
-    return rs.N[0] * rs.N[1] * n / dt, n, dt, oracle.num_threads()
+    return rs.N[0] * rs.N[1] * rs.N[2] * n / dt, n, dt, oracle.num_threads()
 
 
 def main():
@@ -263,7 +324,7 @@ def main():
     ap.add_argument('--cpu-steps', type=int, default=60)
     ap.add_argument('--no-cpu', action='store_true')
     ap.add_argument('--no-compare', action='store_true')
-    ap.add_argument('--workload', default='uppe', choices=['uppe', 'paraxial', 'adjoint', 'batch'])
+    ap.add_argument('--workload', default='uppe', choices=['uppe', 'paraxial', 'adjoint', 'batch', 'cartesian'])
     ap.add_argument('--calibrate-copy', type=int, default=0, metavar='MIB',
                     help='only run a plain device-to-device copy of MIB MiB (for scaling HBM byte counters)')
     args = ap.parse_args()
@@ -289,6 +350,10 @@ def main():
             sys.exit('the adjoint runs on the fused paths only')
         log('adjoint gradient cost (%s), radial Kerr 513 x 128 and radial O2 2049 x 256' % prec)
         print(json.dumps(dict(kerr=adjoint_cost(prec), ionizing=adjoint_cost(prec, workload='ionizing'))))
+        return
+
+    if args.workload == 'cartesian':
+        print(json.dumps(cartesian_workload(args)))
         return
 
     if args.workload == 'batch':
