@@ -1,7 +1,9 @@
 '''Host side of the spectral-phase gradient (searay_amd.adjoint), on the CPU: the Legendre basis, and the
-assembly dL/dc_n = -sum P_n Im(conj(G_0) A_0) from the cotangents, checked by central differences with a linear
-stand-in for the stepper (finish returns M a0 per radial cell, the adjoint M^H g).'''
+assembly dL/dc_n = -sum P_n Im(conj(G_0) A_0) from the cotangents of the on-axis spectrum and radial fluence
+objectives, checked by central differences with a linear stand-in for the stepper (finish returns M a0 per radial
+cell, the adjoint M^H g); the batched workload's seeded draws.'''
 import numpy as np
+import pytest
 
 from conftest import rel_l2
 from searay_amd import adjoint, configs
@@ -51,13 +53,27 @@ class _LinearStepper:
         pass
 
 
-def test_gradient_assembly_against_central_differences():
+@pytest.mark.parametrize('objective', ['spectrum', 'fluence'])
+def test_gradient_assembly_against_central_differences(objective):
     vol = configs.radial_kerr_air(scale=16, steps=2)
     nw = vol['wave grid'][0]
-    p = adjoint.SpectralPhaseProblem(vol, configs.kerr_pulse_sigma(), n_phase=8,
+    p = adjoint.SpectralPhaseProblem(vol, configs.kerr_pulse_sigma(), n_phase=8, objective=objective,
                                      stepper_factory=lambda *a, **k: _LinearStepper(nw))
     c = 0.3 * np.random.default_rng(2).standard_normal(8)
     loss, g = p.value_and_grad(c)
     fd = np.array([(p.loss(c + 1e-6 * e) - p.loss(c - 1e-6 * e)) / 2e-6 for e in np.eye(8)])
     assert loss > 0 and np.linalg.norm(g) > 0
     assert rel_l2(g, fd) < 1e-6
+
+
+def test_batched_workload_draws():
+    '''The 64-instance workload is seeded: its grids, powers and gases are the same on every call, within the
+    stated ranges, and every instance is a radial vol_dict with fixed steps.'''
+    vols, drawn = configs.batched_radial(n=64)
+    vols2, drawn2 = configs.batched_radial(n=64)
+    assert drawn == drawn2
+    assert {d[0] for d in drawn} == {64, 128, 256, 512}
+    assert all(0.1 <= d[1] <= 5.0 for d in drawn) and {d[2] for d in drawn} == {'O2', 'Ar'}
+    for v, d in zip(vols, drawn):
+        assert v['wave grid'][:3] == (1025, d[0], 1) and v['radial modes'] == d[0]
+        assert v['minimum step'] == v['size'][2] / v['subcycles']
