@@ -35,6 +35,9 @@ struct StepState {
   unsigned long long dzraw;  // bits of the running minimum of the per-point estimate
   int sub, sub_end, active, fixed;
   int nanflag, nan_steps, steps, sched_n, sched_pos, rec_cap, error, max_steps;
+  int pend;               // the last step ran; its end (z advance, records, bookkeeping) is not committed yet
+  int reserved;
+  const double* sched;    // replayed steps (device), or null
 };
 
 // per-instance scalars of the source kernel in a batched launch (chi3/n_ref, rate model and its prefactor)
@@ -145,6 +148,102 @@ enum { EPI_NONE = 0, EPI_STAGE = 1, EPI_ADJ = 2, EPI_ADJ_FIN = 3 };
 __device__ inline unsigned long long dbits(double x) { return (unsigned long long)__double_as_longlong(x); }
 __device__ inline double bitsd(unsigned long long b) { return __longlong_as_double((long long)b); }
 
+// ---------------------------------------------------------------------------------------------------
+// device step control (uppe_kernel.py:218-227, 299-325). No launch of its own: every launch of a step reads the
+// step state. The step size is a function of the committed state (stage 1's r2k epilogue leaves the minimum
+// per-point estimate in dzraw), so every launch from stage 2 on forms it itself. The end of a step (z advance,
+// subcycle bookkeeping, records) is committed by the first block of the next step's stage-1 source launch, which
+// is the first launch that reads nothing of the state; until then the step is marked pending, and the next
+// step's stage-1 k2r launch advances a local copy. A chunk of steps ends with one commit launch (f_ctl_commit).
+// ---------------------------------------------------------------------------------------------------
+
+// dz of the current step: the replayed schedule, the subcycle's remainder for fixed steps, or the estimate
+// clamped to [dzmin, L - z] (uppe_kernel.py:221-227, dzmin first, then the subcycle end: Q5)
+__device__ inline double step_dz(const StepState& s) {
+  if (s.sched_n > 0) return s.steps < s.sched_n ? s.sched[s.steps] : 0.0;
+  if (s.fixed) return s.L - s.z;
+  double dz = bitsd(s.dzraw);
+  if (dz < s.dzmin) dz = s.dzmin;
+  if (s.z + dz > s.L) dz = s.L - s.z;
+  return dz;
+}
+
+// end of a step of size dz: z advance, NaN count, step count, the termination test (z - L)^2 > (L/1e6)^2 and the
+// next subcycle (uppe_kernel.py:299-325)
+__device__ inline void end_step(StepState& s, double dz) {
+  s.z = s.z + dz;
+  s.steps++;
+  if (s.nanflag) {
+    s.nan_steps++;
+    s.nanflag = 0;
+  }
+  s.dzraw = dbits(INFINITY);
+  const double z = s.z, L = s.L;
+  if (s.max_steps > 0 && s.steps >= s.max_steps) {
+    s.z_last = z;
+    s.active = 0;
+    return;
+  }
+  if (!(z < L && (z - L) * (z - L) > (L / 1e6) * (L / 1e6))) {
+    s.z_last = z;
+    s.sub++;
+    if (s.sub >= s.sub_end) {
+      s.active = 0;
+    } else {
+      s.z = 0.0;
+      s.zoff = (s.zi + (double)s.sub * L) - s.zi;
+    }
+  }
+}
+
+// where a launch of `stage` stands: phase origin, z, dz (stages 2-4) and whether the run is active. Stage 1
+// advances a pending step in a local copy.
+struct StepPos {
+  double zoff, z, dz;
+  int active;
+};
+__device__ inline StepPos step_pos(const StepState* st, int stage) {
+  if (stage == 1) {
+    if (!st->pend) return StepPos{st->zoff, st->z, 0.0, st->active};
+    StepState loc = *st;
+    if (loc.active) {
+      if (loc.sched_n > 0 && loc.steps >= loc.sched_n)
+        loc.active = 0;
+      else
+        end_step(loc, step_dz(loc));
+    }
+    return StepPos{loc.zoff, loc.z, 0.0, loc.active};
+  }
+  return StepPos{st->zoff, st->z, step_dz(*st), st->active};
+}
+
+// commit a pending step (one thread): records of its dz and phase origin, the end of step, and the maxima
+// (qmax[2] holds max|q| over the columns the r2k direction never updates: the current is zero there)
+__device__ inline void step_commit(StepState* st, unsigned long long* qmax, double* rec, double* zrec, int zrec_cap) {
+  if (!st->pend) return;
+  st->pend = 0;
+  if (!st->active) return;
+  if (st->sched_n > 0 && st->steps >= st->sched_n) {
+    st->error = 1;
+    st->active = 0;
+    return;
+  }
+  StepState s = *st;
+  const double dz = step_dz(s);
+  if (rec && s.steps < s.rec_cap) rec[s.steps] = dz;
+  if (zrec && s.steps < zrec_cap) {
+    // the phase origin of the step, for the adjoint's replay of the stage phases
+    zrec[3 * s.steps] = s.zoff;
+    zrec[3 * s.steps + 1] = s.z;
+    zrec[3 * s.steps + 2] = dz;
+  }
+  s.dz = dz;
+  end_step(s, dz);
+  *st = s;
+  qmax[0] = qmax[1] > qmax[2] ? qmax[1] : qmax[2];
+  qmax[1] = 0ull;
+}
+
 // exp(i phs z) exp(-ki z) with the phase formed in FP64 (long-z cancellation). The FP32 path reduces the
 // phase to a fraction of a turn in FP64 and evaluates the hardware sine, cosine (their input is in turns)
 // and base-2 exponential: absolute error ~1e-6 in the factor, against an RHS that is itself a small
@@ -188,7 +287,6 @@ __global__ __launch_bounds__(256) void f_gemm(const T* __restrict__ A, const cpl
                                               cplx<T>* __restrict__ k2, cplx<T>* __restrict__ k3,
                                               StepState* __restrict__ st, unsigned long long* __restrict__ qmax,
                                               unsigned long long* __restrict__ qmax_next, int stage, double zfix) {
-  if (st && !st->active) return;
   __shared__ T As[BK][BM + 1];
   __shared__ T Bs[BK][BN];
   __shared__ double red[8];
@@ -196,9 +294,11 @@ __global__ __launch_bounds__(256) void f_gemm(const T* __restrict__ A, const cpl
   const int row0 = blockIdx.y * BM, col0 = blockIdx.x * BN;
   double zph = zfix, cu = 0.0, dz = 0.0;
   if (st) {
-    dz = st->dz;
+    const StepPos sp = step_pos(st, stage);
+    if (!sp.active) return;
+    dz = sp.dz;
     double cs = stage == 1 ? 0.0 : (stage == 4 ? 1.0 : 0.5);
-    zph = st->zoff + (st->z + cs * dz);
+    zph = sp.zoff + (sp.z + cs * dz);
     cu = stage == 1 ? 0.0 : (stage == 4 ? dz : dz / 2);
   }
   T acc[4][4];
@@ -328,10 +428,13 @@ __global__ __launch_bounds__(256) void f_gemm(const T* __restrict__ A, const cpl
   if (tid == 0) {
     double v = red[0];
     for (int i = 1; i < 4; i++) v = is_min ? fmin(v, red[i]) : fmax(v, red[i]);
-    if (is_min)
+    if (is_min) {
       atomicMin(&st->dzraw, dbits(v));
-    else
+    } else {
       atomicMax(qmax_next, dbits(v));
+      // the step has run; the next step's stage-1 source launch commits its end
+      if (blockIdx.x == 0 && blockIdx.y == 0) st->pend = 1;
+    }
   }
 }
 
@@ -581,9 +684,10 @@ __global__ __launch_bounds__(BN > 16 ? 512 : 1024) void f_hankel(
   const int nmt = min(BW / 16, (min(fend - fc0, hcw) + HC - 1) / HC);
   double zph = zfix, cu = 0.0, dz = dzfix;
   if (st) {
-    dz = st->dz;
+    const StepPos sp = step_pos(st, stage);
+    dz = sp.dz;
     double cs = stage == 1 ? 0.0 : (stage == 4 ? 1.0 : 0.5);
-    zph = st->zoff + (st->z + cs * dz);
+    zph = sp.zoff + (sp.z + cs * dz);
     cu = stage == 1 ? 0.0 : (stage == 4 ? dz : dz / 2);
   }
 
@@ -840,10 +944,13 @@ __global__ __launch_bounds__(BN > 16 ? 512 : 1024) void f_hankel(
   if (tid == 0) {
     double v = red[0];
     for (int i = 1; i < nwv; i++) v = is_min ? fmin(v, red[i]) : fmax(v, red[i]);
-    if (is_min)
+    if (is_min) {
       atomicMin(&st->dzraw, dbits(v));
-    else
+    } else {
       atomicMax(qmax_next, dbits(v));
+      // the step has run; the next step's stage-1 source launch commits its end
+      if (blockIdx.x == 0 && blockIdx.y == 0) st->pend = 1;
+    }
   }
 }
 
@@ -949,9 +1056,10 @@ __global__ __launch_bounds__(1024) void f_tfft(
   const long base = (axis == 1 ? (long)blockIdx.y : (long)blockIdx.y * ny) * nwp + f0;
   double zph = zfix, cu = 0.0, dz = 0.0;
   if (st) {
-    dz = st->dz;
+    const StepPos sp = step_pos(st, stage);
+    dz = sp.dz;
     const double cs = stage == 1 ? 0.0 : (stage == 4 ? 1.0 : 0.5);
-    zph = st->zoff + (st->z + cs * dz);
+    zph = sp.zoff + (sp.z + cs * dz);
     cu = stage == 1 ? 0.0 : (stage == 4 ? dz : dz / 2);
   }
   for (int i = t; i < tab_len; i += nth) tab[i] = tab_g[i];
@@ -1005,10 +1113,13 @@ __global__ __launch_bounds__(1024) void f_tfft(
   if (t == 0) {
     double v = red[0];
     for (int i = 1; i < nwv; i++) v = is_min ? fmin(v, red[i]) : fmax(v, red[i]);
-    if (is_min)
+    if (is_min) {
       atomicMin(&st->dzraw, dbits(v));
-    else
+    } else {
       atomicMax(qmax_next, dbits(v));
+      // the step has run; the next step's stage-1 source launch commits its end
+      if (blockIdx.x == 0 && blockIdx.y == 0) st->pend = 1;
+    }
   }
 }
 
@@ -1282,8 +1393,14 @@ __global__ __launch_bounds__(1024) void f_source(const cplx<T>* __restrict__ er,
                                                  int lgn, int nwp, double redchi3, double dt, uppe_ionization ion,
                                                  double log_pre, const RotRow<T>* __restrict__ rot_tab,
                                                  int rot_states, double rot_fc, const SrcInst* __restrict__ inst,
-                                                 long zs) {
-  // no early exit for a finished run: the kernel only writes scratch (J_r, diagnostics)
+                                                 long zs, StepState* commit, unsigned long long* cq, double* rec,
+                                                 double* zrec, int zrec_cap) {
+  // no early exit for a finished run: the kernel only writes scratch (J_r, diagnostics). The stage-1 launch
+  // commits the previous step (commit non-null; nothing else of this launch reads the step state).
+  if (commit && blockIdx.x == 0 && threadIdx.x == 0) {
+    const int z = inst ? blockIdx.y : 0;
+    step_commit(commit + z, cq + 3 * z, rec, zrec, zrec_cap);
+  }
   extern __shared__ char smem[];
   cplx<T>* Y = reinterpret_cast<cplx<T>*>(smem);
   __shared__ T wsum[16];
@@ -1483,7 +1600,10 @@ __global__ __launch_bounds__(1024) void f_source_env(const cplx<T>* __restrict__
                                                      const double* __restrict__ ng, int nt, int lgn, int nwp,
                                                      double redchi3, double dt, uppe_ionization ion, double log_pre,
                                                      const RotRow<T>* __restrict__ rot_tab, int rot_states,
-                                                     double rot_fc) {
+                                                     double rot_fc, StepState* commit, unsigned long long* cq,
+                                                     double* rec, double* zrec, int zrec_cap) {
+  // the stage-1 launch commits the previous step (commit non-null; nothing else of this launch reads it)
+  if (commit && blockIdx.x == 0 && threadIdx.x == 0) step_commit(commit, cq, rec, zrec, zrec_cap);
   extern __shared__ char smem[];
   cplx<T>* Y1 = reinterpret_cast<cplx<T>*>(smem);  // E, then P
   cplx<T>* Y2 = Y1 + nt;                            // A, then Q
@@ -1847,68 +1967,10 @@ __global__ __launch_bounds__(1024) void f_source_adj_ion(const cplx<T>* __restri
 // control, layout and reduction kernels
 // ---------------------------------------------------------------------------------------------------
 
-// step size after the first RHS (uppe_kernel.py:221-227)
-// (one block per instance in a batched launch, which records nothing)
-__global__ void f_ctl_dz(StepState* st, const double* sched, double* rec, double* zrec, int zrec_cap) {
-  st += blockIdx.x;
-  if (!st->active) return;
-  double dz;
-  if (st->sched_n > 0) {
-    if (st->sched_pos >= st->sched_n) {
-      st->error = 1;
-      st->active = 0;
-      return;
-    }
-    dz = sched[st->sched_pos++];
-  } else if (st->fixed) {
-    dz = st->L - st->z;
-  } else {
-    dz = bitsd(st->dzraw);
-    if (dz < st->dzmin) dz = st->dzmin;
-    if (st->z + dz > st->L) dz = st->L - st->z;
-  }
-  if (rec && st->steps < st->rec_cap) rec[st->steps] = dz;
-  if (zrec && st->steps < zrec_cap) {
-    // the phase origin of the step, for the adjoint's replay of the stage phases
-    zrec[3 * st->steps] = st->zoff;
-    zrec[3 * st->steps + 1] = st->z;
-    zrec[3 * st->steps + 2] = dz;
-  }
-  st->dz = dz;
-}
-
-// end of step: advance z, subcycle bookkeeping (uppe_kernel.py:299-325)
-// qmax[2] holds max|q| over the columns the r2k direction never updates (the current is zero there)
-__global__ void f_ctl_end(StepState* st, unsigned long long* qmax, unsigned long long* qmax_next) {
-  st += blockIdx.x;
-  qmax += 3 * blockIdx.x;
-  qmax_next += 3 * blockIdx.x;
-  if (!st->active) return;
-  st->z = st->z + st->dz;
-  st->steps++;
-  if (st->nanflag) {
-    st->nan_steps++;
-    st->nanflag = 0;
-  }
-  *qmax = *qmax_next > qmax[2] ? *qmax_next : qmax[2];
-  *qmax_next = 0ull;
-  st->dzraw = dbits(INFINITY);
-  const double z = st->z, L = st->L;
-  if (st->max_steps > 0 && st->steps >= st->max_steps) {
-    st->z_last = z;
-    st->active = 0;
-    return;
-  }
-  if (!(z < L && (z - L) * (z - L) > (L / 1e6) * (L / 1e6))) {
-    st->z_last = z;
-    st->sub++;
-    if (st->sub >= st->sub_end) {
-      st->active = 0;
-    } else {
-      st->z = 0.0;
-      st->zoff = (st->zi + (double)st->sub * L) - st->zi;
-    }
-  }
+// commit the pending step of a chunk's last step (one block per instance in a batched launch, which records
+// nothing)
+__global__ void f_ctl_commit(StepState* st, unsigned long long* qmax, double* rec, double* zrec, int zrec_cap) {
+  step_commit(st + blockIdx.x, qmax + 3 * blockIdx.x, rec, zrec, zrec_cap);
 }
 
 // max |q| over the state (set_field)
@@ -2365,8 +2427,13 @@ struct Fused {
 
   // the source kernel over every strip; a batched launch (nz instances zs elements apart, per-instance scalars
   // inst) may ask for spt_req samples per thread
-  static void source(uppe_handle* h, FusedState* s, const C* er, C* jr, bool diag, StepState* st, int nz = 1,
+  static void source(uppe_handle* h, FusedState* s, const C* er, C* jr, bool diag, StepState* commit, int nz = 1,
                      const SrcInst* inst = nullptr, long zs = 0, int spt_req = 0) {
+    // the stage-1 launch commits the previous step; a standalone handle's advance also records dz (and the
+    // adjoint's phase origins) there
+    double* rec = (commit && !inst) ? s->rec_active : nullptr;
+    double* zrec = (commit && !inst) ? s->zrec_active : nullptr;
+    const int zrec_cap = s->zrec_cap / 3;
     const int N = h->nt;
     // four samples per thread (Nt >= 256): short strips then get short workgroups, several per CU
     const int nth = spt_req > 0 ? N / spt_req : std::max(std::min(N, 64), N / 4);
@@ -2379,7 +2446,6 @@ struct Fused {
     const double log_pre = s->ion.kind != UPPE_ION_EXPLICIT ? 0.0
                            : sizeof(T) == 4                  ? std::log2(s->ion.rate_conv * s->ion.c_pre)
                                                              : std::log(s->ion.rate_conv * s->ion.c_pre);
-    (void)st;
     const bool rot = h->rot_states > 2;
     const dim3 grid((unsigned)h->P, (unsigned)nz), block(nth);
     if (h->paraxial) {
@@ -2392,7 +2458,7 @@ struct Fused {
     launchf(h, UPPE_KC_SOURCE, kern, grid, block, shm_env, er, jr, diag ? (T*)s->chi_t : (T*)nullptr,             \
             diag ? (T*)s->ne_t : (T*)nullptr, (const C*)s->tw, (const T*)s->w, (const double*)s->ng, N, lgn,      \
             s->nwp, s->redchi3, s->dt, s->ion, log_pre, (const RotRow<T>*)s->rot_tab, h->rot_states,              \
-            0.5 * h->rot_dalpha / h->rot_hbar);                                                                  \
+            0.5 * h->rot_dalpha / h->rot_hbar, commit, s->qmax, rec, zrec, zrec_cap);                            \
     return;                                                                                                      \
   }
       UPPE_SRC_ENV(1) UPPE_SRC_ENV(2) UPPE_SRC_ENV(4)
@@ -2406,7 +2472,8 @@ struct Fused {
     launchf(h, UPPE_KC_SOURCE, kern, grid, block, shm, er, jr, diag ? (T*)s->chi_t : (T*)nullptr,                 \
             diag ? (T*)s->ne_t : (T*)nullptr, (const C*)s->tw, (const T*)s->w, (const T*)s->nlf,                  \
             (const double*)s->ng, N, lgn, s->nwp, s->redchi3, s->dt, s->ion, log_pre,                             \
-            (const RotRow<T>*)s->rot_tab, h->rot_states, 0.5 * h->rot_dalpha / h->rot_hbar, inst, zs);          \
+            (const RotRow<T>*)s->rot_tab, h->rot_states, 0.5 * h->rot_dalpha / h->rot_hbar, inst, zs, commit,   \
+            s->qmax, rec, zrec, zrec_cap);                                                                       \
     return;                                                                                                      \
   }
     UPPE_SRC(1) UPPE_SRC(2) UPPE_SRC(4)
@@ -2510,16 +2577,12 @@ struct Fused {
         hankel_mfma<CM, PRO_STAGE_ENV, EPI_NONE>(h, s, 0, (const C*)s->q, er, kp[stage], stage, 0.0, s->st);
       else
         hankel_mfma<CM, PRO_STAGE, EPI_NONE>(h, s, 0, (const C*)s->q, er, kp[stage], stage, 0.0, s->st);
-      source(h, s, (const C*)er, (C*)s->jr, false, s->st, 1, nullptr, 0, s->spt);
+      source(h, s, (const C*)er, (C*)s->jr, false, stage == 1 ? s->st : nullptr, 1, nullptr, 0, s->spt);
       hankel_mfma<CM, PRO_NONE, EPI_STAGE>(h, s, 1, (const C*)s->jr, (C*)nullptr, nullptr, stage, 0.0, s->st);
-      if (stage == 1)
-        launchf(h, UPPE_KC_REDUCE, f_ctl_dz, dim3(1), dim3(1), 0, s->st, (const double*)s->sched, s->rec_active,
-                s->zrec_active, s->zrec_cap / 3);
     }
-    launchf(h, UPPE_KC_REDUCE, f_ctl_end, dim3(1), dim3(1), 0, s->st, s->qmax, s->qmax_next);
   }
 
-  // one RK4 step: 4 x (k2r, source, r2k) + 2 control kernels
+  // one RK4 step: 4 x (k2r, source, r2k); the step control runs inside them
   // one 1-D pass of a Cartesian transverse transform over columns [0, fend); sign +1 inverse (scaled 1/Na)
   template <int PRO, int EPI>
   static void tpass(uppe_handle* h, FusedState* s, int axis, const C* src, C* dst, int fend, int sign,
@@ -2549,14 +2612,10 @@ struct Fused {
       else
         tpass<PRO_STAGE, EPI_NONE>(h, s, 2, q, er, h->nw, +1, kp[stage], stage, 0.0, s->st);
       tpass<PRO_NONE, EPI_NONE>(h, s, 1, er, er, h->nw, +1, nullptr, stage, 0.0, s->st);
-      source(h, s, er, jr, false, s->st);
+      source(h, s, er, jr, false, stage == 1 ? s->st : nullptr);
       tpass<PRO_NONE, EPI_NONE>(h, s, 1, jr, jr, s->f1, -1, nullptr, stage, 0.0, s->st);
       tpass<PRO_NONE, EPI_STAGE>(h, s, 2, jr, (C*)nullptr, s->f1, -1, nullptr, stage, 0.0, s->st);
-      if (stage == 1)
-        launchf(h, UPPE_KC_REDUCE, f_ctl_dz, dim3(1), dim3(1), 0, s->st, (const double*)s->sched, s->rec_active,
-                s->zrec_active, s->zrec_cap / 3);
     }
-    launchf(h, UPPE_KC_REDUCE, f_ctl_end, dim3(1), dim3(1), 0, s->st, s->qmax, s->qmax_next);
   }
 
   static void step(uppe_handle* h, FusedState* s) {
@@ -2583,16 +2642,12 @@ struct Fused {
               (const C*)s->q, (C*)s->er, h->n1, h->nk, s->nwp, (const double*)s->phs, (const C*)s->kzc,
               (const T*)s->cond, (C*)nullptr, (const C*)kp[stage], (C*)nullptr, (C*)nullptr, (C*)nullptr, s->st,
               s->qmax, s->qmax_next, stage, 0.0);
-      source(h, s, (const C*)s->er, (C*)s->jr, false, s->st);
+      source(h, s, (const C*)s->er, (C*)s->jr, false, stage == 1 ? s->st : nullptr);
       launchf(h, UPPE_KC_HANKEL, f_gemm<T, PRO_NONE, EPI_STAGE>, g_r2k, dim3(256), 0, (const T*)s->Hf,
               (const C*)s->jr, (C*)nullptr, h->nk, h->n1, s->nwp, (const double*)s->phs, (const C*)s->kzc,
               (const T*)s->cond, (C*)s->q, (const C*)nullptr, (C*)s->k1, (C*)s->k2, (C*)s->k3, s->st, s->qmax,
               s->qmax_next, stage, 0.0);
-      if (stage == 1)
-        launchf(h, UPPE_KC_REDUCE, f_ctl_dz, dim3(1), dim3(1), 0, s->st, (const double*)s->sched, s->rec_active,
-                s->zrec_active, s->zrec_cap / 3);
     }
-    launchf(h, UPPE_KC_REDUCE, f_ctl_end, dim3(1), dim3(1), 0, s->st, s->qmax, s->qmax_next);
   }
 
   static void set_field(uppe_handle* h, FusedState* s, const double* a_r) {
@@ -2648,7 +2703,15 @@ struct Fused {
     }
     if (c->n_schedule > 0)
       UPPE_HIP(hipMemcpy(s->sched, c->dz_schedule, sizeof(double) * c->n_schedule, hipMemcpyHostToDevice));
+    sh->pend = 0;
+    sh->sched = c->n_schedule > 0 ? s->sched : nullptr;
     return sh;
+  }
+
+  // commit the last step of a chunk (the steps before it were committed by the next step's source launch)
+  static void commit(uppe_handle* h, FusedState* s) {
+    launchf(h, UPPE_KC_REDUCE, f_ctl_commit, dim3(1), dim3(1), 0, s->st, s->qmax, s->rec_active, s->zrec_active,
+            s->zrec_cap / 3);
   }
 
   template <typename V>
@@ -2694,6 +2757,7 @@ struct Fused {
                               h->stream));
       s->zrec_active = s->zrec;
       for (int i = 0; i < every; i++) step_mfma<CM>(h, s, (C*)s->er_store + (long)i * 4 * stride);
+      commit(h, s);
       s->zrec_active = nullptr;
       UPPE_HIP(hipMemcpyAsync(sh, s->st, sizeof(StepState), hipMemcpyDeviceToHost, h->stream));
       UPPE_HIP(hipStreamSynchronize(h->stream));
@@ -2843,6 +2907,8 @@ struct Fused {
     }
     if (c->n_schedule > 0)
       UPPE_HIP(hipMemcpy(s->sched, c->dz_schedule, sizeof(double) * c->n_schedule, hipMemcpyHostToDevice));
+    sh->pend = 0;
+    sh->sched = c->n_schedule > 0 ? s->sched : nullptr;
     if (sh->rec_cap > s->rec_cap) {
       if (s->rec) UPPE_HIP(hipFree(s->rec));
       UPPE_HIP(hipMalloc(&s->rec, sizeof(double) * sh->rec_cap));
@@ -2857,6 +2923,7 @@ struct Fused {
     if (sh->max_steps > 0) chunk = std::min(chunk, sh->max_steps);
     while (true) {
       for (int i = 0; i < chunk; i++) step(h, s);
+      commit(h, s);
       UPPE_HIP(hipMemcpyAsync(sh, s->st, sizeof(StepState), hipMemcpyDeviceToHost, h->stream));
       UPPE_HIP(hipStreamSynchronize(h->stream));
       guard += chunk;
@@ -3110,15 +3177,12 @@ struct BatchOps {
       F::template hk_launch<CM, PRO_STAGE, EPI_NONE>(h, s, s->frb_hi, s->frb_lo, s->kp_b, h->n1, h->nk, h->nw,
                                                      r.plan[0], (const C*)q, (C*)s->er, kp[stage], q, k1, k2, k3,
                                                      s->st, stage, 0.0, 0.0, 0.0, r.count, zs);
-      F::source(h, s, (const C*)s->er, (C*)s->jr, false, s->st, r.count, G.src + r.first, zs, r.spt);
+      F::source(h, s, (const C*)s->er, (C*)s->jr, false, stage == 1 ? s->st : nullptr, r.count, G.src + r.first, zs,
+                r.spt);
       F::template hk_launch<CM, PRO_NONE, EPI_STAGE>(h, s, s->frf_hi, s->frf_lo, s->kp_f, h->nk, h->n1, s->f1,
                                                      r.plan[1], (const C*)s->jr, (C*)nullptr, nullptr, q, k1, k2,
                                                      k3, s->st, stage, 0.0, 0.0, 0.0, r.count, zs);
-      if (stage == 1)
-        launchf(h, UPPE_KC_REDUCE, f_ctl_dz, dim3((unsigned)r.count), dim3(1), 0, s->st, (const double*)nullptr,
-                (double*)nullptr, (double*)nullptr, 0);
     }
-    launchf(h, UPPE_KC_REDUCE, f_ctl_end, dim3((unsigned)r.count), dim3(1), 0, s->st, s->qmax, s->qmax_next);
   }
 
   static void step_run(uppe_batch* b, const BatchRun& r) {
@@ -3234,7 +3298,7 @@ struct BatchOps {
       if (src && !getenv("SEARAY_UPPE_SOURCE_SPT")) {  // a pinned shape is kept
         auto source = [&](int spt) {
           return time_launch(h->stream, e0, e1, [&] {
-            F::source(h, s, (const C*)s->er, (C*)s->jr, false, s->st, nz, src, zs, spt);
+            F::source(h, s, (const C*)s->er, (C*)s->jr, false, nullptr, nz, src, zs, spt);
           }, reps);
         };
         // the incoming choice (0: the default shape, max(min(Nt, 64), Nt/4) threads)
@@ -3449,9 +3513,15 @@ void fused_batch_advance(uppe_batch* b, const uppe_step_control* c, int* steps_o
       for (; ran < chunk; ran++) step_all(b);
     }
     for (const BatchRun& r : fb->runs) {
+      // commit the chunk's last step of every member of the run, then bring the states back
       const BatchGroup& G = fb->groups[(size_t)r.group];
+      uppe_handle* h0 = b->members[(size_t)G.members[(size_t)r.first]];
+      const hipStream_t sr = gs > 0 ? fb->main : run_stream(b, r);
+      hipLaunchKernelGGL(f_ctl_commit, dim3((unsigned)r.count), dim3(1), 0, sr, h0->fused->st, h0->fused->qmax,
+                         (double*)nullptr, (double*)nullptr, 0);
+      UPPE_LAUNCH_CHECK();
       UPPE_HIP(hipMemcpyAsync(G.st_host + r.first, static_cast<StepState*>(G.slab[SL_ST]) + r.first,
-                              sizeof(StepState) * r.count, hipMemcpyDeviceToHost, gs > 0 ? fb->main : run_stream(b, r)));
+                              sizeof(StepState) * r.count, hipMemcpyDeviceToHost, sr));
     }
     if (gs > 0)
       UPPE_HIP(hipStreamSynchronize(fb->main));
