@@ -12,8 +12,8 @@
 //      P, J_p -> one complex FFT for both rffts -> spectral current and filter (uppe_kernel.py:139-186);
 //   3. r2k GEMM; its epilogue forms the ODE right-hand side (uppe.cl:73-91) and the RK bookkeeping, the
 //      per-point step estimate (paraxial.cl:40-58) and the field conditioner (uppe_kernel.py:131-137).
-// Step control (uppe_kernel.py:218-227, 299-325) runs on the device: two single-thread control kernels per
-// step, so a whole subcycle runs without host synchronisation.
+// Step control (uppe_kernel.py:218-227, 299-325) runs on the device inside these three launches (step_pos,
+// step_commit), so a whole subcycle runs without host synchronisation and without launches of its own.
 #include <hip/hip_ext.h>
 
 #include "uppe_internal.h"
