@@ -34,9 +34,10 @@ struct StepState {
   double z_last;
   unsigned long long dzraw;  // bits of the running minimum of the per-point estimate
   int sub, sub_end, active, fixed;
-  int nanflag, nan_steps, steps, sched_n, sched_pos, rec_cap, error, max_steps;
+  int nanflag, nan_steps;
+  int steps;              // steps taken in this advance (also the index of the current step's replayed dz)
+  int sched_n, rec_cap, error, max_steps;
   int pend;               // the last step ran; its end (z advance, records, bookkeeping) is not committed yet
-  int reserved;
   const double* sched;    // replayed steps (device), or null
 };
 
