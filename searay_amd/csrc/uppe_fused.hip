@@ -2671,9 +2671,11 @@ struct Fused {
     UPPE_HIP(hipStreamSynchronize(h->stream));
   }
 
-  // the device step state of a new advance call (uppe_kernel.py:299-325), uploaded on the handle's stream
+  // the host copy of the step state of a new advance call (uppe_kernel.py:299-325), with the replayed schedule
+  // uploaded; the caller uploads the state itself
   static StepState* init_state(uppe_handle* h, FusedState* s, const uppe_step_control* c) {
     if (sizeof(T) == 4) {
+      // the right-hand side carries exp(+Im kz z) (uppe.cl:86-90); keep it inside the FP32 range
       const double zmax = ((c->zi + (double)(c->sub0 + c->n_sub) * c->length) - c->zi);
       if (s->max_ki * zmax > 80.0)
         throw Error{UPPE_ERR_UNSUPPORTED,
@@ -2876,40 +2878,8 @@ struct Fused {
   }
 
   static int advance(uppe_handle* h, FusedState* s, const uppe_step_control* c) {
-    if (sizeof(T) == 4) {
-      // the right-hand side carries exp(+Im kz z) (uppe.cl:86-90); keep it inside the FP32 range
-      const double zmax = ((c->zi + (double)(c->sub0 + c->n_sub) * c->length) - c->zi);
-      if (s->max_ki * zmax > 80.0)
-        throw Error{UPPE_ERR_UNSUPPORTED,
-                    "FP32 range: exp(Im kz z) would exceed 3e38 in this interval (max Im kz * z = " +
-                        std::to_string(s->max_ki * zmax) + "); use FP64 or more diagnostic planes"};
-    }
-    StepState* sh = s->st_host;
-    std::memset(sh, 0, sizeof(StepState));
-    sh->zi = c->zi;
-    sh->L = c->length;
-    sh->dzmin = c->dzmin;
-    sh->dphi = c->dphi;
-    sh->rel_amin = c->rel_amin;
-    sh->sub = c->sub0;
-    sh->sub_end = c->sub0 + c->n_sub;
-    sh->z = 0.0;
-    sh->zoff = (c->zi + (double)c->sub0 * c->length) - c->zi;
-    sh->active = 1;
-    sh->sched_n = c->n_schedule;
-    sh->fixed = (c->n_schedule == 0 && c->dzmin >= c->length) ? 1 : 0;
-    sh->dzraw = 0x7ff0000000000000ull;  // +inf
+    StepState* sh = init_state(h, s, c);
     sh->rec_cap = c->dz_record ? c->record_cap : 0;
-    sh->max_steps = c->max_steps > 0 ? c->max_steps : 0;
-    if (c->n_schedule > s->sched_cap) {
-      if (s->sched) UPPE_HIP(hipFree(s->sched));
-      UPPE_HIP(hipMalloc(&s->sched, sizeof(double) * c->n_schedule));
-      s->sched_cap = c->n_schedule;
-    }
-    if (c->n_schedule > 0)
-      UPPE_HIP(hipMemcpy(s->sched, c->dz_schedule, sizeof(double) * c->n_schedule, hipMemcpyHostToDevice));
-    sh->pend = 0;
-    sh->sched = c->n_schedule > 0 ? s->sched : nullptr;
     if (sh->rec_cap > s->rec_cap) {
       if (s->rec) UPPE_HIP(hipFree(s->rec));
       UPPE_HIP(hipMalloc(&s->rec, sizeof(double) * sh->rec_cap));
