@@ -185,3 +185,28 @@ def test_batch_tile_widths_match_standalone(precision, width, monkeypatch):
         n, q = _standalone(vol, precision)
         assert steps[i] == n
         assert rel_l2(qs[i], q) < 1e-12, (i, rel_l2(qs[i], q))
+
+
+def test_batch_members_with_different_bands_split_runs(monkeypatch):
+    '''Members of one grid whose nonlinear bands differ cannot share a launch: the batch splits them into runs
+    (consecutive members with equal tables) and every member still equals its standalone run.'''
+    monkeypatch.setenv('SEARAY_UPPE_SOURCE_SPT', '4')
+    vols, drawn = configs.batched_radial(n=6, scale=8, steps=3)
+    for v in vols:
+        v['wave grid'] = (v['wave grid'][0], 32, 1, 2)
+        v['radial modes'] = 32
+    band = vols[0]['frequency band']
+    for i in (2, 3):
+        vols[i]['nonlinear band'] = (0.0, 0.6 * band[1])
+    _say('batch with split runs (nonlinear bands differ)')
+    with batch.Batch(vols, precision='fp32', autotune=False) as b:
+        b.set_fields()
+        steps = b.advance()
+        qs = [m.get_state() for m in b.members]
+        plans = [b.plan(i) for i in range(len(vols))]
+    assert len({p['group'] for p in plans}) == 1
+    assert [(p['first_slot'], p['run_length']) for p in plans] == [(0, 2), (0, 2), (2, 2), (2, 2), (4, 2), (4, 2)]
+    for i, vol in enumerate(vols):
+        n, q = _standalone(vol, 'fp32')
+        assert steps[i] == n
+        assert rel_l2(qs[i], q) < 1e-12, (i, rel_l2(qs[i], q))
