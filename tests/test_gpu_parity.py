@@ -115,6 +115,27 @@ def test_radial_ionization_replayed_schedule(oracle_mod, precision):
     assert rel_l2(np.fft.rfft(ne_t, axis=0), ref[3][..., 1]) < FP32_NE_TOL
 
 
+@pytest.mark.parametrize('precision', ['fp64', 'fp32'])
+def test_radial_ionization_simt_contraction(oracle_mod, precision, monkeypatch):
+    '''The SIMT radial contraction (SEARAY_UPPE_SIMT_HANKEL=1, the fallback of the matrix-core path): the same
+    adaptive run against the oracle, with its dz schedule (FP64) or within the adaptive FP32 tolerances.'''
+    monkeypatch.setenv('SEARAY_UPPE_SIMT_HANKEL', '1')
+    vol = configs.radial_o2_ionization(scale=4, steps=4, rate='explicit', length='1 mm', power=3.0)
+    _say('radial_ionization simt contraction %s' % precision)
+    rec_ref = []
+    ref = oracle_mod.oracle_track(vol, record=rec_ref)
+    rec = []
+    got = _run(vol, precision, rec)
+    if precision == 'fp64':
+        assert len(rec) == len(rec_ref)
+        assert np.allclose(rec, rec_ref, rtol=1e-8, atol=0)
+        tol, tol_ne = FP64_TOL, FP64_TOL
+    else:
+        tol, tol_ne = 1e-4, 1e-3
+    assert rel_l2(got[0][..., 1], ref[0][..., 1]) < tol
+    assert rel_l2(got[3][..., 1], ref[3][..., 1]) < tol_ne
+
+
 # ---- committed golden fixtures (tests/golden) --------------------------------------------------------------
 
 import golden_cases  # noqa: E402
