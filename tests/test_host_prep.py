@@ -3,6 +3,9 @@
 * refractive-index spot values of test/dispersion_test.py:6-44;
 * grid generators of test/grid_test.py:7-19;
 * Landau-Lifshitz hydrogen and ADK helium rates of test/ionization_test.py:8-15;
+* the hybrid rate theories (PPT, PMPB) by their limits: the tunnelling-limit rate as the Keldysh parameter goes to
+  zero (second order for PPT, first for PMPB) and the K-photon power law E^(2K) with its Coulomb factor as it
+  grows; the stitched PPT's switch at gamma = 1/2;
 * the discrete Hankel basis: orthogonality, the eigen-equation of the symmetrised radial Laplacian solved by an
   independent LAPACK routine, and the Parseval energies stored in
   extras/explore/explore-discrete-hankel.ipynb (7.017482258163533e-06 J in time, 7.0174822581635315e-06 J modal);
@@ -195,6 +198,72 @@ class TestIonization:
         r_ls = np.linalg.norm(V @ c_ls - y)
         assert r_fit == pytest.approx(r_ls, rel=1e-6)
         assert np.allclose(V @ np.array(q['c']), V @ c_ls, atol=1e-5 * np.max(np.abs(y)))
+
+    # the hybrid theories (PPT, PMPB: photoionization_base.py:337-412) have no stored values in the reference;
+    # they are pinned by their two asymptotic regimes
+    @staticmethod
+    def _log_slope(gen, E, h=1e-5):
+        c = gen.cutoff_field
+        r1, r2 = gen.Rate(np.array([E * (1 - h) - c, E * (1 + h) - c]))
+        return np.log(r2 / r1) / np.log((1 + h) / (1 - h))
+
+    @pytest.mark.parametrize('uion_ev,Z', [(15.76, 1.0), (12.07, 0.53)])
+    @pytest.mark.parametrize('averaging', [True, False])
+    def test_hybrid_tunnelling_limit(self, uion_ev, Z, averaging):
+        '''Keldysh gamma -> 0: PPT reaches the tunnelling-limit rate (photoionization_base.py:314-335) at second
+        order in gamma, PMPB at first order (its Coulomb factor (1 + 2 gamma/e)^(-2 n*)). The Fourier sum needs
+        about 30/alpha terms there, alpha = 2 (asinh gamma - gamma/sqrt(1 + gamma^2)) ~ 2 gamma^3/3.'''
+        au = izb.AtomicUnits()
+        U = au.energy_to_au(uion_ev)
+        E = 0.04
+        dev = {}
+        for gam in (0.1, 0.05):
+            w = gam * E / np.sqrt(2 * U)
+            alpha = 2 * (np.arcsinh(gam) - gam / np.sqrt(1 + gam ** 2))
+            terms = int(30 / alpha) + 10
+            tun = izb.PPT_Tunneling(averaging, U, Z, 0, 0, 0, w)
+            Es = np.array([E - tun.cutoff_field])
+            ref = tun.Rate(Es)[0]
+            for cls in (izb.PPT, izb.PMPB):
+                dev[cls, gam] = cls(averaging, U, Z, 0, 0, 0, w, terms).Rate(Es)[0] / ref - 1
+        for gam in (0.1, 0.05):
+            assert abs(dev[izb.PPT, gam]) < 3 * gam ** 2, (gam, dev[izb.PPT, gam])
+            assert abs(dev[izb.PMPB, gam]) < 1.5 * gam, (gam, dev[izb.PMPB, gam])
+        assert 3.0 < dev[izb.PPT, 0.1] / dev[izb.PPT, 0.05] < 5.5  # second order in gamma
+        assert 1.6 < dev[izb.PMPB, 0.1] / dev[izb.PMPB, 0.05] < 2.4  # first order in gamma
+
+    @pytest.mark.parametrize('uion_ev,Z', [(15.76, 1.0), (12.07, 0.53)])
+    @pytest.mark.parametrize('averaging', [True, False])
+    def test_hybrid_multiphoton_power_law(self, uion_ev, Z, averaging):
+        '''Keldysh gamma >> 1: the rate goes as E^(2K) with K = ceil(nu) photons, times the Coulomb factor.
+        d ln w / d ln E -> 2K - 2 n* for PPT (its factor (2/F)^(2 n*)) and 2K for PMPB, whose
+        (1 + 2 gamma/e)^(-2 n*) cancels that factor; the static rate has 1/2 less (the cycle-averaging factor
+        goes as sqrt(E)). The approach is O(1/gamma^2) for PPT and O(1/gamma) for PMPB.'''
+        au = izb.AtomicUnits()
+        U = au.energy_to_au(uion_ev)
+        ns = Z / np.sqrt(2 * U)
+        w = 0.2
+        off = 0.0 if averaging else 0.5
+        for gam in (30, 100):
+            E = np.sqrt(2 * U) * w / gam
+            K = np.ceil(U / w * (1 + 1 / (2 * gam ** 2)))
+            s_ppt = self._log_slope(izb.PPT(averaging, U, Z, 0, 0, 0, w, 80), E)
+            s_pmpb = self._log_slope(izb.PMPB(averaging, U, Z, 0, 0, 0, w, 80), E)
+            assert abs(s_ppt - (2 * K - 2 * ns - off)) < 20 / gam ** 2, (gam, s_ppt, 2 * K - 2 * ns - off)
+            assert abs(s_pmpb - (2 * K - off)) < 4 / gam, (gam, s_pmpb, 2 * K - off)
+
+    def test_stitched_ppt_switches_at_gamma_half(self):
+        '''StitchedPPT is the tunnelling-limit rate below gamma = 1/2 and the full PPT rate above
+        (photoionization_base.py:414-426).'''
+        au = izb.AtomicUnits()
+        U = au.energy_to_au(12.07)
+        w = 0.057
+        E = np.sqrt(2 * U) * w / np.array([0.2, 0.45, 0.55, 2.0])
+        st = izb.StitchedPPT(True, U, 0.53, 0, 0, 0, w, 80)
+        Es = E - st.cutoff_field
+        tun = izb.PPT_Tunneling(True, U, 0.53, 0, 0, 0, w).Rate(Es)
+        full = izb.PPT(True, U, 0.53, 0, 0, 0, w, 80).Rate(Es)
+        assert np.array_equal(st.Rate(Es), np.concatenate([tun[:2], full[2:]]))
 
     def test_stitched_ppt_uses_closed_form(self):
         mks = configs.MKS_LENGTH
