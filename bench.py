@@ -15,7 +15,9 @@ Printed JSON line (one):
                 events lengthen the step, so `value` is taken without them). The transform is also reported
                 against the matrix-core peak of the type it issues. `traffic` is the measured HBM bytes per
                 launch (rocprofv3 FETCH_SIZE + WRITE_SIZE, copy-calibrated) read from the committed profile
-                summary of the same kernel and workload, or null
+                summary of the same kernel and workload, or null. `mfma_busy_frac` is the committed
+                rocprofv3 SQ_VALU_MFMA_BUSY_CYCLES per transform launch over (SIMDs x engine clock x this run's
+                average launch duration); `mfma_flop_per_launch_counted` the counted MFMA flops per launch
   cpu_baseline  the FP64 oracle (reference algorithm and pass structure, OpenMP) on a bounded sample
   extras        speedup over the plain FP64 GPU stepper, relative L2 error of the state against the fused FP64
                 stepper on the same step schedule; the same on the radial Kerr workload (Nw = 513, Nr = 128,
@@ -84,6 +86,19 @@ def hankel_bytes(rs, el):
 def step_bytes(rs, el):
     '''Algorithmic bytes of one RK4 step: 4 right-hand sides x 96 B per grid point in FP32.'''
     return 384.0 * (el / 4) * rs.N[0] * rs.N[1]
+
+
+def measured_mfma(prec):
+    '''MFMA busy cycles and counted MFMA flops per radial-transform launch (the average of a k->r and an r->k
+    launch), the SIMD count and the engine clock, from the committed rocprofv3 counter summary of the radial O2
+    workload, if present.'''
+    path = os.path.join(ROOT, 'profiles', 'mfma_radial_o2_2049x256.json')
+    try:
+        with open(path) as f:
+            tab = json.load(f)
+        return tab[prec]['hankel'], tab['simds'], tab['clock_mhz']
+    except (OSError, KeyError, ValueError):
+        return None
 
 
 def measured_traffic(prec, kernel, workload='uppe'):
@@ -420,6 +435,13 @@ def main():
         kname = 'hankel (radial transform: MFMA contraction with fused stage prologue / RK epilogue)'
         extra = dict(mfma_achieved_tflops=mfma, mfma_peak_tflops=pk, mfma_frac=mfma / pk,
                      flop_per_launch=F * issued)
+        mm = measured_mfma(prec) if args.workload == 'uppe' else None
+        if mm:
+            # counter figures over this run's launch time: the r->k launch contracts only the nonlinear band,
+            # so the counted flops fall below the algorithmic 4 Nw Nk Nr per launch
+            h, simds, mhz = mm
+            extra['mfma_flop_per_launch_counted'] = h['flop_per_launch']
+            extra['mfma_busy_frac'] = h['busy_cycles_per_launch'] / (simds * avg_s * mhz * 1e6)
     else:
         B = source_bytes(rs, el)
         kname = dom
