@@ -1161,13 +1161,35 @@ __device__ inline double ion_rate<float>(double ea, const uppe_ionization& ion, 
   return ion.rate_conv * (double)expf((float)poly);
 }
 
+// FP32 rate at field modulus ea (sim units). The explicit form runs in FP32 throughout: forming the field in
+// atomic units in FP32 adds one rounding (~1e-7 relative, times the rate's logarithmic slope), below the FP32
+// evaluation error of the exponent itself. The series form is ion_rate<float>.
+__device__ inline float ion_rate_f32(float ea, const uppe_ionization& ion, double log_pre) {
+  if (ion.kind == UPPE_ION_EXPLICIT) {
+    const float emag = fmaf((float)ion.e_conv, ea, (float)ion.e_cutoff);
+    const float x = (float)log_pre + (float)ion.c_pow * __builtin_amdgcn_logf(emag) +
+                    (float)(ion.c_exp * 1.4426950408889634) * __builtin_amdgcn_rcpf(emag);
+    return __builtin_amdgcn_exp2f(x);
+  }
+  return (float)ion_rate<float>((double)ea, ion, log_pre);
+}
+
+// 1 - exp(-x) for x >= 0 in FP32 (the depletion factor of the electron density): below 1/16 the Taylor
+// polynomial to x^5 (truncation below 3e-8 relative), above it the hardware base-2 exponential, whose ~1 ulp
+// the cancellation enlarges at most 16-fold. The library expm1f costs about twice the instructions.
+__device__ inline float one_minus_exp_neg(float x) {
+  const float p = fmaf(x, fmaf(x, fmaf(x, fmaf(x, 1.0f / 120.0f, -1.0f / 24.0f), 1.0f / 6.0f), -0.5f), 1.0f);
+  const float big = 1.0f - __builtin_amdgcn_exp2f(-x * 1.4426950408889634f);
+  return x < 0.0625f ? x * p : big;
+}
+
 // exp(+2 pi i k/N), the split twiddle of the half-length inverse real transform: FP32 from the hardware
 // sine and cosine of k/N turns (about 2 ulp), FP64 from sincospi
 template <typename T>
 __device__ inline cplx<T> twiddle_inv(int k, int N);
 template <>
 __device__ inline cplx<float> twiddle_inv<float>(int k, int N) {
-  const float t = (float)k / (float)N;
+  const float t = (float)k * (1.0f / (float)N);  // N is a power of two: the reciprocal and product are exact
   return cmk(__builtin_amdgcn_cosf(t), __builtin_amdgcn_sinf(t));
 }
 template <>
@@ -1496,7 +1518,11 @@ __global__ __launch_bounds__(1024) void f_source(const cplx<T>* __restrict__ er,
 #pragma unroll
     for (int j = 0; j < SPT; j++) {
       const int n = a0 + j;
-      const R v = (R)ion_rate<T>(fabs((double)e[j]), ion, log_pre);
+      R v;
+      if constexpr (sizeof(T) == 4)
+        v = ion_rate_f32(fabsf(e[j]), ion, log_pre);
+      else
+        v = ion_rate<T>(fabs(e[j]), ion, log_pre);
       r[j] = (n >= N - 1) ? R(0) : v;  // last sample zeroed (ionization.cl:147)
       tot += r[j];
     }
@@ -1514,7 +1540,7 @@ __global__ __launch_bounds__(1024) void f_source(const cplx<T>* __restrict__ er,
       // where the integral is tiny), the FP64 path as the reference does
       R x;
       if constexpr (sizeof(T) == 4)
-        x = ngp * -expm1f(-integral);
+        x = ngp * one_minus_exp_neg(integral);
       else
         x = ngp * (1.0 - exp(-integral));
       // SoftTimeWindow(4, 64)
@@ -1577,9 +1603,16 @@ __global__ __launch_bounds__(1024) void f_source(const cplx<T>* __restrict__ er,
     T wk = wv[i];
     cplx<T> out;
     if (ion.kind != UPPE_ION_NONE) {
+      // J_p / (0.1 - i w): FP32 multiplies by the hardware reciprocal of |0.1 - i w|^2 (1 ulp; the denominator
+      // is at least 0.01), FP64 divides
       T cr = T(0.1), ci = -wk;
       T den = cr * cr + ci * ci;
-      out = cmk((jp.re * cr + jp.im * ci) / den, (jp.im * cr - jp.re * ci) / den);
+      if constexpr (sizeof(T) == 4) {
+        const float rd = __builtin_amdgcn_rcpf(den);
+        out = cmk((jp.re * cr + jp.im * ci) * rd, (jp.im * cr - jp.re * ci) * rd);
+      } else {
+        out = cmk((jp.re * cr + jp.im * ci) / den, (jp.im * cr - jp.re * ci) / den);
+      }
       out = cadd(out, cmk(wk * pk.im, -wk * pk.re));
     } else {
       out = cmk(wk * pk.im, -wk * pk.re);
@@ -1671,7 +1704,7 @@ __global__ __launch_bounds__(1024) void f_source_env(const cplx<T>* __restrict__
       const R integral = (later + R(0.5) * r[j]) * dtr;
       later += r[j];
       if constexpr (sizeof(T) == 4)
-        ne[j] = ngp * -expm1f(-integral);
+        ne[j] = ngp * one_minus_exp_neg(integral);
       else
         ne[j] = ngp * (1.0 - exp(-integral));
     }
