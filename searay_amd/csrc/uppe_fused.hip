@@ -1408,7 +1408,8 @@ __device__ inline void rot_chi(const R* drive, R ngp, R* dchi, const RotRow<R>* 
 }
 
 // one workgroup per strip, Nt/4 threads (at least min(Nt, 64)); thread t owns SPT = Nt / threads samples
-template <typename T, int SPT, bool DIAG, bool ROT>
+// LGC > 0: an instance for Nt = 2^LGC (block Nt / SPT), whose transforms unroll with constant spans
+template <typename T, int SPT, bool DIAG, bool ROT, int LGC = 0>
 __global__ __launch_bounds__(1024) void f_source(const cplx<T>* __restrict__ er, cplx<T>* __restrict__ jr,
                                                  T* __restrict__ chi_out, T* __restrict__ ne_out,
                                                  const cplx<T>* __restrict__ tw, const T* __restrict__ w,
@@ -1427,7 +1428,8 @@ __global__ __launch_bounds__(1024) void f_source(const cplx<T>* __restrict__ er,
   extern __shared__ char smem[];
   cplx<T>* Y = reinterpret_cast<cplx<T>*>(smem);
   __shared__ T wsum[16];
-  const int N = nt, M = nt / 2, lgm = lgn - 1;
+  if (LGC > 0) lgn = LGC;
+  const int N = LGC > 0 ? (1 << LGC) : nt, M = N / 2, lgm = lgn - 1;
   const long p = blockIdx.x;
   if (inst) {
     // batched launch: instance blockIdx.y, strips zs elements apart, its own medium scalars
@@ -1439,7 +1441,9 @@ __global__ __launch_bounds__(1024) void f_source(const cplx<T>* __restrict__ er,
     log_pre = inst[z].log_pre;
     ion = inst[z].ion;
   }
-  const int tid = threadIdx.x, nth = blockDim.x, lane = tid & 63, wave = tid >> 6, nwv = nth >> 6;
+  constexpr int NTH = LGC > 0 ? (1 << LGC) / SPT : 0;
+  const int tid = threadIdx.x, nth = NTH > 0 ? NTH : (int)blockDim.x, lane = tid & 63, wave = tid >> 6,
+            nwv = nth >> 6;
   const cplx<T>* X = er + p * nwp;
   // every global load the kernel makes up front, issued together: this strip's spectrum, the frequency and
   // filter of the bins this thread writes at the end, and (FP32) the twiddle tables bound for LDS
@@ -1463,7 +1467,7 @@ __global__ __launch_bounds__(1024) void f_source(const cplx<T>* __restrict__ er,
   const cplx<T>* tws = tw;
   if constexpr (sizeof(T) == 4) {
     constexpr int TPT = (3 * SPT + 1) / 2;  // table entries per thread, N + N/2 = 1.5 SPT nth
-    cplx<T>* tl = Y + nt;
+    cplx<T>* tl = Y + N;
     cplx<T> tv[TPT];
 #pragma unroll
     for (int i = 0; i < TPT; i++) {
@@ -1494,7 +1498,7 @@ __global__ __launch_bounds__(1024) void f_source(const cplx<T>* __restrict__ er,
     Y[lds_swz<T>(k)] = cmk(ze.re - zo.im, ze.im + zo.re);
   }
   __syncthreads();
-  lds_fft_r8(Y, M, lgm, tws + r8_table_len(lgn), +1, tid);
+  lds_fft_r8<T, (LGC > 0 ? LGC - 1 : 0), NTH>(Y, M, lgm, tws + r8_table_len(lgn), +1, tid);
   const T invm = T(1) / T(M);
   // thread t owns samples [a0, a0 + SPT) with a0 = (nth - 1 - t) SPT: the depletion integral runs from the
   // last sample backward, which makes it a prefix sum over the threads in lane order
@@ -1589,7 +1593,7 @@ __global__ __launch_bounds__(1024) void f_source(const cplx<T>* __restrict__ er,
   }
   lds_write_reals<T, 2 * SPT>(Y, 2 * a0, yrun);
   __syncthreads();
-  lds_fft_r8(Y, N, lgn, tws, -1, tid);
+  lds_fft_r8<T, LGC, NTH>(Y, N, lgn, tws, -1, tid);
 
   // 4. J = F_NL [ rfft(J_p)/(0.1 - i w) + (-i w) rfft(P) ] (fft.cl:282-329, uppe_kernel.py:170-186)
 #pragma unroll
@@ -2238,6 +2242,16 @@ constexpr size_t kMaxLds = 160 * 1024;  // LDS of one CU (gfx950)
 template <typename T>
 void tune_single(uppe_handle* h, FusedState* s);
 
+// the FP32 stepping source kernel (no diagnostics, no rotator) for Nt = 2^LG, where the block of Nt / S threads
+// fits 64..1024; otherwise the kernel for any Nt
+template <int S, int LG>
+constexpr auto source_fixed() {
+  if constexpr ((1 << LG) / S <= 1024 && (1 << LG) / S >= 64)
+    return f_source<float, S, false, false, LG>;
+  else
+    return f_source<float, S, false, false, 0>;
+}
+
 template <typename T>
 struct Fused {
   using C = cplx<T>;
@@ -2503,6 +2517,9 @@ struct Fused {
   if (spt == S) {                                                                                                \
     auto kern = rot ? (diag ? f_source<T, S, true, true> : f_source<T, S, false, true>)                         \
                     : (diag ? f_source<T, S, true, false> : f_source<T, S, false, false>);                      \
+    if constexpr (sizeof(T) == 4)                                                                                \
+      if (!rot && !diag && lgn >= 10 && lgn <= 12)                                                               \
+        kern = lgn == 10 ? source_fixed<S, 10>() : (lgn == 11 ? source_fixed<S, 11>() : source_fixed<S, 12>());  \
     launchf(h, UPPE_KC_SOURCE, kern, grid, block, shm, er, jr, diag ? (T*)s->chi_t : (T*)nullptr,                 \
             diag ? (T*)s->ne_t : (T*)nullptr, (const C*)s->tw, (const T*)s->w, (const T*)s->nlf,                  \
             (const double*)s->ng, N, lgn, s->nwp, s->redchi3, s->dt, s->ion, log_pre,                             \

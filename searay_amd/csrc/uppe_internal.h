@@ -173,13 +173,17 @@ __device__ inline void dft8(const cplx<T>* v, cplx<T>* y, int sign) {
   y[7] = b3;
 }
 
-template <typename T>
-__device__ inline void lds_fft_r8(cplx<T>* buf, int n, int lg, const cplx<T>* tab, int sign, int tid) {
+// LG > 0 fixes lg = LG (and n, and the block size NTH when nonzero) at compile time: the passes then unroll
+// with their spans, strides and table offsets as constants.
+template <typename T, int LG = 0, int NTH = 0>
+__device__ inline void lds_fft_r8(cplx<T>* buf, int n_rt, int lg_rt, const cplx<T>* tab, int sign, int tid) {
+  const int lg = LG > 0 ? LG : lg_rt;
+  const int n = LG > 0 ? (1 << LG) : n_rt;
   const int r0 = lg % 3;
   int Ns = 1;
   if (r0 > 0) {
     // up to two butterflies per thread (a radix-2 pass has n/2 of them, the block at least n/4 threads)
-    const int R = 1 << r0, nb = n >> r0, nth = blockDim.x;
+    const int R = 1 << r0, nb = n >> r0, nth = NTH > 0 ? NTH : (int)blockDim.x;
     cplx<T> v[2][4];
 #pragma unroll
     for (int b = 0; b < 2; b++) {
@@ -209,7 +213,7 @@ __device__ inline void lds_fft_r8(cplx<T>* buf, int n, int lg, const cplx<T>* ta
     Ns = R;
   }
   const cplx<T>* tp = tab;
-  for (; Ns < n; Ns *= 8) {
+  auto pass = [&](int Ns) {
     const int nb = n >> 3;
     cplx<T> v[8], y[8];
     if (tid < nb)
@@ -231,6 +235,12 @@ __device__ inline void lds_fft_r8(cplx<T>* buf, int n, int lg, const cplx<T>* ta
     }
     __syncthreads();
     tp += 7 * Ns;
+  };
+  if constexpr (LG > 0) {
+#pragma unroll
+    for (int Nc = 1 << (LG % 3); Nc < (1 << LG); Nc *= 8) pass(Nc);
+  } else {
+    for (; Ns < n; Ns *= 8) pass(Ns);
   }
 }
 
