@@ -1631,7 +1631,7 @@ __global__ __launch_bounds__(1024) void f_source(const cplx<T>* __restrict__ er,
 // E(w) = i w A(w) (DtSpectral, sgn -1) and A(w) inverse-transformed side by side in LDS, the rate of |E|, the
 // depletion scan, the rotator driven by |E|^2 / 2 and the Kerr term 0.75 n_g chi3 |E|^2 give P = chi E and
 // Q = n_e (|A|^2/4 - 1) A; the reference's J = FFT(IFFT(-i w FFT(P)) + Q) is formed as -i w FFT(P) + FFT(Q).
-template <typename T, int SPT, bool DIAG, bool ROT>
+template <typename T, int SPT, bool DIAG, bool ROT, int LGC = 0>
 __global__ __launch_bounds__(1024) void f_source_env(const cplx<T>* __restrict__ ar, cplx<T>* __restrict__ jr,
                                                      T* __restrict__ chi_out, T* __restrict__ ne_out,
                                                      const cplx<T>* __restrict__ tw, const T* __restrict__ w,
@@ -1643,12 +1643,14 @@ __global__ __launch_bounds__(1024) void f_source_env(const cplx<T>* __restrict__
   // the stage-1 launch commits the previous step (commit non-null; nothing else of this launch reads it)
   if (commit && blockIdx.x == 0 && threadIdx.x == 0) step_commit(commit, cq, rec, zrec, zrec_cap);
   extern __shared__ char smem[];
+  if (LGC > 0) lgn = LGC;
+  const int N = LGC > 0 ? (1 << LGC) : nt;
   cplx<T>* Y1 = reinterpret_cast<cplx<T>*>(smem);  // E, then P
-  cplx<T>* Y2 = Y1 + nt;                            // A, then Q
+  cplx<T>* Y2 = Y1 + N;                             // A, then Q
   __shared__ T wsum[16];
-  const int N = nt;
   const long p = blockIdx.x;
-  const int tid = threadIdx.x, nth = blockDim.x, lane = tid & 63, wave = tid >> 6;
+  constexpr int NTH = LGC > 0 ? (1 << LGC) / SPT : 0;
+  const int tid = threadIdx.x, nth = NTH > 0 ? NTH : (int)blockDim.x, lane = tid & 63, wave = tid >> 6;
   const cplx<T>* X = ar + p * nwp;
   cplx<T> xa[SPT];
   T wv[SPT];
@@ -1673,8 +1675,8 @@ __global__ __launch_bounds__(1024) void f_source_env(const cplx<T>* __restrict__
     Y2[lds_swz<T>(k)] = xa[i];
   }
   __syncthreads();
-  lds_fft_r8(Y1, N, lgn, tws, +1, tid);
-  lds_fft_r8(Y2, N, lgn, tws, +1, tid);
+  lds_fft_r8<T, LGC, NTH>(Y1, N, lgn, tws, +1, tid);
+  lds_fft_r8<T, LGC, NTH>(Y2, N, lgn, tws, +1, tid);
   const T invn = T(1) / T(N);
   const int a0 = (nth - 1 - tid) * SPT;
   cplx<T> e[SPT];
@@ -1748,8 +1750,8 @@ __global__ __launch_bounds__(1024) void f_source_env(const cplx<T>* __restrict__
     }
   }
   __syncthreads();
-  lds_fft_r8(Y1, N, lgn, tws, -1, tid);
-  lds_fft_r8(Y2, N, lgn, tws, -1, tid);
+  lds_fft_r8<T, LGC, NTH>(Y1, N, lgn, tws, -1, tid);
+  lds_fft_r8<T, LGC, NTH>(Y2, N, lgn, tws, -1, tid);
   // J = -i w FFT(P) + FFT(Q) (DtSpectral, sgn +1)
 #pragma unroll
   for (int i = 0; i < SPT; i++) {
@@ -1769,7 +1771,7 @@ __global__ __launch_bounds__(1024) void f_source_env(const cplx<T>* __restrict__
 // e and g_P come from one unnormalised inverse transform of Z = E^ext + i Y^ext (Hermitian extensions, Y = R with
 // its inner bins halved), g_e from one forward transform. One workgroup per strip, max(min(Nt, 64), Nt/4)
 // threads.
-template <typename T, int SPT>
+template <typename T, int SPT, int LGC = 0>
 __global__ __launch_bounds__(1024) void f_source_adj(const cplx<T>* __restrict__ er, const cplx<T>* __restrict__ gj,
                                                      cplx<T>* __restrict__ ge, const cplx<T>* __restrict__ tw,
                                                      const T* __restrict__ w, const T* __restrict__ nlf,
@@ -1777,9 +1779,11 @@ __global__ __launch_bounds__(1024) void f_source_adj(const cplx<T>* __restrict__
                                                      double redchi3) {
   extern __shared__ char smem[];
   cplx<T>* Z = reinterpret_cast<cplx<T>*>(smem);
-  const int N = nt, M = nt / 2;
+  if (LGC > 0) lgn = LGC;
+  const int N = LGC > 0 ? (1 << LGC) : nt, M = N / 2;
   const long p = blockIdx.x;
-  const int tid = threadIdx.x, nth = blockDim.x;
+  constexpr int NTH = LGC > 0 ? (1 << LGC) / SPT : 0;
+  const int tid = threadIdx.x, nth = NTH > 0 ? NTH : (int)blockDim.x;
   const cplx<T>* X = er + p * nwp;
   const cplx<T>* G = gj + p * nwp;
   const cplx<T>* tws = tw;
@@ -1808,7 +1812,7 @@ __global__ __launch_bounds__(1024) void f_source_adj(const cplx<T>* __restrict__
     Z[lds_swz<T>(k)] = cmk(x.re - y.im, x.im + y.re);
   }
   __syncthreads();
-  lds_fft_r8(Z, N, lgn, tws, +1, tid);
+  lds_fft_r8<T, LGC, NTH>(Z, N, lgn, tws, +1, tid);
   const T kc3 = (T)(3.0 * ng[p] * redchi3), invn = T(1) / T(N);
   T gev[SPT];
 #pragma unroll
@@ -1821,7 +1825,7 @@ __global__ __launch_bounds__(1024) void f_source_adj(const cplx<T>* __restrict__
 #pragma unroll
   for (int i = 0; i < SPT; i++) Z[lds_swz<T>(tid + i * nth)] = cmk(gev[i], T(0));
   __syncthreads();
-  lds_fft_r8(Z, N, lgn, tws, -1, tid);
+  lds_fft_r8<T, LGC, NTH>(Z, N, lgn, tws, -1, tid);
   for (int k = tid; k <= M; k += nth) {
     const cplx<T> f = Z[lds_swz<T>(k)];
     ge[p * nwp + k] = (k == 0 || k == M) ? cmk(f.re * invn, T(0)) : cscale(f, T(2) * invn);
@@ -1837,6 +1841,25 @@ __device__ inline double ion_rate_slope(double ea, const uppe_ionization& ion, d
   const double l = log(emag);
   const double dp = ion.c[1] + l * (2.0 * ion.c[2] + l * (3.0 * ion.c[3] + l * (4.0 * ion.c[4] + l * 5.0 * ion.c[5])));
   return rate * dp / emag * ion.e_conv;
+}
+
+// FP32 rate and d rate / d|e| at field modulus ea (sim units): the explicit form in FP32 with one hardware
+// reciprocal, w = C E^p exp(c/E), dw/dE = w (p - c/E)/E times the field's unit factor; the series form through
+// ion_rate<float> and ion_rate_slope
+__device__ inline void ion_rate_and_slope_f32(float ea, const uppe_ionization& ion, double log_pre, float& rate,
+                                              float& slope) {
+  if (ion.kind == UPPE_ION_EXPLICIT) {
+    const float emag = fmaf((float)ion.e_conv, ea, (float)ion.e_cutoff);
+    const float re = __builtin_amdgcn_rcpf(emag);
+    const float cp = (float)ion.c_pow, ce = (float)ion.c_exp;
+    rate = __builtin_amdgcn_exp2f((float)log_pre + cp * __builtin_amdgcn_logf(emag) +
+                                  (float)(ion.c_exp * 1.4426950408889634) * re);
+    slope = rate * (cp - ce * re) * re * (float)ion.e_conv;
+    return;
+  }
+  const double r = ion_rate<float>((double)ea, ion, log_pre);
+  rate = (float)r;
+  slope = (float)ion_rate_slope((double)ea, ion, r);
 }
 
 // the soft time window applied to the electron density, SoftTimeWindow(4, 64) (fft.cl:331-355)
@@ -1887,7 +1910,7 @@ __device__ inline T block_excl_prefix(T v, T* ws, T* total) {
 // e and g_P come from one inverse transform of E^ext + i R^ext, g_Jp from a second of R'^ext (Hermitian extensions,
 // inner bins halved). Thread t owns samples t SPT .. t SPT + SPT - 1, so the forward suffix sum and the transposed
 // prefix sum are block scans in thread order. One workgroup per strip, max(min(Nt, 64), Nt/4) threads.
-template <typename T, int SPT>
+template <typename T, int SPT, int LGC = 0>
 __global__ __launch_bounds__(1024) void f_source_adj_ion(const cplx<T>* __restrict__ er, const cplx<T>* __restrict__ gj,
                                                          cplx<T>* __restrict__ ge, const cplx<T>* __restrict__ tw,
                                                          const T* __restrict__ w, const T* __restrict__ nlf,
@@ -1897,10 +1920,12 @@ __global__ __launch_bounds__(1024) void f_source_adj_ion(const cplx<T>* __restri
   extern __shared__ char smem[];
   cplx<T>* Z = reinterpret_cast<cplx<T>*>(smem);
   __shared__ T ws[16];
-  const int N = nt, M = nt / 2;
+  if (LGC > 0) lgn = LGC;
+  const int N = LGC > 0 ? (1 << LGC) : nt, M = N / 2;
   cplx<T>* Z2 = Z + N;
   const long p = blockIdx.x;
-  const int tid = threadIdx.x, nth = blockDim.x;
+  constexpr int NTH = LGC > 0 ? (1 << LGC) / SPT : 0;
+  const int tid = threadIdx.x, nth = NTH > 0 ? NTH : (int)blockDim.x;
   const cplx<T>* X = er + p * nwp;
   const cplx<T>* G = gj + p * nwp;
   const cplx<T>* tws = tw;
@@ -1917,7 +1942,11 @@ __global__ __launch_bounds__(1024) void f_source_adj_ion(const cplx<T>* __restri
     const T f = nlf[kk], wk = w[kk];
     const T wf = wk * f;
     cplx<T> y = cmk(-wf * g.im, wf * g.re);  // i w F_NL G
-    const T d = f / (T(0.01) + wk * wk);      // F_NL G (0.1 - i w) / (0.01 + w^2)
+    T d;                                      // F_NL G (0.1 - i w) / (0.01 + w^2)
+    if constexpr (sizeof(T) == 4)
+      d = f * __builtin_amdgcn_rcpf(0.01f + wk * wk);
+    else
+      d = f / (0.01 + wk * wk);
     cplx<T> y2 = cmk(d * (T(0.1) * g.re + wk * g.im), d * (T(0.1) * g.im - wk * g.re));
     if (kk == 0 || kk == M) {
       x.im = T(0);
@@ -1936,8 +1965,8 @@ __global__ __launch_bounds__(1024) void f_source_adj_ion(const cplx<T>* __restri
     Z2[lds_swz<T>(k)] = y2;
   }
   __syncthreads();
-  lds_fft_r8(Z, N, lgn, tws, +1, tid);
-  lds_fft_r8(Z2, N, lgn, tws, +1, tid);
+  lds_fft_r8<T, LGC, NTH>(Z, N, lgn, tws, +1, tid);
+  lds_fft_r8<T, LGC, NTH>(Z2, N, lgn, tws, +1, tid);
   const double ngp = ng[p];
   const T kc3 = (T)(3.0 * ngp * redchi3), invn = T(1) / T(N), ngt = (T)ngp, dtt = (T)dt;
   const int a0 = tid * SPT;
@@ -1950,10 +1979,16 @@ __global__ __launch_bounds__(1024) void f_source_adj_ion(const cplx<T>* __restri
     e[j] = z.re * invn;
     gp[j] = z.im;
     gjp[j] = Z2[lds_swz<T>(n)].re;
-    const double ea = fabs((double)e[j]);
-    const double rate = ion_rate<T>(ea, ion, log_pre);
-    r[j] = n >= N - 1 ? T(0) : (T)rate;
-    dr[j] = n >= N - 1 ? T(0) : (T)(ion_rate_slope(ea, ion, rate) * (e[j] > T(0) ? 1.0 : (e[j] < T(0) ? -1.0 : 0.0)));
+    T rate, slope;
+    if constexpr (sizeof(T) == 4) {
+      ion_rate_and_slope_f32(fabsf(e[j]), ion, log_pre, rate, slope);
+    } else {
+      const double ea = fabs(e[j]);
+      rate = ion_rate<T>(ea, ion, log_pre);
+      slope = ion_rate_slope(ea, ion, rate);
+    }
+    r[j] = n >= N - 1 ? T(0) : rate;
+    dr[j] = n >= N - 1 ? T(0) : slope * (e[j] > T(0) ? T(1) : (e[j] < T(0) ? T(-1) : T(0)));
     rtot += r[j];
   }
   // forward depletion integral: samples after this thread's own hold the rest of the suffix sum
@@ -1970,8 +2005,8 @@ __global__ __launch_bounds__(1024) void f_source_adj_ion(const cplx<T>* __restri
     const T win = soft_window<T>(n, N);
     T ex, dep;
     if constexpr (sizeof(T) == 4) {
-      ex = expf(-I);
-      dep = -expm1f(-I);
+      ex = __builtin_amdgcn_exp2f(-I * 1.4426950408889634f);
+      dep = one_minus_exp_neg(I);
     } else {
       ex = exp(-I);
       dep = 1.0 - exp(-I);
@@ -1994,7 +2029,7 @@ __global__ __launch_bounds__(1024) void f_source_adj_ion(const cplx<T>* __restri
 #pragma unroll
   for (int j = 0; j < SPT; j++) Z[lds_swz<T>(a0 + j)] = cmk(gev[j], T(0));
   __syncthreads();
-  lds_fft_r8(Z, N, lgn, tws, -1, tid);
+  lds_fft_r8<T, LGC, NTH>(Z, N, lgn, tws, -1, tid);
   for (int k = tid; k <= M; k += nth) {
     const cplx<T> f = Z[lds_swz<T>(k)];
     ge[p * nwp + k] = (k == 0 || k == M) ? cmk(f.re * invn, T(0)) : cscale(f, T(2) * invn);
@@ -2252,6 +2287,31 @@ constexpr auto source_fixed() {
     return f_source<float, S, false, false, 0>;
 }
 
+// the same for the FP32 envelope source kernel (Nt = Nw = 256 .. 1024)
+template <int S, int LG>
+constexpr auto env_fixed() {
+  if constexpr ((1 << LG) / S <= 1024 && (1 << LG) / S >= 64)
+    return f_source_env<float, S, false, false, LG>;
+  else
+    return f_source_env<float, S, false, false, 0>;
+}
+
+// the same for the FP32 adjoint source kernels (Kerr, and with photoionisation)
+template <int S, int LG>
+constexpr auto adj_fixed() {
+  if constexpr ((1 << LG) / S <= 1024 && (1 << LG) / S >= 64)
+    return f_source_adj<float, S, LG>;
+  else
+    return f_source_adj<float, S, 0>;
+}
+template <int S, int LG>
+constexpr auto adj_ion_fixed() {
+  if constexpr ((1 << LG) / S <= 1024 && (1 << LG) / S >= 64)
+    return f_source_adj_ion<float, S, LG>;
+  else
+    return f_source_adj_ion<float, S, 0>;
+}
+
 template <typename T>
 struct Fused {
   using C = cplx<T>;
@@ -2503,6 +2563,9 @@ struct Fused {
   if (spt == S) {                                                                                                \
     auto kern = rot ? (diag ? f_source_env<T, S, true, true> : f_source_env<T, S, false, true>)                 \
                     : (diag ? f_source_env<T, S, true, false> : f_source_env<T, S, false, false>);              \
+    if constexpr (sizeof(T) == 4)                                                                                \
+      if (!rot && !diag && lgn >= 8 && lgn <= 10)                                                                \
+        kern = lgn == 8 ? env_fixed<S, 8>() : (lgn == 9 ? env_fixed<S, 9>() : env_fixed<S, 10>());             \
     launchf(h, UPPE_KC_SOURCE, kern, grid, block, shm_env, er, jr, diag ? (T*)s->chi_t : (T*)nullptr,             \
             diag ? (T*)s->ne_t : (T*)nullptr, (const C*)s->tw, (const T*)s->w, (const double*)s->ng, N, lgn,      \
             s->nwp, s->redchi3, s->dt, s->ion, log_pre, (const RotRow<T>*)s->rot_tab, h->rot_states,              \
@@ -2542,7 +2605,11 @@ struct Fused {
     const dim3 grid((unsigned)h->P), block(nth);
 #define UPPE_SRC_ADJ(S)                                                                                          \
   if (spt == S) {                                                                                                \
-    launchf(h, UPPE_KC_SOURCE, f_source_adj<T, S>, grid, block, shm, er, gj, ge, (const C*)s->tw, (const T*)s->w, \
+    auto kern = f_source_adj<T, S>;                                                                              \
+    if constexpr (sizeof(T) == 4)                                                                                \
+      if (lgn >= 10 && lgn <= 12)                                                                                \
+        kern = lgn == 10 ? adj_fixed<S, 10>() : (lgn == 11 ? adj_fixed<S, 11>() : adj_fixed<S, 12>());          \
+    launchf(h, UPPE_KC_SOURCE, kern, grid, block, shm, er, gj, ge, (const C*)s->tw, (const T*)s->w,               \
             (const T*)s->nlf, (const double*)s->ng, N, lgn, s->nwp, s->redchi3);                                  \
     return;                                                                                                      \
   }
@@ -2554,7 +2621,11 @@ struct Fused {
                                                                : std::log(s->ion.rate_conv * s->ion.c_pre);
 #define UPPE_SRC_ADJ_ION(S)                                                                                      \
   if (spt == S) {                                                                                                \
-    launchf(h, UPPE_KC_SOURCE, f_source_adj_ion<T, S>, grid, block, shm_ion, er, gj, ge, (const C*)s->tw,         \
+    auto kern = f_source_adj_ion<T, S>;                                                                          \
+    if constexpr (sizeof(T) == 4)                                                                                \
+      if (lgn >= 10 && lgn <= 12)                                                                                \
+        kern = lgn == 10 ? adj_ion_fixed<S, 10>() : (lgn == 11 ? adj_ion_fixed<S, 11>() : adj_ion_fixed<S, 12>()); \
+    launchf(h, UPPE_KC_SOURCE, kern, grid, block, shm_ion, er, gj, ge, (const C*)s->tw,                          \
             (const T*)s->w, (const T*)s->nlf, (const double*)s->ng, N, lgn, s->nwp, s->redchi3, s->dt, s->ion,    \
             log_pre);                                                                                            \
     return;                                                                                                      \
