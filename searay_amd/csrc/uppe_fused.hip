@@ -2598,7 +2598,8 @@ struct Fused {
   // adjoint of the Kerr source on every strip (f_source_adj): er = the stage's E_r, gj = cotangent of J_r
   static void source_adj(uppe_handle* h, FusedState* s, const C* er, const C* gj, C* ge) {
     const int N = h->nt;
-    const int nth = std::max(std::min(N, 64), N / 4);
+    // the forward source kernel's shape (wide workgroups when the strips are fewer than the CUs)
+    const int nth = s->spt > 0 ? N / s->spt : std::max(std::min(N, 64), N / 4);
     const int spt = N / nth;
     const size_t shm = sizeof(C) * (sizeof(T) == 4 ? 2 * N : N);
     const int lgn = ilog2i(N);
