@@ -488,19 +488,26 @@ __device__ inline void stage_point(cplx<T> J, cplx<T> kz, double ph, cplx<T> Q, 
                                    cplx<T>* __restrict__ k3, const StepState* st, double amin, double& loc,
                                    bool& nan) {
   cplx<T> t = cmul(linear_factor<T>(ph, kz.im, zph, -1), J);
-  cplx<T> S = cmk((T(-0.5) * t.im) / kz.re, (T(0.5) * t.re) / kz.re);
+  cplx<T> S;
+  if constexpr (sizeof(T) == 4) {
+    const float rk = __builtin_amdgcn_rcpf(kz.re);  // 1 ulp
+    S = cmk(-0.5f * t.im * rk, 0.5f * t.re * rk);
+  } else {
+    S = cmk((T(-0.5) * t.im) / kz.re, (T(0.5) * t.re) / kz.re);
+  }
   if (stage == 1) {
     k1[idx] = S;
     double qr = (double)Q.re, qi = (double)Q.im;
-    double mod = sqrt(qr * qr + qi * qi);
-    if (mod != mod) nan = true;
+    if (qr != qr || qi != qi) nan = true;  // |q| is NaN exactly when a part is
     if (!st->fixed) {
+      // loc keeps the largest rate Kr: the smallest per-point step dphi / Kr is dphi over it (a correctly
+      // rounded quotient falls monotonically with the divisor), formed once after the loop
       double sr = (double)S.re, si = (double)S.im;
-      double den = qr * qr + qi * qi + amin * amin;
-      double alpha = (qr * sr + qi * si) / den;
-      double keff = (qr * si - qi * sr) / den;
+      const double rden = 1.0 / (qr * qr + qi * qi + amin * amin);
+      double alpha = (qr * sr + qi * si) * rden;
+      double keff = (qr * si - qi * sr) * rden;
       double Kr = 0.5 * st->dphi / st->L + fabs(keff) - alpha * (double)(alpha < 0.0);
-      loc = fmin(loc, st->dphi / Kr);
+      loc = fmax(loc, Kr);
     }
   } else if (stage == 2) {
     k2[idx] = S;
@@ -518,7 +525,7 @@ __device__ inline void stage_point(cplx<T> J, cplx<T> kz, double ph, cplx<T> Q, 
     im = im + c6 * S.im;
     cplx<T> qn = cmk(re * cf, im * cf);
     q[idx] = qn;
-    loc = fmax(loc, sqrt((double)qn.re * qn.re + (double)qn.im * qn.im));
+    loc = fmax(loc, (double)qn.re * qn.re + (double)qn.im * qn.im);  // |q|^2: the root of the largest, after the loop
   }
 }
 
@@ -889,7 +896,7 @@ __global__ __launch_bounds__(BN > 16 ? 512 : 1024) void f_hankel(
   }
   if (EPI == EPI_STAGE && !st->active) return;
   const double amin = (EPI == EPI_STAGE && stage == 1 && !st->fixed) ? st->rel_amin * bitsd(*qmax) : 0.0;
-  double loc = (EPI == EPI_STAGE && stage == 4) ? 0.0 : INFINITY;
+  double loc = 0.0;  // stage 1: the largest step rate Kr; stage 4: the largest |q|^2
   bool nan = false;
   // points are taken EPB per thread per batch, all of a batch's loads issued before its arithmetic
   constexpr int EPB = sizeof(T) == 8 ? 1 : 2;
@@ -936,6 +943,7 @@ __global__ __launch_bounds__(BN > 16 ? 512 : 1024) void f_hankel(
   if (stage == 2 || stage == 3) return;
   if (stage == 1 && st->fixed) return;
   const bool is_min = stage == 1;
+  loc = is_min ? (loc > 0.0 ? st->dphi / loc : INFINITY) : sqrt(loc);  // smallest step, or max |q|
   for (int o = 32; o > 0; o >>= 1) {
     double v = __shfl_xor(loc, o);
     loc = is_min ? fmin(loc, v) : fmax(loc, v);
@@ -1088,7 +1096,7 @@ __global__ __launch_bounds__(1024) void f_tfft(
   // store, or the RK stage epilogue (left out once the run has finished)
   if (EPI == EPI_STAGE && !st->active) return;
   const double amin = (EPI == EPI_STAGE && stage == 1 && !st->fixed) ? st->rel_amin * bitsd(*qmax) : 0.0;
-  double loc = (EPI == EPI_STAGE && stage == 4) ? 0.0 : INFINITY;
+  double loc = 0.0;  // stage 1: the largest step rate Kr; stage 4: the largest |q|^2
   bool nan = false;
   for (int e = t; e < ne; e += nth) {
     const int a = e / CW, c = e % CW, f = f0 + c;
@@ -1105,6 +1113,7 @@ __global__ __launch_bounds__(1024) void f_tfft(
   if (stage == 2 || stage == 3) return;
   if (stage == 1 && st->fixed) return;
   const bool is_min = stage == 1;
+  loc = is_min ? (loc > 0.0 ? st->dphi / loc : INFINITY) : sqrt(loc);  // smallest step, or max |q|
   for (int o = 32; o > 0; o >>= 1) {
     double v = __shfl_xor(loc, o);
     loc = is_min ? fmin(loc, v) : fmax(loc, v);
