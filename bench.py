@@ -29,6 +29,8 @@ Printed JSON line (one):
                 instances, autotuned launch shapes, HIP graphs) against the same instances run one after another
 
 ``--workload batch`` runs only the batched workload (its JSON line carries the drawn instances).
+``--workload kerr`` runs only the radial Kerr workload (Nw = 513, Nr = 128, 100 fixed steps) against the plain FP64
+stepper, as the bench line's `radial_kerr_513x128`.
 ``--workload cartesian`` measures the 3-D Cartesian filamentation workload (Nw = 513, 256 x 256, Kerr + O2) the
 same way, its roofline on the transverse FFT pass (28 B per grid point per launch in FP32: 40 B on the passes
 with the stage prologue or epilogue, 16 B on the plain ones).
@@ -339,7 +341,7 @@ def main():
     ap.add_argument('--cpu-steps', type=int, default=60)
     ap.add_argument('--no-cpu', action='store_true')
     ap.add_argument('--no-compare', action='store_true')
-    ap.add_argument('--workload', default='uppe', choices=['uppe', 'paraxial', 'adjoint', 'batch', 'cartesian'])
+    ap.add_argument('--workload', default='uppe', choices=['uppe', 'paraxial', 'adjoint', 'batch', 'cartesian', 'kerr'])
     ap.add_argument('--calibrate-copy', type=int, default=0, metavar='MIB',
                     help='only run a plain device-to-device copy of MIB MiB (for scaling HBM byte counters)')
     args = ap.parse_args()
@@ -358,6 +360,14 @@ def main():
     if args.gpus != 1:
         print('bench.py runs on one GPU only: this stepper has no multi-GPU path (use --gpus 1)', file=sys.stderr)
         sys.exit(2)
+
+    if args.workload == 'kerr':
+        prec = args.precision or 'bf16x3'
+        if prec == 'fp64-plain':
+            sys.exit('the Kerr workload compares a fused path with the plain FP64 stepper')
+        log('radial Kerr workload (Nw=513, Nr=128): %s vs plain FP64' % prec)
+        print(json.dumps(kerr_513x128_speedup(prec, args.plain_steps)))
+        return
 
     if args.workload == 'adjoint':
         prec = args.precision or 'fp32'
