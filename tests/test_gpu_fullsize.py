@@ -1,6 +1,7 @@
 '''Full-size properties on the photoionisation workload (Nw = 2049, Nr = 256), where the oracle is too slow to
 run to completion: agreement of the reduced-precision paths with the fused FP64 path on one step schedule,
-FP64 fused against FP64 plain over the first steps, and conservation of energy in the linear regime.'''
+FP64 fused against FP64 plain over the first steps, and conservation of energy in the linear regime; and the
+radial Kerr workload (Nw = 513, Nr = 128) at full size against fused FP64 and the plain stepper.'''
 import sys
 
 import numpy as np
@@ -50,6 +51,24 @@ def test_reduced_precision_on_fp64_schedule(c2, precision):
     n64, rec, q64, info64 = _steps(c2, 'fp64', 200, record=True)
     n, _, q, info = _steps(c2, precision, 200, schedule=rec)
     assert n == n64 == 200
+    assert info['nan_steps'] == 0 and info64['nan_steps'] == 0
+    assert rel_l2(q, q64) < 1e-5
+
+
+@pytest.mark.parametrize('precision', ['fp32', 'bf16x3'])
+def test_radial_kerr_full_size(precision):
+    '''The radial Kerr workload at full size (Nw = 513, Nr = 128, 100 fixed steps over 1 m): fused FP64 against
+    the plain FP64 stepper over the first 10 steps (1e-10), and the whole run in the reduced precision against
+    fused FP64 (1e-5; fixed steps give both the same schedule).'''
+    rs = uppe_kernel.RunSetup(configs.radial_kerr_air())
+    _say('full size radial Kerr: fp64 fused vs plain (10 steps), %s vs fp64 (100 steps)' % precision)
+    n0, _, q0, _ = _steps(rs, 'fp64-plain', 10)
+    n1, _, q1, _ = _steps(rs, 'fp64', 10)
+    assert n0 == n1 == 10
+    assert rel_l2(q1, q0) < 1e-10
+    n64, _, q64, info64 = _steps(rs, 'fp64', 100)
+    n, _, q, info = _steps(rs, precision, 100)
+    assert n == n64 == 100
     assert info['nan_steps'] == 0 and info64['nan_steps'] == 0
     assert rel_l2(q, q64) < 1e-5
 
