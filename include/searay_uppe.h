@@ -163,14 +163,28 @@ int uppe_set_field(uppe_handle* h, const double* a_r);
  * steps_out (nullable) receives the steps taken by this call. */
 int uppe_advance(uppe_handle* h, const uppe_step_control* c, int* steps_out);
 
+/* uppe_advance that allocates nothing. No counterpart in the reference: it is for callers that cannot allocate
+ * device memory (code under HIP graph capture of the surrounding work, tight serving loops, harnesses that forbid
+ * hipMalloc). Same arguments, stepping and results as uppe_advance (the same kernels in the same order), but the
+ * call's work arrays (the RK slopes k1..k3, the r-space field and current), the replayed schedule and the dz
+ * record live in one static device workspace of the library, 48 MiB per GPU, held from the library's load until
+ * the process ends. A call needs 5 * W + S + R bytes of it, each term rounded up to 256 bytes: W = P * NWP *
+ * (8 bytes in FP32 / BF16x3, 16 in FP64) with P = n1 * n2 and NWP = nw rounded up to a multiple of 32, S = 8 *
+ * n_schedule, R = 8 * record_cap (0 without dz_record); larger requests return UPPE_ERR_ARG with the sizes in
+ * uppe_last_error. Standalone handles on the fused precisions (FP64, FP32, BF16X3) only; no event timing. Every
+ * call on one GPU shares its workspace, which keeps the last call's contents (the call clears what must start
+ * at zero): calls that overlap on one GPU, from several threads or streams, race. */
+int uppe_advance_static(uppe_handle* h, const uppe_step_control* c, int* steps_out);
+
 /* finish() semantics (uppe_kernel.py:234-258, paraxial_kernel.py:222-249): propagate to z_total = zf - zi,
  * transform to r-space and evaluate the diagnostics. Outputs (each nullable): a_r, j_r complex128
  * [nw][n1][n2]; chi_t, ne_t float64 [nt][n1][n2] (the caller forms rfft (UPPE) or fft (paraxial) along
  * axis 0 as the reference does on the host). */
 int uppe_finish(uppe_handle* h, double z_total, double* a_r, double* j_r, double* chi_t, double* ne_t);
 
-/* Gradients (no reference counterpart; the adjoint the input spectral-phase objectives need). Kerr-only UPPE
- * media on the radial matrix-core paths (FP64, FP32, BF16x3).
+/* Gradients (no reference counterpart; the adjoint the input spectral-phase objectives need). UPPE media with
+ * Kerr, photoionisation (explicit and series rates) and the plasma current, without a rotator, on the radial
+ * matrix-core paths (FP64, FP32, BF16x3).
  *
  * uppe_advance_checkpointed: uppe_advance that also keeps what the adjoint needs: the state every
  * checkpoint_every steps and the phase origin and dz of every step. The step sizes are treated as constants by

@@ -64,7 +64,8 @@ class uppe_info(ct.Structure):
 
 ENTRY_POINTS = ('create', 'set_hankel', 'set_medium', 'set_paraxial_medium', 'set_rotator', 'set_field', 'advance',
                 'finish', 'get_state', 'get_info', 'destroy', 'last_error')
-HIP_ONLY_ENTRY_POINTS = ('profile_enable', 'profile_read', 'synchronize', 'advance_checkpointed', 'adjoint')
+HIP_ONLY_ENTRY_POINTS = ('profile_enable', 'profile_read', 'synchronize', 'advance_checkpointed', 'adjoint',
+                         'advance_static')
 BATCH_ENTRY_POINTS = ('batch_create', 'batch_member', 'batch_advance', 'batch_set_autotune', 'batch_set_graph',
                       'batch_plan', 'batch_destroy')
 
@@ -175,13 +176,18 @@ class Stepper:
 
     def advance(self, zi, length, sub0, n_sub, dzmin, dphi=0.4, rel_amin=1e-2, dz_schedule=None, record_cap=0,
                 max_steps=0):
+        return self._advance_with(self._f['advance'], 'advance', zi, length, sub0, n_sub, dzmin, dphi, rel_amin,
+                                  dz_schedule, record_cap, max_steps)
+
+    def _advance_with(self, fn, what, zi, length, sub0, n_sub, dzmin, dphi, rel_amin, dz_schedule, record_cap,
+                      max_steps):
         sched = None if dz_schedule is None else _f64(dz_schedule)
         rec = np.zeros(max(record_cap, 0)) if record_cap > 0 else None
         c = uppe_step_control(float(zi), float(length), int(sub0), int(max_steps), float(dzmin), float(dphi), float(rel_amin),
                               int(n_sub), 0 if sched is None else len(sched), _ptr(sched), _ptr(rec),
                               record_cap if rec is not None else 0, 0)
         n = ct.c_int(0)
-        self._check(self._f['advance'](self.h, ct.byref(c), ct.byref(n)), 'advance')
+        self._check(fn(self.h, ct.byref(c), ct.byref(n)), what)
         if rec is not None:
             return n.value, rec[:min(n.value, record_cap)].copy()
         return n.value, None

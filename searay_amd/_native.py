@@ -47,6 +47,13 @@ class HipStepper(_abi.Stepper):
     def synchronize(self):
         self._check(self.lib.uppe_synchronize(self.h), 'synchronize')
 
+    def advance_static(self, zi, length, sub0, n_sub, dzmin, dphi=0.4, rel_amin=1e-2, dz_schedule=None,
+                       record_cap=0, max_steps=0):
+        '''advance() through uppe_advance_static: the same stepping with the work arrays in the library's static
+        device workspace (no device allocation). Returns (steps, recorded dz or None).'''
+        return self._advance_with(self.lib.uppe_advance_static, 'advance_static', zi, length, sub0, n_sub, dzmin,
+                                  dphi, rel_amin, dz_schedule, record_cap, max_steps)
+
     def advance_checkpointed(self, zi, length, sub0, n_sub, dzmin, every, dphi=0.4, rel_amin=1e-2, dz_schedule=None,
                              record_cap=0):
         '''advance() keeping what adjoint() needs: the state every `every` steps and each step's phase origin and

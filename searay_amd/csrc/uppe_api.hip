@@ -281,6 +281,40 @@ int uppe_advance(uppe_handle* h, const uppe_step_control* c, int* steps_out) {
   return rc;
 }
 
+int uppe_advance_static(uppe_handle* h, const uppe_step_control* c, int* steps_out) {
+  if (steps_out) *steps_out = 0;
+  if (!h) return UPPE_ERR_ARG;
+  if (!c) {
+    h->err = "null argument";
+    return UPPE_ERR_ARG;
+  }
+  if (!h->fused || h->batch) {
+    h->err = "uppe_advance_static: standalone handles on the fused paths only (not fp64-plain, not batch members)";
+    return UPPE_ERR_UNSUPPORTED;
+  }
+  if (c->n_sub < 1 || !(c->length > 0.0) || (c->n_schedule > 0 && !c->dz_schedule) || c->n_schedule < 0) {
+    h->err = "bad step control";
+    return UPPE_ERR_ARG;
+  }
+  const size_t need = fused_static_need(h, c), cap = fused_static_capacity();
+  if (need > cap) {
+    h->err = "uppe_advance_static: this grid and step control need " + std::to_string(need) +
+             " bytes of workspace; the static workspace holds " + std::to_string(cap) + " bytes";
+    return UPPE_ERR_ARG;
+  }
+  if (!h->have_field || !h->have_medium) {
+    h->err = "field or medium not set";
+    return UPPE_ERR_STATE;
+  }
+  int taken = 0;
+  int rc = guarded(h, [&] {
+    UPPE_HIP(hipSetDevice(h->d.device));
+    taken = fused_advance_static(h, c);
+  });
+  if (steps_out) *steps_out = taken;
+  return rc;
+}
+
 int uppe_advance_checkpointed(uppe_handle* h, const uppe_step_control* c, int checkpoint_every, int* steps_out) {
   if (!h) return UPPE_ERR_ARG;
   if (!c || checkpoint_every < 1) {

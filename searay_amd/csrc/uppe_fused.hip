@@ -20,6 +20,7 @@
 
 #include <algorithm>
 #include <cstring>
+#include <mutex>
 
 namespace uppe {
 
@@ -2055,6 +2056,11 @@ __global__ void f_ctl_commit(StepState* st, unsigned long long* qmax, double* re
   step_commit(st + blockIdx.x, qmax + 3 * blockIdx.x, rec, zrec, zrec_cap);
 }
 
+// zero n 8-byte words (a fill kernel, so that a stream the caller captures into a graph holds no memset node)
+__global__ void f_fill_zero(unsigned long long* p, long n) {
+  for (long i = blockIdx.x * (long)blockDim.x + threadIdx.x; i < n; i += (long)gridDim.x * blockDim.x) p[i] = 0ull;
+}
+
 // max |q| over the state (set_field)
 template <typename T>
 __global__ void f_qmax(const cplx<T>* q, long n, unsigned long long* out) {
@@ -2284,7 +2290,7 @@ static size_t hk_lds_bytes(int cm, int kp, int bm, int bn) {
 constexpr size_t kMaxLds = 160 * 1024;  // LDS of one CU (gfx950)
 
 template <typename T>
-void tune_single(uppe_handle* h, FusedState* s);
+void tune_single(uppe_handle* h, FusedState* s, bool may_time = true);
 
 // the FP32 stepping source kernel (no diagnostics, no rotator) for Nt = 2^LG, where the block of Nt / S threads
 // fits 64..1024; otherwise the kernel for any Nt
@@ -3008,7 +3014,64 @@ struct Fused {
     }
   }
 
-  static int advance(uppe_handle* h, FusedState* s, const uppe_step_control* c) {
+  // bytes of the static workspace one uppe_advance_static call takes: the five work arrays (slopes k1..k3, E_r,
+  // J_r), the replayed schedule and the dz record, each on a 256-byte boundary
+  static size_t static_need(const uppe_handle* h, const FusedState* s, const uppe_step_control* c) {
+    auto al = [](size_t b) { return (b + 255) & ~(size_t)255; };
+    const size_t cs = al(sizeof(C) * (size_t)h->P * s->nwp);
+    const size_t ns = c->n_schedule > 0 ? (size_t)c->n_schedule : 0;
+    const size_t nr = (c->dz_record && c->record_cap > 0) ? (size_t)c->record_cap : 0;
+    return 5 * cs + al(sizeof(double) * ns) + al(sizeof(double) * nr);
+  }
+
+  // advance with the work arrays, schedule and record carved out of the static workspace ws (its device
+  // address) instead of the handle's allocations; nothing is allocated or freed. The slopes are cleared first
+  // (columns the r2k direction never writes must read zero), with a fill kernel.
+  static int advance_static(uppe_handle* h, FusedState* s, const uppe_step_control* c, char* ws, size_t ws_bytes) {
+    const size_t need = static_need(h, s, c);
+    if (need > ws_bytes)
+      throw Error{UPPE_ERR_ARG, "uppe_advance_static: this grid needs " + std::to_string(need) +
+                                    " bytes of workspace; the static workspace holds " + std::to_string(ws_bytes)};
+    auto al = [](size_t b) { return (b + 255) & ~(size_t)255; };
+    const size_t cs = al(sizeof(C) * (size_t)h->P * s->nwp);
+    struct Swap {
+      FusedState* s;
+      uppe_handle* h;
+      void *k1, *k2, *k3, *er, *jr;
+      double *sched, *rec;
+      int sched_cap, rec_cap;
+      bool prof;
+      ~Swap() {
+        s->k1 = k1;
+        s->k2 = k2;
+        s->k3 = k3;
+        s->er = er;
+        s->jr = jr;
+        s->sched = sched;
+        s->rec = rec;
+        s->sched_cap = sched_cap;
+        s->rec_cap = rec_cap;
+        s->rec_active = nullptr;
+        h->prof = prof;
+      }
+    } keep{s, h, s->k1, s->k2, s->k3, s->er, s->jr, s->sched, s->rec, s->sched_cap, s->rec_cap, h->prof};
+    h->prof = false;  // event timing would create events
+    s->k1 = ws;
+    s->k2 = ws + cs;
+    s->k3 = ws + 2 * cs;
+    s->er = ws + 3 * cs;
+    s->jr = ws + 4 * cs;
+    s->sched = reinterpret_cast<double*>(ws + 5 * cs);
+    s->sched_cap = c->n_schedule > 0 ? c->n_schedule : 0;
+    s->rec = reinterpret_cast<double*>(ws + 5 * cs + al(sizeof(double) * (size_t)s->sched_cap));
+    s->rec_cap = (c->dz_record && c->record_cap > 0) ? c->record_cap : 0;
+    const long words = (long)(3 * cs / 8);
+    launchf(h, UPPE_KC_POINT, f_fill_zero, dim3((unsigned)std::min<long>(1024, (words + 255) / 256)), dim3(256), 0,
+            reinterpret_cast<unsigned long long*>(ws), words);
+    return advance(h, s, c, false);
+  }
+
+  static int advance(uppe_handle* h, FusedState* s, const uppe_step_control* c, bool may_time = true) {
     StepState* sh = init_state(h, s, c);
     sh->rec_cap = c->dz_record ? c->record_cap : 0;
     if (sh->rec_cap > s->rec_cap) {
@@ -3019,7 +3082,7 @@ struct Fused {
     double* rec_dev = sh->rec_cap > 0 ? s->rec : nullptr;
     s->rec_active = rec_dev;
     UPPE_HIP(hipMemcpyAsync(s->st, sh, sizeof(StepState), hipMemcpyHostToDevice, h->stream));
-    tune_single<T>(h, s);
+    tune_single<T>(h, s, may_time);
     long long guard = 0;
     int chunk = c->n_sub;
     if (sh->max_steps > 0) chunk = std::min(chunk, sh->max_steps);
@@ -3464,8 +3527,13 @@ static int default_spt(const uppe_handle* h, const FusedState* s, long strips) {
 // the candidates on the single-instance workloads found no plan faster by more than the run-to-run spread, and
 // fixed plans keep every run's launch shapes (and the measured traffic per launch) the same.
 template <typename T>
-void tune_single(uppe_handle* h, FusedState* s) {
+void tune_single(uppe_handle* h, FusedState* s, bool may_time) {
   if (s->cm < 0 || h->d.geometry != UPPE_RADIAL || s->tuned_f1 == s->f1) return;
+  if (!may_time) {
+    // the source shape only; the plans stay untimed until an advance that may time them
+    s->spt = default_spt(h, s, h->P);
+    return;
+  }
   const char* env = getenv("SEARAY_UPPE_TUNE");
   if (env && env[0] == '1') {
     int unused = 0;
@@ -3783,6 +3851,45 @@ void fused_set_field(uppe_handle* h, const double* a_r) {
 
 int fused_advance(uppe_handle* h, const uppe_step_control* c) {
   return is_fp64(h) ? Fused<double>::advance(h, h->fused, c) : Fused<float>::advance(h, h->fused, c);
+}
+
+// ---------------------------------------------------------------------------------------------------
+// Static workspace of uppe_advance_static: the work arrays of an advance that allocates nothing. Each GPU the
+// library runs on holds one copy until the process ends; every call on that GPU shares it (calls that overlap
+// on one GPU race). Host code never passes the array's name to a kernel: its device address comes from
+// hipGetSymbolAddress, looked up once per device with that device current.
+// ---------------------------------------------------------------------------------------------------
+constexpr size_t kStaticWorkspaceBytes = (size_t)48 << 20;
+static __device__ __align__(256) unsigned char scratch[kStaticWorkspaceBytes];
+
+static std::mutex g_scratch_mu;
+static std::vector<char*> g_scratch_ptr;  // per device ordinal; null until looked up
+
+// the device address of this device's workspace (the current device is h->d.device)
+static char* scratch_address(int device) {
+  std::lock_guard<std::mutex> lock(g_scratch_mu);
+  if ((int)g_scratch_ptr.size() <= device) g_scratch_ptr.resize((size_t)device + 1, nullptr);
+  if (!g_scratch_ptr[(size_t)device]) {
+    void* ptr = nullptr;
+    const hipError_t e = hipGetSymbolAddress(&ptr, HIP_SYMBOL(scratch));
+    if (e != hipSuccess || !ptr)
+      throw Error{UPPE_ERR_DEVICE, std::string("static workspace: hipGetSymbolAddress failed: ") +
+                                       hipGetErrorString(e)};
+    g_scratch_ptr[(size_t)device] = static_cast<char*>(ptr);
+  }
+  return g_scratch_ptr[(size_t)device];
+}
+
+size_t fused_static_need(uppe_handle* h, const uppe_step_control* c) {
+  return is_fp64(h) ? Fused<double>::static_need(h, h->fused, c) : Fused<float>::static_need(h, h->fused, c);
+}
+
+size_t fused_static_capacity() { return kStaticWorkspaceBytes; }
+
+int fused_advance_static(uppe_handle* h, const uppe_step_control* c) {
+  char* ws = scratch_address(h->d.device);
+  return is_fp64(h) ? Fused<double>::advance_static(h, h->fused, c, ws, kStaticWorkspaceBytes)
+                    : Fused<float>::advance_static(h, h->fused, c, ws, kStaticWorkspaceBytes);
 }
 
 void fused_finish(uppe_handle* h, double z_total, double* a_r, double* j_r, double* chi_t, double* ne_t) {

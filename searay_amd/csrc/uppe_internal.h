@@ -363,6 +363,11 @@ int fused_advance_checkpointed(uppe_handle* h, const uppe_step_control* c, int e
 void fused_adjoint(uppe_handle* h, double z_total, const double* g_a_r, double* g_a0_r);
 void fused_set_field(uppe_handle* h, const double* a_r);
 int fused_advance(uppe_handle* h, const uppe_step_control* c);
+// uppe_advance_static: advance with its work arrays in the library's static device workspace (no allocation);
+// need = the workspace bytes the call takes, capacity = the workspace size
+size_t fused_static_need(uppe_handle* h, const uppe_step_control* c);
+size_t fused_static_capacity();
+int fused_advance_static(uppe_handle* h, const uppe_step_control* c);
 void fused_finish(uppe_handle* h, double z_total, double* a_r, double* j_r, double* chi_t, double* ne_t);
 void fused_get_state(uppe_handle* h, double* q);
 
