@@ -265,6 +265,37 @@ class TestIonization:
         full = izb.PPT(True, U, 0.53, 0, 0, 0, w, 80).Rate(Es)
         assert np.array_equal(st.Rate(Es), np.concatenate([tun[:2], full[2:]]))
 
+    @pytest.mark.parametrize('l,m', [(0, 0), (1, 0), (1, 1), (2, 1), (2, 2)])
+    @pytest.mark.parametrize('Z', [0.53, 1.0, 2.0])
+    def test_ppt_tunnelling_bound_state_factor_against_adk(self, Z, l, m):
+        '''PPT_Tunneling (photoionization_base.py:314-335) writes the bound-state factor with Gamma functions,
+        |C_n*l*|^2 = 2^(2n*) / (n* G(n*+l*+1) G(n*-l*)); ADK (:271-287) writes the same rate in its asymptotic form,
+        with no Gamma function. They share the power and the exponent; their prefactors differ only by the
+        Stirling approximation of the Gamma functions: for l* = 0 the ratio is exactly
+        (2 pi / n*) (n*/e)^(2 n*) / G(n*)^2 = 1 - 1/(6 n*) + ..., and for l* > 0 ADK's ((n*+l*)/(n*-l*))^(l*+1/2)
+        adds ((n*-l*)/(n*+l*))^(2 l*+1) on top of the Stirling form 2 pi ((n*+l*)/(n*-l*))^(1/2)
+        (n*+l*)^(n*+l*) (n*-l*)^(n*-l*) e^(-2 n*) of G(n*+l*+1) G(n*-l*). This pins PPT's Gamma form, its
+        angular factor (the 2^-m), (F0/2)^-(m+3/2) and the Coulomb factor (2 F0)^(2 n*) independently of PPT's own
+        gamma -> 0 limit, which shares them.'''
+        from scipy.special import gammaln
+        for nstar in (0.8, 1.3, 2.0, 3.7, 8.0, 16.0):
+            for lstar in (0.0, 0.3, min(l, 0.9 * nstar)):
+                U = 0.5 * (Z / nstar) ** 2
+                for avg in (True, False):
+                    adk = izb.ADK(avg, U, Z, lstar, l, m)
+                    ppt = izb.PPT_Tunneling(avg, U, Z, lstar, l, m)
+                    assert ppt.C_pow == pytest.approx(adk.C_pow, rel=0, abs=1e-14)
+                    assert ppt.C_exp == pytest.approx(adk.C_exp, rel=1e-14)
+                    n, L = nstar, lstar
+                    log_st = (np.log(2 * np.pi) + 0.5 * np.log((n + L) / (n - L)) + (n + L) * np.log(n + L) +
+                              (n - L) * np.log(n - L) - 2 * n - gammaln(n + L + 1) - gammaln(n - L))
+                    expect = np.exp(log_st + (2 * L + 1) * np.log((n - L) / (n + L)))
+                    assert ppt.C_pre / adk.C_pre == pytest.approx(expect, rel=1e-12), (nstar, lstar)
+        # the ratio tends to 1 as n* grows (l* = 0): 1 - 1/(6 n*) at leading order
+        r = [izb.PPT_Tunneling(True, 0.5 / n ** 2, 1.0).C_pre / izb.ADK(True, 0.5 / n ** 2, 1.0).C_pre
+             for n in (8.0, 16.0)]
+        assert abs((r[0] - 1) * 8 + 1 / 6) < 3e-3 and abs((r[1] - 1) * 16 + 1 / 6) < 1.5e-3
+
     def test_stitched_ppt_uses_closed_form(self):
         mks = configs.MKS_LENGTH
         helper = input_tools.InputHelper(mks)

@@ -10,11 +10,15 @@ Each check is chosen so that a plausible restatement mistake fails it:
 * the nonlinear current (uppe_kernel.py:139-186) in the time domain: Kerr J = -d/dtau (chi E) and plasma
   0.1 J - dJ/dtau = n_e E, with tau the (reversed) time index, checked by finite differences;
 * the density scan (ionization.cl:127-155) vs its closed form for a constant rate, including the zeroed last
-  sample and the soft time window, and the device rate formulas vs the host rate models.
+  sample and the soft time window, and the device rate formulas vs the host rate models;
+* the UPPE right-hand side (E = i kz A, the Kerr current, S = (i/2) exp(+Im kz z) exp(-i(Re kz + kg) z) J / Re kz;
+  uppe.cl:73-108) vs the closed-form plane-wave solutions of tests/plane_wave.py in BK7: planting E = i w A, a
+  dropped 1/2, 1/w for 1/Re kz, a flipped exponent sign or exp(-Im kz z) in a copy of the oracle fails them.
 '''
 import numpy as np
 import pytest
 
+import plane_wave as pw
 from conftest import rel_l2
 from searay_amd import _abi, configs, grid_tools, ionization, uppe_kernel
 
@@ -447,3 +451,67 @@ class TestRotationalRamanContinuum:
         e1, e2 = self._error(oracle_mod, 129, rot), self._error(oracle_mod, 257, rot)
         assert e2 < 1e-3 and 1.8 < e1 / e2 < 2.2
         assert self._error(oracle_mod, 257, rot, decay_sum=False) > 0.1
+
+
+class TestPlaneWaveRHS:
+    '''The UPPE right-hand side assembled by the oracle (E = i kz A, uppe.cl:93-108; the Kerr current of
+    update_current, uppe_kernel.py:139-186; S = (i/2) exp(+Im kz z) exp(-i(Re kz + kg) z) J / Re kz, uppe.cl:73-91)
+    against the closed-form plane-wave solutions of tests/plane_wave.py, in BK7 (kz differs from w by 50 %):
+    the self-phase of a transparent medium, self-phase with absorption (amplitude and phase), and third-harmonic
+    growth with phase mismatch. Planting a dropped 1/2, 1/w for 1/Re kz, E = i w A, a flipped exponent sign or
+    exp(-Im kz z) in the oracle fails these checks by orders of magnitude beyond their tolerances.'''
+
+    BAND = (6, 11)  # nonlinear band: the fundamental's bin 8 and its neighbours only
+
+    def _spm(self, geometry, absorb, phase, steps):
+        from oracle import oracle
+        kz = pw.bk7_kz(absorb)
+        st, shape, nrm = pw.make(oracle.oracle_factory, geometry, kz, self.BAND)
+        c = pw.spm_rate(kz[pw.M], nrm)
+        L = 400.0
+        u0 = phase / (c * L)                     # nonlinear phase `phase` over L without absorption
+        q0 = np.sqrt(u0) * np.exp(0.3j)
+        q = pw.run(st, pw.field(shape, q0 / nrm), L, steps)
+        st.close()
+        expect = pw.spm_solution(c, kz[pw.M], q0, L)
+        got = q[pw.M, 0, 0]
+        others = np.delete(q.reshape(pw.NW, -1), pw.M, axis=0)
+        return got, expect, q0, others, q
+
+    @pytest.mark.parametrize('geometry', ['cartesian', 'radial'])
+    def test_self_phase_transparent(self, geometry):
+        got, expect, q0, others, q = self._spm(geometry, 0.0, 1.0, 400)
+        assert abs(got - expect) < 1e-11 * abs(q0)
+        assert abs(abs(got) - abs(q0)) < 1e-12 * abs(q0)
+        assert np.angle(got / q0) == pytest.approx(1.0, abs=1e-10)
+        assert np.abs(others).max() < 1e-12 * abs(q0)
+        if geometry == 'radial':
+            # identity basis: every radial cell carries the same plane wave
+            assert np.allclose(q[pw.M], q[pw.M, 0, 0], rtol=1e-14, atol=0)
+
+    @pytest.mark.parametrize('geometry', ['cartesian', 'radial'])
+    def test_self_phase_absorbing(self, geometry):
+        kz1 = pw.bk7_kz(3.8e-3)[pw.M]
+        assert 0.8 < 2 * kz1.imag * 400.0 < 1.2
+        got, expect, q0, others, q = self._spm(geometry, 3.8e-3, 1.0, 400)
+        # |q| falls by the Im kz / Re kz part of kz / Re kz; the phase rate decays as exp(-2 Im kz z)
+        assert 1e-4 < 1 - abs(got) / abs(q0) < 1e-3
+        assert abs(got - expect) < 1e-11 * abs(q0)
+        transparent = np.angle(pw.spm_solution(pw.spm_rate(kz1, 1.0), kz1.real + 0j, 1.0, 400.0))
+        assert abs(np.angle(got / q0) - transparent) > 0.1
+
+    def test_third_harmonic(self):
+        from oracle import oracle
+        kz = pw.bk7_kz()
+        st, shape, nrm = pw.make(oracle.oracle_factory, 'cartesian', kz, (0, pw.NW))
+        c = pw.spm_rate(kz[pw.M], nrm)
+        L = 50.0
+        q0 = np.sqrt(1e-4 / (c * L)) * np.exp(0.3j)   # weak: self-phase 1e-4 rad over L
+        q = pw.run(st, pw.field(shape, q0 / nrm), L, 400)
+        st.close()
+        d = 3 * kz[pw.M] - kz[3 * pw.M]
+        assert 2.0 < abs(d) * L / np.pi < 3.0           # a few coherence lengths: the growth oscillates
+        expect = pw.third_harmonic(kz, q0, 1e-3, nrm, L)
+        got = q[3 * pw.M, 0, 0]
+        assert abs(got - expect) < 1e-3 * abs(expect)
+
