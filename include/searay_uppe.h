@@ -36,7 +36,7 @@ enum { UPPE_RADIAL = 0, UPPE_CARTESIAN = 1 };
 enum {
   UPPE_PREC_FP64_PLAIN = 0, /* one kernel per reference operation, reference layout, FP64, host-driven dz */
   UPPE_PREC_FP64 = 1,       /* fused MI355X kernels in FP64 */
-  UPPE_PREC_FP32 = 2,       /* fused kernels in FP32 (FP64 linear phase, FP64 plasma scan) */
+  UPPE_PREC_FP32 = 2,       /* fused kernels in FP32 (FP64 linear phase; FP32 plasma scan as a tree of sums) */
   UPPE_PREC_BF16X3 = 3      /* as FP32, Hankel contraction as three BF16 MFMA products, FP32 accumulation */
 };
 
@@ -206,9 +206,11 @@ int uppe_get_info(uppe_handle* h, uppe_info* info);
 /* Per-kernel-class timing with HIP events on the handle's stream (for measurement): in the fused paths each
  * dispatch stamps its own start and end events, the plain path records an event pair around each launch.
  * enable != 0 starts recording and clears totals; uppe_profile_read returns total milliseconds and launch
- * count of a class. */
+ * count of a class. The radial transforms are two classes: UPPE_KC_HANKEL for the k->r direction (and the
+ * one-off transforms of set_field and finish), UPPE_KC_HANKEL_R2K for the r->k direction with the RK stage
+ * epilogue. */
 enum { UPPE_KC_HANKEL = 0, UPPE_KC_SOURCE = 1, UPPE_KC_POINT = 2, UPPE_KC_REDUCE = 3, UPPE_KC_TRANSVERSE_FFT = 4,
-       UPPE_KC_COUNT = 5 };
+       UPPE_KC_HANKEL_R2K = 5, UPPE_KC_COUNT = 6 };
 int uppe_profile_enable(uppe_handle* h, int enable);
 int uppe_profile_read(uppe_handle* h, int kernel_class, double* total_ms, long long* launches);
 

@@ -17,9 +17,9 @@ PRECISIONS = {'fp64-plain': PREC_FP64_PLAIN, 'fp64': PREC_FP64, 'fp32': PREC_FP3
 MODEL_UPPE = 0
 MODEL_PARAXIAL = 1
 
-KC_HANKEL, KC_SOURCE, KC_POINT, KC_REDUCE, KC_TRANSVERSE_FFT = range(5)
+KC_HANKEL, KC_SOURCE, KC_POINT, KC_REDUCE, KC_TRANSVERSE_FFT, KC_HANKEL_R2K = range(6)
 KERNEL_CLASSES = {'hankel': KC_HANKEL, 'source': KC_SOURCE, 'point': KC_POINT, 'reduce': KC_REDUCE,
-                  'transverse_fft': KC_TRANSVERSE_FFT}
+                  'transverse_fft': KC_TRANSVERSE_FFT, 'hankel_r2k': KC_HANKEL_R2K}
 
 _dp = ct.POINTER(ct.c_double)
 

@@ -45,12 +45,15 @@ class SpectralPhaseProblem:
         self.rs = uppe_kernel.RunSetup(vol)
         if self.rs.diagnostic_steps != 1:
             raise ValueError('the objective is defined on one diagonal interval (wave grid with two planes)')
+        if not self.rs.medium_is_uniform(0):
+            raise ValueError('the density varies along z: the adjoint path loads the medium once')
         self.every = int(checkpoint_every)
         self.st = self.rs.make_stepper(stepper_factory or uppe_kernel.hip_stepper_factory(precision))
         self.basis = legendre_basis(self.rs.w, w0, sigma, n_phase)
         self.zi, self.zf, self.L = self.rs.plane(0)
         self._a0 = None
         self._g = None
+        self._token = None  # identifies the forward() whose recorded run the device holds
         if target is None:
             s0 = self.spectrum(np.zeros(n_phase))
             rng = np.random.default_rng(target_seed)
@@ -100,6 +103,7 @@ class SpectralPhaseProblem:
         a0, a_r = self._run(c, True)
         d = self._observable(a_r) - self.target
         self._a0, self._g = a0, self._cotangent(a_r, d)
+        self._token = object()
         return float(np.sum(d ** 2))
 
     def backward(self):
@@ -127,11 +131,19 @@ def spectral_phase_loss_function():
         def forward(ctx, c, problem):
             ctx.problem = problem
             ctx.dtype = c.dtype
-            return torch.tensor(problem.forward(c.detach().cpu().double().numpy()), dtype=c.dtype)
+            ctx.device = c.device
+            ctx.c = c.detach().cpu().double().numpy().copy()
+            loss = problem.forward(ctx.c)
+            ctx.token = problem._token
+            return torch.tensor(loss, dtype=c.dtype, device=c.device)
 
         @staticmethod
         def backward(ctx, grad_out):
-            g = torch.as_tensor(ctx.problem.backward(), dtype=ctx.dtype)
+            p = ctx.problem
+            if p._g is None or p._token is not ctx.token:
+                # another forward ran on this problem since (its recorded run replaced this one's): record again
+                p.forward(ctx.c)
+            g = torch.as_tensor(p.backward(), dtype=ctx.dtype, device=ctx.device)
             return grad_out * g, None
 
     return SpectralPhaseLoss

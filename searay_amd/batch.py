@@ -79,7 +79,12 @@ class Batch:
         return c
 
     def advance(self, max_steps=0, plane=0):
-        '''Step every member through its interval `plane` (or max_steps steps); returns the step counts.'''
+        '''Step every member through its interval `plane` (or max_steps steps); returns the step counts. The members
+        keep the medium of their first subcycle, so an instance whose density changes along z is refused.'''
+        for i, rs in enumerate(self.setups):
+            if not rs.medium_is_uniform(plane):
+                raise ValueError('instance %d: the density varies along z in interval %d; a batch loads the medium '
+                                 'once (step such an instance with track)' % (i, plane))
         c = self.controls(max_steps, plane)
         steps = (ct.c_int * len(self.setups))()
         self._check(self.lib.uppe_batch_advance(self.b, c, steps), 'advance')

@@ -1,5 +1,6 @@
 // uppe_api.hip -- extern "C" entry points of libsearay_uppe.so (declared in include/searay_uppe.h).
 #include <cstring>
+#include <exception>
 
 #include "uppe_internal.h"
 
@@ -26,14 +27,21 @@ void prof_push(uppe_handle* h, int kc, hipEvent_t a, hipEvent_t b) {
 
 ProfScope::ProfScope(uppe_handle* h_, int kc_) : h(h_), kc(kc_) {
   if (!h->prof) return;
+  if (h->prof_pending.size() > 4096) prof_collect(h);  // the destructor queues pairs without collecting
   prof_take(h, &a, &b);
   UPPE_HIP(hipEventRecord(a, h->stream));
 }
 
-ProfScope::~ProfScope() noexcept(false) {
+ProfScope::~ProfScope() noexcept {
   if (!a) return;
-  UPPE_HIP(hipEventRecord(b, h->stream));
-  prof_push(h, kc, a, b);
+  // never throws: a launch that failed while this scope was open unwinds through here, and a failed record
+  // (after a sticky device error) only loses this timing pair
+  if (std::uncaught_exceptions() > 0 || hipEventRecord(b, h->stream) != hipSuccess) {
+    h->prof_free.push_back(a);
+    h->prof_free.push_back(b);
+    return;
+  }
+  h->prof_pending.push_back(ProfRecord{kc, a, b});
 }
 
 void prof_collect(uppe_handle* h) {

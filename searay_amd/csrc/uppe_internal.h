@@ -330,7 +330,7 @@ struct ProfScope {
   int kc;
   hipEvent_t a = nullptr, b = nullptr;
   ProfScope(uppe_handle* h_, int kc_);
-  ~ProfScope() noexcept(false);
+  ~ProfScope() noexcept;
 };
 
 void prof_collect(uppe_handle* h);  // synchronise and fold finished events into totals

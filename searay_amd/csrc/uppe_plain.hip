@@ -673,7 +673,7 @@ static void kspacex(uppe_handle* h, double* a) {
          h->nm);
   UPPE_HIP(hipMemcpyAsync(s->hscratch, a, sizeof(double) * 2 * npt, hipMemcpyDeviceToDevice, h->stream));
   UPPE_HIP(hipMemsetAsync(a, 0, sizeof(double) * 2 * npt, h->stream));
-  launch(h, UPPE_KC_HANKEL, plain_radial_transform, grid_for((long)h->nw * h->nk * h->nm), kBlock, 0,
+  launch(h, UPPE_KC_HANKEL_R2K, plain_radial_transform, grid_for((long)h->nw * h->nk * h->nm), kBlock, 0,
          (const double*)s->H, (const double*)s->hscratch, a, h->nw, h->nk, h->n1, h->nm);
   launch(h, UPPE_KC_POINT, plain_root_volume, grid_for(npt), kBlock, 0, a, (const double*)s->lam, 1, npt, h->n1,
          h->nm);

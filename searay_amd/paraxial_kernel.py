@@ -121,7 +121,8 @@ def _propagate(ctx, ctool, a, mat, ionizer, rotator, subcycles, zi, zf, record=N
         n = 1
         while sub + n < subcycles and np.array_equal(dens[sub + n], dens[sub]):
             n += 1
-        key = dens[sub].tobytes()
+        # everything set_paraxial_medium uploads: the density, the source time step, chi3 and the rate model
+        key = (dens[sub].tobytes(), float(dt), float(mat.redchi3), repr(sorted(ion.items())))
         if key != ctx.medium_key:
             mat.UpdateMaterial(zi + sub * L + 0.5 * L, dens[sub])
             st.set_paraxial_medium(w, mat.kz, mat.k0, mat.ngas, mat.redchi3, dt, ion)

@@ -136,7 +136,8 @@ def _propagate(ctx, ctool, vg, a, mat, ionizer, NL_band, subcycles, zi, zf, dzmi
         n = 1
         while sub + n < subcycles and np.array_equal(dens[sub + n], dens[sub]):
             n += 1
-        key = dens[sub].tobytes()
+        # everything set_medium uploads: the density, the filters, the source time step, chi3 and the rate model
+        key = (dens[sub].tobytes(), nl.tobytes(), cond.tobytes(), float(dt), float(mat.redchi3), repr(sorted(ion.items())))
         if key != ctx.medium_key:
             mat.UpdateMaterial(zi + sub * L + 0.5 * L, dens[sub])
             st.set_medium(w, mat.kz, mat.kg, mat.ngas, nl, cond, mat.redchi3, dt, ion)
@@ -227,6 +228,15 @@ class RunSetup:
         zi = self.diagnostic_walls[k]
         zf = self.diagnostic_walls[k + 1]
         return zi, zf, (zf - zi) / self.subcycles
+
+    def medium_is_uniform(self, plane):
+        '''Whether every subcycle of diagnostic interval `plane` sees the density make_stepper loads (the midpoint of
+        plane 0's first subcycle). The batched and adjoint paths load the medium once, where the reference updates
+        it for every subcycle (uppe_kernel.py:300).'''
+        zi0, _, L0 = self.plane(0)
+        d0 = self.mat.density(zi0 + 0.5 * L0)
+        zi, _, L = self.plane(plane)
+        return all(np.array_equal(self.mat.density(zi + sub * L + 0.5 * L), d0) for sub in range(self.subcycles))
 
     def make_stepper(self, stepper_factory):
         '''A stepper with the transverse basis and the medium at the first subcycle loaded.'''

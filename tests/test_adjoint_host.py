@@ -77,3 +77,34 @@ def test_batched_workload_draws():
     for v, d in zip(vols, drawn):
         assert v['wave grid'][:3] == (1025, d[0], 1) and v['radial modes'] == d[0]
         assert v['minimum step'] == v['size'][2] / v['subcycles']
+
+
+def _ramped(vol):
+    '''The same workload with a gas density that rises along z.'''
+    class Ramp:
+        def GetRelDensity(self, xp):
+            return 1.0 + 0.1 * xp[..., 3] / vol['size'][2] + 0.0 * xp[..., 1]
+    vol = dict(vol)
+    vol['object'] = Ramp()
+    return vol
+
+
+def test_uniform_medium_detection():
+    from searay_amd import uppe_kernel
+    vol = configs.radial_kerr_air(scale=16, steps=4)
+    assert uppe_kernel.RunSetup(vol).medium_is_uniform(0)
+    assert not uppe_kernel.RunSetup(_ramped(vol)).medium_is_uniform(0)
+
+
+def test_batch_and_adjoint_refuse_z_varying_density():
+    '''The batch and the adjoint load each instance's medium once (at the first subcycle's midpoint); an instance
+    whose density changes along z is refused before any device work, instead of being stepped in the wrong medium.'''
+    from searay_amd import batch, uppe_kernel
+    vol = _ramped(configs.radial_kerr_air(scale=16, steps=4))
+
+    class Stub:
+        setups = [uppe_kernel.RunSetup(configs.radial_kerr_air(scale=16, steps=4)), uppe_kernel.RunSetup(vol)]
+    with pytest.raises(ValueError, match='instance 1: the density varies along z'):
+        batch.Batch.advance(Stub())
+    with pytest.raises(ValueError, match='density varies along z'):
+        adjoint.SpectralPhaseProblem(vol, configs.kerr_pulse_sigma(), stepper_factory=lambda *a, **k: None)
