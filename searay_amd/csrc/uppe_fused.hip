@@ -107,7 +107,7 @@ struct FusedState {
   void *tw_x = nullptr, *tw_y = nullptr;
   int twlen_x = 0, twlen_y = 0;
   void* rot_tab = nullptr;  // rotator rows (device, RotRow<T> per state j >= 2), when the handle has a rotator
-  // adjoint (Kerr-only radial MFMA path): fragments of Hf^T (the shape of Hb) and Hb^T (the shape of Hf)
+  // adjoint (radial MFMA path): fragments of Hf^T (the shape of Hb) and Hb^T (the shape of Hf)
   void *fab_hi = nullptr, *fab_lo = nullptr, *faf_hi = nullptr, *faf_lo = nullptr;
   // checkpointed forward record: q, the step state and qmax every ck_every steps; (zoff, z, dz) of every step;
   // the stage fields E_r of one segment (4 per step); cotangent work arrays
@@ -637,24 +637,34 @@ __device__ inline int xcd_tile(int b, int G) {
 // B-slab elements each thread loads per batch (all loads of a batch are in flight together)
 constexpr int hankel_batch(int cm, int rt) { return (cm == CM_F64 || rt == 2) ? 2 : 4; }
 
+// where the r2k epilogue's reductions go: the running minimum of the step estimate, the NaN flag and the
+// "step has run" flag (null: not kept)
+struct StepOut {
+  unsigned long long* dzraw;
+  int* nanflag;
+  int* pend;
+};
+
+// One output tile of the contraction (see f_hankel): column tile `tile` of `ntile`, row block `rowblk`,
+// instance `zi` of a batched launch.
 template <int CM, int PRO, int EPI, int BN, int RT>
-__global__ __launch_bounds__(BN > 16 ? 512 : 1024) void f_hankel(
+__device__ inline void hankel_tile(
     const typename MfmaTraits<CM>::frag_t* __restrict__ Ahi, const typename MfmaTraits<CM>::frag_t* __restrict__ Alo,
     const cplx<typename MfmaTraits<CM>::T>* __restrict__ B, cplx<typename MfmaTraits<CM>::T>* __restrict__ out,
     int M, int K, int Kp, int rw, int nwp, int f0, int fend, const double* __restrict__ phs,
     const cplx<typename MfmaTraits<CM>::T>* __restrict__ kzc, const typename MfmaTraits<CM>::T* __restrict__ cond,
     cplx<typename MfmaTraits<CM>::T>* __restrict__ q, const cplx<typename MfmaTraits<CM>::T>* __restrict__ kprev,
     cplx<typename MfmaTraits<CM>::T>* __restrict__ k1, cplx<typename MfmaTraits<CM>::T>* __restrict__ k2,
-    cplx<typename MfmaTraits<CM>::T>* __restrict__ k3, StepState* __restrict__ st,
-    unsigned long long* __restrict__ qmax, unsigned long long* __restrict__ qmax_next, int stage, double zfix,
-    double dzfix, double dzprev, int wide, long zs) {
+    cplx<typename MfmaTraits<CM>::T>* __restrict__ k3, const StepState* st, StepOut so,
+    const unsigned long long* __restrict__ qmax, unsigned long long* __restrict__ qmax_next, int stage, double zfix,
+    double dzfix, double dzprev, int wide, long zs, int tile, int ntile, int rowblk, int zi) {
   using T = typename MfmaTraits<CM>::T;
   using C = cplx<T>;
   using acc_t = typename MfmaTraits<CM>::acc_t;
   if (zs) {
-    // batched launch: instance blockIdx.z of equally shaped instances whose arrays lie zs elements apart
+    // batched launch: instance zi of equally shaped instances whose arrays lie zs elements apart
     // (state, tables), one fragment set apart (operator), one step state and three maxima apart
-    const long z = blockIdx.z, zo = z * zs;
+    const long z = zi, zo = z * zs;
     const long fs = (long)((M + 15) / 16) * (CM == CM_BF16X3 ? Kp / 32 : Kp / 16) * 64;
     Ahi = zshift(Ahi, z * fs);
     Alo = zshift(Alo, z * fs);
@@ -670,6 +680,9 @@ __global__ __launch_bounds__(BN > 16 ? 512 : 1024) void f_hankel(
     st = zshift(st, z);
     qmax = zshift(qmax, 3 * z);
     qmax_next = zshift(qmax_next, 3 * z);
+    if (so.dzraw) so.dzraw = &const_cast<StepState*>(st)->dzraw;
+    if (so.nanflag) so.nanflag = &const_cast<StepState*>(st)->nanflag;
+    if (so.pend) so.pend = &const_cast<StepState*>(st)->pend;
   }
   // no early exit for a finished run: the contraction only writes scratch (E_r) or stops before its
   // epilogue, so its loads need not wait for the step state
@@ -678,7 +691,7 @@ __global__ __launch_bounds__(BN > 16 ? 512 : 1024) void f_hankel(
   const int nth = blockDim.x, nwv = nth >> 6;
   const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
   const int bm = nwv * rw;
-  const int row0 = blockIdx.y * bm;
+  const int row0 = rowblk * bm;
   // column tiles of HC complex columns (one 16-column MFMA tile); the last tile of the grid also takes the
   // remainder (fewer than HC columns), so that the grid never needs a workgroup for a sliver. Wide tiles take
   // HC << wide columns each (wide = 1: two MFMA column tiles; wide = 2 with BN = 32: four), the last one what is
@@ -686,10 +699,9 @@ __global__ __launch_bounds__(BN > 16 ? 512 : 1024) void f_hankel(
   // nmt = MFMA column tiles holding columns below fend.
   constexpr int HC = 8;
   constexpr int BW = 2 * BN;       // LDS row width in real columns: BN / 8 MFMA column tiles
-  const int tile = xcd_tile(blockIdx.x, gridDim.x);
   const int tcw = HC << wide;      // complex columns per tile
   const int fc0 = f0 + tile * tcw; // first complex column of this tile
-  const int hcw = wide ? tcw : ((tile == (int)gridDim.x - 1 && fend - fc0 > HC) ? 2 * HC : HC);  // staged columns
+  const int hcw = wide ? tcw : ((tile == ntile - 1 && fend - fc0 > HC) ? 2 * HC : HC);  // staged columns
   const int nmt = min(BW / 16, (min(fend - fc0, hcw) + HC - 1) / HC);
   double zph = zfix, cu = 0.0, dz = dzfix;
   if (st) {
@@ -940,7 +952,7 @@ __global__ __launch_bounds__(BN > 16 ? 512 : 1024) void f_hankel(
                        k3, st, amin, loc, nan);
   }
   if (EPI == EPI_NONE) return;
-  if (stage == 1 && nan) atomicOr(&st->nanflag, 1);
+  if (stage == 1 && nan) atomicOr(so.nanflag, 1);
   if (stage == 2 || stage == 3) return;
   if (stage == 1 && st->fixed) return;
   const bool is_min = stage == 1;
@@ -955,13 +967,30 @@ __global__ __launch_bounds__(BN > 16 ? 512 : 1024) void f_hankel(
     double v = red[0];
     for (int i = 1; i < nwv; i++) v = is_min ? fmin(v, red[i]) : fmax(v, red[i]);
     if (is_min) {
-      atomicMin(&st->dzraw, dbits(v));
+      atomicMin(so.dzraw, dbits(v));
     } else {
       atomicMax(qmax_next, dbits(v));
       // the step has run; the next step's stage-1 source launch commits its end
-      if (blockIdx.x == 0 && blockIdx.y == 0) st->pend = 1;
+      if (so.pend && tile == 0 && rowblk == 0) *so.pend = 1;
     }
   }
+}
+
+template <int CM, int PRO, int EPI, int BN, int RT>
+__global__ __launch_bounds__(BN > 16 ? 512 : 1024) void f_hankel(
+    const typename MfmaTraits<CM>::frag_t* __restrict__ Ahi, const typename MfmaTraits<CM>::frag_t* __restrict__ Alo,
+    const cplx<typename MfmaTraits<CM>::T>* __restrict__ B, cplx<typename MfmaTraits<CM>::T>* __restrict__ out,
+    int M, int K, int Kp, int rw, int nwp, int f0, int fend, const double* __restrict__ phs,
+    const cplx<typename MfmaTraits<CM>::T>* __restrict__ kzc, const typename MfmaTraits<CM>::T* __restrict__ cond,
+    cplx<typename MfmaTraits<CM>::T>* __restrict__ q, const cplx<typename MfmaTraits<CM>::T>* __restrict__ kprev,
+    cplx<typename MfmaTraits<CM>::T>* __restrict__ k1, cplx<typename MfmaTraits<CM>::T>* __restrict__ k2,
+    cplx<typename MfmaTraits<CM>::T>* __restrict__ k3, StepState* __restrict__ st,
+    unsigned long long* __restrict__ qmax, unsigned long long* __restrict__ qmax_next, int stage, double zfix,
+    double dzfix, double dzprev, int wide, long zs) {
+  const StepOut so{st ? &st->dzraw : nullptr, st ? &st->nanflag : nullptr, st ? &st->pend : nullptr};
+  hankel_tile<CM, PRO, EPI, BN, RT>(Ahi, Alo, B, out, M, K, Kp, rw, nwp, f0, fend, phs, kzc, cond, q, kprev, k1,
+                                           k2, k3, st, so, qmax, qmax_next, stage, zfix, dzfix, dzprev, wide, zs,
+                                           xcd_tile(blockIdx.x, gridDim.x), gridDim.x, blockIdx.y, blockIdx.z);
 }
 
 // ---------------------------------------------------------------------------------------------------
@@ -1131,6 +1160,108 @@ __global__ __launch_bounds__(1024) void f_tfft(
       // the step has run; the next step's stage-1 source launch commits its end
       if (blockIdx.x == 0 && blockIdx.y == 0) st->pend = 1;
     }
+  }
+}
+
+// ---------------------------------------------------------------------------------------------------
+// strip transforms on one wavefront (the wave-per-strip source kernel) and the four-step block transform
+// ---------------------------------------------------------------------------------------------------
+__device__ inline void wave_sync() {
+  __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+  __builtin_amdgcn_wave_barrier();
+  __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+}
+
+// twiddles of the radix-8 passes of an n = 2^LG point transform that lane `lane` applies: pass q (span
+// Ns = 2^(LG mod 3) 8^q), butterfly b (j = lane + 64 b), factor r = 1..7, from the tables of lds_fft_r8
+template <int LG>
+struct WaveTw {
+  static constexpr int NP = LG / 3;                               // radix-8 passes
+  static constexpr int NB = (1 << LG) / 8;                        // butterflies per pass
+  static constexpr int BPL = NB >= 64 ? NB / 64 : 1;              // per lane
+  cplx<float> t[NP > 0 ? NP : 1][BPL][7];
+};
+
+template <int LG>
+__device__ inline void wave_tw_load(WaveTw<LG>& W, const cplx<float>* __restrict__ tab, int lane) {
+  using WT = WaveTw<LG>;
+  int off = 0;
+#pragma unroll
+  for (int q = 0; q < WT::NP; q++) {
+    const int Ns = (1 << (LG % 3)) << (3 * q);
+#pragma unroll
+    for (int b = 0; b < WT::BPL; b++) {
+      const int j = lane + 64 * b, k = j & (Ns - 1);
+#pragma unroll
+      for (int r = 1; r < 8; r++) W.t[q][b][r - 1] = j < WT::NB ? tab[off + (r - 1) * Ns + k] : cmk(1.0f, 0.0f);
+    }
+    off += 7 * Ns;
+  }
+}
+
+// STRIDE > 1: the transform of the strided view whose element e is buf[lds_swz(STRIDE e + off)]
+template <int LG, int STRIDE = 1>
+__device__ inline void wave_fft_r8(cplx<float>* buf, const WaveTw<LG>& W, int sign, int lane, int off = 0) {
+  using WT = WaveTw<LG>;
+  constexpr int n = 1 << LG, r0 = LG % 3;
+  auto at = [&](int e) -> cplx<float>& { return buf[lds_swz<float>(STRIDE * e + off)]; };
+  if constexpr (r0 > 0) {
+    constexpr int R = 1 << r0, nb = n >> r0, BP = (nb + 63) / 64;
+    cplx<float> v[BP][4];
+#pragma unroll
+    for (int b = 0; b < BP; b++) {
+      const int j = lane + 64 * b;
+#pragma unroll
+      for (int r = 0; r < R; r++)
+        if (j < nb) v[b][r] = at(j + r * nb);
+    }
+    wave_sync();
+#pragma unroll
+    for (int b = 0; b < BP; b++) {
+      const int j = lane + 64 * b;
+      if (j >= nb) continue;
+      if constexpr (R == 4) {
+        dft4(v[b][0], v[b][1], v[b][2], v[b][3], sign);
+      } else {
+        const cplx<float> s0 = cadd(v[b][0], v[b][1]), s1 = csub(v[b][0], v[b][1]);
+        v[b][0] = s0;
+        v[b][1] = s1;
+      }
+#pragma unroll
+      for (int r = 0; r < R; r++) at(j * R + r) = v[b][r];
+    }
+    wave_sync();
+  }
+#pragma unroll
+  for (int q = 0; q < WT::NP; q++) {
+    const int Ns = (1 << r0) << (3 * q);
+    cplx<float> v[WT::BPL][8];
+#pragma unroll
+    for (int b = 0; b < WT::BPL; b++) {
+      const int j = lane + 64 * b;
+      if (j < WT::NB)
+#pragma unroll
+        for (int r = 0; r < 8; r++) v[b][r] = at(j + r * WT::NB);
+    }
+    wave_sync();
+#pragma unroll
+    for (int b = 0; b < WT::BPL; b++) {
+      const int j = lane + 64 * b;
+      if (j >= WT::NB) continue;
+      const int k = j & (Ns - 1);
+      cplx<float> y[8];
+#pragma unroll
+      for (int r = 1; r < 8; r++) {
+        cplx<float> w = W.t[q][b][r - 1];
+        if (sign > 0) w.im = -w.im;
+        v[b][r] = cmul(v[b][r], w);
+      }
+      dft8(v[b], y, sign);
+      const int o = (j - k) * 8 + k;
+#pragma unroll
+      for (int r = 0; r < 8; r++) at(o + r * Ns) = y[r];
+    }
+    wave_sync();
   }
 }
 
@@ -1550,8 +1681,8 @@ __global__ __launch_bounds__(1024) void f_source(const cplx<T>* __restrict__ er,
       const int n = a0 + j;
       const R integral = (later + R(0.5) * r[j]) * dtr;
       later += r[j];
-      // n_e = n_g (1 - exp(-integral)); the FP32 path evaluates the depletion as -expm1 (accurate also
-      // where the integral is tiny), the FP64 path as the reference does
+      // n_e = n_g (1 - exp(-integral)); the FP32 path evaluates the depletion with one_minus_exp_neg (accurate
+      // also where the integral is tiny), the FP64 path as the reference does
       R x;
       if constexpr (sizeof(T) == 4)
         x = ngp * one_minus_exp_neg(integral);
@@ -1633,6 +1764,175 @@ __global__ __launch_bounds__(1024) void f_source(const cplx<T>* __restrict__ er,
     }
     jr[p * nwp + k] = cscale(out, fv[i]);
   }
+}
+
+// ---------------------------------------------------------------------------------------------------
+// Wave-per-strip source (FP32 stepping, Nt = 2^LG, 256 <= Nt <= 1024, no rotator, no diagnostics)
+//
+// One wavefront owns one strip and a private LDS buffer; a workgroup holds WPB such waves and they share
+// nothing, so no workgroup barrier is ever taken. The strip transforms run the Stockham passes of lds_fft_r8
+// with each lane taking its share of the butterflies, ordered by wavefront-scope fences (the LDS processes
+// one wave's instructions in order). The twiddles of both transforms are loaded into registers up front, with
+// the strip, so that no pass waits on a global load. Same arithmetic as f_source up to the FFT summation
+// order; the depletion scan is one DPP wave scan.
+// ---------------------------------------------------------------------------------------------------
+// one strip of the source on one wave: E_r spectrum X (Nw = Nt/2 + 1 bins) to the current spectrum J
+// (uppe_kernel.py:139-186), as f_source
+template <int LG>
+__device__ inline void source_strip_wave(const cplx<float>* __restrict__ X, cplx<float>* __restrict__ J,
+                                         cplx<float>* Y, const cplx<float>* __restrict__ tw,
+                                         const float* __restrict__ w, const float* __restrict__ nlf, float ngp,
+                                         double redchi3, double dt, const uppe_ionization& ion, double log_pre,
+                                         int lane) {
+  constexpr int N = 1 << LG, M = N / 2, SPT = N / 64, KPL = M / 64;
+  // every global load up front: the strip, the frequencies and filter of the bins this lane writes, and the
+  // twiddles of both transforms
+  cplx<float> xk[KPL], xm[KPL];
+  float wv[KPL + 1], fv[KPL + 1];
+#pragma unroll
+  for (int i = 0; i < KPL; i++) {
+    const int k = lane + 64 * i;
+    xk[i] = X[k];
+    xm[i] = X[M - k];
+  }
+#pragma unroll
+  for (int i = 0; i <= KPL; i++) {
+    const int k = min(lane + 64 * i, M);
+    wv[i] = w[k];
+    fv[i] = nlf[k];
+  }
+  WaveTw<LG - 1> twh;
+  WaveTw<LG> twf;
+  wave_tw_load<LG - 1>(twh, tw + r8_table_len(LG), lane);
+  wave_tw_load<LG>(twf, tw, lane);
+
+  // 1. irfft: the half-length complex transform with the standard split (fft.cl:271-278 semantics)
+#pragma unroll
+  for (int i = 0; i < KPL; i++) {
+    const int k = lane + 64 * i;
+    cplx<float> xa = xk[i], xb = xm[i];
+    if (k == 0) {
+      xa.im = 0.0f;
+      xb.im = 0.0f;
+    }
+    const cplx<float> ze = cscale(cadd(xa, cconj(xb)), 0.5f);
+    const cplx<float> zo = cmul(cscale(csub(xa, cconj(xb)), 0.5f), twiddle_inv<float>(k, N));
+    Y[lds_swz<float>(k)] = cmk(ze.re - zo.im, ze.im + zo.re);
+  }
+  wave_sync();
+  wave_fft_r8<LG - 1>(Y, twh, +1, lane);
+  // lane l owns samples [a0, a0 + SPT), a0 = (63 - l) SPT: the depletion integral runs from the last sample
+  // backward, a prefix sum in lane order
+  const int a0 = (63 - lane) * SPT;
+  float e[SPT];
+  lds_read_reals<float, SPT>(Y, a0, e);
+  const float invm = 1.0f / (float)M;
+#pragma unroll
+  for (int j = 0; j < SPT; j++) e[j] *= invm;
+  wave_sync();
+
+  // 2. rate, backward trapezoid with depletion (ionization.cl:127-155), soft window (fft.cl:331-355)
+  float ne[SPT];
+  if (ion.kind != UPPE_ION_NONE) {
+    const float dtr = (float)dt;
+    float r[SPT];
+    float tot = 0.0f;
+#pragma unroll
+    for (int j = 0; j < SPT; j++) {
+      const int n = a0 + j;
+      const float v = ion_rate_f32(fabsf(e[j]), ion, log_pre);
+      r[j] = (n >= N - 1) ? 0.0f : v;  // last sample zeroed (ionization.cl:147)
+      tot += r[j];
+    }
+    float later = wave_excl_prefix(tot);
+#pragma unroll
+    for (int j = SPT - 1; j >= 0; j--) {
+      const int n = a0 + j;
+      const float integral = (later + 0.5f * r[j]) * dtr;
+      later += r[j];
+      float x = ngp * one_minus_exp_neg(integral);
+      if (n < 4 || n > N - 5) x = 0.0f;
+      if (n >= 4 && n < 68) {
+        const float qq = (float)(n - 4) / 64.0f;
+        x *= 10.0f * qq * qq * qq - 15.0f * qq * qq * qq * qq + 6.0f * qq * qq * qq * qq * qq;
+      }
+      if (n <= N - 5 && n > N - 69) {
+        const float qq = (float)(N - 5 - n) / 64.0f;
+        x *= 10.0f * qq * qq * qq - 15.0f * qq * qq * qq * qq + 6.0f * qq * qq * qq * qq * qq;
+      }
+      ne[j] = x;
+    }
+  } else {
+#pragma unroll
+    for (int j = 0; j < SPT; j++) ne[j] = 0.0f;
+  }
+
+  // 3. Kerr chi = ng chi3/nref E^2 (uppe.cl:132-151), P = chi E, J_p = ne E, packed as P + i J_p
+  const float kc = (float)((double)ngp * redchi3);
+  float yrun[2 * SPT];
+#pragma unroll
+  for (int j = 0; j < SPT; j++) {
+    const float chi = kc * e[j] * e[j];
+    yrun[2 * j] = chi * e[j];
+    yrun[2 * j + 1] = ne[j] * e[j];
+  }
+  lds_write_reals<float, 2 * SPT>(Y, 2 * a0, yrun);
+  wave_sync();
+  wave_fft_r8<LG>(Y, twf, -1, lane);
+
+  // 4. J = F_NL [ rfft(J_p)/(0.1 - i w) + (-i w) rfft(P) ] (fft.cl:282-329, uppe_kernel.py:170-186)
+#pragma unroll
+  for (int i = 0; i <= KPL; i++) {
+    const int k = lane + 64 * i;
+    if (k > M) continue;
+    const cplx<float> yk = Y[lds_swz<float>(k)], ym = Y[lds_swz<float>((N - k) & (N - 1))];
+    const cplx<float> pk = cscale(cadd(yk, cconj(ym)), 0.5f);
+    const cplx<float> d = csub(yk, cconj(ym));
+    const cplx<float> jp = cmk(0.5f * d.im, -0.5f * d.re);
+    const float wk = wv[i];
+    cplx<float> out;
+    if (ion.kind != UPPE_ION_NONE) {
+      const float cr = 0.1f, ci = -wk;
+      const float rd = __builtin_amdgcn_rcpf(cr * cr + ci * ci);
+      out = cmk((jp.re * cr + jp.im * ci) * rd, (jp.im * cr - jp.re * ci) * rd);
+      out = cadd(out, cmk(wk * pk.im, -wk * pk.re));
+    } else {
+      out = cmk(wk * pk.im, -wk * pk.re);
+    }
+    J[k] = cscale(out, fv[i]);
+  }
+}
+
+template <int LG, int WPB>
+__global__ __launch_bounds__(64 * WPB) void f_source_w(const cplx<float>* __restrict__ er, cplx<float>* __restrict__ jr,
+                                                       const cplx<float>* __restrict__ tw, const float* __restrict__ w,
+                                                       const float* __restrict__ nlf, const double* __restrict__ ng,
+                                                       int nstrips, int nwp, double redchi3, double dt,
+                                                       uppe_ionization ion, double log_pre,
+                                                       const SrcInst* __restrict__ inst, long zs, StepState* commit,
+                                                       unsigned long long* cq, double* rec, double* zrec,
+                                                       int zrec_cap) {
+  // the stage-1 launch commits the previous step (nothing else of this launch reads the step state); a batched
+  // launch takes instance blockIdx.y, whose strips lie zs elements apart, with its own medium scalars
+  if (commit && blockIdx.x == 0 && threadIdx.x == 0) {
+    const int z = inst ? blockIdx.y : 0;
+    step_commit(commit + z, cq + 3 * z, inst ? nullptr : rec, inst ? nullptr : zrec, zrec_cap);
+  }
+  __shared__ cplx<float> ybuf[WPB][1 << LG];
+  const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
+  const long p = (long)blockIdx.x * WPB + wv;
+  if (p >= nstrips) return;  // no workgroup barrier follows: an idle wave may leave
+  if (inst) {
+    const int z = blockIdx.y;
+    er += z * zs;
+    jr += z * zs;
+    ng += (long)z * nstrips;
+    redchi3 = inst[z].redchi3;
+    log_pre = inst[z].log_pre;
+    ion = inst[z].ion;
+  }
+  source_strip_wave<LG>(er + p * nwp, jr + p * nwp, ybuf[wv], tw, w, nlf, (float)ng[p], redchi3, dt, ion, log_pre,
+                        lane);
 }
 
 // Paraxial envelope source (paraxial_kernel.py:140-184): one workgroup per strip of N = Nw complex samples,
@@ -2292,6 +2592,17 @@ constexpr size_t kMaxLds = 160 * 1024;  // LDS of one CU (gfx950)
 template <typename T>
 void tune_single(uppe_handle* h, FusedState* s, bool may_time = true);
 
+// the wave-per-strip source kernel for many short FP32 strips (SEARAY_UPPE_WAVE_SOURCE=0 selects the workgroup
+// kernel, =2 forces the wave kernel whatever the strip count)
+static bool wave_source_enabled() {
+  const char* env = getenv("SEARAY_UPPE_WAVE_SOURCE");
+  return !(env && env[0] == '0');
+}
+static bool wave_source_forced() {
+  const char* env = getenv("SEARAY_UPPE_WAVE_SOURCE");
+  return env && env[0] == '2';
+}
+
 // the FP32 stepping source kernel (no diagnostics, no rotator) for Nt = 2^LG, where the block of Nt / S threads
 // fits 64..1024; otherwise the kernel for any Nt
 template <int S, int LG>
@@ -2570,6 +2881,19 @@ struct Fused {
                            : sizeof(T) == 4                  ? std::log2(s->ion.rate_conv * s->ion.c_pre)
                                                              : std::log(s->ion.rate_conv * s->ion.c_pre);
     const bool rot = h->rot_states > 2;
+    if constexpr (sizeof(T) == 4) {
+      // many short strips (at least four per CU): one wavefront per strip, no workgroup barriers. With fewer
+      // strips the workgroup kernel, whose strip transforms spread over more waves, is as fast or faster.
+      if (!rot && !diag && !h->paraxial && lgn >= 8 && lgn <= 10 &&
+          ((long)h->P * nz >= 4L * s->n_cu || wave_source_forced()) && wave_source_enabled()) {
+        const dim3 g((unsigned)h->P, (unsigned)nz);
+        auto kern = lgn == 8 ? f_source_w<8, 1> : (lgn == 9 ? f_source_w<9, 1> : f_source_w<10, 1>);
+        launchf(h, UPPE_KC_SOURCE, kern, g, dim3(64), 0, (const cplx<float>*)er, (cplx<float>*)jr,
+                (const cplx<float>*)s->tw, (const float*)s->w, (const float*)s->nlf, (const double*)s->ng, (int)h->P,
+                s->nwp, s->redchi3, s->dt, s->ion, log_pre, inst, zs, commit, s->qmax, rec, zrec, zrec_cap);
+        return;
+      }
+    }
     const dim3 grid((unsigned)h->P, (unsigned)nz), block(nth);
     if (h->paraxial) {
       // two N-point complex strips (E and A), plus the N-point tables in FP32
@@ -2674,7 +2998,8 @@ struct Fused {
     dim3 grid((unsigned)pl.tiles, (unsigned)((M + bm - 1) / bm), (unsigned)nz);
     const size_t shm = hk_lds_bytes(CM, kp, bm, pl.wide == 2 ? 32 : 16);
 #define UPPE_HK(BNV, RTV)                                                                                        \
-  launchf(h, UPPE_KC_HANKEL, f_hankel<CM, PRO, EPI, BNV, RTV>, grid, dim3(64 * pl.nwv), shm, (const F*)ahi,    \
+  launchf(h, EPI == EPI_NONE ? UPPE_KC_HANKEL : UPPE_KC_HANKEL_R2K, f_hankel<CM, PRO, EPI, BNV, RTV>, grid,      \
+          dim3(64 * pl.nwv), shm, (const F*)ahi,                                                               \
           (const F*)alo, B, out, M, K, kp, rw, s->nwp, 0, ncols, (const double*)s->phs, (const C*)s->kzc,         \
           (const T*)s->cond, q, kprev, k1, k2, k3, st, s->qmax, s->qmax_next, stage, zfix, dzfix, dzprev, pl.wide, \
           zs)
@@ -2781,7 +3106,7 @@ struct Fused {
               (const T*)s->cond, (C*)nullptr, (const C*)kp[stage], (C*)nullptr, (C*)nullptr, (C*)nullptr, s->st,
               s->qmax, s->qmax_next, stage, 0.0);
       source(h, s, (const C*)s->er, (C*)s->jr, false, stage == 1 ? s->st : nullptr);
-      launchf(h, UPPE_KC_HANKEL, f_gemm<T, PRO_NONE, EPI_STAGE>, g_r2k, dim3(256), 0, (const T*)s->Hf,
+      launchf(h, UPPE_KC_HANKEL_R2K, f_gemm<T, PRO_NONE, EPI_STAGE>, g_r2k, dim3(256), 0, (const T*)s->Hf,
               (const C*)s->jr, (C*)nullptr, h->nk, h->n1, s->nwp, (const double*)s->phs, (const C*)s->kzc,
               (const T*)s->cond, (C*)s->q, (const C*)nullptr, (C*)s->k1, (C*)s->k2, (C*)s->k3, s->st, s->qmax,
               s->qmax_next, stage, 0.0);
@@ -2869,7 +3194,7 @@ struct Fused {
     *cap = n;
   }
 
-  // forward run that keeps what the adjoint needs (Kerr-only radial MFMA path): q, the step state and qmax at the
+  // forward run that keeps what the adjoint needs (radial MFMA path, no rotator): q, the step state and qmax at the
   // start of every segment of `every` steps, (zoff, z, dz) of every step, and the stage fields E_r of the last
   // segment
   template <int CM>

@@ -156,3 +156,23 @@ def test_adjoint_rejects_rotator():
     vol = configs.radial_raman_air(scale=8, steps=2)
     with pytest.raises(_abi.UppeError):
         adjoint.SpectralPhaseProblem(vol, configs.kerr_pulse_sigma(), precision='fp32').value_and_grad(np.zeros(32))
+
+
+def test_config5_fp32_gradient_against_central_differences_full_size():
+    '''The gradient check as the workload states it: the radial Kerr workload at full size (Nw = 513, Nr = 128,
+    100 steps over 1 m), the FP32 adjoint with checkpoints every 10 steps against central finite differences
+    (h = 1e-4) of the FP64 forward on all 32 Legendre coefficients: 1e-4 relative L2 (the FP32 rounding of the
+    forward and adjoint sweeps; the FP64 adjoint matches the same differences to 1e-7).'''
+    _say('adjoint full size: fp32 (checkpoints every 10) vs central FD of the fp64 forward, 32 coefficients')
+    vol = configs.radial_kerr_air()
+    sig = configs.kerr_pulse_sigma()
+    c = _coeffs()
+    p64 = adjoint.SpectralPhaseProblem(vol, sig, precision='fp64', checkpoint_every=10)
+    _, g64 = p64.value_and_grad(c)
+    fd = np.array([(p64.loss(c + 1e-4 * e) - p64.loss(c - 1e-4 * e)) / 2e-4 for e in np.eye(32)])
+    p32 = adjoint.SpectralPhaseProblem(vol, sig, precision='fp32', checkpoint_every=10, target=p64.target)
+    _, g32 = p32.value_and_grad(c)
+    p64.close()
+    p32.close()
+    assert rel_l2(g64, fd) < 1e-7
+    assert rel_l2(g32, fd) < 1e-4
