@@ -155,8 +155,9 @@ def radial_o2_ionization(scale=1, steps=None, seed=1001, rate='explicit', precis
     subcycles = 400 if steps is None else steps
     air = dispersion.SimpleAir(MKS_LENGTH)
     if uv_opacity:
-        # the UV absorption band of examples/uppe/air_fil.py:36 (Im kz ~ 1.3e-4: exp(Im kz z) in the RHS
-        # leaves the FP32 range after ~9 cm, so FP32 runs need more diagnostic planes with it)
+        # the UV absorption band of examples/uppe/air_fil.py:36 (Im kz up to 1.56e-4: exp(Im kz z) in the RHS
+        # would leave the FP32 range after ~6.5 cm; the FP32 stepper keeps its state relative to a reference
+        # point that moves with the run)
         air.add_opacity_region(helper.dnum('0.1 cm'), helper.dnum('0.01 um'), helper.dnum('0.3 um'))
     vol = _base(helper, band, (helper.dnum('2 mm'), helper.dnum('2 mm'), L), (nw, nr, 1, 2),
                 (0.0, min(4.0, band[1])), subcycles, 1.0, 'cylindrical', nr)

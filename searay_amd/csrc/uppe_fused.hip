@@ -40,7 +40,15 @@ struct StepState {
   int sched_n, rec_cap, error, max_steps;
   int pend;               // the last step ran; its end (z advance, records, bookkeeping) is not committed yet
   const double* sched;    // replayed steps (device), or null
+  // FP32 runs: the state of every column f < rb_f1 is kept as q e^{-Im kz qbase} (q the interaction-picture
+  // state), so that the factors e^{+Im kz z} of the right-hand side stay within a span of the reference point
+  double qbase;           // the reference point (0: the state is q itself)
+  double rb_kmax;         // largest Im kz of the medium
+  int rebase, rb_f1;      // the reference moves with the run (rebase), and the columns it applies to
 };
+
+// the exponent span Im kz (z - qbase) after which a step's update moves the reference point to the step's end
+constexpr double kRebaseSpan = 8.0;
 
 // per-instance scalars of the source kernel in a batched launch (chi3/n_ref, rate model and its prefactor)
 struct SrcInst {
@@ -78,6 +86,10 @@ struct FusedState {
   void* tw = nullptr;      // [Nt/2] complex T
   double redchi3 = 0, dt = 0;
   double max_ki = 0;  // largest Im kz of the loaded medium
+  // FP32: the state's columns f < f1 are kept as q e^{-Im kz qbase} (the step state's reference point, brought
+  // back after every advance; 0: the state is q itself)
+  double qbase = 0;
+  std::vector<double> ki_h;  // Im kz of the loaded medium, point order of the host arrays
   uppe_ionization ion{};
   // state and work buffers: complex T [P][nwp]
   void *q = nullptr, *k1 = nullptr, *k2 = nullptr, *k3 = nullptr, *er = nullptr, *jr = nullptr;
@@ -170,9 +182,17 @@ __device__ inline double step_dz(const StepState& s) {
   return dz;
 }
 
-// end of a step of size dz: z advance, NaN count, step count, the termination test (z - L)^2 > (L/1e6)^2 and the
-// next subcycle (uppe_kernel.py:299-325)
+// does the update of the step ending at zend move the reference point of the rebased columns
+__device__ inline bool rebase_at(const StepState& s, double zend) {
+  return s.rebase && (zend - s.qbase) * s.rb_kmax > kRebaseSpan;
+}
+
+// end of a step of size dz: the reference point of the rebased columns (moved to the step's end, zoff + (z + dz),
+// the stage-4 phase origin, when its span is used up), z advance, NaN count, step count, the termination test
+// (z - L)^2 > (L/1e6)^2 and the next subcycle (uppe_kernel.py:299-325)
 __device__ inline void end_step(StepState& s, double dz) {
+  const double zend = s.zoff + (s.z + dz);
+  if (rebase_at(s, zend)) s.qbase = zend;
   s.z = s.z + dz;
   s.steps++;
   if (s.nanflag) {
@@ -198,15 +218,16 @@ __device__ inline void end_step(StepState& s, double dz) {
   }
 }
 
-// where a launch of `stage` stands: phase origin, z, dz (stages 2-4) and whether the run is active. Stage 1
-// advances a pending step in a local copy.
+// where a launch of `stage` stands: phase origin, z, dz (stages 2-4), whether the run is active, the reference
+// point of the rebased columns f < f1, and (stage 4) whether this step's update moves it. Stage 1 advances a
+// pending step in a local copy.
 struct StepPos {
-  double zoff, z, dz;
-  int active;
+  double zoff, z, dz, qb;
+  int active, f1, rb_end;
 };
 __device__ inline StepPos step_pos(const StepState* st, int stage) {
   if (stage == 1) {
-    if (!st->pend) return StepPos{st->zoff, st->z, 0.0, st->active};
+    if (!st->pend) return StepPos{st->zoff, st->z, 0.0, st->qbase, st->active, st->rb_f1, 0};
     StepState loc = *st;
     if (loc.active) {
       if (loc.sched_n > 0 && loc.steps >= loc.sched_n)
@@ -214,9 +235,58 @@ __device__ inline StepPos step_pos(const StepState* st, int stage) {
       else
         end_step(loc, step_dz(loc));
     }
-    return StepPos{loc.zoff, loc.z, 0.0, loc.active};
+    return StepPos{loc.zoff, loc.z, 0.0, loc.qbase, loc.active, loc.rb_f1, 0};
   }
-  return StepPos{st->zoff, st->z, step_dz(*st), st->active};
+  const double dz = step_dz(*st);
+  const int end = stage == 4 && rebase_at(*st, st->zoff + (st->z + dz));
+  return StepPos{st->zoff, st->z, dz, st->qbase, st->active, st->rb_f1, end};
+}
+
+// a value every lane of the wavefront holds, moved to scalar registers
+__device__ inline double uniform_f64(double x) {
+  const long long b = __double_as_longlong(x);
+  const int lo = __builtin_amdgcn_readfirstlane((int)b), hi = __builtin_amdgcn_readfirstlane((int)(b >> 32));
+  return __longlong_as_double((long long)(((unsigned long long)(unsigned)hi << 32) | (unsigned)lo));
+}
+
+// the reference of one column in a launch at phase origin zph: the amplitude origin za (zph - qb for a rebased
+// column, zph otherwise), the reference point qb (0 for a column that is q itself) and whether this step's update
+// moves it to zph (stage 4 only)
+struct ColRef {
+  double za, qb;
+  bool end;
+};
+__device__ inline ColRef col_ref(int f, int f1, double zph, double qb, int rb_end) {
+  if (f < f1) return ColRef{zph - qb, qb, rb_end != 0};
+  return ColRef{zph, 0.0, false};
+}
+
+// e^x for FP64 x as the hardware exp2 of the remainder of x log2(e) in [-1/2, 1/2] times an exact power of two:
+// relative error ~2e-7 over the whole FP64 exponent range, in a few instructions
+__device__ inline double exp_split(double x) {
+  const double y = x * 1.4426950408889634;
+  const double n = fmin(fmax(rint(y), -2100.0), 2100.0);
+  return ldexp((double)__builtin_amdgcn_exp2f((float)(y - n)), (int)n);
+}
+
+// the stage-4 state of one point: moved to the new reference point zph (factor e^{-Im kz za}) when the step's
+// update moves it; returns |q|^2 of the interaction-picture state, e^{2 Im kz qb} |q_kept|^2, in FP64
+template <typename T>
+__device__ inline double ref_update(cplx<T>& qn, T ki, const ColRef& cr, double zph) {
+  double qb = cr.qb;
+  if (cr.end) {
+    const T g = (T)exp_split(-(double)ki * cr.za);
+    qn = cmk(qn.re * g, qn.im * g);
+    qb = zph;
+  }
+  double m2 = (double)qn.re * qn.re + (double)qn.im * qn.im;
+  if (qb != 0.0) m2 *= exp_split(2.0 * (double)ki * qb);
+  return m2;
+}
+
+// amin of the step estimate against a point kept relative to the reference point qb: amin e^{-Im kz qb}
+__device__ inline double ref_amin(double amin, double ki, double qb) {
+  return qb != 0.0 ? amin * exp_split(-ki * qb) : amin;
 }
 
 // commit a pending step (one thread): records of its dz and phase origin, the end of step, and the maxima
@@ -248,15 +318,17 @@ __device__ inline void step_commit(StepState* st, unsigned long long* qmax, doub
 
 // exp(i phs z) exp(-ki z) with the phase formed in FP64 (long-z cancellation). The FP32 path reduces the
 // phase to a fraction of a turn in FP64 and evaluates the hardware sine, cosine (their input is in turns)
-// and base-2 exponential: absolute error ~1e-6 in the factor, against an RHS that is itself a small
-// correction to the state, well inside the FP32 path's 1e-5 field tolerance.
+// and base-2 exponential: error ~1e-6 in the factor, against an RHS that is itself a small correction to the
+// state, well inside the FP32 path's 1e-5 field tolerance.
+// The amplitude takes its own origin za: z itself, or z - qbase for a column kept relative to the reference point
+// qbase.
 template <typename T>
-__device__ inline cplx<T> linear_factor(double phs, T ki, double z, int sgn);
+__device__ inline cplx<T> linear_factor(double phs, T ki, double z, double za, int sgn);
 
 template <>
-__device__ inline cplx<double> linear_factor<double>(double phs, double ki, double z, int sgn) {
+__device__ inline cplx<double> linear_factor<double>(double phs, double ki, double z, double za, int sgn) {
   // the reference forms ar = -Ki z, ai = (Kr + kg) z and exp(ar)(cos ai, sin ai) (uppe.cl:66-70, 86-90)
-  double ar = sgn > 0 ? -ki * z : ki * z;
+  double ar = sgn > 0 ? -ki * za : ki * za;
   double ai = sgn > 0 ? phs * z : -(phs * z);
   double ex = exp(ar);
   double s, c;
@@ -265,10 +337,10 @@ __device__ inline cplx<double> linear_factor<double>(double phs, double ki, doub
 }
 
 template <>
-__device__ inline cplx<float> linear_factor<float>(double phs, float ki, double z, int sgn) {
+__device__ inline cplx<float> linear_factor<float>(double phs, float ki, double z, double za, int sgn) {
   const double turns = phs * z * 0.15915494309189535;
   float f = (float)(turns - rint(turns));  // in [-1/2, 1/2]
-  float ar = (float)(-(double)ki * z);
+  float ar = (float)(-(double)ki * za);
   if (sgn < 0) {
     f = -f;
     ar = -ar;
@@ -288,13 +360,15 @@ __global__ __launch_bounds__(256) void f_gemm(const T* __restrict__ A, const cpl
                                               const cplx<T>* __restrict__ kprev, cplx<T>* __restrict__ k1,
                                               cplx<T>* __restrict__ k2, cplx<T>* __restrict__ k3,
                                               StepState* __restrict__ st, unsigned long long* __restrict__ qmax,
-                                              unsigned long long* __restrict__ qmax_next, int stage, double zfix) {
+                                              unsigned long long* __restrict__ qmax_next, int stage, double zfix,
+                                              double qbfix, int f1fix) {
   __shared__ T As[BK][BM + 1];
   __shared__ T Bs[BK][BN];
   __shared__ double red[8];
   const int tid = threadIdx.x, tx = tid & 15, ty = tid >> 4;
   const int row0 = blockIdx.y * BM, col0 = blockIdx.x * BN;
-  double zph = zfix, cu = 0.0, dz = 0.0;
+  double zph = zfix, cu = 0.0, dz = 0.0, qb = sizeof(T) == 4 ? qbfix : 0.0;
+  int f1r = f1fix, rb_end = 0;
   if (st) {
     const StepPos sp = step_pos(st, stage);
     if (!sp.active) return;
@@ -302,6 +376,11 @@ __global__ __launch_bounds__(256) void f_gemm(const T* __restrict__ A, const cpl
     double cs = stage == 1 ? 0.0 : (stage == 4 ? 1.0 : 0.5);
     zph = sp.zoff + (sp.z + cs * dz);
     cu = stage == 1 ? 0.0 : (stage == 4 ? dz : dz / 2);
+    if constexpr (sizeof(T) == 4) {  // only FP32 states are rebased
+      qb = sp.qb;
+      f1r = sp.f1;
+      rb_end = sp.rb_end;
+    }
   }
   T acc[4][4];
 #pragma unroll
@@ -329,11 +408,11 @@ __global__ __launch_bounds__(256) void f_gemm(const T* __restrict__ A, const cpl
             v = cscale(cmk(v.re + c * kp.re, v.im + c * kp.im), cond[f]);
           }
           cplx<T> kz = kzc[idx];
-          v = cmul(linear_factor<T>(phs[idx], kz.im, zph, +1), v);
+          v = cmul(linear_factor<T>(phs[idx], kz.im, zph, col_ref(f, f1r, zph, qb, 0).za, +1), v);
           if (PRO == PRO_STAGE) v = cmul(cmk(-kz.im, kz.re), v);  // E = i kz A
         } else if (PRO == PRO_FINISH) {
           cplx<T> kz = kzc[idx];
-          v = cmul(linear_factor<T>(phs[idx], kz.im, zph, +1), v);
+          v = cmul(linear_factor<T>(phs[idx], kz.im, zph, col_ref(f, f1r, zph, qb, 0).za, +1), v);
         }
       }
       Bs[kk][2 * cf] = v.re;
@@ -372,7 +451,8 @@ __global__ __launch_bounds__(256) void f_gemm(const T* __restrict__ A, const cpl
       }
       // S = 0.5 i exp(-i(kz+kg) z) J / Re kz (uppe.cl:73-91)
       cplx<T> kz = kzc[idx];
-      cplx<T> t = cmul(linear_factor<T>(phs[idx], kz.im, zph, -1), J);
+      const ColRef cr = col_ref(f, f1r, zph, qb, rb_end);
+      cplx<T> t = cmul(linear_factor<T>(phs[idx], kz.im, zph, cr.za, -1), J);
       cplx<T> S = cmk((T(-0.5) * t.im) / kz.re, (T(0.5) * t.re) / kz.re);
       if (stage == 1) {
         k1[idx] = S;
@@ -382,7 +462,7 @@ __global__ __launch_bounds__(256) void f_gemm(const T* __restrict__ A, const cpl
         if (mod != mod) nan = true;
         if (!st->fixed) {
           // LoadStepSize (paraxial.cl:40-58) with amin = rel_amin * max|q| (uppe_kernel.py:221)
-          double amin = st->rel_amin * bitsd(*qmax);
+          double amin = ref_amin(st->rel_amin * bitsd(*qmax), (double)kz.im, cr.qb);
           double sr = (double)S.re, si = (double)S.im;
           double den = qr * qr + qi * qi + amin * amin;
           double alpha = (qr * sr + qi * si) / den;
@@ -409,9 +489,9 @@ __global__ __launch_bounds__(256) void f_gemm(const T* __restrict__ A, const cpl
         im = im + c6 * S.im;
         T cf = cond[f];
         cplx<T> qn = cmk(re * cf, im * cf);
+        const double m2 = ref_update<T>(qn, kz.im, cr, zph);
         q[idx] = qn;
-        double m = sqrt((double)qn.re * qn.re + (double)qn.im * qn.im);
-        loc = fmax(loc, m);
+        loc = fmax(loc, sqrt(m2));
       }
     }
   }
@@ -484,11 +564,11 @@ struct MfmaTraits<CM_BF16X3> {
 // update of q with the conditioner and max|q|)
 template <typename T>
 __device__ inline void stage_point(cplx<T> J, cplx<T> kz, double ph, cplx<T> Q, cplx<T> a1, cplx<T> a2,
-                                   cplx<T> a3, T cf, long idx, int stage, double zph, double dz,
+                                   cplx<T> a3, T cf, long idx, int stage, double zph, const ColRef& cr, double dz,
                                    cplx<T>* __restrict__ q, cplx<T>* __restrict__ k1, cplx<T>* __restrict__ k2,
                                    cplx<T>* __restrict__ k3, const StepState* st, double amin, double& loc,
                                    bool& nan) {
-  cplx<T> t = cmul(linear_factor<T>(ph, kz.im, zph, -1), J);
+  cplx<T> t = cmul(linear_factor<T>(ph, kz.im, zph, cr.za, -1), J);
   cplx<T> S;
   if constexpr (sizeof(T) == 4) {
     const float rk = __builtin_amdgcn_rcpf(kz.re);  // 1 ulp
@@ -504,7 +584,8 @@ __device__ inline void stage_point(cplx<T> J, cplx<T> kz, double ph, cplx<T> Q, 
       // loc keeps the largest rate Kr: the smallest per-point step dphi / Kr is dphi over it (a correctly
       // rounded quotient falls monotonically with the divisor), formed once after the loop
       double sr = (double)S.re, si = (double)S.im;
-      const double rden = 1.0 / (qr * qr + qi * qi + amin * amin);
+      const double am = ref_amin(amin, (double)kz.im, cr.qb);
+      const double rden = 1.0 / (qr * qr + qi * qi + am * am);
       double alpha = (qr * sr + qi * si) * rden;
       double keff = (qr * si - qi * sr) * rden;
       double Kr = 0.5 * st->dphi / st->L + fabs(keff) - alpha * (double)(alpha < 0.0);
@@ -525,14 +606,15 @@ __device__ inline void stage_point(cplx<T> J, cplx<T> kz, double ph, cplx<T> Q, 
     re = re + c6 * S.re;
     im = im + c6 * S.im;
     cplx<T> qn = cmk(re * cf, im * cf);
+    const double m2 = ref_update<T>(qn, kz.im, cr, zph);
     q[idx] = qn;
-    loc = fmax(loc, (double)qn.re * qn.re + (double)qn.im * qn.im);  // |q|^2: the root of the largest, after the loop
+    loc = fmax(loc, m2);  // |q|^2: the root of the largest, after the loop
   }
 }
 
 // one point of the r2k epilogue (loads, then stage_point)
 template <typename T>
-__device__ inline void stage_epilogue(cplx<T> J, long idx, int f, int stage, double zph, double dz,
+__device__ inline void stage_epilogue(cplx<T> J, long idx, int f, int stage, double zph, const ColRef& cr, double dz,
                                       const double* __restrict__ phs, const cplx<T>* __restrict__ kzc,
                                       const T* __restrict__ cond, cplx<T>* __restrict__ q,
                                       cplx<T>* __restrict__ k1, cplx<T>* __restrict__ k2,
@@ -548,7 +630,7 @@ __device__ inline void stage_epilogue(cplx<T> J, long idx, int f, int stage, dou
     a3 = k3[idx];
     cf = cond[f];
   }
-  stage_point<T>(J, kzc[idx], phs[idx], Q, a1, a2, a3, cf, idx, stage, zph, dz, q, k1, k2, k3, st, amin, loc,
+  stage_point<T>(J, kzc[idx], phs[idx], Q, a1, a2, a3, cf, idx, stage, zph, cr, dz, q, k1, k2, k3, st, amin, loc,
                  nan);
 }
 
@@ -564,7 +646,7 @@ template <typename T, int EPI>
 __device__ inline void adj_point(cplx<T> acc, cplx<T> kz, double ph, T cf, long idx, int stage, double zph, double dz,
                                  double dzprev, cplx<T>* __restrict__ gq, cplx<T>* __restrict__ gs,
                                  cplx<T>* __restrict__ gk) {
-  const cplx<T> back = cmul(linear_factor<T>(ph, -kz.im, zph, -1), acc);  // conj(e^{i (Re kz + kg) z} e^{-Im kz z})
+  const cplx<T> back = cmul(linear_factor<T>(ph, -kz.im, zph, zph, -1), acc);  // conj(e^{i (Re kz + kg) z} e^{-Im kz z})
   if (EPI == EPI_ADJ_FIN) {
     const cplx<T> g = cscale(back, cf);
     gs[idx] = g;
@@ -601,8 +683,8 @@ __device__ inline void adj_point(cplx<T> acc, cplx<T> kz, double ph, T cf, long 
 
 // B-operand element (k, f) with the prologue of the k2r direction applied
 template <typename T, int PRO>
-__device__ inline cplx<T> load_b(const cplx<T>* __restrict__ B, long idx, int f, int stage, double zph, double cu,
-                                 const double* __restrict__ phs, const cplx<T>* __restrict__ kzc,
+__device__ inline cplx<T> load_b(const cplx<T>* __restrict__ B, long idx, int f, int stage, double zph, double za,
+                                 double cu, const double* __restrict__ phs, const cplx<T>* __restrict__ kzc,
                                  const T* __restrict__ cond, const cplx<T>* __restrict__ kprev) {
   cplx<T> v = B[idx];
   if (pro_stage(PRO)) {
@@ -612,11 +694,11 @@ __device__ inline cplx<T> load_b(const cplx<T>* __restrict__ B, long idx, int f,
       v = cscale(cmk(v.re + c * kp.re, v.im + c * kp.im), cond[f]);
     }
     cplx<T> kz = kzc[idx];
-    v = cmul(linear_factor<T>(phs[idx], kz.im, zph, +1), v);
+    v = cmul(linear_factor<T>(phs[idx], kz.im, zph, za, +1), v);
     if (PRO == PRO_STAGE) v = cmul(cmk(-kz.im, kz.re), v);  // E = i kz A
   } else if (PRO == PRO_FINISH) {
     cplx<T> kz = kzc[idx];
-    v = cmul(linear_factor<T>(phs[idx], kz.im, zph, +1), v);
+    v = cmul(linear_factor<T>(phs[idx], kz.im, zph, za, +1), v);
   }
   return v;
 }
@@ -703,14 +785,27 @@ __device__ inline void hankel_tile(
   const int fc0 = f0 + tile * tcw; // first complex column of this tile
   const int hcw = wide ? tcw : ((tile == ntile - 1 && fend - fc0 > HC) ? 2 * HC : HC);  // staged columns
   const int nmt = min(BW / 16, (min(fend - fc0, hcw) + HC - 1) / HC);
-  double zph = zfix, cu = 0.0, dz = dzfix;
+  double zph = zfix, cu = 0.0, dz = dzfix, qb = 0.0;
+  int f1r = 0, rb_end = 0;  // launches without a step state (set-up, adjoint) see the state itself
   if (st) {
     const StepPos sp = step_pos(st, stage);
     dz = sp.dz;
     double cs = stage == 1 ? 0.0 : (stage == 4 ? 1.0 : 0.5);
     zph = sp.zoff + (sp.z + cs * dz);
     cu = stage == 1 ? 0.0 : (stage == 4 ? dz : dz / 2);
+    if constexpr (sizeof(T) == 4) {  // only FP32 states are rebased
+      qb = sp.qb;
+      f1r = sp.f1;
+      rb_end = sp.rb_end;
+    }
   }
+  // the step position is uniform over the workgroup: keep it in scalar registers
+  zph = uniform_f64(zph);
+  cu = uniform_f64(cu);
+  dz = uniform_f64(dz);
+  qb = uniform_f64(qb);
+  f1r = __builtin_amdgcn_readfirstlane(f1r);
+  rb_end = __builtin_amdgcn_readfirstlane(rb_end);
 
   // ---- A fragments: a ring of RING contraction chunks per row tile, the first RING issued before the B
   // slab is staged so their latency overlaps it
@@ -777,13 +872,13 @@ __device__ inline void hankel_tile(
           T c = (T)cu;
           v = cscale(cmk(v.re + c * vk[j].re, v.im + c * vk[j].im), vc[j]);
         }
-        v = cmul(linear_factor<T>(vp[j], vz[j].im, zph, +1), v);
+        v = cmul(linear_factor<T>(vp[j], vz[j].im, zph, col_ref(fc0 + cf, f1r, zph, qb, 0).za, +1), v);
         if (PRO == PRO_STAGE) v = cmul(cmk(-vz[j].im, vz[j].re), v);  // E = i kz A
       } else if (PRO == PRO_FINISH) {
-        v = cmul(linear_factor<T>(vp[j], vz[j].im, zph, +1), v);
+        v = cmul(linear_factor<T>(vp[j], vz[j].im, zph, col_ref(fc0 + cf, f1r, zph, qb, 0).za, +1), v);
       } else if (PRO == PRO_ADJ) {
         // conj of S = (i/2) e^{-i kz z} J / Re kz: (-i/2) e^{i (Re kz + kg) z} e^{Im kz z} g / Re kz
-        const C a = cmul(linear_factor<T>(vp[j], -vz[j].im, zph, +1), v);
+        const C a = cmul(linear_factor<T>(vp[j], -vz[j].im, zph, zph, +1), v);
         v = cmk((T(0.5) * a.im) / vz[j].re, (T(-0.5) * a.re) / vz[j].re);
       }
       if constexpr (CM == CM_BF16X3) {
@@ -911,6 +1006,8 @@ __device__ inline void hankel_tile(
   const double amin = (EPI == EPI_STAGE && stage == 1 && !st->fixed) ? st->rel_amin * bitsd(*qmax) : 0.0;
   double loc = 0.0;  // stage 1: the largest step rate Kr; stage 4: the largest |q|^2
   bool nan = false;
+  // every column of the epilogue lies below fend = f1, the rebased band (or the state is not rebased: qb = 0)
+  const ColRef cr_band{zph - qb, qb, rb_end != 0};
   // points are taken EPB per thread per batch, all of a batch's loads issued before its arithmetic
   constexpr int EPB = sizeof(T) == 8 ? 1 : 2;
   const int npts = bm * hcw;
@@ -948,8 +1045,8 @@ __device__ inline void hankel_tile(
 #pragma unroll
     for (int j = 0; j < EPB; j++)
       if (ok[j])
-        stage_point<T>(J[j], kz[j], ph[j], Q[j], a1[j], a2[j], a3[j], cfv[j], idx[j], stage, zph, dz, q, k1, k2,
-                       k3, st, amin, loc, nan);
+        stage_point<T>(J[j], kz[j], ph[j], Q[j], a1[j], a2[j], a3[j], cfv[j], idx[j], stage, zph, cr_band, dz, q, k1,
+                       k2, k3, st, amin, loc, nan);
   }
   if (EPI == EPI_NONE) return;
   if (stage == 1 && nan) atomicOr(so.nanflag, 1);
@@ -1083,7 +1180,7 @@ __global__ __launch_bounds__(1024) void f_tfft(
     const cplx<T>* __restrict__ kzc, const T* __restrict__ cond, cplx<T>* __restrict__ q,
     const cplx<T>* __restrict__ kprev, cplx<T>* __restrict__ k1, cplx<T>* __restrict__ k2,
     cplx<T>* __restrict__ k3, StepState* __restrict__ st, unsigned long long* __restrict__ qmax,
-    unsigned long long* __restrict__ qmax_next, int stage, double zfix) {
+    unsigned long long* __restrict__ qmax_next, int stage, double zfix, double qbfix, int f1fix) {
   using C = cplx<T>;
   extern __shared__ char smem[];
   C* buf = reinterpret_cast<C*>(smem);
@@ -1093,13 +1190,19 @@ __global__ __launch_bounds__(1024) void f_tfft(
   const int f0 = blockIdx.x * CW;
   const long sa = axis == 1 ? (long)ny * nwp : (long)nwp;
   const long base = (axis == 1 ? (long)blockIdx.y : (long)blockIdx.y * ny) * nwp + f0;
-  double zph = zfix, cu = 0.0, dz = 0.0;
+  double zph = zfix, cu = 0.0, dz = 0.0, qb = sizeof(T) == 4 ? qbfix : 0.0;
+  int f1r = f1fix, rb_end = 0;
   if (st) {
     const StepPos sp = step_pos(st, stage);
     dz = sp.dz;
     const double cs = stage == 1 ? 0.0 : (stage == 4 ? 1.0 : 0.5);
     zph = sp.zoff + (sp.z + cs * dz);
     cu = stage == 1 ? 0.0 : (stage == 4 ? dz : dz / 2);
+    if constexpr (sizeof(T) == 4) {  // only FP32 states are rebased
+      qb = sp.qb;
+      f1r = sp.f1;
+      rb_end = sp.rb_end;
+    }
   }
   for (int i = t; i < tab_len; i += nth) tab[i] = tab_g[i];
   // the tile, with the prologue applied; four elements per thread per batch, their loads issued together
@@ -1112,7 +1215,9 @@ __global__ __launch_bounds__(1024) void f_tfft(
       v[j] = cmk(T(0), T(0));
       if (e < ne && f < fend) {
         const long idx = base + a * sa + c;
-        v[j] = PRO == PRO_NONE ? src[idx] : load_b<T, PRO>(src, idx, f, stage, zph, cu, phs, kzc, cond, kprev);
+        v[j] = PRO == PRO_NONE ? src[idx]
+                               : load_b<T, PRO>(src, idx, f, stage, zph, col_ref(f, f1r, zph, qb, 0).za, cu, phs,
+                                                kzc, cond, kprev);
       }
     }
 #pragma unroll
@@ -1128,6 +1233,8 @@ __global__ __launch_bounds__(1024) void f_tfft(
   const double amin = (EPI == EPI_STAGE && stage == 1 && !st->fixed) ? st->rel_amin * bitsd(*qmax) : 0.0;
   double loc = 0.0;  // stage 1: the largest step rate Kr; stage 4: the largest |q|^2
   bool nan = false;
+  // the epilogue covers f < fend = f1, the rebased band (or the state is not rebased: qb = 0)
+  const ColRef cr_band{zph - qb, qb, rb_end != 0};
   for (int e = t; e < ne; e += nth) {
     const int a = e / CW, c = e % CW, f = f0 + c;
     if (f >= fend) continue;
@@ -1136,7 +1243,7 @@ __global__ __launch_bounds__(1024) void f_tfft(
     if (EPI == EPI_NONE)
       dst[idx] = v;
     else
-      stage_epilogue<T>(v, idx, f, stage, zph, dz, phs, kzc, cond, q, k1, k2, k3, st, amin, loc, nan);
+      stage_epilogue<T>(v, idx, f, stage, zph, cr_band, dz, phs, kzc, cond, q, k1, k2, k3, st, amin, loc, nan);
   }
   if (EPI == EPI_NONE) return;
   if (stage == 1 && nan) atomicOr(&st->nanflag, 1);
@@ -2400,14 +2507,34 @@ __global__ void f_from_host(const double* src, cplx<T>* dst, int nw, long P, int
     dst[i] = cmk(T(0), T(0));
 }
 
+// (kzc non-null: a state kept relative to the reference point qb in its columns f < f1, returned as the
+// interaction-picture state e^{Im kz qb} q in FP64)
 template <typename T>
-__global__ void f_to_host(const cplx<T>* src, double* dst, int nw, long P, int nwp) {
+__global__ void f_to_host(const cplx<T>* src, double* dst, int nw, long P, int nwp, const cplx<T>* kzc,
+                          int f1, double qb) {
   long i = blockIdx.x * (long)blockDim.x + threadIdx.x;
   if (i >= P * nw) return;
   long f = i / P, p = i % P;
   cplx<T> v = src[p * nwp + f];
-  dst[2 * i] = (double)v.re;
-  dst[2 * i + 1] = (double)v.im;
+  const double g = (kzc && f < f1) ? exp((double)kzc[p * nwp + f].im * qb) : 1.0;
+  dst[2 * i] = (double)v.re * g;
+  dst[2 * i + 1] = (double)v.im * g;
+}
+
+// move a state from reference point qa of its columns f < f1a under the Im kz of kza to reference point qbn of
+// its columns f < f1b under the Im kz of kzb: q e^{Im kza qa [f < f1a] - Im kzb qbn [f < f1b]}, the exponent in
+// FP64 (after a change of medium, or back to the state itself)
+template <typename T>
+__global__ void f_reref(cplx<T>* q, const cplx<T>* kza, int f1a, double qa, const cplx<T>* kzb, int f1b, double qbn,
+                        long n, int nwp) {
+  long i = blockIdx.x * (long)blockDim.x + threadIdx.x;
+  if (i >= n) return;
+  const int f = (int)(i % nwp);
+  const double ex = (f < f1a ? (double)kza[i].im * qa : 0.0) - (f < f1b ? (double)kzb[i].im * qbn : 0.0);
+  if (ex == 0.0) return;
+  const T g = (T)exp(ex);
+  cplx<T> v = q[i];
+  q[i] = cmk(v.re * g, v.im * g);
 }
 
 template <typename T>
@@ -2783,6 +2910,7 @@ struct Fused {
     UPPE_HIP(hipMalloc(&kg, sizeof(double) * h->nw));
     UPPE_HIP(hipMemcpy(kg, m->kg, sizeof(double) * h->nw, hipMemcpyHostToDevice));
     UPPE_HIP(hipMemcpy(s->stage, m->kz, sizeof(double) * 2 * h->npt, hipMemcpyHostToDevice));
+    const bool moved = begin_medium_change(h, s, m->kz);
     launchf(h, UPPE_KC_POINT, f_medium<T>, dim3(grid_for(P * s->nwp)), dim3(kBlock), 0, (const double*)s->stage,
             (const double*)kg, s->phs, (C*)s->kzc, h->nw, P, s->nwp, (const double*)nullptr);
     UPPE_HIP(hipStreamSynchronize(h->stream));
@@ -2795,6 +2923,7 @@ struct Fused {
     // columns at and above f1 carry zero current: the r2k contraction and the stage updates skip them.
     // f1 covers the conditioner's four bins.
     const int f1 = std::min(h->nw, std::max(last_nl + 1, 4));
+    if (moved) end_medium_change(h, s, f1);
     if (f1 != s->f1) {
       s->f1 = f1;
       if (h->have_hankel) s->plan[1] = plan_hankel(h->nk, s->f1, s->n_cu);
@@ -2823,6 +2952,7 @@ struct Fused {
     UPPE_HIP(hipMalloc(&k0, sizeof(double) * h->nw));
     UPPE_HIP(hipMemcpy(k0, m->k0, sizeof(double) * h->nw, hipMemcpyHostToDevice));
     UPPE_HIP(hipMemcpy(s->stage, m->kz, sizeof(double) * 2 * h->npt, hipMemcpyHostToDevice));
+    const bool moved = begin_medium_change(h, s, m->kz);
     launchf(h, UPPE_KC_POINT, f_medium<T>, dim3(grid_for(P * s->nwp)), dim3(kBlock), 0, (const double*)s->stage,
             (const double*)nullptr, s->phs, (C*)s->kzc, h->nw, P, s->nwp, (const double*)k0);
     UPPE_HIP(hipStreamSynchronize(h->stream));
@@ -2832,6 +2962,7 @@ struct Fused {
     s->ion = m->ion;
     s->max_ki = 0.0;
     for (long i = 0; i < h->npt; i++) s->max_ki = std::max(s->max_ki, m->kz[2 * i + 1]);
+    if (moved) end_medium_change(h, s, h->nw);
     if (h->nw != s->f1) {
       s->f1 = h->nw;
       if (h->have_hankel) s->plan[1] = plan_hankel(h->nk, s->f1, s->n_cu);
@@ -2841,6 +2972,34 @@ struct Fused {
       UPPE_HIP(hipMemset(s->k3, 0, cs));
       if (h->have_field) static_qmax(h, s);
     }
+  }
+
+  // A change of medium under a state kept relative to its reference point (FP32, qbase != 0): the state is q
+  // e^{-Im kz_old qbase} in the old band, and must become q e^{-Im kz_new qbase} in the new one, which keeps q
+  // across the change as the reference's subcycles do (uppe_kernel.py:300-305). begin_medium_change saves the old
+  // Im kz (complex kz into the zero stage slope k1) before the new medium is loaded and checks the factor's range
+  // against the host copy; end_medium_change applies it and clears k1.
+  static bool begin_medium_change(uppe_handle* h, FusedState* s, const double* kz_new) {
+    const bool moved = sizeof(T) == 4 && s->qbase != 0.0 && h->have_field;
+    if (moved) {
+      double ex = 0.0;
+      for (long i = 0; i < h->npt; i++) ex = std::max(ex, (s->ki_h[(size_t)i] - kz_new[2 * i + 1]) * s->qbase);
+      if (ex > 80.0)
+        throw Error{UPPE_ERR_UNSUPPORTED, "FP32 range: this change of medium scales the state by up to exp(" +
+                                              std::to_string(ex) + "); use FP64"};
+      UPPE_HIP(hipMemcpyAsync(s->k1, s->kzc, sizeof(C) * h->P * s->nwp, hipMemcpyDeviceToDevice, h->stream));
+    }
+    s->ki_h.resize((size_t)h->npt);
+    for (long i = 0; i < h->npt; i++) s->ki_h[(size_t)i] = kz_new[2 * i + 1];
+    return moved;
+  }
+
+  static void end_medium_change(uppe_handle* h, FusedState* s, int f1_new) {
+    const long n = (long)h->P * s->nwp;
+    launchf(h, UPPE_KC_POINT, f_reref<T>, dim3(grid_for(n)), dim3(kBlock), 0, (C*)s->q, (const C*)s->k1, s->f1,
+            s->qbase, (const C*)s->kzc, f1_new, s->qbase, n, s->nwp);
+    UPPE_HIP(hipMemsetAsync(s->k1, 0, sizeof(C) * n, h->stream));
+    UPPE_HIP(hipStreamSynchronize(h->stream));
   }
 
   static void static_qmax(uppe_handle* h, FusedState* s) {
@@ -2856,7 +3015,7 @@ struct Fused {
     launchf(h, UPPE_KC_HANKEL, f_gemm<T, PRO_NONE, EPI_NONE>, grid, dim3(256), 0, (const T*)s->Hf, in, out, h->nk,
             h->n1, s->nwp, (const double*)nullptr, (const C*)nullptr, (const T*)nullptr, (C*)nullptr,
             (const C*)nullptr, (C*)nullptr, (C*)nullptr, (C*)nullptr, (StepState*)nullptr,
-            (unsigned long long*)nullptr, (unsigned long long*)nullptr, 0, 0.0);
+            (unsigned long long*)nullptr, (unsigned long long*)nullptr, 0, 0.0, 0.0, 0);
   }
 
   // the source kernel over every strip; a batched launch (nz instances zs elements apart, per-instance scalars
@@ -3061,7 +3220,7 @@ struct Fused {
     launchf(h, UPPE_KC_TRANSVERSE_FFT, f_tfft<T, CW, PRO, EPI>, grid, dim3(nth), shm, src, dst, axis, na, ilog2i(na),
             h->n2, s->nwp, fend, (const C*)tab, tl, sign, scale, (const double*)s->phs, (const C*)s->kzc,
             (const T*)s->cond, (C*)s->q, kprev, (C*)s->k1, (C*)s->k2, (C*)s->k3, st, s->qmax, s->qmax_next, stage,
-            zfix);
+            zfix, s->qbase, s->f1);
   }
 
   // one RK4 step on a Cartesian grid: per RHS, the inverse transform (y with the stage prologue, then x), the
@@ -3104,17 +3263,18 @@ struct Fused {
               g_k2r, dim3(256), 0, (const T*)s->Hb,
               (const C*)s->q, (C*)s->er, h->n1, h->nk, s->nwp, (const double*)s->phs, (const C*)s->kzc,
               (const T*)s->cond, (C*)nullptr, (const C*)kp[stage], (C*)nullptr, (C*)nullptr, (C*)nullptr, s->st,
-              s->qmax, s->qmax_next, stage, 0.0);
+              s->qmax, s->qmax_next, stage, 0.0, 0.0, 0);
       source(h, s, (const C*)s->er, (C*)s->jr, false, stage == 1 ? s->st : nullptr);
       launchf(h, UPPE_KC_HANKEL_R2K, f_gemm<T, PRO_NONE, EPI_STAGE>, g_r2k, dim3(256), 0, (const T*)s->Hf,
               (const C*)s->jr, (C*)nullptr, h->nk, h->n1, s->nwp, (const double*)s->phs, (const C*)s->kzc,
               (const T*)s->cond, (C*)s->q, (const C*)nullptr, (C*)s->k1, (C*)s->k2, (C*)s->k3, s->st, s->qmax,
-              s->qmax_next, stage, 0.0);
+              s->qmax_next, stage, 0.0, 0.0, 0);
     }
   }
 
   static void set_field(uppe_handle* h, FusedState* s, const double* a_r) {
     const long P = h->P;
+    s->qbase = 0.0;
     UPPE_HIP(hipMemcpy(s->stage, a_r, sizeof(double) * 2 * h->npt, hipMemcpyHostToDevice));
     launchf(h, UPPE_KC_POINT, f_from_host<T>, dim3(grid_for(P * s->nwp)), dim3(kBlock), 0, (const double*)s->stage,
             (C*)s->er, h->nw, P, s->nwp);
@@ -3135,14 +3295,40 @@ struct Fused {
 
   // the host copy of the step state of a new advance call (uppe_kernel.py:299-325), with the replayed schedule
   // uploaded; the caller uploads the state itself
-  static StepState* init_state(uppe_handle* h, FusedState* s, const uppe_step_control* c) {
+  // move the state to reference point qn under the loaded medium (a launch on the handle's stream, waited for)
+  static void rereference(uppe_handle* h, FusedState* s, double qn) {
+    if (s->qbase == qn) return;
+    const long n = (long)h->P * s->nwp;
+    launchf(h, UPPE_KC_POINT, f_reref<T>, dim3(grid_for(n)), dim3(kBlock), 0, (C*)s->q, (const C*)s->kzc, s->f1,
+            s->qbase, (const C*)s->kzc, s->f1, qn, n, s->nwp);
+    UPPE_HIP(hipStreamSynchronize(h->stream));
+    s->qbase = qn;
+  }
+
+  // The right-hand side carries exp(+Im kz z) (uppe.cl:86-90). FP32 runs keep it in range by moving the state's
+  // reference point with the run (rebase): the factors then span at most one subcycle plus kRebaseSpan. A run
+  // that replays its own steps (the checkpointed forward run of the adjoint) keeps the state itself and takes
+  // the whole interval's range.
+  static StepState* init_state(uppe_handle* h, FusedState* s, const uppe_step_control* c, bool rebase_ok = true) {
+    const bool rebase = sizeof(T) == 4 && rebase_ok;
     if (sizeof(T) == 4) {
-      // the right-hand side carries exp(+Im kz z) (uppe.cl:86-90); keep it inside the FP32 range
-      const double zmax = ((c->zi + (double)(c->sub0 + c->n_sub) * c->length) - c->zi);
-      if (s->max_ki * zmax > 80.0)
-        throw Error{UPPE_ERR_UNSUPPORTED,
-                    "FP32 range: exp(Im kz z) would exceed 3e38 in this interval (max Im kz * z = " +
-                        std::to_string(s->max_ki * zmax) + "); use FP64 or more diagnostic planes"};
+      if (rebase) {
+        if (s->max_ki * c->length > 40.0)
+          throw Error{UPPE_ERR_UNSUPPORTED, "FP32 range: max Im kz times the subcycle length is " +
+                                                std::to_string(s->max_ki * c->length) +
+                                                " (at most 40); use FP64 or more subcycles"};
+      } else {
+        if (s->max_ki * std::fabs(s->qbase) > 80.0)
+          throw Error{UPPE_ERR_UNSUPPORTED, "FP32 range: the state's reference point is too far for this run (" +
+                                                std::to_string(s->max_ki * std::fabs(s->qbase)) +
+                                                "); set the field again"};
+        rereference(h, s, 0.0);
+        const double zmax = ((c->zi + (double)(c->sub0 + c->n_sub) * c->length) - c->zi);
+        if (s->max_ki * zmax > 80.0)
+          throw Error{UPPE_ERR_UNSUPPORTED,
+                      "FP32 range: exp(Im kz z) would exceed 3e38 in this interval (max Im kz * z = " +
+                          std::to_string(s->max_ki * zmax) + "); use FP64 or more diagnostic planes"};
+      }
     }
     StepState* sh = s->st_host;
     std::memset(sh, 0, sizeof(StepState));
@@ -3161,6 +3347,10 @@ struct Fused {
     sh->dzraw = 0x7ff0000000000000ull;  // +inf
     sh->rec_cap = 0;
     sh->max_steps = c->max_steps > 0 ? c->max_steps : 0;
+    sh->qbase = rebase ? s->qbase : 0.0;
+    sh->rb_kmax = s->max_ki;
+    sh->rebase = rebase ? 1 : 0;
+    sh->rb_f1 = rebase ? s->f1 : 0;
     if (c->n_schedule > s->sched_cap) {
       if (s->sched) UPPE_HIP(hipFree(s->sched));
       UPPE_HIP(hipMalloc(&s->sched, sizeof(double) * c->n_schedule));
@@ -3200,7 +3390,7 @@ struct Fused {
   template <int CM>
   static int advance_ck(uppe_handle* h, FusedState* s, const uppe_step_control* c, int every) {
     const long P = h->P, stride = P * (long)s->nwp;
-    StepState* sh = init_state(h, s, c);
+    StepState* sh = init_state(h, s, c, false);  // the checkpoints hold the state itself
     s->rec_active = nullptr;
     UPPE_HIP(hipMemcpyAsync(s->st, sh, sizeof(StepState), hipMemcpyHostToDevice, h->stream));
     tune_single<T>(h, s);
@@ -3304,7 +3494,7 @@ struct Fused {
     hk_launch<CM, PRO_NONE, EPI_NONE>(h, s, s->fab_hi, s->fab_lo, s->kp_b, nr, nk, h->nw, pl_a0, gq, g2, nullptr,
                                       nullptr, nullptr, nullptr, nullptr, nullptr, 0, 0.0, 0.0, 0.0);
     launchf(h, UPPE_KC_POINT, f_to_host<T>, dim3(grid_for(P * h->nw)), dim3(kBlock), 0, (const C*)g2, s->stage,
-            h->nw, P, s->nwp);
+            h->nw, P, s->nwp, (const C*)nullptr, 0, 0.0);
     UPPE_HIP(hipMemcpyAsync(g_a0_r, s->stage, sizeof(double) * 2 * h->npt, hipMemcpyDeviceToHost, h->stream));
     UPPE_HIP(hipStreamSynchronize(h->stream));
   }
@@ -3434,6 +3624,7 @@ struct Fused {
     h->evals += 4LL * sh->steps;
     h->nan_steps += sh->nan_steps;
     h->z_last = sh->z_last;
+    s->qbase = sh->qbase;
     return sh->steps;
   }
 
@@ -3449,7 +3640,8 @@ struct Fused {
       launchf(h, UPPE_KC_HANKEL, f_gemm<T, PRO_FINISH, EPI_NONE>, g_k2r, dim3(256), 0, (const T*)s->Hb,
               (const C*)s->q, (C*)s->er, h->n1, h->nk, s->nwp, (const double*)s->phs, (const C*)s->kzc,
               (const T*)s->cond, (C*)nullptr, (const C*)nullptr, (C*)nullptr, (C*)nullptr, (C*)nullptr,
-              (StepState*)nullptr, (unsigned long long*)nullptr, (unsigned long long*)nullptr, 0, z_total);
+              (StepState*)nullptr, (unsigned long long*)nullptr, (unsigned long long*)nullptr, 0, z_total, s->qbase,
+              s->f1);
     }
     if (h->paraxial) {
       // the envelope source takes A itself (paraxial_kernel.py:243-245)
@@ -3461,7 +3653,7 @@ struct Fused {
     }
     auto out_c = [&](const void* src, double* dst) {
       launchf(h, UPPE_KC_POINT, f_to_host<T>, dim3(grid_for(P * h->nw)), dim3(kBlock), 0, (const C*)src, s->stage,
-              h->nw, P, s->nwp);
+              h->nw, P, s->nwp, (const C*)nullptr, 0, 0.0);
       UPPE_HIP(hipMemcpyAsync(dst, s->stage, sizeof(double) * 2 * h->npt, hipMemcpyDeviceToHost, h->stream));
       UPPE_HIP(hipStreamSynchronize(h->stream));
     };
@@ -3483,7 +3675,7 @@ struct Fused {
   static void get_state(uppe_handle* h, FusedState* s, double* q) {
     const long P = h->P;
     launchf(h, UPPE_KC_POINT, f_to_host<T>, dim3(grid_for(P * h->nw)), dim3(kBlock), 0, (const C*)s->q, s->stage,
-            h->nw, P, s->nwp);
+            h->nw, P, s->nwp, s->qbase != 0.0 ? (const C*)s->kzc : (const C*)nullptr, s->f1, s->qbase);
     UPPE_HIP(hipMemcpyAsync(q, s->stage, sizeof(double) * 2 * h->npt, hipMemcpyDeviceToHost, h->stream));
     UPPE_HIP(hipStreamSynchronize(h->stream));
   }
@@ -4044,6 +4236,7 @@ void fused_batch_advance(uppe_batch* b, const uppe_step_control* c, int* steps_o
     uppe_handle* h = b->members[(size_t)i];
     const StepState& sh = fb->groups[(size_t)fb->group_of[(size_t)i]].st_host[fb->slot_of[(size_t)i]];
     if (sh.error) throw Error{UPPE_ERR_STATE, "batch member " + std::to_string(i) + ": step control error"};
+    h->fused->qbase = sh.qbase;
     h->steps += sh.steps;
     h->evals += 4LL * sh.steps;
     h->nan_steps += sh.nan_steps;
