@@ -160,7 +160,9 @@ int uppe_set_rotator(uppe_handle* h, const uppe_rotator* r);
 int uppe_set_field(uppe_handle* h, const double* a_r);
 
 /* RK4 interaction-picture stepping with step control (uppe_kernel.py:299-325, load_source :188-232).
- * steps_out (nullable) receives the steps taken by this call. */
+ * steps_out (nullable) receives the steps taken by this call. FP32 handles keep the state of the nonlinear band
+ * relative to a reference point that moves with the run, so absorbing media (factors exp(+Im kz z)) stay in
+ * range; UPPE_ERR_UNSUPPORTED when max(Im kz) times one subcycle's length exceeds 40. */
 int uppe_advance(uppe_handle* h, const uppe_step_control* c, int* steps_out);
 
 /* uppe_advance that allocates nothing. No counterpart in the reference: it is for callers that cannot allocate
