@@ -739,13 +739,14 @@ __device__ inline void hankel_tile(
     cplx<typename MfmaTraits<CM>::T>* __restrict__ k1, cplx<typename MfmaTraits<CM>::T>* __restrict__ k2,
     cplx<typename MfmaTraits<CM>::T>* __restrict__ k3, const StepState* st, StepOut so,
     const unsigned long long* __restrict__ qmax, unsigned long long* __restrict__ qmax_next, int stage, double zfix,
-    double dzfix, double dzprev, int wide, long zs, int tile, int ntile, int rowblk, int zi) {
+    double dzfix, double dzprev, int wide, long zs, int zst, int tile, int ntile, int rowblk, int zi) {
   using T = typename MfmaTraits<CM>::T;
   using C = cplx<T>;
   using acc_t = typename MfmaTraits<CM>::acc_t;
   if (zs) {
     // batched launch: instance zi of equally shaped instances whose arrays lie zs elements apart
-    // (state, tables), one fragment set apart (operator), one step state and three maxima apart
+    // (state, tables), one fragment set apart (operator); batch members (zst = 1) also one step state and three
+    // maxima apart, the azimuthal modes of one grid (zst = 0) share them
     const long z = zi, zo = z * zs;
     const long fs = (long)((M + 15) / 16) * (CM == CM_BF16X3 ? Kp / 32 : Kp / 16) * 64;
     Ahi = zshift(Ahi, z * fs);
@@ -759,12 +760,14 @@ __device__ inline void hankel_tile(
     k1 = zshift(k1, zo);
     k2 = zshift(k2, zo);
     k3 = zshift(k3, zo);
-    st = zshift(st, z);
-    qmax = zshift(qmax, 3 * z);
-    qmax_next = zshift(qmax_next, 3 * z);
-    if (so.dzraw) so.dzraw = &const_cast<StepState*>(st)->dzraw;
-    if (so.nanflag) so.nanflag = &const_cast<StepState*>(st)->nanflag;
-    if (so.pend) so.pend = &const_cast<StepState*>(st)->pend;
+    if (zst) {
+      st = zshift(st, z);
+      qmax = zshift(qmax, 3 * z);
+      qmax_next = zshift(qmax_next, 3 * z);
+      if (so.dzraw) so.dzraw = &const_cast<StepState*>(st)->dzraw;
+      if (so.nanflag) so.nanflag = &const_cast<StepState*>(st)->nanflag;
+      if (so.pend) so.pend = &const_cast<StepState*>(st)->pend;
+    }
   }
   // no early exit for a finished run: the contraction only writes scratch (E_r) or stops before its
   // epilogue, so its loads need not wait for the step state
@@ -1083,10 +1086,10 @@ __global__ __launch_bounds__(BN > 16 ? 512 : 1024) void f_hankel(
     cplx<typename MfmaTraits<CM>::T>* __restrict__ k1, cplx<typename MfmaTraits<CM>::T>* __restrict__ k2,
     cplx<typename MfmaTraits<CM>::T>* __restrict__ k3, StepState* __restrict__ st,
     unsigned long long* __restrict__ qmax, unsigned long long* __restrict__ qmax_next, int stage, double zfix,
-    double dzfix, double dzprev, int wide, long zs) {
+    double dzfix, double dzprev, int wide, long zs, int zst) {
   const StepOut so{st ? &st->dzraw : nullptr, st ? &st->nanflag : nullptr, st ? &st->pend : nullptr};
   hankel_tile<CM, PRO, EPI, BN, RT>(Ahi, Alo, B, out, M, K, Kp, rw, nwp, f0, fend, phs, kzc, cond, q, kprev, k1,
-                                           k2, k3, st, so, qmax, qmax_next, stage, zfix, dzfix, dzprev, wide, zs,
+                                           k2, k3, st, so, qmax, qmax_next, stage, zfix, dzfix, dzprev, wide, zs, zst,
                                            xcd_tile(blockIdx.x, gridDim.x), gridDim.x, blockIdx.y, blockIdx.z);
 }
 
@@ -2521,6 +2524,52 @@ __global__ void f_to_host(const cplx<T>* src, double* dst, int nw, long P, int n
   dst[2 * i + 1] = (double)v.im * g;
 }
 
+// cos(2 pi e / 16), e < 16: the twiddles of the azimuthal transforms (N2 <= 16 points; e scaled by 16 / N2)
+__constant__ double kCos16[16] = {1.0,
+                                  0.92387953251128675613,
+                                  0.70710678118654752440,
+                                  0.38268343236508977173,
+                                  0.0,
+                                  -0.38268343236508977173,
+                                  -0.70710678118654752440,
+                                  -0.92387953251128675613,
+                                  -1.0,
+                                  -0.92387953251128675613,
+                                  -0.70710678118654752440,
+                                  -0.38268343236508977173,
+                                  0.0,
+                                  0.38268343236508977173,
+                                  0.70710678118654752440,
+                                  0.92387953251128675613};
+
+// The azimuthal transform of a radial grid with N2 > 1 points in phi, in place on a [Nr N2][nwp] array: for every
+// (r, f < ncols), numpy's fft over the N2 rows r N2 + m (sign -1) or its ifft (sign +1, scaled by 1/N2), as the
+// reference's FFT_axis2 / IFFT_axis2 (fft.cl:401-441) take before and after the per-mode radial transforms
+// (grid_tools.py:308-325). A direct N2-point DFT per thread.
+template <typename T, int N2>
+__global__ void f_azim(cplx<T>* a, int nr, int nwp, int ncols, int sign) {
+  const long i = blockIdx.x * (long)blockDim.x + threadIdx.x;
+  if (i >= (long)nr * ncols) return;
+  const int r = (int)(i / ncols), f = (int)(i % ncols);
+  cplx<T>* p = a + (long)r * N2 * nwp + f;
+  cplx<T> v[N2];
+#pragma unroll
+  for (int m = 0; m < N2; m++) v[m] = p[(long)m * nwp];
+  const T sg = (T)sign, sc = sign > 0 ? T(1) / T(N2) : T(1);
+#pragma unroll
+  for (int k = 0; k < N2; k++) {
+    T re = T(0), im = T(0);
+#pragma unroll
+    for (int m = 0; m < N2; m++) {
+      const int e = ((m * k) % N2) * (16 / N2);
+      const T c = (T)kCos16[e], sn = sg * (T)kCos16[(e + 12) & 15];  // sin(x) = cos(x - pi/2)
+      re += v[m].re * c - v[m].im * sn;
+      im += v[m].re * sn + v[m].im * c;
+    }
+    p[(long)k * nwp] = cmk(re * sc, im * sc);
+  }
+}
+
 // move a state from reference point qa of its columns f < f1a under the Im kz of kza to reference point qbn of
 // its columns f < f1b under the Im kz of kzb: q e^{Im kza qa [f < f1a] - Im kzb qbn [f < f1b]}, the exponent in
 // FP64 (after a change of medium, or back to the state itself)
@@ -2813,8 +2862,9 @@ struct Fused {
     UPPE_HIP(hipMalloc(&s->tw, sizeof(C) * t.size()));
     UPPE_HIP(hipMemcpy(s->tw, t.data(), sizeof(C) * t.size(), hipMemcpyHostToDevice));
     if (h->d.geometry == UPPE_RADIAL) {
-      UPPE_HIP(hipMalloc(&s->Hb, sizeof(T) * (size_t)h->n1 * h->nk));
-      UPPE_HIP(hipMalloc(&s->Hf, sizeof(T) * (size_t)h->n1 * h->nk));
+      // one operator per azimuthal mode
+      UPPE_HIP(hipMalloc(&s->Hb, sizeof(T) * (size_t)h->n1 * h->nk * h->n2));
+      UPPE_HIP(hipMalloc(&s->Hf, sizeof(T) * (size_t)h->n1 * h->nk * h->n2));
     } else {
       std::vector<C> tx = r8_table_one<T>(h->n1), ty = r8_table_one<T>(h->n2);
       s->twlen_x = (int)tx.size();
@@ -2827,14 +2877,19 @@ struct Fused {
   }
 
   static void set_hankel(uppe_handle* h, FusedState* s) {
-    const int nr = h->n1, nk = h->nk;
-    std::vector<double> hb((size_t)nr * nk), hf((size_t)nk * nr);
-    for (int k = 0; k < nk; k++)
-      for (int j = 0; j < nr; j++) {
-        double v = h->H[(size_t)k * nr + j];
-        hb[(size_t)j * nk + k] = v * h->lam[k] / h->lam[j];
-        hf[(size_t)k * nr + j] = v * h->lam[j] / h->lam[k];
-      }
+    const int nr = h->n1, nk = h->nk, nm = h->n2;
+    // per azimuthal mode m (H is [Nk][Nr][N2]): Hb_m = (Lambda_k / Lambda_j) H_m^T and Hf_m = (Lambda_j / Lambda_k) H_m,
+    // the root-volume weighting of RootVolumeMultiply / RootVolumeDivide folded in (grid_tools.py:308-325), the
+    // modes' operators one after another
+    const size_t mat = (size_t)nr * nk;
+    std::vector<double> hb(mat * nm), hf(mat * nm);
+    for (int m = 0; m < nm; m++)
+      for (int k = 0; k < nk; k++)
+        for (int j = 0; j < nr; j++) {
+          double v = h->H[((size_t)k * nr + j) * nm + m];
+          hb[m * mat + (size_t)j * nk + k] = v * h->lam[k] / h->lam[j];
+          hf[m * mat + (size_t)k * nr + j] = v * h->lam[j] / h->lam[k];
+        }
     std::vector<T> hbt(hb.begin(), hb.end()), hft(hf.begin(), hf.end());
     UPPE_HIP(hipMemcpy(s->Hb, hbt.data(), sizeof(T) * hbt.size(), hipMemcpyHostToDevice));
     UPPE_HIP(hipMemcpy(s->Hf, hft.data(), sizeof(T) * hft.size(), hipMemcpyHostToDevice));
@@ -2844,8 +2899,9 @@ struct Fused {
     s->frb_hi = s->frb_lo = s->frf_hi = s->frf_lo = nullptr;
     s->fab_hi = s->fab_lo = s->faf_hi = s->faf_lo = nullptr;
     if (s->cm < 0) return;
-    // the adjoint's transposes: Hf^T has the shape of Hb, Hb^T that of Hf
-    std::vector<double> hft_((size_t)nr * nk), hbt_((size_t)nk * nr);
+    // the adjoint's transposes (one mode: the adjoint takes no azimuthal modes): Hf^T has the shape of Hb, Hb^T
+    // that of Hf
+    std::vector<double> hft_(mat), hbt_(mat);
     for (int k = 0; k < nk; k++)
       for (int j = 0; j < nr; j++) {
         hft_[(size_t)j * nk + k] = hf[(size_t)k * nr + j];
@@ -2859,14 +2915,34 @@ struct Fused {
       UPPE_HIP(hipMemcpy(d, v.data(), sizeof(v[0]) * v.size(), hipMemcpyHostToDevice));
       return d;
     };
+    // the fragment sets of the modes, one after another (the contraction's instance stride)
+    auto modes = [&](const std::vector<double>& all, auto pack) {
+      std::vector<double> one(mat);
+      decltype(pack(one)) out;
+      for (int m = 0; m < nm; m++) {
+        std::copy(all.begin() + m * mat, all.begin() + (m + 1) * mat, one.begin());
+        auto part = pack(one);
+        out.first.insert(out.first.end(), part.first.begin(), part.first.end());
+        out.second.insert(out.second.end(), part.second.begin(), part.second.end());
+      }
+      return out;
+    };
     if (s->cm == CM_BF16X3) {
+      auto pb = modes(hb, [&](const std::vector<double>& v) {
+        std::pair<std::vector<uint16_t>, std::vector<uint16_t>> r;
+        pack_bf16x3(v, nr, nk, r.first, r.second, &s->kp_b);
+        return r;
+      });
+      s->frb_hi = place(pb.first, SL_FRB_HI);
+      s->frb_lo = place(pb.second, SL_FRB_LO);
+      auto pf = modes(hf, [&](const std::vector<double>& v) {
+        std::pair<std::vector<uint16_t>, std::vector<uint16_t>> r;
+        pack_bf16x3(v, nk, nr, r.first, r.second, &s->kp_f);
+        return r;
+      });
+      s->frf_hi = place(pf.first, SL_FRF_HI);
+      s->frf_lo = place(pf.second, SL_FRF_LO);
       std::vector<uint16_t> hi, lo;
-      pack_bf16x3(hb, nr, nk, hi, lo, &s->kp_b);
-      s->frb_hi = place(hi, SL_FRB_HI);
-      s->frb_lo = place(lo, SL_FRB_LO);
-      pack_bf16x3(hf, nk, nr, hi, lo, &s->kp_f);
-      s->frf_hi = place(hi, SL_FRF_HI);
-      s->frf_lo = place(lo, SL_FRF_LO);
       pack_bf16x3(hft_, nr, nk, hi, lo, &kpa);
       s->fab_hi = upload(hi);
       s->fab_lo = upload(lo);
@@ -2874,11 +2950,19 @@ struct Fused {
       s->faf_hi = upload(hi);
       s->faf_lo = upload(lo);
     } else {
+      auto pb = modes(hb, [&](const std::vector<double>& v) {
+        std::pair<std::vector<T>, std::vector<T>> r;
+        pack_k4<T>(v, nr, nk, r.first, &s->kp_b);
+        return r;
+      });
+      s->frb_hi = place(pb.first, SL_FRB_HI);
+      auto pf = modes(hf, [&](const std::vector<double>& v) {
+        std::pair<std::vector<T>, std::vector<T>> r;
+        pack_k4<T>(v, nk, nr, r.first, &s->kp_f);
+        return r;
+      });
+      s->frf_hi = place(pf.first, SL_FRF_HI);
       std::vector<T> fr;
-      pack_k4<T>(hb, nr, nk, fr, &s->kp_b);
-      s->frb_hi = place(fr, SL_FRB_HI);
-      pack_k4<T>(hf, nk, nr, fr, &s->kp_f);
-      s->frf_hi = place(fr, SL_FRF_HI);
       pack_k4<T>(hft_, nr, nk, fr, &kpa);
       s->fab_hi = upload(fr);
       pack_k4<T>(hbt_, nk, nr, fr, &kpa);
@@ -3009,13 +3093,25 @@ struct Fused {
     UPPE_HIP(hipStreamSynchronize(h->stream));
   }
 
-  // SIMT contraction (the FP64 path and the one-off transforms of set_field/finish)
+  // SIMT contraction (the one-off transforms of set_field/finish): one launch per azimuthal mode m over the rows
+  // r N2 + m, with that mode's operator
   static void gemm_r2k_plain(uppe_handle* h, FusedState* s, const C* in, C* out) {
     dim3 grid(2 * s->nwp / BN, (h->nk + BM - 1) / BM);
-    launchf(h, UPPE_KC_HANKEL, f_gemm<T, PRO_NONE, EPI_NONE>, grid, dim3(256), 0, (const T*)s->Hf, in, out, h->nk,
-            h->n1, s->nwp, (const double*)nullptr, (const C*)nullptr, (const T*)nullptr, (C*)nullptr,
-            (const C*)nullptr, (C*)nullptr, (C*)nullptr, (C*)nullptr, (StepState*)nullptr,
-            (unsigned long long*)nullptr, (unsigned long long*)nullptr, 0, 0.0, 0.0, 0);
+    const int row = h->n2 * s->nwp;
+    for (int m = 0; m < h->n2; m++)
+      launchf(h, UPPE_KC_HANKEL, f_gemm<T, PRO_NONE, EPI_NONE>, grid, dim3(256), 0,
+              (const T*)s->Hf + (size_t)m * h->n1 * h->nk, in + (long)m * s->nwp, out + (long)m * s->nwp, h->nk,
+              h->n1, row, (const double*)nullptr, (const C*)nullptr, (const T*)nullptr, (C*)nullptr,
+              (const C*)nullptr, (C*)nullptr, (C*)nullptr, (C*)nullptr, (StepState*)nullptr,
+              (unsigned long long*)nullptr, (unsigned long long*)nullptr, 0, 0.0, 0.0, 0);
+  }
+
+  // numpy fft (sign -1) or ifft (sign +1) over the azimuthal points of every (r, f < ncols); nothing for N2 = 1
+  static void azim(uppe_handle* h, FusedState* s, C* a, int ncols, int sign) {
+    if (h->n2 == 1) return;
+    const long n = (long)h->n1 * ncols;
+    auto kern = h->n2 == 2 ? f_azim<T, 2> : (h->n2 == 4 ? f_azim<T, 4> : (h->n2 == 8 ? f_azim<T, 8> : f_azim<T, 16>));
+    launchf(h, UPPE_KC_TRANSVERSE_FFT, kern, dim3(grid_for(n)), dim3(kBlock), 0, a, h->n1, s->nwp, ncols, sign);
   }
 
   // the source kernel over every strip; a batched launch (nz instances zs elements apart, per-instance scalars
@@ -3146,6 +3242,16 @@ struct Fused {
                         C* k2, C* k3, StepState* st, int stage, double zfix, double dzfix, double dzprev,
                         int nz = 1, long zs = 0) {
     using F = typename MfmaTraits<CM>::frag_t;
+    // a single grid with azimuthal modes: one contraction per mode m (grid z), each over the rows r N2 + m of the
+    // [Nr N2][nwp] arrays with its own operator, all sharing the step state
+    int zst = 1;
+    long row = s->nwp;
+    if (zs == 0 && h->n2 > 1) {
+      nz = h->n2;
+      zs = s->nwp;
+      zst = 0;
+      row = (long)h->n2 * s->nwp;
+    }
     const int rw = 16 * pl.rt;
     const int bm = pl.nwv * rw;
     // narrow tiles: 8 columns each, the last one up to 15; wide tiles: 8 << wide columns each, the last one what
@@ -3159,9 +3265,9 @@ struct Fused {
 #define UPPE_HK(BNV, RTV)                                                                                        \
   launchf(h, EPI == EPI_NONE ? UPPE_KC_HANKEL : UPPE_KC_HANKEL_R2K, f_hankel<CM, PRO, EPI, BNV, RTV>, grid,      \
           dim3(64 * pl.nwv), shm, (const F*)ahi,                                                               \
-          (const F*)alo, B, out, M, K, kp, rw, s->nwp, 0, ncols, (const double*)s->phs, (const C*)s->kzc,         \
+          (const F*)alo, B, out, M, K, kp, rw, (int)row, 0, ncols, (const double*)s->phs, (const C*)s->kzc,       \
           (const T*)s->cond, q, kprev, k1, k2, k3, st, s->qmax, s->qmax_next, stage, zfix, dzfix, dzprev, pl.wide, \
-          zs)
+          zs, zst)
     if (pl.wide == 2) {
       if (pl.rt == 1)
         UPPE_HK(32, 1);
@@ -3199,7 +3305,9 @@ struct Fused {
         hankel_mfma<CM, PRO_STAGE_ENV, EPI_NONE>(h, s, 0, (const C*)s->q, er, kp[stage], stage, 0.0, s->st);
       else
         hankel_mfma<CM, PRO_STAGE, EPI_NONE>(h, s, 0, (const C*)s->q, er, kp[stage], stage, 0.0, s->st);
+      azim(h, s, er, h->nw, +1);  // azimuthal modes to phi (IFFT_axis2 after the radial transform)
       source(h, s, (const C*)er, (C*)s->jr, false, stage == 1 ? s->st : nullptr, 1, nullptr, 0, s->spt);
+      azim(h, s, (C*)s->jr, s->f1, -1);  // phi to modes over the nonlinear band (FFT_axis2 first)
       hankel_mfma<CM, PRO_NONE, EPI_STAGE>(h, s, 1, (const C*)s->jr, (C*)nullptr, nullptr, stage, 0.0, s->st);
     }
   }
@@ -3284,6 +3392,7 @@ struct Fused {
       tpass<PRO_NONE, EPI_NONE>(h, s, 1, (const C*)s->er, (C*)s->er, h->nw, -1, nullptr, 0, 0.0, nullptr);
       tpass<PRO_NONE, EPI_NONE>(h, s, 2, (const C*)s->er, (C*)s->q, h->nw, -1, nullptr, 0, 0.0, nullptr);
     } else {
+      azim(h, s, (C*)s->er, h->nw, -1);  // FFT_axis2 first (grid_tools.py:308-316)
       gemm_r2k_plain(h, s, (const C*)s->er, (C*)s->q);
     }
     UPPE_HIP(hipMemsetAsync(s->qmax, 0, sizeof(unsigned long long) * 3, h->stream));
@@ -3500,8 +3609,8 @@ struct Fused {
   }
 
   static void check_adjoint(uppe_handle* h, FusedState* s) {
-    if (h->d.geometry != UPPE_RADIAL || s->cm < 0)
-      throw Error{UPPE_ERR_UNSUPPORTED, "adjoint: radial geometry on the matrix-core path only"};
+    if (h->d.geometry != UPPE_RADIAL || s->cm < 0 || h->n2 != 1)
+      throw Error{UPPE_ERR_UNSUPPORTED, "adjoint: radial geometry with one azimuthal mode, on the matrix-core path"};
     if (h->paraxial || h->rot_states > 2)
       throw Error{UPPE_ERR_UNSUPPORTED, "adjoint: UPPE media without a rotator"};
   }
@@ -3637,11 +3746,16 @@ struct Fused {
       tpass<PRO_NONE, EPI_NONE>(h, s, 1, (const C*)s->er, (C*)s->er, h->nw, +1, nullptr, 0, z_total, nullptr);
     } else {
       dim3 g_k2r(2 * s->nwp / BN, (h->n1 + BM - 1) / BM);
-      launchf(h, UPPE_KC_HANKEL, f_gemm<T, PRO_FINISH, EPI_NONE>, g_k2r, dim3(256), 0, (const T*)s->Hb,
-              (const C*)s->q, (C*)s->er, h->n1, h->nk, s->nwp, (const double*)s->phs, (const C*)s->kzc,
-              (const T*)s->cond, (C*)nullptr, (const C*)nullptr, (C*)nullptr, (C*)nullptr, (C*)nullptr,
-              (StepState*)nullptr, (unsigned long long*)nullptr, (unsigned long long*)nullptr, 0, z_total, s->qbase,
-              s->f1);
+      const int row = h->n2 * s->nwp;
+      for (int m = 0; m < h->n2; m++) {
+        const long o = (long)m * s->nwp;
+        launchf(h, UPPE_KC_HANKEL, f_gemm<T, PRO_FINISH, EPI_NONE>, g_k2r, dim3(256), 0,
+                (const T*)s->Hb + (size_t)m * h->n1 * h->nk, (const C*)s->q + o, (C*)s->er + o, h->n1, h->nk, row,
+                (const double*)s->phs + o, (const C*)s->kzc + o, (const T*)s->cond, (C*)nullptr, (const C*)nullptr,
+                (C*)nullptr, (C*)nullptr, (C*)nullptr, (StepState*)nullptr, (unsigned long long*)nullptr,
+                (unsigned long long*)nullptr, 0, z_total, s->qbase, s->f1);
+      }
+      azim(h, s, (C*)s->er, h->nw, +1);
     }
     if (h->paraxial) {
       // the envelope source takes A itself (paraxial_kernel.py:243-245)
@@ -4248,8 +4362,8 @@ void fused_batch_advance(uppe_batch* b, const uppe_step_control* c, int* steps_o
 static bool is_fp64(uppe_handle* h) { return h->fused->prec == UPPE_PREC_FP64; }
 
 void fused_create(uppe_handle* h, const MemberSlots* slots) {
-  if (h->d.geometry == UPPE_RADIAL && h->n2 != 1)
-    throw Error{UPPE_ERR_UNSUPPORTED, "fused path: radial geometry with one azimuthal mode only"};
+  if (h->d.geometry == UPPE_RADIAL && (h->n2 < 1 || h->n2 > 16 || (h->n2 & (h->n2 - 1))))
+    throw Error{UPPE_ERR_UNSUPPORTED, "fused path: radial geometry with 1, 2, 4, 8 or 16 azimuthal points"};
   if (h->d.geometry == UPPE_CARTESIAN && (h->n1 < 2 || h->n2 < 2 || h->n1 > 512 || h->n2 > 512))
     throw Error{UPPE_ERR_UNSUPPORTED, "fused Cartesian path supports 2 <= Nx, Ny <= 512"};
   if (h->nt > kMaxNtFused || h->nt < 4) throw Error{UPPE_ERR_UNSUPPORTED, "fused path supports 4 <= Nt <= 4096"};
@@ -4267,6 +4381,8 @@ void fused_create(uppe_handle* h, const MemberSlots* slots) {
     s->cm = CM_BF16X3;
   else
     s->cm = simt ? -1 : CM_F32;
+  if (s->cm < 0 && h->d.geometry == UPPE_RADIAL && h->n2 > 1)
+    throw Error{UPPE_ERR_UNSUPPORTED, "fused path: azimuthal modes on the matrix-core contraction only"};
   if (is_fp64(h))
     Fused<double>::create(h, s);
   else

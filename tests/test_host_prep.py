@@ -122,6 +122,27 @@ class TestHankelBasis:
         from scipy.special import jn_zeros
         assert np.allclose(kr[:4] * R, jn_zeros(0, 4), rtol=1e-4)
 
+    def test_azimuthal_modes(self):
+        '''mmax = 2 (N2 = 4 azimuthal points): every mode's basis is orthonormal and diagonalises its own operator
+        (the (m/r)^2 term included); mode index 3 is m = -1, a copy of m = 1, and index 2 the unpaired m = 2
+        (grid_tools.py:266-298); each mode's smallest wavenumbers approach the zeros of J_m at the wall.'''
+        Nr, R = 512, 1.0
+        T = grid_tools.HankelTransformTool(Nr, R / Nr, 2, None, Nr)
+        assert T.H.shape == (Nr, Nr, 4)
+        from scipy.special import jn_zeros
+        for idx, m in enumerate([0, 1, 2, -1]):
+            H = T.H[:, :, idx]
+            assert np.allclose(H @ H.T, np.eye(Nr), atol=1e-11)
+            diag, off = T.tridiagonal(abs(m))
+            S = np.diag(diag) + np.diag(off, 1) + np.diag(off, -1)
+            vals = -T.kr2()[:, idx]
+            resid = S @ H.T - H.T * vals[None, :]
+            assert np.max(np.abs(resid)) < 1e-10 * np.max(np.abs(vals))
+            kr = np.sqrt(np.sort(T.kr2()[:, idx]))
+            assert np.allclose(kr[:4] * R, jn_zeros(abs(m), 4), rtol=1e-4)
+        assert np.array_equal(T.H[:, :, 3], T.H[:, :, 1])
+        assert not np.allclose(T.kr2()[:4, 2], T.kr2()[:4, 1])
+
     def test_parseval_notebook_known_answer(self):
         '''Energy of the notebook's test pulse in the time domain and through the modal transform.'''
         w00, dw00, r00, rmax, Nw, Nr = 0.1, .03, 100, 300, 65, 16
