@@ -162,6 +162,15 @@ enum { EPI_NONE = 0, EPI_STAGE = 1, EPI_ADJ = 2, EPI_ADJ_FIN = 3 };
 __device__ inline unsigned long long dbits(double x) { return (unsigned long long)__double_as_longlong(x); }
 __device__ inline double bitsd(unsigned long long b) { return __longlong_as_double((long long)b); }
 
+// The maxima of one run, kQW words: [0] the committed max|q|, [2] max|q| over the columns the r2k direction never
+// updates, then kSlots words (kSlotStride apart) for the stage-4 maxima of the running step. Each workgroup of a
+// stage-4 launch takes one slot, so that the workgroups do not serialise on one word; the commit takes the
+// maximum over the slots. The values are bit patterns of non-negative doubles, whose integer order is their order.
+// (The stage-1 step estimate keeps one word in the step state: every later launch of the step reads it.)
+constexpr int kSlots = 16, kSlotStride = 8, kQmaxSlot = 8, kQW = kQmaxSlot + kSlots * kSlotStride;
+
+__device__ inline int red_slot(int block) { return (block & (kSlots - 1)) * kSlotStride; }
+
 // ---------------------------------------------------------------------------------------------------
 // device step control (uppe_kernel.py:218-227, 299-325). No launch of its own: every launch of a step reads the
 // step state. The step size is a function of the committed state (stage 1's r2k epilogue leaves the minimum
@@ -289,8 +298,9 @@ __device__ inline double ref_amin(double amin, double ki, double qb) {
   return qb != 0.0 ? amin * exp_split(-ki * qb) : amin;
 }
 
-// commit a pending step (one thread): records of its dz and phase origin, the end of step, and the maxima
-// (qmax[2] holds max|q| over the columns the r2k direction never updates: the current is zero there)
+// commit a pending step (one thread): records of its dz and phase origin, the end of step, and the committed
+// max|q| over the stage-4 slots and qmax[2] (max|q| over the columns the r2k direction never updates: the current
+// is zero there), the slots cleared for the next step
 __device__ inline void step_commit(StepState* st, unsigned long long* qmax, double* rec, double* zrec, int zrec_cap) {
   if (!st->pend) return;
   st->pend = 0;
@@ -312,8 +322,14 @@ __device__ inline void step_commit(StepState* st, unsigned long long* qmax, doub
   s.dz = dz;
   end_step(s, dz);
   *st = s;
-  qmax[0] = qmax[1] > qmax[2] ? qmax[1] : qmax[2];
-  qmax[1] = 0ull;
+  unsigned long long m = qmax[2];
+#pragma unroll
+  for (int i = 0; i < kSlots; i++) {
+    const unsigned long long v = qmax[kQmaxSlot + i * kSlotStride];
+    m = v > m ? v : m;
+    qmax[kQmaxSlot + i * kSlotStride] = 0ull;
+  }
+  qmax[0] = m;
 }
 
 // exp(i phs z) exp(-ki z) with the phase formed in FP64 (long-z cancellation). The FP32 path reduces the
@@ -513,7 +529,7 @@ __global__ __launch_bounds__(256) void f_gemm(const T* __restrict__ A, const cpl
     if (is_min) {
       atomicMin(&st->dzraw, dbits(v));
     } else {
-      atomicMax(qmax_next, dbits(v));
+      atomicMax(&qmax_next[red_slot(blockIdx.x + gridDim.x * blockIdx.y)], dbits(v));
       // the step has run; the next step's stage-1 source launch commits its end
       if (blockIdx.x == 0 && blockIdx.y == 0) st->pend = 1;
     }
@@ -762,8 +778,8 @@ __device__ inline void hankel_tile(
     k3 = zshift(k3, zo);
     if (zst) {
       st = zshift(st, z);
-      qmax = zshift(qmax, 3 * z);
-      qmax_next = zshift(qmax_next, 3 * z);
+      qmax = zshift(qmax, (long)kQW * z);
+      qmax_next = zshift(qmax_next, (long)kQW * z);
       if (so.dzraw) so.dzraw = &const_cast<StepState*>(st)->dzraw;
       if (so.nanflag) so.nanflag = &const_cast<StepState*>(st)->nanflag;
       if (so.pend) so.pend = &const_cast<StepState*>(st)->pend;
@@ -1069,7 +1085,7 @@ __device__ inline void hankel_tile(
     if (is_min) {
       atomicMin(so.dzraw, dbits(v));
     } else {
-      atomicMax(qmax_next, dbits(v));
+      atomicMax(qmax_next + red_slot(tile + ntile * rowblk), dbits(v));
       // the step has run; the next step's stage-1 source launch commits its end
       if (so.pend && tile == 0 && rowblk == 0) *so.pend = 1;
     }
@@ -1266,7 +1282,7 @@ __global__ __launch_bounds__(1024) void f_tfft(
     if (is_min) {
       atomicMin(&st->dzraw, dbits(v));
     } else {
-      atomicMax(qmax_next, dbits(v));
+      atomicMax(&qmax_next[red_slot(blockIdx.x + gridDim.x * blockIdx.y)], dbits(v));
       // the step has run; the next step's stage-1 source launch commits its end
       if (blockIdx.x == 0 && blockIdx.y == 0) st->pend = 1;
     }
@@ -1674,7 +1690,7 @@ __global__ __launch_bounds__(1024) void f_source(const cplx<T>* __restrict__ er,
   // commits the previous step (commit non-null; nothing else of this launch reads the step state).
   if (commit && blockIdx.x == 0 && threadIdx.x == 0) {
     const int z = inst ? blockIdx.y : 0;
-    step_commit(commit + z, cq + 3 * z, rec, zrec, zrec_cap);
+    step_commit(commit + z, cq + (long)kQW * z, rec, zrec, zrec_cap);
   }
   extern __shared__ char smem[];
   cplx<T>* Y = reinterpret_cast<cplx<T>*>(smem);
@@ -2026,7 +2042,7 @@ __global__ __launch_bounds__(64 * WPB) void f_source_w(const cplx<float>* __rest
   // launch takes instance blockIdx.y, whose strips lie zs elements apart, with its own medium scalars
   if (commit && blockIdx.x == 0 && threadIdx.x == 0) {
     const int z = inst ? blockIdx.y : 0;
-    step_commit(commit + z, cq + 3 * z, inst ? nullptr : rec, inst ? nullptr : zrec, zrec_cap);
+    step_commit(commit + z, cq + (long)kQW * z, inst ? nullptr : rec, inst ? nullptr : zrec, zrec_cap);
   }
   __shared__ cplx<float> ybuf[WPB][1 << LG];
   const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
@@ -2463,7 +2479,7 @@ __global__ __launch_bounds__(1024) void f_source_adj_ion(const cplx<T>* __restri
 // commit the pending step of a chunk's last step (one block per instance in a batched launch, which records
 // nothing)
 __global__ void f_ctl_commit(StepState* st, unsigned long long* qmax, double* rec, double* zrec, int zrec_cap) {
-  step_commit(st + blockIdx.x, qmax + 3 * blockIdx.x, rec, zrec, zrec_cap);
+  step_commit(st + blockIdx.x, qmax + (long)kQW * blockIdx.x, rec, zrec, zrec_cap);
 }
 
 // zero n 8-byte words (a fill kernel, so that a stream the caller captures into a graph holds no memset node)
@@ -2850,9 +2866,9 @@ struct Fused {
     UPPE_HIP(hipMalloc(&s->chi_t, sizeof(T) * h->ntt));
     UPPE_HIP(hipMalloc(&s->ne_t, sizeof(T) * h->ntt));
     UPPE_HIP(hipMalloc(&s->stage, sizeof(double) * std::max(2 * h->npt, h->ntt)));
-    take((void**)&s->qmax, SL_QMAX, sizeof(unsigned long long) * 3);
-    s->qmax_next = s->qmax + 1;
-    UPPE_HIP(hipMemset(s->qmax, 0, sizeof(unsigned long long) * 3));
+    take((void**)&s->qmax, SL_QMAX, sizeof(unsigned long long) * kQW);
+    s->qmax_next = s->qmax + kQmaxSlot;
+    UPPE_HIP(hipMemset(s->qmax, 0, sizeof(unsigned long long) * kQW));
     take((void**)&s->st, SL_ST, sizeof(StepState));
     UPPE_HIP(hipMemset(s->st, 0, sizeof(StepState)));
     UPPE_HIP(hipHostMalloc(&s->st_host, sizeof(StepState), hipHostMallocDefault));
@@ -3395,7 +3411,7 @@ struct Fused {
       azim(h, s, (C*)s->er, h->nw, -1);  // FFT_axis2 first (grid_tools.py:308-316)
       gemm_r2k_plain(h, s, (const C*)s->er, (C*)s->q);
     }
-    UPPE_HIP(hipMemsetAsync(s->qmax, 0, sizeof(unsigned long long) * 3, h->stream));
+    UPPE_HIP(hipMemsetAsync(s->qmax, 0, sizeof(unsigned long long) * kQW, h->stream));
     launchf(h, UPPE_KC_REDUCE, f_qmax<T>, dim3(256), dim3(kBlock), 0, (const C*)s->q, P * s->nwp, s->qmax);
     launchf(h, UPPE_KC_REDUCE, f_qmax_cols<T>, dim3(256), dim3(kBlock), 0, (const C*)s->q, P, s->nwp, s->f1,
             s->qmax + 2);
@@ -3514,10 +3530,10 @@ struct Fused {
       grow(&s->zrec, &s->zrec_cap, 3 * (seg + 1) * every, sizeof(double), h->stream);
       grow((char**)&s->ck_q, &s->ckq_cap, seg + 1, sizeof(C) * stride, h->stream);
       grow(&s->ck_st, &s->ckst_cap, seg + 1, sizeof(StepState), h->stream);
-      grow(&s->ck_qmax, &s->ckqm_cap, 3 * (seg + 1), sizeof(unsigned long long), h->stream);
+      grow(&s->ck_qmax, &s->ckqm_cap, (long)kQW * (seg + 1), sizeof(unsigned long long), h->stream);
       UPPE_HIP(hipMemcpyAsync((C*)s->ck_q + seg * stride, s->q, sizeof(C) * stride, hipMemcpyDeviceToDevice, h->stream));
       UPPE_HIP(hipMemcpyAsync(s->ck_st + seg, s->st, sizeof(StepState), hipMemcpyDeviceToDevice, h->stream));
-      UPPE_HIP(hipMemcpyAsync(s->ck_qmax + 3 * seg, s->qmax, 3 * sizeof(unsigned long long), hipMemcpyDeviceToDevice,
+      UPPE_HIP(hipMemcpyAsync(s->ck_qmax + (long)kQW * seg, s->qmax, kQW * sizeof(unsigned long long), hipMemcpyDeviceToDevice,
                               h->stream));
       s->zrec_active = s->zrec;
       for (int i = 0; i < every; i++) step_mfma<CM>(h, s, (C*)s->er_store + (long)i * 4 * stride);
@@ -3578,7 +3594,7 @@ struct Fused {
         UPPE_HIP(hipMemcpyAsync(s->q, (C*)s->ck_q + seg * stride, sizeof(C) * stride, hipMemcpyDeviceToDevice,
                                 h->stream));
         UPPE_HIP(hipMemcpyAsync(s->st, s->ck_st + seg, sizeof(StepState), hipMemcpyDeviceToDevice, h->stream));
-        UPPE_HIP(hipMemcpyAsync(s->qmax, s->ck_qmax + 3 * seg, 3 * sizeof(unsigned long long),
+        UPPE_HIP(hipMemcpyAsync(s->qmax, s->ck_qmax + (long)kQW * seg, kQW * sizeof(unsigned long long),
                                 hipMemcpyDeviceToDevice, h->stream));
         s->rec_active = nullptr;
         s->zrec_active = nullptr;
@@ -3880,7 +3896,7 @@ void fused_batch_create(uppe_batch* b, const uppe_desc* descs, int n, MemberFact
     G.stride[SL_NG] = 8 * P;
     for (int k : {SL_KZC, SL_Q, SL_K1, SL_K2, SL_K3, SL_ER, SL_JR}) G.stride[k] = cs;
     G.stride[SL_ST] = sizeof(StepState);
-    G.stride[SL_QMAX] = 3 * sizeof(unsigned long long);
+    G.stride[SL_QMAX] = kQW * sizeof(unsigned long long);
     G.stride[SL_FRB_HI] = fbytes * (size_t)frag_count(G.cm, G.n1, G.nk);
     G.stride[SL_FRF_HI] = fbytes * (size_t)frag_count(G.cm, G.nk, G.n1);
     G.stride[SL_FRB_LO] = G.cm == CM_BF16X3 ? G.stride[SL_FRB_HI] : 0;
