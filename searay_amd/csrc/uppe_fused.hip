@@ -1730,9 +1730,9 @@ __global__ __launch_bounds__(1024) void f_source(const cplx<T>* __restrict__ er,
     }
   }
   // per-pass twiddle tables of the N- and N/2-point transforms (fewer than N + N/2 entries): in LDS for
-  // FP32, read from L2 for FP64 (LDS would not fit)
+  // FP32 strips up to 4096 samples, read from L2 for FP64 and the 8192-sample strips (LDS would not fit)
   const cplx<T>* tws = tw;
-  if constexpr (sizeof(T) == 4) {
+  if constexpr (sizeof(T) == 4 && SPT <= 4) {
     constexpr int TPT = (3 * SPT + 1) / 2;  // table entries per thread, N + N/2 = 1.5 SPT nth
     cplx<T>* tl = Y + N;
     cplx<T> tv[TPT];
@@ -1765,7 +1765,7 @@ __global__ __launch_bounds__(1024) void f_source(const cplx<T>* __restrict__ er,
     Y[lds_swz<T>(k)] = cmk(ze.re - zo.im, ze.im + zo.re);
   }
   __syncthreads();
-  lds_fft_r8<T, (LGC > 0 ? LGC - 1 : 0), NTH>(Y, M, lgm, tws + r8_table_len(lgn), +1, tid);
+  lds_fft_r8<T, (LGC > 0 ? LGC - 1 : 0), NTH, (SPT >= 8 ? 4 : 2)>(Y, M, lgm, tws + r8_table_len(lgn), +1, tid);
   const T invm = T(1) / T(M);
   // thread t owns samples [a0, a0 + SPT) with a0 = (nth - 1 - t) SPT: the depletion integral runs from the
   // last sample backward, which makes it a prefix sum over the threads in lane order
@@ -1860,7 +1860,7 @@ __global__ __launch_bounds__(1024) void f_source(const cplx<T>* __restrict__ er,
   }
   lds_write_reals<T, 2 * SPT>(Y, 2 * a0, yrun);
   __syncthreads();
-  lds_fft_r8<T, LGC, NTH>(Y, N, lgn, tws, -1, tid);
+  lds_fft_r8<T, LGC, NTH, (SPT >= 8 ? 4 : 2)>(Y, N, lgn, tws, -1, tid);
 
   // 4. J = F_NL [ rfft(J_p)/(0.1 - i w) + (-i w) rfft(P) ] (fft.cl:282-329, uppe_kernel.py:170-186)
 #pragma unroll
@@ -3140,12 +3140,13 @@ struct Fused {
     double* zrec = (commit && !inst) ? s->zrec_active : nullptr;
     const int zrec_cap = s->zrec_cap / 3;
     const int N = h->nt;
-    // four samples per thread (Nt >= 256): short strips then get short workgroups, several per CU
-    const int nth = spt_req > 0 ? N / spt_req : std::max(std::min(N, 64), N / 4);
+    // four samples per thread (Nt >= 256): short strips then get short workgroups, several per CU; eight for the
+    // 8192-sample strips (a workgroup holds at most 1024 threads)
+    const int nth = spt_req > 0 ? N / spt_req : std::max(std::min(N, 64), std::min(N / 4, 1024));
     const int spt = N / nth;
     if (spt_req > 0 && (h->rot_states > 2 || h->paraxial || nth > 1024 || nth < 64))
       throw Error{UPPE_ERR_ARG, "source kernel: unsupported samples per thread"};
-    const size_t shm = sizeof(C) * (sizeof(T) == 4 ? N + N + N / 2 : N);
+    const size_t shm = sizeof(C) * (sizeof(T) == 4 && N <= kMaxNtTables ? N + N + N / 2 : N);
     const int lgn = ilog2i(N);
     // the explicit-rate prefactor, in the base of the exponential each precision uses
     const double log_pre = s->ion.kind != UPPE_ION_EXPLICIT ? 0.0
@@ -3190,7 +3191,7 @@ struct Fused {
   if (spt == S) {                                                                                                \
     auto kern = rot ? (diag ? f_source<T, S, true, true> : f_source<T, S, false, true>)                         \
                     : (diag ? f_source<T, S, true, false> : f_source<T, S, false, false>);                      \
-    if constexpr (sizeof(T) == 4)                                                                                \
+    if constexpr (sizeof(T) == 4 && S <= 4)                                                                      \
       if (!rot && !diag && lgn >= 10 && lgn <= 12)                                                               \
         kern = lgn == 10 ? source_fixed<S, 10>() : (lgn == 11 ? source_fixed<S, 11>() : source_fixed<S, 12>());  \
     launchf(h, UPPE_KC_SOURCE, kern, grid, block, shm, er, jr, diag ? (T*)s->chi_t : (T*)nullptr,                 \
@@ -3200,7 +3201,7 @@ struct Fused {
             s->qmax, rec, zrec, zrec_cap);                                                                       \
     return;                                                                                                      \
   }
-    UPPE_SRC(1) UPPE_SRC(2) UPPE_SRC(4)
+    UPPE_SRC(1) UPPE_SRC(2) UPPE_SRC(4) UPPE_SRC(8)
 #undef UPPE_SRC
     throw Error{UPPE_ERR_UNSUPPORTED, "fused source kernel: unsupported Nt"};
   }
@@ -3629,6 +3630,7 @@ struct Fused {
       throw Error{UPPE_ERR_UNSUPPORTED, "adjoint: radial geometry with one azimuthal mode, on the matrix-core path"};
     if (h->paraxial || h->rot_states > 2)
       throw Error{UPPE_ERR_UNSUPPORTED, "adjoint: UPPE media without a rotator"};
+    if (h->nt > kMaxNtTables) throw Error{UPPE_ERR_UNSUPPORTED, "adjoint: Nt <= 4096"};
   }
 
   static int advance_checkpointed(uppe_handle* h, FusedState* s, const uppe_step_control* c, int every) {
@@ -4113,10 +4115,10 @@ struct BatchOps {
             F::source(h, s, (const C*)s->er, (C*)s->jr, false, nullptr, nz, src, zs, spt);
           }, reps);
         };
-        // the incoming choice (0: the default shape, max(min(Nt, 64), Nt/4) threads)
-        int cur = *spt_out > 0 ? *spt_out : h->nt / std::max(std::min(h->nt, 64), h->nt / 4);
+        // the incoming choice (0: the default shape, max(min(Nt, 64), min(Nt/4, 1024)) threads)
+        int cur = *spt_out > 0 ? *spt_out : h->nt / std::max(std::min(h->nt, 64), std::min(h->nt / 4, 1024));
         best = source(cur);
-        for (int spt = 1; spt <= 4; spt *= 2) {
+        for (int spt = 1; spt <= 8; spt *= 2) {
           const int nth = h->nt / spt;
           if (spt == cur || nth > 1024 || nth < 64) continue;
           const float t = source(spt);
@@ -4164,7 +4166,7 @@ static int default_spt(const uppe_handle* h, const FusedState* s, long strips) {
   if (h->rot_states > 2 || h->paraxial) return 0;
   const char* env = getenv("SEARAY_UPPE_SOURCE_SPT");
   const int want = env ? std::atoi(env) : (strips < s->n_cu ? std::max(1, h->nt / 1024) : 0);
-  if (want > 0 && (want & (want - 1)) == 0 && want <= 4 && h->nt / want <= 1024 && h->nt / want >= 64) return want;
+  if (want > 0 && (want & (want - 1)) == 0 && want <= 8 && h->nt / want <= 1024 && h->nt / want >= 64) return want;
   return 0;
 }
 
@@ -4382,7 +4384,9 @@ void fused_create(uppe_handle* h, const MemberSlots* slots) {
     throw Error{UPPE_ERR_UNSUPPORTED, "fused path: radial geometry with 1, 2, 4, 8 or 16 azimuthal points"};
   if (h->d.geometry == UPPE_CARTESIAN && (h->n1 < 2 || h->n2 < 2 || h->n1 > 512 || h->n2 > 512))
     throw Error{UPPE_ERR_UNSUPPORTED, "fused Cartesian path supports 2 <= Nx, Ny <= 512"};
-  if (h->nt > kMaxNtFused || h->nt < 4) throw Error{UPPE_ERR_UNSUPPORTED, "fused path supports 4 <= Nt <= 4096"};
+  if (h->nt > kMaxNtFused || h->nt < 4) throw Error{UPPE_ERR_UNSUPPORTED, "fused path supports 4 <= Nt <= 8192"};
+  if (h->paraxial && h->nt > kMaxNtTables)
+    throw Error{UPPE_ERR_UNSUPPORTED, "fused envelope path supports Nw <= 4096"};
   if (h->d.geometry == UPPE_RADIAL && h->n1 > 512)
     throw Error{UPPE_ERR_UNSUPPORTED, "fused path supports up to 512 radial cells"};
   auto* s = new FusedState();

@@ -14,7 +14,8 @@
 namespace uppe {
 
 constexpr int kBlock = 256;        // threads per workgroup for point and strip kernels (4 wavefronts)
-constexpr int kMaxNtFused = 4096;  // fused source kernel keeps one Nt-point complex strip in LDS
+constexpr int kMaxNtFused = 8192;  // fused source kernel keeps one Nt-point complex strip in LDS
+constexpr int kMaxNtTables = 4096;  // FP32 strips up to this length also keep their twiddle tables in LDS
 constexpr int kMaxNtPlain = 16384; // plain strip FFT keeps Nt/2 complex points in LDS (FP64)
 
 struct Error {
@@ -122,7 +123,8 @@ __device__ inline int lds_swz(int i) {
 // exp(sign 2 pi i r (j mod Ns)/(R Ns)), forms the R-point DFT and writes element
 // (j - j mod Ns) R + j mod Ns + r Ns (autosorted). Each pass loads its inputs into registers, waits for the
 // block, then stores, so no second buffer is needed; the block has at least n/8 threads for the radix-8 passes
-// and n/4 for the first pass (one or two butterflies per thread).
+// and n/(2 MAXB) for the first pass (up to MAXB butterflies per thread; MAXB = 2 by default, 4 for the
+// 8192-sample strips of a 1024-thread block).
 // Twiddles: per-pass tables, the radix-8 pass of span Ns holding exp(-2 pi i r k/(8 Ns)) at
 // [(Ns - R0) + (r - 1) Ns + k], r = 1..7, k < Ns, with R0 = 2^(lg mod 3) the span of the first radix-8
 // pass; the table of an n-point transform has R0 (8^(lg/3) - 1) entries. Threads of a pass read consecutive
@@ -174,19 +176,20 @@ __device__ inline void dft8(const cplx<T>* v, cplx<T>* y, int sign) {
 }
 
 // LG > 0 fixes lg = LG (and n, and the block size NTH when nonzero) at compile time: the passes then unroll
-// with their spans, strides and table offsets as constants.
-template <typename T, int LG = 0, int NTH = 0>
+// with their spans, strides and table offsets as constants. MAXB bounds the butterflies a thread takes in the
+// first pass.
+template <typename T, int LG = 0, int NTH = 0, int MAXB = 2>
 __device__ inline void lds_fft_r8(cplx<T>* buf, int n_rt, int lg_rt, const cplx<T>* tab, int sign, int tid) {
   const int lg = LG > 0 ? LG : lg_rt;
   const int n = LG > 0 ? (1 << LG) : n_rt;
   const int r0 = lg % 3;
   int Ns = 1;
   if (r0 > 0) {
-    // up to two butterflies per thread (a radix-2 pass has n/2 of them, the block at least n/4 threads)
+    // up to MAXB butterflies per thread (a radix-2 pass has n/2 of them)
     const int R = 1 << r0, nb = n >> r0, nth = NTH > 0 ? NTH : (int)blockDim.x;
-    cplx<T> v[2][4];
+    cplx<T> v[MAXB][4];
 #pragma unroll
-    for (int b = 0; b < 2; b++) {
+    for (int b = 0; b < MAXB; b++) {
       const int j = tid + b * nth;
       if (j < nb)
 #pragma unroll
@@ -195,7 +198,7 @@ __device__ inline void lds_fft_r8(cplx<T>* buf, int n_rt, int lg_rt, const cplx<
     }
     __syncthreads();
 #pragma unroll
-    for (int b = 0; b < 2; b++) {
+    for (int b = 0; b < MAXB; b++) {
       const int j = tid + b * nth;
       if (j >= nb) continue;
       if (R == 4) {

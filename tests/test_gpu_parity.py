@@ -277,3 +277,19 @@ def test_paraxial_filament(oracle_mod, coords, precision):
     assert rel_l2(got[0][..., 1], ref[0][..., 1]) < tol
     assert rel_l2(got[2][..., 1], ref[2][..., 1]) < tol_ne
     assert rel_l2(got[3][..., 1], ref[3][..., 1]) < tol_ne
+
+
+@pytest.mark.parametrize('precision', ['fp64-plain', 'fp64', 'fp32', 'bf16x3'])
+def test_radial_ionization_8192_samples(oracle_mod, precision):
+    '''Strips of Nt = 8192 samples (Nw = 4097: the photoionisation workload with its band doubled at the same
+    frequency step), where the source kernel takes eight samples per thread and reads its twiddles from L2.'''
+    vol = configs.radial_o2_ionization(nr=16, steps=2, length='1 mm')
+    vol['wave grid'] = (4097,) + tuple(vol['wave grid'][1:])
+    vol['frequency band'] = (0.0, 2 * vol['frequency band'][1])
+    _say('radial_ionization Nt=8192 nr=16 %s' % precision)
+    ref = oracle_mod.oracle_track(vol)
+    got = _run(vol, precision)
+    fp64 = precision.startswith('fp64')
+    assert rel_l2(got[0][..., 1], ref[0][..., 1]) < (FP64_TOL if fp64 else FP32_FIELD_TOL)
+    assert rel_l2(got[3][..., 1], ref[3][..., 1]) < (FP64_TOL if fp64 else FP32_NE_TOL)
+    assert got[5] == ref[5]
