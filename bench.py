@@ -72,6 +72,8 @@ def run_steps(rs, st, K, schedule=None, record=False):
 def nl_columns(rs):
     '''Complex columns the r->k contraction covers: up to one past the nonlinear filter's last nonzero bin, and at
     least the conditioner's four bins.'''
+    if getattr(rs, 'nl_filter', None) is None:
+        return int(rs.N[0])  # the envelope model filters nothing
     nz = np.nonzero(np.asarray(rs.nl_filter) != 0.0)[0]
     return int(min(rs.N[0], max((nz[-1] + 1) if nz.size else 0, 4)))
 
@@ -173,13 +175,13 @@ def pcie_inclusive(rs, prec, K):
 CLASSES = ('hankel', 'hankel_r2k', 'source', 'point', 'reduce', 'transverse_fft')
 
 
-def radial_measure(vol, prec, K, W, plain_steps, compare=True, counters=None, error_steps=None):
+def radial_measure(vol, prec, K, W, plain_steps, compare=True, counters=None, error_steps=None, module=uppe_kernel):
     '''One radial workload on one handle: K timed steps after W warm-up steps (inputs resident in HBM), the same K
     steps again with dispatch-stamped HIP events on every launch (per-kernel averages and the roofline of the
     dominant kernel), then the same-precision fused FP64 and the FP32 steppers over the same K steps, the plain FP64
     stepper over plain_steps steps, and the field error of prec (and FP32) against fused FP64 on FP64's step schedule
-    after error_steps steps (default K).'''
-    rs = uppe_kernel.RunSetup(vol)
+    after error_steps steps (default K). module: uppe_kernel, or paraxial_kernel for the envelope model.'''
+    rs = module.RunSetup(vol)
     pts = rs.N[0] * rs.N[1]
     st = rs.make_stepper(uppe_kernel.hip_stepper_factory(prec))
     if W > 0:
@@ -495,7 +497,7 @@ def main():
     if args.workload == 'paraxial':
         vol = configs.paraxial_air_filament(steps=400, rate=args.rate)
         log('paraxial workload: %d timed steps (%s)' % (K, prec))
-        rs, r = radial_measure(vol, prec, K, W, args.plain_steps, compare)
+        rs, r = radial_measure(vol, prec, K, W, args.plain_steps, compare, module=paraxial_kernel)
         cpu = None
         if not args.no_cpu:
             cv, cn, ct_, threads = cpu_baseline(paraxial_kernel, vol, 60)

@@ -2111,8 +2111,7 @@ __global__ __launch_bounds__(1024) void f_source_env(const cplx<T>* __restrict__
     Y2[lds_swz<T>(k)] = xa[i];
   }
   __syncthreads();
-  lds_fft_r8<T, LGC, NTH>(Y1, N, lgn, tws, +1, tid);
-  lds_fft_r8<T, LGC, NTH>(Y2, N, lgn, tws, +1, tid);
+  lds_fft_r8_pair<T, LGC, NTH>(Y1, Y2, N, lgn, tws, +1, tid);
   const T invn = T(1) / T(N);
   const int a0 = (nth - 1 - tid) * SPT;
   cplx<T> e[SPT];
@@ -2186,8 +2185,7 @@ __global__ __launch_bounds__(1024) void f_source_env(const cplx<T>* __restrict__
     }
   }
   __syncthreads();
-  lds_fft_r8<T, LGC, NTH>(Y1, N, lgn, tws, -1, tid);
-  lds_fft_r8<T, LGC, NTH>(Y2, N, lgn, tws, -1, tid);
+  lds_fft_r8_pair<T, LGC, NTH>(Y1, Y2, N, lgn, tws, -1, tid);
   // J = -i w FFT(P) + FFT(Q) (DtSpectral, sgn +1)
 #pragma unroll
   for (int i = 0; i < SPT; i++) {
@@ -2401,8 +2399,7 @@ __global__ __launch_bounds__(1024) void f_source_adj_ion(const cplx<T>* __restri
     Z2[lds_swz<T>(k)] = y2;
   }
   __syncthreads();
-  lds_fft_r8<T, LGC, NTH>(Z, N, lgn, tws, +1, tid);
-  lds_fft_r8<T, LGC, NTH>(Z2, N, lgn, tws, +1, tid);
+  lds_fft_r8_pair<T, LGC, NTH>(Z, Z2, N, lgn, tws, +1, tid);
   const double ngp = ng[p];
   const T kc3 = (T)(3.0 * ngp * redchi3), invn = T(1) / T(N), ngt = (T)ngp, dtt = (T)dt;
   const int a0 = tid * SPT;

@@ -179,14 +179,15 @@ __device__ inline void dft8(const cplx<T>* v, cplx<T>* y, int sign) {
 // with their spans, strides and table offsets as constants. MAXB bounds the butterflies a thread takes in the
 // first pass.
 template <typename T, int LG = 0, int NTH = 0, int MAXB = 2>
-__device__ inline void lds_fft_r8(cplx<T>* buf, int n_rt, int lg_rt, const cplx<T>* tab, int sign, int tid) {
+__device__ inline void lds_fft_r8(cplx<T>* buf, int n_rt, int lg_rt, const cplx<T>* tab, int sign, int tid,
+                                  int nth_rt = 0) {
   const int lg = LG > 0 ? LG : lg_rt;
   const int n = LG > 0 ? (1 << LG) : n_rt;
   const int r0 = lg % 3;
   int Ns = 1;
   if (r0 > 0) {
-    // up to MAXB butterflies per thread (a radix-2 pass has n/2 of them)
-    const int R = 1 << r0, nb = n >> r0, nth = NTH > 0 ? NTH : (int)blockDim.x;
+    // up to MAXB butterflies per thread (a radix-2 pass has n/2 of them); the block's threads, or nth_rt of them
+    const int R = 1 << r0, nb = n >> r0, nth = NTH > 0 ? NTH : (nth_rt > 0 ? nth_rt : (int)blockDim.x);
     cplx<T> v[MAXB][4];
 #pragma unroll
     for (int b = 0; b < MAXB; b++) {
@@ -244,6 +245,25 @@ __device__ inline void lds_fft_r8(cplx<T>* buf, int n_rt, int lg_rt, const cplx<
     for (int Nc = 1 << (LG % 3); Nc < (1 << LG); Nc *= 8) pass(Nc);
   } else {
     for (; Ns < n; Ns *= 8) pass(Ns);
+  }
+}
+
+// Two transforms of the same length and direction (buffers a and b, one table), each on one half of the block
+// when a half covers a pass (n/8 butterflies of the radix-8 passes, and the first pass's within 2 per thread): both
+// then share every pass's barriers, where one after the other they would take twice as many. Otherwise in turn.
+template <typename T, int LG = 0, int NTH = 0>
+__device__ inline void lds_fft_r8_pair(cplx<T>* a, cplx<T>* b, int n_rt, int lg_rt, const cplx<T>* tab, int sign,
+                                       int tid) {
+  const int lg = LG > 0 ? LG : lg_rt;
+  const int n = LG > 0 ? (1 << LG) : n_rt;
+  const int nth = NTH > 0 ? NTH : (int)blockDim.x, half = nth / 2;
+  const int nb0 = (lg % 3) ? n >> (lg % 3) : 0;
+  if (half >= n / 8 && 2 * half >= nb0 && half >= 64) {
+    const bool second = tid >= half;
+    lds_fft_r8<T, LG, NTH / 2>(second ? b : a, n, lg, tab, sign, second ? tid - half : tid, half);
+  } else {
+    lds_fft_r8<T, LG, NTH>(a, n, lg, tab, sign, tid);
+    lds_fft_r8<T, LG, NTH>(b, n, lg, tab, sign, tid);
   }
 }
 
