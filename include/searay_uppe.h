@@ -223,8 +223,9 @@ int uppe_profile_read(uppe_handle* h, int kernel_class, double* total_ms, long l
  * with uppe_finish / uppe_get_state / uppe_get_info; uppe_destroy refuses them (uppe_batch_destroy releases
  * them). uppe_batch_advance runs uppe_advance on every member at once, c[i] being member i's step control
  * (adaptive or fixed steps: no schedule, no record; members may differ in subcycles and step counts) and
- * steps_out[i] (nullable) its step count: consecutive members of a group with the same nonlinear band and
- * frequency tables share every launch. Rotators are not taken. With autotune on (the default) the first
+ * steps_out[i] (nullable) its step count: consecutive members of a group with the same nonlinear band,
+ * frequency tables and presence of a rotator (uppe_set_rotator, per member) share every launch. With autotune on
+ * (the default) the first
  * advance of each run times the candidate contraction tilings and source launch shapes on the run and keeps
  * the fastest; uppe_batch_plan reports member i's run as {k2r tiles, waves, row tiles per wave, r2k tiles,
  * waves, row tiles per wave, source samples per thread (0 = default), run length, first slot, group}.

@@ -54,6 +54,9 @@ constexpr double kRebaseSpan = 8.0;
 struct SrcInst {
   double redchi3, log_pre;
   uppe_ionization ion;
+  const void* rot_tab;  // the rotator rows (RotRow<T>), its state count and 0.5 dalpha / hbar, when it has one
+  int rot_states;
+  double rot_fc;
 };
 
 // arrays a batch member keeps inside its batch's slabs (one slab per kind, instances a fixed stride apart)
@@ -1707,6 +1710,11 @@ __global__ __launch_bounds__(1024) void f_source(const cplx<T>* __restrict__ er,
     redchi3 = inst[z].redchi3;
     log_pre = inst[z].log_pre;
     ion = inst[z].ion;
+    if (ROT) {
+      rot_tab = static_cast<const RotRow<T>*>(inst[z].rot_tab);
+      rot_states = inst[z].rot_states;
+      rot_fc = inst[z].rot_fc;
+    }
   }
   constexpr int NTH = LGC > 0 ? (1 << LGC) / SPT : 0;
   const int tid = threadIdx.x, nth = NTH > 0 ? NTH : (int)blockDim.x, lane = tid & 63, wave = tid >> 6,
@@ -4106,7 +4114,8 @@ struct BatchOps {
           plan[1] = p;
         }
       }
-      if (src && !getenv("SEARAY_UPPE_SOURCE_SPT")) {  // a pinned shape is kept
+      if (src && !getenv("SEARAY_UPPE_SOURCE_SPT") && h->rot_states <= 2) {  // a pinned shape is kept, and the
+                                                                               // rotator's
         auto source = [&](int spt) {
           return time_launch(h->stream, e0, e1, [&] {
             F::source(h, s, (const C*)s->er, (C*)s->jr, false, nullptr, nz, src, zs, spt);
@@ -4239,7 +4248,8 @@ static void build_graph(uppe_batch* b, int steps) {
 // whether members a and b can share one launch: the same nonlinear band, frequency tables and source step
 static bool same_run(const uppe_handle* a, const uppe_handle* b) {
   const FusedState *x = a->fused, *y = b->fused;
-  return x->f1 == y->f1 && x->w_h == y->w_h && x->nl_h == y->nl_h && x->cond_h == y->cond_h && x->dt == y->dt;
+  return x->f1 == y->f1 && x->w_h == y->w_h && x->nl_h == y->nl_h && x->cond_h == y->cond_h && x->dt == y->dt &&
+         (a->rot_states > 2) == (b->rot_states > 2);
 }
 
 void fused_batch_advance(uppe_batch* b, const uppe_step_control* c, int* steps_out) {
@@ -4252,7 +4262,6 @@ void fused_batch_advance(uppe_batch* b, const uppe_step_control* c, int* steps_o
     const std::string who = "batch member " + std::to_string(i) + ": ";
     if (!h->have_hankel || !h->have_medium || !h->have_field)
       throw Error{UPPE_ERR_STATE, who + "needs the Hankel basis, the medium and the field first"};
-    if (h->rot_states > 2) throw Error{UPPE_ERR_UNSUPPORTED, who + "batches take no rotator"};
     if (c[i].n_schedule != 0 || c[i].dz_record)
       throw Error{UPPE_ERR_UNSUPPORTED, who + "batches run adaptive or fixed steps (no schedule, no record)"};
     if (c[i].n_sub < 1 || !(c[i].length > 0.0)) throw Error{UPPE_ERR_ARG, who + "bad step control"};
@@ -4265,9 +4274,13 @@ void fused_batch_advance(uppe_batch* b, const uppe_step_control* c, int* steps_o
   for (BatchGroup& G : fb->groups) {
     std::vector<SrcInst> src(G.members.size());
     for (size_t j = 0; j < G.members.size(); j++) {
-      const FusedState* s = b->members[(size_t)G.members[j]]->fused;
+      const uppe_handle* hm = b->members[(size_t)G.members[j]];
+      const FusedState* s = hm->fused;
       src[j].redchi3 = s->redchi3;
       src[j].ion = s->ion;
+      src[j].rot_tab = hm->rot_states > 2 ? s->rot_tab : nullptr;
+      src[j].rot_states = hm->rot_states;
+      src[j].rot_fc = hm->rot_states > 2 ? 0.5 * hm->rot_dalpha / hm->rot_hbar : 0.0;
       src[j].log_pre = s->ion.kind != UPPE_ION_EXPLICIT ? 0.0
                        : G.prec == UPPE_PREC_FP64       ? std::log(s->ion.rate_conv * s->ion.c_pre)
                                                         : std::log2(s->ion.rate_conv * s->ion.c_pre);

@@ -210,3 +210,31 @@ def test_batch_members_with_different_bands_split_runs(monkeypatch):
         n, q = _standalone(vol, 'fp32')
         assert steps[i] == n
         assert rel_l2(qs[i], q) < 1e-12, (i, rel_l2(qs[i], q))
+
+
+@pytest.mark.parametrize('precision', ['fp64', 'fp32', 'bf16x3'])
+def test_batch_with_rotator_matches_standalone(precision):
+    '''Members 2 (alone in its grid) and 4 (the middle one of three Nr = 64 members) carry the 32-state N2 rotator:
+    a member with a rotator splits its group into runs of its own, and each instance's rotator rows come from the
+    per-instance table of the batched source launch. Same source launch shape in the batch and standalone
+    (autotune off), so 1e-12 as above.'''
+    vols, drawn = _mixed_vols(False)
+    assert [d[0] for d in drawn[3:6]] == [64, 64, 64]
+    for i in (2, 4):
+        vols[i]['rotator'] = configs.n2_rotator(32)
+    _say('batch with rotator vs standalone %s grids %s' % (precision, [d[0] for d in drawn]))
+    with batch.Batch(vols, precision=precision, autotune=False) as b:
+        b.set_fields()
+        steps = b.advance()
+        qs = [m.get_state() for m in b.members]
+    for i, vol in enumerate(vols):
+        n, q = _standalone(vol, precision)
+        assert steps[i] == n
+        assert np.all(np.isfinite(q))
+        assert rel_l2(qs[i], q) < 1e-12, (i, rel_l2(qs[i], q))
+    # the rotator changes the member's field: it is not lost in the comparison
+    i = 4
+    vol = dict(vols[i])
+    del vol['rotator']
+    _, q0 = _standalone(vol, precision)
+    assert rel_l2(qs[i], q0) > 1e-6
