@@ -14,8 +14,8 @@ seeded generator.
 The forward run goes through uppe_advance_checkpointed (the state every `checkpoint_every` steps), finish gives
 A(z_f), and uppe_adjoint carries the cotangent G = dL/dRe A + i dL/dIm A (4 (|A|^2 - S) A on axis, or
 4 (F - F_tgt) w^2 A) back to the initial field; then dL/dc_n = -sum_{w,r} P_n(x(w)) Im(conj(G_0) A_0).
-``spectral_phase_loss_function`` puts the pair behind a torch.autograd.Function. Radial UPPE media with Kerr,
-photoionisation and the plasma current (no rotator) on the matrix-core paths.
+``spectral_phase_loss_function`` puts the pair behind a torch.autograd.Function. Radial UPPE media with Kerr, the
+rotational Raman susceptibility, photoionisation and the plasma current, on the matrix-core paths.
 '''
 import numpy as np
 from scipy.special import eval_legendre

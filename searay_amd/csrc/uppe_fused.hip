@@ -1677,6 +1677,91 @@ __device__ inline void rot_chi(const R* drive, R ngp, R* dchi, const RotRow<R>* 
   }
 }
 
+// Adjoint of rot_chi, for a thread that owns the samples n = n0 .. n0 + SPT - 1 with n0 = SPT t (m = N - 1 - n
+// then falls along the thread, the reverse of rot_chi's order). With gam the cotangent of chi_rot at these samples,
+// it adds to gd the cotangent of the drive. Writing rot_chi's state j (w_j its row's (wr, wi)) as
+//   P_j(m) = sum_{m' < m} F(m') E^(2m'+1),   g_j(m) = c_j n_g Im(-i dt P_j(m) conj(E^(2m)) w_j)
+//          = -c_j n_g dt Re(P_j(m) conj(E^(2m)) w_j),   chi_rot(m) = sum_{m'' <= m} sum_j g_j(m''),   F = fc drive,
+// the reverse sweep is, with Gamma(m) = sum_{m' >= m} gam(m') and U_j(m) = sum_{m' > m} Gamma(m') conj(E^(2m')) w_j,
+//   d L / d drive(m) = -fc dt sum_j c_j n_g Re(E^(2m+1) U_j(m)).
+// Sums over larger m are sums over smaller n: Gamma is an inclusive and U_j an exclusive prefix sum in this
+// thread order (a DPP wave scan plus the earlier waves' totals, one barrier per G states). The per-state rows are
+// read from L2 (the forward's staging buffer keeps its own LDS).
+template <typename R, int SPT>
+__device__ inline void rot_chi_adj(const R* gam, R ngp, R* gd, const RotRow<R>* __restrict__ rot_tab, int rot_states,
+                                   double rot_fc, double dt, int N, int n0) {
+  const int tid = threadIdx.x, nth = blockDim.x, lane = tid & 63, wave = tid >> 6, nwv = nth >> 6;
+  constexpr int G = sizeof(R) == 4 ? ROT_G32 : ROT_G64;
+  __shared__ R adj_wsr[16];
+  __shared__ cplx<R> adj_ws[2][G][16];
+  const R dtr = (R)dt, fc = (R)rot_fc;
+  const int wv_u = __builtin_amdgcn_readfirstlane(wave);
+  // Gamma at this thread's samples: the inclusive prefix sum of gam in n
+  R gt = R(0);
+#pragma unroll
+  for (int jj = 0; jj < SPT; jj++) gt += gam[jj];
+  R before = wave_excl_prefix(gt);
+  if (lane == 63) adj_wsr[wave] = before + gt;
+  __syncthreads();
+  for (int w2 = 0; w2 < wave; w2++) before += adj_wsr[w2];
+  R Gm[SPT];
+#pragma unroll
+  for (int jj = 0; jj < SPT; jj++) {
+    before += gam[jj];
+    Gm[jj] = before;
+  }
+  // this thread's smallest m (its last sample): SPT (nth - 1 - t), as for the thread nth - 1 - t of rot_chi
+  const int tm = nth - 1 - tid, mlow = N - n0 - SPT;
+  R out[SPT];
+#pragma unroll
+  for (int jj = 0; jj < SPT; jj++) out[jj] = R(0);
+  int par = 0;
+  for (int q0 = 2; q0 < rot_states; q0 += G) {
+    // pass 1: this thread's sum of Gamma conj(E^(2m)) W, its exclusive scan over the wave, the wave totals
+    cplx<R> run[G];
+#pragma unroll
+    for (int g = 0; g < G; g++) {
+      const RotRow<R> rw = rot_tab[min(q0 + g, rot_states - 1) - 2];  // past the last state: a repeat, not used
+      const cplx<R> e2 = rw.e[1], wz = cmk(rw.wr, rw.wi);
+      cplx<R> pw = rot_start(rw, tm, mlow), tot = cmk(R(0), R(0));
+#pragma unroll
+      for (int jj = SPT - 1; jj >= 0; jj--) {  // m = mlow + (SPT - 1 - jj)
+        tot = cadd(tot, cscale(cmul(cconj(pw), wz), Gm[jj]));
+        pw = cmul(pw, e2);
+      }
+      run[g] = cmk(wave_excl_prefix(tot.re), wave_excl_prefix(tot.im));
+      if (lane == 63) adj_ws[par][g][wave] = cadd(run[g], tot);
+    }
+    __syncthreads();
+    // pass 2: U_j at each sample (the sum over smaller n), then its term of the drive's cotangent
+#pragma unroll
+    for (int g = 0; g < G; g++) {
+      if (q0 + g < rot_states) {
+        const RotRow<R> rw = rot_tab[q0 + g - 2];
+        const R cj = rw.c * ngp;
+        const cplx<R> e1 = rw.e[0], e2 = rw.e[1], wz = cmk(rw.wr, rw.wi);
+        const cplx<R> wt = lane < nwv ? adj_ws[par][g][lane] : cmk(R(0), R(0));
+        cplx<R> u = cadd(run[g], cmk(lanes_below(wt.re, wv_u), lanes_below(wt.im, wv_u)));
+        cplx<R> pws[SPT];
+        cplx<R> pw = rot_start(rw, tm, mlow);
+#pragma unroll
+        for (int jj = SPT - 1; jj >= 0; jj--) {
+          pws[jj] = pw;
+          pw = cmul(pw, e2);
+        }
+#pragma unroll
+        for (int jj = 0; jj < SPT; jj++) {  // n ascending: u holds the sum over the smaller n
+          out[jj] -= cj * cmul(cmul(pws[jj], e1), u).re;
+          u = cadd(u, cscale(cmul(cconj(pws[jj]), wz), Gm[jj]));
+        }
+      }
+    }
+    par ^= 1;
+  }
+#pragma unroll
+  for (int jj = 0; jj < SPT; jj++) gd[jj] += fc * dtr * out[jj];
+}
+
 // one workgroup per strip, Nt/4 threads (at least min(Nt, 64)); thread t owns SPT = Nt / threads samples
 // LGC > 0: an instance for Nt = 2^LGC (block Nt / SPT), whose transforms unroll with constant spans
 template <typename T, int SPT, bool DIAG, bool ROT, int LGC = 0>
@@ -2352,13 +2437,14 @@ __device__ inline T block_excl_prefix(T v, T* ws, T* total) {
 // e and g_P come from one inverse transform of E^ext + i R^ext, g_Jp from a second of R'^ext (Hermitian extensions,
 // inner bins halved). Thread t owns samples t SPT .. t SPT + SPT - 1, so the forward suffix sum and the transposed
 // prefix sum are block scans in thread order. One workgroup per strip, max(min(Nt, 64), Nt/4) threads.
-template <typename T, int SPT, int LGC = 0>
+template <typename T, int SPT, int LGC = 0, bool ROT = false>
 __global__ __launch_bounds__(1024) void f_source_adj_ion(const cplx<T>* __restrict__ er, const cplx<T>* __restrict__ gj,
                                                          cplx<T>* __restrict__ ge, const cplx<T>* __restrict__ tw,
                                                          const T* __restrict__ w, const T* __restrict__ nlf,
                                                          const double* __restrict__ ng, int nt, int lgn, int nwp,
                                                          double redchi3, double dt, uppe_ionization ion,
-                                                         double log_pre) {
+                                                         double log_pre, const RotRow<T>* __restrict__ rot_tab,
+                                                         int rot_states, double rot_fc) {
   extern __shared__ char smem[];
   cplx<T>* Z = reinterpret_cast<cplx<T>*>(smem);
   __shared__ T ws[16];
@@ -2420,13 +2506,15 @@ __global__ __launch_bounds__(1024) void f_source_adj_ion(const cplx<T>* __restri
     e[j] = z.re * invn;
     gp[j] = z.im;
     gjp[j] = Z2[lds_swz<T>(n)].re;
-    T rate, slope;
-    if constexpr (sizeof(T) == 4) {
-      ion_rate_and_slope_f32(fabsf(e[j]), ion, log_pre, rate, slope);
-    } else {
-      const double ea = fabs(e[j]);
-      rate = ion_rate<T>(ea, ion, log_pre);
-      slope = ion_rate_slope(ea, ion, rate);
+    T rate = T(0), slope = T(0);  // no rate model: a rotator's medium without photoionisation
+    if (ion.kind != UPPE_ION_NONE) {
+      if constexpr (sizeof(T) == 4) {
+        ion_rate_and_slope_f32(fabsf(e[j]), ion, log_pre, rate, slope);
+      } else {
+        const double ea = fabs(e[j]);
+        rate = ion_rate<T>(ea, ion, log_pre);
+        slope = ion_rate_slope(ea, ion, rate);
+      }
     }
     r[j] = n >= N - 1 ? T(0) : rate;
     dr[j] = n >= N - 1 ? T(0) : slope * (e[j] > T(0) ? T(1) : (e[j] < T(0) ? T(-1) : T(0)));
@@ -2465,6 +2553,35 @@ __global__ __launch_bounds__(1024) void f_source_adj_ion(const cplx<T>* __restri
     const T gr = (earlier + T(0.5) * gI[j]) * dtt;
     earlier += gI[j];
     gev[j] = kc3 * e[j] * e[j] * gp[j] + ne[j] * gjp[j] + gr * dr[j];
+  }
+  if constexpr (ROT) {
+    // chi_rot of the forward, by rot_chi in its own thread order (thread t takes the samples of thread nth-1-t
+    // here, read from the strip), handed over through the free second strip; then the cotangent of chi_rot,
+    // gam = e g_P (P = (chi_rot + kc e^2) e), back to the drive e^2: g_e += chi_rot g_P + 2 e d L / d drive
+    const int a0m = (nth - 1 - tid) * SPT;
+    T drive[SPT], dchi[SPT];
+#pragma unroll
+    for (int j = 0; j < SPT; j++) {
+      const T em = Z[lds_swz<T>(a0m + j)].re * invn;
+      drive[j] = em * em;
+      dchi[j] = T(0);
+    }
+    rot_chi<T, SPT>(drive, (T)ngp, dchi, rot_tab, rot_states, rot_fc, dt, N, a0m);
+    __syncthreads();  // every thread has read its R' samples from the second strip
+#pragma unroll
+    for (int j = 0; j < SPT; j++) Z2[lds_swz<T>(a0m + j)] = cmk(dchi[j], T(0));
+    __syncthreads();
+    T gam[SPT], gd[SPT];
+#pragma unroll
+    for (int j = 0; j < SPT; j++) {
+      const T chr = Z2[lds_swz<T>(a0 + j)].re;
+      gev[j] += chr * gp[j];
+      gam[j] = e[j] * gp[j];
+      gd[j] = T(0);
+    }
+    rot_chi_adj<T, SPT>(gam, (T)ngp, gd, rot_tab, rot_states, rot_fc, dt, N, a0);
+#pragma unroll
+    for (int j = 0; j < SPT; j++) gev[j] += T(2) * e[j] * gd[j];
   }
   __syncthreads();
 #pragma unroll
@@ -3230,21 +3347,24 @@ struct Fused {
             (const T*)s->nlf, (const double*)s->ng, N, lgn, s->nwp, s->redchi3);                                  \
     return;                                                                                                      \
   }
-    if (s->ion.kind != UPPE_ION_NONE) {
-      // E, G and R' strips, and the FP32 tables
+    const bool rot = h->rot_states > 2;
+    const RotRow<T>* rtab = (const RotRow<T>*)s->rot_tab;
+    const double rfc = rot ? 0.5 * h->rot_dalpha / h->rot_hbar : 0.0;
+    if (s->ion.kind != UPPE_ION_NONE || rot) {
+      // E, G and R' strips, and the FP32 tables; a rotator takes this kernel with or without photoionisation
       const size_t shm_ion = sizeof(C) * (sizeof(T) == 4 ? 3 * N : 2 * N);
       const double log_pre = s->ion.kind != UPPE_ION_EXPLICIT ? 0.0
                              : sizeof(T) == 4                  ? std::log2(s->ion.rate_conv * s->ion.c_pre)
                                                                : std::log(s->ion.rate_conv * s->ion.c_pre);
 #define UPPE_SRC_ADJ_ION(S)                                                                                      \
   if (spt == S) {                                                                                                \
-    auto kern = f_source_adj_ion<T, S>;                                                                          \
+    auto kern = rot ? f_source_adj_ion<T, S, 0, true> : f_source_adj_ion<T, S>;                                  \
     if constexpr (sizeof(T) == 4)                                                                                \
-      if (lgn >= 10 && lgn <= 12)                                                                                \
+      if (!rot && lgn >= 10 && lgn <= 12)                                                                        \
         kern = lgn == 10 ? adj_ion_fixed<S, 10>() : (lgn == 11 ? adj_ion_fixed<S, 11>() : adj_ion_fixed<S, 12>()); \
     launchf(h, UPPE_KC_SOURCE, kern, grid, block, shm_ion, er, gj, ge, (const C*)s->tw,                          \
             (const T*)s->w, (const T*)s->nlf, (const double*)s->ng, N, lgn, s->nwp, s->redchi3, s->dt, s->ion,    \
-            log_pre);                                                                                            \
+            log_pre, rtab, h->rot_states, rfc);                                                                  \
     return;                                                                                                      \
   }
       UPPE_SRC_ADJ_ION(1) UPPE_SRC_ADJ_ION(2) UPPE_SRC_ADJ_ION(4)
@@ -3633,8 +3753,7 @@ struct Fused {
   static void check_adjoint(uppe_handle* h, FusedState* s) {
     if (h->d.geometry != UPPE_RADIAL || s->cm < 0 || h->n2 != 1)
       throw Error{UPPE_ERR_UNSUPPORTED, "adjoint: radial geometry with one azimuthal mode, on the matrix-core path"};
-    if (h->paraxial || h->rot_states > 2)
-      throw Error{UPPE_ERR_UNSUPPORTED, "adjoint: UPPE media without a rotator"};
+    if (h->paraxial) throw Error{UPPE_ERR_UNSUPPORTED, "adjoint: the UPPE model only"};
     if (h->nt > kMaxNtTables) throw Error{UPPE_ERR_UNSUPPORTED, "adjoint: Nt <= 4096"};
   }
 

@@ -1,8 +1,9 @@
 '''Adjoint gradients of the spectral-phase objective (searay_amd.adjoint) through the HIP library.
 
 The FP64 adjoint is exact for the discrete forward map, so it must match central finite differences of the FP64
-forward to their truncation error (h = 1e-4: ~1e-8 relative, tested at 1e-6), with Kerr only and through the
-photoionisation rate (explicit and series), depletion integral, soft window and plasma current, for the on-axis
+forward to their truncation error (h = 1e-4: ~1e-8 relative, tested at 1e-6), with Kerr only, with the rotational
+Raman susceptibility, and through the photoionisation rate (explicit and series), depletion integral, soft window
+and plasma current, for the on-axis
 spectrum and the radial fluence objectives. FP32 and BF16x3 gradients are
 compared with the FP64 adjoint (1e-4 and 1e-3 relative L2). Replays from checkpoints are deterministic, so the
 gradient does not depend on the checkpoint interval.
@@ -13,7 +14,7 @@ import numpy as np
 import pytest
 
 from conftest import rel_l2
-from searay_amd import _abi, adjoint, configs
+from searay_amd import adjoint, configs
 
 pytestmark = pytest.mark.gpu
 
@@ -100,10 +101,12 @@ def test_autograd_function():
 
 
 def _ionizing(precision, every, steps=4, scale=4, nr=16, rate='explicit', objective='spectrum', target=None,
-              length='0.5 mm'):
+              length='0.5 mm', rotator=False):
     '''The radial O2 photoionisation workload (Kerr, PPT rate, depletion, plasma current) at reduced size with fixed
-    steps (the adjoint holds the step sizes constant).'''
+    steps (the adjoint holds the step sizes constant), with the N2 rotator on request.'''
     vol = configs.radial_o2_ionization(scale=scale, steps=steps, rate=rate, length=length, nr=nr)
+    if rotator:
+        vol['rotator'] = configs.n2_rotator(32)
     vol['frequency band'] = (0.0, 2.0)
     vol['nonlinear band'] = (0.0, 2.0)
     L = vol['size'][2]
@@ -151,11 +154,57 @@ def test_reduced_precision_ionizing_gradient(precision, tol):
     assert rel_l2(g, g64) < tol, rel_l2(g, g64)
 
 
-def test_adjoint_rejects_rotator():
-    _say('adjoint rejects a medium with a rotator')
-    vol = configs.radial_raman_air(scale=8, steps=2)
-    with pytest.raises(_abi.UppeError):
-        adjoint.SpectralPhaseProblem(vol, configs.kerr_pulse_sigma(), precision='fp32').value_and_grad(np.zeros(32))
+def _raman(precision, every, steps=4, scale=8, target=None, rotator=True):
+    vol = configs.radial_raman_air(scale=scale, steps=steps)
+    if not rotator:
+        del vol['rotator']
+    return adjoint.SpectralPhaseProblem(vol, configs.kerr_pulse_sigma(), precision=precision,
+                                        checkpoint_every=every, target=target)
+
+
+def test_fp64_rotator_gradient_against_central_differences():
+    '''Kerr plus the 32-state N2 rotator (rotations.cl:80-129): the adjoint of the Raman susceptibility's two
+    running sums, against central differences of the FP64 forward; the Kerr-only gradient of the same problem
+    differs by more than 1e-3, so the rotator's terms are exercised.'''
+    _say('adjoint fp64 with the rotator vs central FD, radial Raman scale=8 steps=4')
+    c = _coeffs(7)
+    p = _raman('fp64', 2)
+    loss, g = p.value_and_grad(c)
+    fd = np.array([(p.loss(c + 1e-4 * e) - p.loss(c - 1e-4 * e)) / 2e-4 for e in np.eye(32)])
+    pk = _raman('fp64', 2, target=p.target, rotator=False)
+    _, gk = pk.value_and_grad(c)
+    p.close()
+    pk.close()
+    _say('  rotator gradient vs FD %.2e, vs Kerr-only gradient %.2e' % (rel_l2(g, fd), rel_l2(gk, g)))
+    assert loss > 0 and np.linalg.norm(g) > 0
+    assert rel_l2(g, fd) < 1e-6
+    assert rel_l2(gk, g) > 1e-3
+
+
+def test_fp64_rotator_and_ionization_gradient_against_central_differences():
+    '''The rotator and photoionisation together (one adjoint source kernel takes both).'''
+    _say('adjoint fp64 with the rotator and ionisation vs central FD')
+    c = _coeffs(9)
+    p = _ionizing('fp64', 2, rotator=True)
+    _, g = p.value_and_grad(c)
+    fd = np.array([(p.loss(c + 1e-4 * e) - p.loss(c - 1e-4 * e)) / 2e-4 for e in np.eye(32)])
+    p.close()
+    _say('  gradient vs FD %.2e' % rel_l2(g, fd))
+    assert rel_l2(g, fd) < 1e-6
+
+
+@pytest.mark.parametrize('precision,tol', [('fp32', 1e-4), ('bf16x3', 1e-3)])
+def test_reduced_precision_rotator_gradient(precision, tol):
+    _say('adjoint %s with the rotator vs fp64 adjoint, radial Raman scale=4 steps=10' % precision)
+    c = _coeffs(8)
+    p64 = _raman('fp64', 5, steps=10, scale=4)
+    _, g64 = p64.value_and_grad(c)
+    p = _raman(precision, 5, steps=10, scale=4, target=p64.target)
+    _, g = p.value_and_grad(c)
+    p64.close()
+    p.close()
+    _say('  %s gradient vs fp64 adjoint %.2e' % (precision, rel_l2(g, g64)))
+    assert rel_l2(g, g64) < tol
 
 
 def test_config5_fp32_gradient_against_central_differences_full_size():
