@@ -1293,7 +1293,7 @@ __global__ __launch_bounds__(1024) void f_tfft(
 }
 
 // ---------------------------------------------------------------------------------------------------
-// strip transforms on one wavefront (the wave-per-strip source kernel) and the four-step block transform
+// strip transforms on one wavefront (the wave-per-strip source kernel)
 // ---------------------------------------------------------------------------------------------------
 __device__ inline void wave_sync() {
   __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
@@ -1328,12 +1328,14 @@ __device__ inline void wave_tw_load(WaveTw<LG>& W, const cplx<float>* __restrict
   }
 }
 
-// STRIDE > 1: the transform of the strided view whose element e is buf[lds_swz(STRIDE e + off)]
-template <int LG, int STRIDE = 1>
-__device__ inline void wave_fft_r8(cplx<float>* buf, const WaveTw<LG>& W, int sign, int lane, int off = 0) {
+// The Stockham passes of lds_fft_r8 on one wavefront: the n = 2^LG point strip buf (lds_swz layout) is
+// transformed in place by the 64 lanes, each pass's loads ordered before its stores by wavefront-scope fences
+// (wave_sync) instead of workgroup barriers, the twiddles from registers (W)
+template <int LG>
+__device__ inline void wave_fft_r8(cplx<float>* buf, const WaveTw<LG>& W, int sign, int lane) {
   using WT = WaveTw<LG>;
   constexpr int n = 1 << LG, r0 = LG % 3;
-  auto at = [&](int e) -> cplx<float>& { return buf[lds_swz<float>(STRIDE * e + off)]; };
+  auto at = [&](int e) -> cplx<float>& { return buf[lds_swz<float>(e)]; };
   if constexpr (r0 > 0) {
     constexpr int R = 1 << r0, nb = n >> r0, BP = (nb + 63) / 64;
     cplx<float> v[BP][4];
