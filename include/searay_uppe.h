@@ -186,7 +186,7 @@ int uppe_finish(uppe_handle* h, double z_total, double* a_r, double* j_r, double
 
 /* Gradients (no reference counterpart; the adjoint the input spectral-phase objectives need). UPPE media with
  * Kerr, the rotational Raman susceptibility, photoionisation (explicit and series rates) and the plasma current,
- * on the radial matrix-core paths (FP64, FP32, BF16x3) with one azimuthal point and Nt <= 4096.
+ * on the radial matrix-core paths (FP64, FP32, BF16x3) with any azimuthal point count and Nt <= 4096.
  *
  * uppe_advance_checkpointed: uppe_advance that also keeps what the adjoint needs: the state every
  * checkpoint_every steps and the phase origin and dz of every step. The step sizes are treated as constants by

@@ -2683,11 +2683,12 @@ __constant__ double kCos16[16] = {1.0,
                                   0.92387953251128675613};
 
 // The azimuthal transform of a radial grid with N2 > 1 points in phi, in place on a [Nr N2][nwp] array: for every
-// (r, f < ncols), numpy's fft over the N2 rows r N2 + m (sign -1) or its ifft (sign +1, scaled by 1/N2), as the
-// reference's FFT_axis2 / IFFT_axis2 (fft.cl:401-441) take before and after the per-mode radial transforms
-// (grid_tools.py:308-325). A direct N2-point DFT per thread.
+// (r, f < ncols), the N2-point DFT over the rows r N2 + m with exponent sign `sign`, scaled by sc: numpy's fft
+// (sign -1, sc 1) and ifft (sign +1, sc 1/N2), as the reference's FFT_axis2 / IFFT_axis2 (fft.cl:401-441) take
+// before and after the per-mode radial transforms (grid_tools.py:308-325), and their adjoints for the reverse
+// sweep. A direct N2-point DFT per thread.
 template <typename T, int N2>
-__global__ void f_azim(cplx<T>* a, int nr, int nwp, int ncols, int sign) {
+__global__ void f_azim(cplx<T>* a, int nr, int nwp, int ncols, int sign, T sc) {
   const long i = blockIdx.x * (long)blockDim.x + threadIdx.x;
   if (i >= (long)nr * ncols) return;
   const int r = (int)(i / ncols), f = (int)(i % ncols);
@@ -2695,7 +2696,7 @@ __global__ void f_azim(cplx<T>* a, int nr, int nwp, int ncols, int sign) {
   cplx<T> v[N2];
 #pragma unroll
   for (int m = 0; m < N2; m++) v[m] = p[(long)m * nwp];
-  const T sg = (T)sign, sc = sign > 0 ? T(1) / T(N2) : T(1);
+  const T sg = (T)sign;
 #pragma unroll
   for (int k = 0; k < N2; k++) {
     T re = T(0), im = T(0);
@@ -3039,14 +3040,14 @@ struct Fused {
     s->frb_hi = s->frb_lo = s->frf_hi = s->frf_lo = nullptr;
     s->fab_hi = s->fab_lo = s->faf_hi = s->faf_lo = nullptr;
     if (s->cm < 0) return;
-    // the adjoint's transposes (one mode: the adjoint takes no azimuthal modes): Hf^T has the shape of Hb, Hb^T
-    // that of Hf
-    std::vector<double> hft_(mat), hbt_(mat);
-    for (int k = 0; k < nk; k++)
-      for (int j = 0; j < nr; j++) {
-        hft_[(size_t)j * nk + k] = hf[(size_t)k * nr + j];
-        hbt_[(size_t)k * nr + j] = hb[(size_t)j * nk + k];
-      }
+    // the adjoint's transposes, per mode: Hf^T has the shape of Hb, Hb^T that of Hf
+    std::vector<double> hft_(mat * nm), hbt_(mat * nm);
+    for (int m = 0; m < nm; m++)
+      for (int k = 0; k < nk; k++)
+        for (int j = 0; j < nr; j++) {
+          hft_[m * mat + (size_t)j * nk + k] = hf[m * mat + (size_t)k * nr + j];
+          hbt_[m * mat + (size_t)k * nr + j] = hb[m * mat + (size_t)j * nk + k];
+        }
     int kpa = 0;
     // the forward operators of a batch member go to its slots (sizes fixed by the shape: frag_count)
     auto place = [&](const auto& v, int kind) -> void* {
@@ -3082,13 +3083,20 @@ struct Fused {
       });
       s->frf_hi = place(pf.first, SL_FRF_HI);
       s->frf_lo = place(pf.second, SL_FRF_LO);
-      std::vector<uint16_t> hi, lo;
-      pack_bf16x3(hft_, nr, nk, hi, lo, &kpa);
-      s->fab_hi = upload(hi);
-      s->fab_lo = upload(lo);
-      pack_bf16x3(hbt_, nk, nr, hi, lo, &kpa);
-      s->faf_hi = upload(hi);
-      s->faf_lo = upload(lo);
+      auto pab = modes(hft_, [&](const std::vector<double>& v) {
+        std::pair<std::vector<uint16_t>, std::vector<uint16_t>> r;
+        pack_bf16x3(v, nr, nk, r.first, r.second, &kpa);
+        return r;
+      });
+      s->fab_hi = upload(pab.first);
+      s->fab_lo = upload(pab.second);
+      auto paf = modes(hbt_, [&](const std::vector<double>& v) {
+        std::pair<std::vector<uint16_t>, std::vector<uint16_t>> r;
+        pack_bf16x3(v, nk, nr, r.first, r.second, &kpa);
+        return r;
+      });
+      s->faf_hi = upload(paf.first);
+      s->faf_lo = upload(paf.second);
     } else {
       auto pb = modes(hb, [&](const std::vector<double>& v) {
         std::pair<std::vector<T>, std::vector<T>> r;
@@ -3102,11 +3110,18 @@ struct Fused {
         return r;
       });
       s->frf_hi = place(pf.first, SL_FRF_HI);
-      std::vector<T> fr;
-      pack_k4<T>(hft_, nr, nk, fr, &kpa);
-      s->fab_hi = upload(fr);
-      pack_k4<T>(hbt_, nk, nr, fr, &kpa);
-      s->faf_hi = upload(fr);
+      auto pab = modes(hft_, [&](const std::vector<double>& v) {
+        std::pair<std::vector<T>, std::vector<T>> r;
+        pack_k4<T>(v, nr, nk, r.first, &kpa);
+        return r;
+      });
+      s->fab_hi = upload(pab.first);
+      auto paf = modes(hbt_, [&](const std::vector<double>& v) {
+        std::pair<std::vector<T>, std::vector<T>> r;
+        pack_k4<T>(v, nk, nr, r.first, &kpa);
+        return r;
+      });
+      s->faf_hi = upload(paf.first);
     }
     s->plan[0] = plan_hankel(nr, h->nw, s->n_cu);
     s->plan[1] = plan_hankel(nk, s->f1, s->n_cu);
@@ -3246,12 +3261,15 @@ struct Fused {
               (unsigned long long*)nullptr, (unsigned long long*)nullptr, 0, 0.0, 0.0, 0);
   }
 
-  // numpy fft (sign -1) or ifft (sign +1) over the azimuthal points of every (r, f < ncols); nothing for N2 = 1
-  static void azim(uppe_handle* h, FusedState* s, C* a, int ncols, int sign) {
+  // the DFT over the azimuthal points of every (r, f < ncols) with exponent sign `sign`, scaled by sc (default:
+  // numpy's fft for sign -1, its ifft for sign +1); nothing for N2 = 1
+  static void azim(uppe_handle* h, FusedState* s, C* a, int ncols, int sign, double sc = 0.0) {
     if (h->n2 == 1) return;
+    if (sc == 0.0) sc = sign > 0 ? 1.0 / h->n2 : 1.0;
     const long n = (long)h->n1 * ncols;
     auto kern = h->n2 == 2 ? f_azim<T, 2> : (h->n2 == 4 ? f_azim<T, 4> : (h->n2 == 8 ? f_azim<T, 8> : f_azim<T, 16>));
-    launchf(h, UPPE_KC_TRANSVERSE_FFT, kern, dim3(grid_for(n)), dim3(kBlock), 0, a, h->n1, s->nwp, ncols, sign);
+    launchf(h, UPPE_KC_TRANSVERSE_FFT, kern, dim3(grid_for(n)), dim3(kBlock), 0, a, h->n1, s->nwp, ncols, sign,
+            (T)sc);
   }
 
   // the source kernel over every strip; a batched launch (nz instances zs elements apart, per-instance scalars
@@ -3711,6 +3729,8 @@ struct Fused {
     UPPE_HIP(hipMemcpyAsync(s->stage, g_a_r, sizeof(double) * 2 * h->npt, hipMemcpyHostToDevice, h->stream));
     launchf(h, UPPE_KC_POINT, f_from_host<T>, dim3(grid_for(P * s->nwp)), dim3(kBlock), 0, (const double*)s->stage,
             g3, h->nw, P, s->nwp);
+    // azimuthal points: the adjoint of ifft over phi (fft with sign -1 and 1/N2) takes the cotangent to the modes
+    azim(h, s, g3, h->nw, -1, 1.0 / h->n2);
     // finish: a = Hb (e^{i kz z} q)
     hk_launch<CM, PRO_NONE, EPI_ADJ_FIN>(h, s, s->faf_hi, s->faf_lo, s->kp_f, nk, nr, h->nw, pl_ar, g3, nullptr,
                                          nullptr, gq, gs, gk, nullptr, nullptr, 0, z_total, 0.0,
@@ -3736,16 +3756,19 @@ struct Fused {
           const C* ers = (const C*)s->er_store + ((long)(n - n0) * 4 + (stage - 1)) * stride;
           hk_launch<CM, PRO_ADJ, EPI_NONE>(h, s, s->fab_hi, s->fab_lo, s->kp_b, nr, nk, s->f1, pl_ak, gk, g2, nullptr,
                                            nullptr, nullptr, nullptr, nullptr, nullptr, stage, zph, 0.0, 0.0);
+          azim(h, s, g2, s->f1, +1, 1.0);  // adjoint of the fft over phi of J_r: the unscaled sign +1 transform
           source_adj(h, s, ers, g2, g3);
+          azim(h, s, g3, h->nw, -1, 1.0 / h->n2);  // adjoint of the ifft over phi of E_r
           const double dzprev = (stage == 1 && n > 0) ? zr[3 * (n - 1) + 2] : 0.0;
           hk_launch<CM, PRO_NONE, EPI_ADJ>(h, s, s->faf_hi, s->faf_lo, s->kp_f, nk, nr, h->nw, pl_ar, g3, nullptr,
                                            nullptr, gq, gs, gk, nullptr, nullptr, stage, zph, dz, dzprev);
         }
       }
     }
-    // set_field: q0 = Hf a0
+    // set_field: q0 = Hf fft_phi(a0)
     hk_launch<CM, PRO_NONE, EPI_NONE>(h, s, s->fab_hi, s->fab_lo, s->kp_b, nr, nk, h->nw, pl_a0, gq, g2, nullptr,
                                       nullptr, nullptr, nullptr, nullptr, nullptr, 0, 0.0, 0.0, 0.0);
+    azim(h, s, g2, h->nw, +1, 1.0);
     launchf(h, UPPE_KC_POINT, f_to_host<T>, dim3(grid_for(P * h->nw)), dim3(kBlock), 0, (const C*)g2, s->stage,
             h->nw, P, s->nwp, (const C*)nullptr, 0, 0.0);
     UPPE_HIP(hipMemcpyAsync(g_a0_r, s->stage, sizeof(double) * 2 * h->npt, hipMemcpyDeviceToHost, h->stream));
@@ -3753,8 +3776,8 @@ struct Fused {
   }
 
   static void check_adjoint(uppe_handle* h, FusedState* s) {
-    if (h->d.geometry != UPPE_RADIAL || s->cm < 0 || h->n2 != 1)
-      throw Error{UPPE_ERR_UNSUPPORTED, "adjoint: radial geometry with one azimuthal mode, on the matrix-core path"};
+    if (h->d.geometry != UPPE_RADIAL || s->cm < 0)
+      throw Error{UPPE_ERR_UNSUPPORTED, "adjoint: radial geometry on the matrix-core path"};
     if (h->paraxial) throw Error{UPPE_ERR_UNSUPPORTED, "adjoint: the UPPE model only"};
     if (h->nt > kMaxNtTables) throw Error{UPPE_ERR_UNSUPPORTED, "adjoint: Nt <= 4096"};
   }

@@ -58,3 +58,43 @@ def test_azimuthal_ionization(oracle_mod, precision):
     ref = oracle_mod.oracle_track(vol)
     got = uppe_kernel.track_with(vol, uppe_kernel.hip_stepper_factory(precision))
     _check(ref, got, precision)
+
+
+def _coeffs(seed=5):
+    return 0.05 * np.random.default_rng(seed).standard_normal(32)
+
+
+def test_azimuthal_fp64_gradient_against_central_differences():
+    '''The adjoint through the azimuthal modes (the transposed per-mode contractions and the adjoints of the phi
+    transforms): FP64 against central differences of the FP64 forward, on the fluence objective (which sums over
+    every radial cell of phi = 0) and the on-axis spectrum.'''
+    from searay_amd import adjoint
+    for objective in ('spectrum', 'fluence'):
+        vol = with_azimuth(configs.radial_kerr_air(scale=8, steps=4))
+        _say('adjoint with 4 azimuthal points, fp64 vs central FD, %s' % objective)
+        p = adjoint.SpectralPhaseProblem(vol, configs.kerr_pulse_sigma(), precision='fp64', checkpoint_every=2,
+                                         objective=objective)
+        c = _coeffs()
+        loss, g = p.value_and_grad(c)
+        fd = np.array([(p.loss(c + 1e-4 * e) - p.loss(c - 1e-4 * e)) / 2e-4 for e in np.eye(32)])
+        p.close()
+        _say('  gradient vs FD %.2e' % rel_l2(g, fd))
+        assert loss > 0 and np.linalg.norm(g) > 0
+        assert rel_l2(g, fd) < 1e-6
+
+
+@pytest.mark.parametrize('precision,tol', [('fp32', 1e-4), ('bf16x3', 1e-3)])
+def test_azimuthal_reduced_precision_gradient(precision, tol):
+    from searay_amd import adjoint
+    vol = with_azimuth(configs.radial_kerr_air(scale=4, steps=10))
+    _say('adjoint with 4 azimuthal points, %s vs fp64' % precision)
+    sig = configs.kerr_pulse_sigma()
+    c = _coeffs(6)
+    p64 = adjoint.SpectralPhaseProblem(vol, sig, precision='fp64', checkpoint_every=5)
+    _, g64 = p64.value_and_grad(c)
+    p = adjoint.SpectralPhaseProblem(vol, sig, precision=precision, checkpoint_every=5, target=p64.target)
+    _, g = p.value_and_grad(c)
+    p64.close()
+    p.close()
+    _say('  %s gradient vs fp64 adjoint %.2e' % (precision, rel_l2(g, g64)))
+    assert rel_l2(g, g64) < tol
