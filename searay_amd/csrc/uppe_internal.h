@@ -139,7 +139,7 @@ __host__ __device__ inline int r8_table_len(int lg) {
 
 // 4-point DFT in place (sign < 0 forward)
 template <typename T>
-__device__ inline void dft4(cplx<T>& a0, cplx<T>& a1, cplx<T>& a2, cplx<T>& a3, int sign) {
+__device__ __forceinline__ void dft4(cplx<T>& a0, cplx<T>& a1, cplx<T>& a2, cplx<T>& a3, int sign) {
   const cplx<T> A0 = cadd(a0, a2), A1 = csub(a0, a2), A2 = cadd(a1, a3), d = csub(a1, a3);
   const cplx<T> A3 = sign < 0 ? cmk(d.im, -d.re) : cmk(-d.im, d.re);  // -i d forward, +i d inverse
   a0 = cadd(A0, A2);
@@ -150,7 +150,7 @@ __device__ inline void dft4(cplx<T>& a0, cplx<T>& a1, cplx<T>& a2, cplx<T>& a3, 
 
 // 8-point DFT of v[0..7] into y[0..7]: radix-2 split into sums and twiddled differences, then two 4-point DFTs
 template <typename T>
-__device__ inline void dft8(const cplx<T>* v, cplx<T>* y, int sign) {
+__device__ __forceinline__ void dft8(const cplx<T>* v, cplx<T>* y, int sign) {
   const T h = T(0.70710678118654752440);
   cplx<T> a0 = cadd(v[0], v[4]), a1 = cadd(v[1], v[5]), a2 = cadd(v[2], v[6]), a3 = cadd(v[3], v[7]);
   cplx<T> b0 = csub(v[0], v[4]), b1 = csub(v[1], v[5]), b2 = csub(v[2], v[6]), b3 = csub(v[3], v[7]);
@@ -177,10 +177,11 @@ __device__ inline void dft8(const cplx<T>* v, cplx<T>* y, int sign) {
 
 // LG > 0 fixes lg = LG (and n, and the block size NTH when nonzero) at compile time: the passes then unroll
 // with their spans, strides and table offsets as constants. MAXB bounds the butterflies a thread takes in the
-// first pass.
+// first pass. The transform is always inlined: as a call it would keep sign at run time and save the caller's
+// registers to scratch around it.
 template <typename T, int LG = 0, int NTH = 0, int MAXB = 2>
-__device__ inline void lds_fft_r8(cplx<T>* buf, int n_rt, int lg_rt, const cplx<T>* tab, int sign, int tid,
-                                  int nth_rt = 0) {
+__device__ __forceinline__ void lds_fft_r8(cplx<T>* buf, int n_rt, int lg_rt, const cplx<T>* tab, int sign, int tid,
+                                           int nth_rt = 0) {
   const int lg = LG > 0 ? LG : lg_rt;
   const int n = LG > 0 ? (1 << LG) : n_rt;
   const int r0 = lg % 3;
@@ -252,8 +253,8 @@ __device__ inline void lds_fft_r8(cplx<T>* buf, int n_rt, int lg_rt, const cplx<
 // when a half covers a pass (n/8 butterflies of the radix-8 passes, and the first pass's within 2 per thread): both
 // then share every pass's barriers, where one after the other they would take twice as many. Otherwise in turn.
 template <typename T, int LG = 0, int NTH = 0>
-__device__ inline void lds_fft_r8_pair(cplx<T>* a, cplx<T>* b, int n_rt, int lg_rt, const cplx<T>* tab, int sign,
-                                       int tid) {
+__device__ __forceinline__ void lds_fft_r8_pair(cplx<T>* a, cplx<T>* b, int n_rt, int lg_rt, const cplx<T>* tab,
+                                                int sign, int tid) {
   const int lg = LG > 0 ? LG : lg_rt;
   const int n = LG > 0 ? (1 << LG) : n_rt;
   const int nth = NTH > 0 ? NTH : (int)blockDim.x, half = nth / 2;
