@@ -790,7 +790,7 @@ __device__ inline void hankel_tile(
   }
   // no early exit for a finished run: the contraction only writes scratch (E_r) or stops before its
   // epilogue, so its loads need not wait for the step state
-  extern __shared__ char smem[];
+  extern __shared__ __align__(16) char smem[];
   __shared__ double red[16];
   const int nth = blockDim.x, nwv = nth >> 6;
   const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
@@ -1204,7 +1204,7 @@ __global__ __launch_bounds__(1024) void f_tfft(
     cplx<T>* __restrict__ k3, StepState* __restrict__ st, unsigned long long* __restrict__ qmax,
     unsigned long long* __restrict__ qmax_next, int stage, double zfix, double qbfix, int f1fix) {
   using C = cplx<T>;
-  extern __shared__ char smem[];
+  extern __shared__ __align__(16) char smem[];
   C* buf = reinterpret_cast<C*>(smem);
   C* tab = buf + na * CW;
   __shared__ double red[16];
@@ -1671,7 +1671,7 @@ __device__ inline void rot_chi(const R* drive, R ngp, R* dchi, const RotRow<R>* 
   R cum = wave_excl_prefix(gtot);
   if (lane == 63) rot_wsr[wave] = cum + gtot;
   __syncthreads();
-  for (int w2 = 0; w2 < wave; w2++) cum += rot_wsr[w2];
+  cum += lanes_below(lane < nwv ? rot_wsr[lane] : R(0), wv_u);  // the earlier waves' totals
 #pragma unroll
   for (int jj = SPT - 1; jj >= 0; jj--) {
     cum += gs[jj];
@@ -1705,7 +1705,7 @@ __device__ inline void rot_chi_adj(const R* gam, R ngp, R* gd, const RotRow<R>* 
   R before = wave_excl_prefix(gt);
   if (lane == 63) adj_wsr[wave] = before + gt;
   __syncthreads();
-  for (int w2 = 0; w2 < wave; w2++) before += adj_wsr[w2];
+  before += lanes_below(lane < nwv ? adj_wsr[lane] : R(0), wv_u);  // the earlier waves' totals
   R Gm[SPT];
 #pragma unroll
   for (int jj = 0; jj < SPT; jj++) {
@@ -1782,7 +1782,7 @@ __global__ __launch_bounds__(1024) void f_source(const cplx<T>* __restrict__ er,
     const int z = inst ? blockIdx.y : 0;
     step_commit(commit + z, cq + (long)kQW * z, rec, zrec, zrec_cap);
   }
-  extern __shared__ char smem[];
+  extern __shared__ __align__(16) char smem[];
   cplx<T>* Y = reinterpret_cast<cplx<T>*>(smem);
   __shared__ T wsum[16];
   if (LGC > 0) lgn = LGC;
@@ -1807,8 +1807,22 @@ __global__ __launch_bounds__(1024) void f_source(const cplx<T>* __restrict__ er,
   const int tid = threadIdx.x, nth = NTH > 0 ? NTH : (int)blockDim.x, lane = tid & 63, wave = tid >> 6,
             nwv = nth >> 6;
   const cplx<T>* X = er + p * nwp;
-  // every global load the kernel makes up front, issued together: this strip's spectrum, the frequency and
-  // filter of the bins this thread writes at the end, and (FP32) the twiddle tables bound for LDS
+  // every global load the kernel makes up front, issued together: (FP32) the twiddle tables bound for LDS first,
+  // then this strip's spectrum and the frequency and filter of the bins this thread writes at the end. The tables
+  // come from L2 and reach LDS while the strip is still on its way from HBM.
+  // Per-pass twiddle tables of the N- and N/2-point transforms (fewer than N + N/2 entries): in LDS for FP32
+  // strips up to 4096 samples, read from L2 for FP64 and the 8192-sample strips (LDS would not fit)
+  constexpr bool kTabLds = sizeof(T) == 4 && SPT <= 4;
+  constexpr int TPT = kTabLds ? (3 * SPT + 1) / 2 : 1;  // table entries per thread: tlen < N + N/2 = 1.5 SPT nth
+  const int tlen = r8_table_len(lgn) + r8_table_len(lgm);
+  cplx<T> tv[TPT];
+  if constexpr (kTabLds) {
+#pragma unroll
+    for (int i = 0; i < TPT; i++) {
+      const int k = tid + i * nth;
+      if (k < tlen) tv[i] = tw[k];
+    }
+  }
   constexpr int KPT = SPT / 2 + 1;  // spectral bins per thread (M / nth rounded up, plus the Nyquist bin)
   cplx<T> xk[KPT], xm[KPT];
   T wv[KPT], fv[KPT];
@@ -1824,22 +1838,13 @@ __global__ __launch_bounds__(1024) void f_source(const cplx<T>* __restrict__ er,
       fv[i] = nlf[k];
     }
   }
-  // per-pass twiddle tables of the N- and N/2-point transforms (fewer than N + N/2 entries): in LDS for
-  // FP32 strips up to 4096 samples, read from L2 for FP64 and the 8192-sample strips (LDS would not fit)
   const cplx<T>* tws = tw;
-  if constexpr (sizeof(T) == 4 && SPT <= 4) {
-    constexpr int TPT = (3 * SPT + 1) / 2;  // table entries per thread, N + N/2 = 1.5 SPT nth
+  if constexpr (kTabLds) {
     cplx<T>* tl = Y + N;
-    cplx<T> tv[TPT];
 #pragma unroll
     for (int i = 0; i < TPT; i++) {
       const int k = tid + i * nth;
-      if (k < N + M) tv[i] = tw[k];
-    }
-#pragma unroll
-    for (int i = 0; i < TPT; i++) {
-      const int k = tid + i * nth;
-      if (k < N + M) tl[k] = tv[i];
+      if (k < tlen) tl[k] = tv[i];
     }
     tws = tl;
     __syncthreads();
@@ -1896,7 +1901,7 @@ __global__ __launch_bounds__(1024) void f_source(const cplx<T>* __restrict__ er,
     if (lane == 63) wsum[wave] = ex + tot;
     __syncthreads();
     R later = ex;
-    for (int w2 = 0; w2 < wave; w2++) later += wsum[w2];
+    later += lanes_below(lane < nwv ? wsum[lane] : R(0), __builtin_amdgcn_readfirstlane(wave));  // earlier waves
 #pragma unroll
     for (int j = SPT - 1; j >= 0; j--) {
       const int n = a0 + j;
@@ -2173,7 +2178,7 @@ __global__ __launch_bounds__(1024) void f_source_env(const cplx<T>* __restrict__
                                                      double* rec, double* zrec, int zrec_cap) {
   // the stage-1 launch commits the previous step (commit non-null; nothing else of this launch reads it)
   if (commit && blockIdx.x == 0 && threadIdx.x == 0) step_commit(commit, cq, rec, zrec, zrec_cap);
-  extern __shared__ char smem[];
+  extern __shared__ __align__(16) char smem[];
   if (LGC > 0) lgn = LGC;
   const int N = LGC > 0 ? (1 << LGC) : nt;
   cplx<T>* Y1 = reinterpret_cast<cplx<T>*>(smem);  // E, then P
@@ -2181,7 +2186,8 @@ __global__ __launch_bounds__(1024) void f_source_env(const cplx<T>* __restrict__
   __shared__ T wsum[16];
   const long p = blockIdx.x;
   constexpr int NTH = LGC > 0 ? (1 << LGC) / SPT : 0;
-  const int tid = threadIdx.x, nth = NTH > 0 ? NTH : (int)blockDim.x, lane = tid & 63, wave = tid >> 6;
+  const int tid = threadIdx.x, nth = NTH > 0 ? NTH : (int)blockDim.x, lane = tid & 63, wave = tid >> 6,
+            nwv = nth >> 6;
   const cplx<T>* X = ar + p * nwp;
   cplx<T> xa[SPT];
   T wv[SPT];
@@ -2234,7 +2240,7 @@ __global__ __launch_bounds__(1024) void f_source_env(const cplx<T>* __restrict__
     if (lane == 63) wsum[wave] = ex + tot;
     __syncthreads();
     R later = ex;
-    for (int w2 = 0; w2 < wave; w2++) later += wsum[w2];
+    later += lanes_below(lane < nwv ? wsum[lane] : R(0), __builtin_amdgcn_readfirstlane(wave));  // earlier waves
 #pragma unroll
     for (int j = SPT - 1; j >= 0; j--) {
       const R integral = (later + R(0.5) * r[j]) * dtr;
@@ -2306,7 +2312,7 @@ __global__ __launch_bounds__(1024) void f_source_adj(const cplx<T>* __restrict__
                                                      const T* __restrict__ w, const T* __restrict__ nlf,
                                                      const double* __restrict__ ng, int nt, int lgn, int nwp,
                                                      double redchi3) {
-  extern __shared__ char smem[];
+  extern __shared__ __align__(16) char smem[];
   cplx<T>* Z = reinterpret_cast<cplx<T>*>(smem);
   if (LGC > 0) lgn = LGC;
   const int N = LGC > 0 ? (1 << LGC) : nt, M = N / 2;
@@ -2447,7 +2453,7 @@ __global__ __launch_bounds__(1024) void f_source_adj_ion(const cplx<T>* __restri
                                                          double redchi3, double dt, uppe_ionization ion,
                                                          double log_pre, const RotRow<T>* __restrict__ rot_tab,
                                                          int rot_states, double rot_fc) {
-  extern __shared__ char smem[];
+  extern __shared__ __align__(16) char smem[];
   cplx<T>* Z = reinterpret_cast<cplx<T>*>(smem);
   __shared__ T ws[16];
   if (LGC > 0) lgn = LGC;

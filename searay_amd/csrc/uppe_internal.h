@@ -35,8 +35,11 @@ struct Error {
 // ---------------------------------------------------------------------------------------------------
 // complex helpers (interleaved re, im)
 // ---------------------------------------------------------------------------------------------------
+// An FP32 complex value is aligned to its 8-byte size, so that the compiler moves it with one 8-byte access
+// (ds_read_b64 / ds_write_b64 in LDS) instead of two 4-byte ones; FP64 keeps the alignment of double (8), which
+// host buffers of interleaved doubles guarantee.
 template <typename T>
-struct cplx {
+struct alignas(2 * sizeof(T) <= 8 ? 2 * sizeof(T) : alignof(T)) cplx {
   T re, im;
 };
 
