@@ -7,6 +7,10 @@ import ctypes as ct
 import numpy as np
 
 UPPE_OK = 0
+UPPE_ERR_ARG = 1
+UPPE_ERR_DEVICE = 2
+UPPE_ERR_STATE = 3
+UPPE_ERR_UNSUPPORTED = 4
 UPPE_RADIAL = 0
 UPPE_CARTESIAN = 1
 PREC_FP64_PLAIN = 0

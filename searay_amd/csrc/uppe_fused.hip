@@ -758,7 +758,7 @@ __device__ inline void hankel_tile(
     cplx<typename MfmaTraits<CM>::T>* __restrict__ k1, cplx<typename MfmaTraits<CM>::T>* __restrict__ k2,
     cplx<typename MfmaTraits<CM>::T>* __restrict__ k3, const StepState* st, StepOut so,
     const unsigned long long* __restrict__ qmax, unsigned long long* __restrict__ qmax_next, int stage, double zfix,
-    double dzfix, double dzprev, int wide, long zs, int zst, int tile, int ntile, int rowblk, int zi) {
+    double dzfix, double dzprev, int wide, long zs, int zst, int kch, int tile, int ntile, int rowblk, int zi) {
   using T = typename MfmaTraits<CM>::T;
   using C = cplx<T>;
   using acc_t = typename MfmaTraits<CM>::acc_t;
@@ -851,76 +851,7 @@ __device__ inline void hankel_tile(
 #pragma unroll
   for (int d = 0; d < RING; d++) fetch_frag(d, d);
 
-  // ---- stage the B slab (Kp x hcw) in LDS with the prologue applied; loads of a batch are issued together
-  constexpr int EB = hankel_batch(CM, RT);  // register room: elements per thread per batch
-  const int ldb = Kp + 8;     // bf16 row stride (k contiguous), padded
-  const int lgh = __builtin_ctz(hcw);
-  const int ne = Kp * hcw;
-  const int eb = min(EB, (ne + nth - 1) / nth);  // elements per thread per batch: as few as cover the slab
-  for (int base = 0; base < ne; base += eb * nth) {
-    C vb[EB], vk[EB], vz[EB];
-    double vp[EB];
-    T vc[EB];
-#pragma unroll
-    for (int j = 0; j < EB; j++) {
-      const int e = base + tid + j * nth;
-      const int k = e >> lgh, f = fc0 + (e & (hcw - 1));
-      vb[j] = cmk(T(0), T(0));
-      vk[j] = cmk(T(0), T(0));
-      vz[j] = cmk(T(1), T(0));
-      vp[j] = 0.0;
-      vc[j] = T(1);
-      if (j < eb && e < ne && k < K && f < fend) {
-        const long idx = (long)k * nwp + f;
-        vb[j] = B[idx];
-        if (PRO != PRO_NONE) {
-          vz[j] = kzc[idx];
-          vp[j] = phs[idx];
-        }
-        if (pro_stage(PRO) && stage > 1) {
-          vk[j] = kprev[idx];
-          vc[j] = cond[f];
-        }
-      }
-    }
-#pragma unroll
-    for (int j = 0; j < EB; j++) {
-      const int e = base + tid + j * nth;
-      if (j >= eb || e >= ne) break;
-      const int k = e >> lgh, cf = e & (hcw - 1);
-      C v = vb[j];
-      if (pro_stage(PRO)) {
-        if (stage > 1) {
-          T c = (T)cu;
-          v = cscale(cmk(v.re + c * vk[j].re, v.im + c * vk[j].im), vc[j]);
-        }
-        v = cmul(linear_factor<T>(vp[j], vz[j].im, zph, col_ref(fc0 + cf, f1r, zph, qb, 0).za, +1), v);
-        if (PRO == PRO_STAGE) v = cmul(cmk(-vz[j].im, vz[j].re), v);  // E = i kz A
-      } else if (PRO == PRO_FINISH) {
-        v = cmul(linear_factor<T>(vp[j], vz[j].im, zph, col_ref(fc0 + cf, f1r, zph, qb, 0).za, +1), v);
-      } else if (PRO == PRO_ADJ) {
-        // conj of S = (i/2) e^{-i kz z} J / Re kz: (-i/2) e^{i (Re kz + kg) z} e^{Im kz z} g / Re kz
-        const C a = cmul(linear_factor<T>(vp[j], -vz[j].im, zph, zph, +1), v);
-        v = cmk((T(0.5) * a.im) / vz[j].re, (T(-0.5) * a.re) / vz[j].re);
-      }
-      if constexpr (CM == CM_BF16X3) {
-        __bf16* bhi = reinterpret_cast<__bf16*>(smem);
-        __bf16* blo = bhi + BW * ldb;
-        __bf16 hr = (__bf16)v.re, hi_ = (__bf16)v.im;
-        bhi[(2 * cf) * ldb + k] = hr;
-        bhi[(2 * cf + 1) * ldb + k] = hi_;
-        blo[(2 * cf) * ldb + k] = (__bf16)((float)v.re - (float)hr);
-        blo[(2 * cf + 1) * ldb + k] = (__bf16)((float)v.im - (float)hi_);
-      } else {
-        T* bs = reinterpret_cast<T*>(smem);
-        bs[k * BW + 2 * cf] = v.re;
-        bs[k * BW + 2 * cf + 1] = v.im;
-      }
-    }
-  }
-  __syncthreads();
-
-  // ---- MFMA main loop: wave w owns rows [row0 + w rw, row0 + (w+1) rw), rw <= 32
+  // accumulators of the MFMA loop: wave w owns rows [row0 + w rw, row0 + (w+1) rw), rw <= 32
   constexpr int CT = BW / 16;  // column tiles the accumulators can hold; nmt of them are in use
   acc_t acc[RT][CT];
 #pragma unroll
@@ -928,68 +859,145 @@ __device__ inline void hankel_tile(
 #pragma unroll
     for (int c = 0; c < CT; c++) acc[r][c] = acc_t{0, 0, 0, 0};
 
-  if constexpr (CM == CM_BF16X3) {
-    const __bf16* bhi = reinterpret_cast<const __bf16*>(smem);
-    const __bf16* blo = bhi + BW * ldb;
-    const int KC = Kp / 32;
-    for (int kc0 = 0; kc0 < KC; kc0 += RING) {
+  // the contraction runs over chunks of kch rows of the B slab (all Kp rows at once when the slab fits in LDS;
+  // kch is a multiple of the fragment ring's span, so that chunk boundaries keep the ring's slots in step)
+  const int lgh = __builtin_ctz(hcw);
+  for (int kb = 0; kb < Kp; kb += kch) {
+    const int kcn = min(kch, Kp - kb);  // rows of this chunk
+    if (kb > 0) __syncthreads();        // the previous chunk's MFMA loop has read its slab
+    // ---- stage rows [kb, kb + kcn) of the B slab (Kp x hcw) in LDS with the prologue applied; loads of a batch are
+    // issued together
+    constexpr int EB = hankel_batch(CM, RT);  // register room: elements per thread per batch
+    const int ldb = kch + 8;    // bf16 row stride (k contiguous), padded
+    const int ne = kcn * hcw;
+    const int eb = min(EB, (ne + nth - 1) / nth);  // elements per thread per batch: as few as cover the slab
+    for (int base = 0; base < ne; base += eb * nth) {
+      C vb[EB], vk[EB], vz[EB];
+      double vp[EB];
+      T vc[EB];
 #pragma unroll
-      for (int d = 0; d < RING; d++) {
-        const int kc = kc0 + d;
-        if (kc >= KC) break;
-        bf16x8 bh[CT], bl[CT];
-#pragma unroll
-        for (int c = 0; c < CT; c++) {
-          if (c >= nmt) break;
-          const int col = c * 16 + (lane & 15), ko = kc * 32 + 8 * (lane >> 4);
-          bh[c] = *reinterpret_cast<const bf16x8*>(bhi + col * ldb + ko);
-          bl[c] = *reinterpret_cast<const bf16x8*>(blo + col * ldb + ko);
+      for (int j = 0; j < EB; j++) {
+        const int e = base + tid + j * nth;
+        const int k = kb + (e >> lgh), f = fc0 + (e & (hcw - 1));
+        vb[j] = cmk(T(0), T(0));
+        vk[j] = cmk(T(0), T(0));
+        vz[j] = cmk(T(1), T(0));
+        vp[j] = 0.0;
+        vc[j] = T(1);
+        if (j < eb && e < ne && k < K && f < fend) {
+          const long idx = (long)k * nwp + f;
+          vb[j] = B[idx];
+          if (PRO != PRO_NONE) {
+            vz[j] = kzc[idx];
+            vp[j] = phs[idx];
+          }
+          if (pro_stage(PRO) && stage > 1) {
+            vk[j] = kprev[idx];
+            vc[j] = cond[f];
+          }
         }
+      }
 #pragma unroll
-        for (int r = 0; r < RT; r++) {
-          if (mt0 + r >= M16) break;
+      for (int j = 0; j < EB; j++) {
+        const int e = base + tid + j * nth;
+        if (j >= eb || e >= ne) break;
+        const int k = e >> lgh, cf = e & (hcw - 1);  // k: row within the chunk
+        C v = vb[j];
+        if (pro_stage(PRO)) {
+          if (stage > 1) {
+            T c = (T)cu;
+            v = cscale(cmk(v.re + c * vk[j].re, v.im + c * vk[j].im), vc[j]);
+          }
+          v = cmul(linear_factor<T>(vp[j], vz[j].im, zph, col_ref(fc0 + cf, f1r, zph, qb, 0).za, +1), v);
+          if (PRO == PRO_STAGE) v = cmul(cmk(-vz[j].im, vz[j].re), v);  // E = i kz A
+        } else if (PRO == PRO_FINISH) {
+          v = cmul(linear_factor<T>(vp[j], vz[j].im, zph, col_ref(fc0 + cf, f1r, zph, qb, 0).za, +1), v);
+        } else if (PRO == PRO_ADJ) {
+          // conj of S = (i/2) e^{-i kz z} J / Re kz: (-i/2) e^{i (Re kz + kg) z} e^{Im kz z} g / Re kz
+          const C a = cmul(linear_factor<T>(vp[j], -vz[j].im, zph, zph, +1), v);
+          v = cmk((T(0.5) * a.im) / vz[j].re, (T(-0.5) * a.re) / vz[j].re);
+        }
+        if constexpr (CM == CM_BF16X3) {
+          __bf16* bhi = reinterpret_cast<__bf16*>(smem);
+          __bf16* blo = bhi + BW * ldb;
+          __bf16 hr = (__bf16)v.re, hi_ = (__bf16)v.im;
+          bhi[(2 * cf) * ldb + k] = hr;
+          bhi[(2 * cf + 1) * ldb + k] = hi_;
+          blo[(2 * cf) * ldb + k] = (__bf16)((float)v.re - (float)hr);
+          blo[(2 * cf + 1) * ldb + k] = (__bf16)((float)v.im - (float)hi_);
+        } else {
+          T* bs = reinterpret_cast<T*>(smem);
+          bs[k * BW + 2 * cf] = v.re;
+          bs[k * BW + 2 * cf + 1] = v.im;
+        }
+      }
+    }
+    __syncthreads();
+
+    // ---- MFMA main loop over the chunk's contraction chunks (global indices: the ring prefetches across chunks)
+    if constexpr (CM == CM_BF16X3) {
+      const __bf16* bhi = reinterpret_cast<const __bf16*>(smem);
+      const __bf16* blo = bhi + BW * ldb;
+      const int KC = (kb + kcn) / 32;
+      for (int kc0 = kb / 32; kc0 < KC; kc0 += RING) {
+#pragma unroll
+        for (int d = 0; d < RING; d++) {
+          const int kc = kc0 + d;
+          if (kc >= KC) break;
+          bf16x8 bh[CT], bl[CT];
 #pragma unroll
           for (int c = 0; c < CT; c++) {
             if (c >= nmt) break;
-            acc[r][c] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(ring_h[d][r], bh[c], acc[r][c], 0, 0, 0);
-            acc[r][c] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(ring_h[d][r], bl[c], acc[r][c], 0, 0, 0);
-            acc[r][c] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(ring_l[d][r], bh[c], acc[r][c], 0, 0, 0);
+            const int col = c * 16 + (lane & 15), ko = kc * 32 - kb + 8 * (lane >> 4);
+            bh[c] = *reinterpret_cast<const bf16x8*>(bhi + col * ldb + ko);
+            bl[c] = *reinterpret_cast<const bf16x8*>(blo + col * ldb + ko);
           }
-        }
-        fetch_frag(kc + RING, d);
-      }
-    }
-  } else {
-    const T* bs = reinterpret_cast<const T*>(smem);
-    const int KQ = Kp / 16;
-    for (int kq0 = 0; kq0 < KQ; kq0 += RING) {
 #pragma unroll
-      for (int d = 0; d < RING; d++) {
-        const int kq = kq0 + d;
-        if (kq >= KQ) break;
-#pragma unroll
-        for (int j = 0; j < 4; j++) {
-          const int krow = kq * 16 + 4 * j + (lane >> 4);
-          T b[CT];
-#pragma unroll
-          for (int c = 0; c < CT; c++) b[c] = c < nmt ? bs[krow * BW + c * 16 + (lane & 15)] : T(0);
-#pragma unroll
-          for (int r = 0; r < 2; r++) {
-            if (r >= rt_n || mt0 + r >= M16) break;
+          for (int r = 0; r < RT; r++) {
+            if (mt0 + r >= M16) break;
 #pragma unroll
             for (int c = 0; c < CT; c++) {
               if (c >= nmt) break;
-              if constexpr (CM == CM_F64)
-                acc[r][c] = __builtin_amdgcn_mfma_f64_16x16x4f64(ring_h[d][r][j], b[c], acc[r][c], 0, 0, 0);
-              else
-                acc[r][c] = __builtin_amdgcn_mfma_f32_16x16x4f32(ring_h[d][r][j], b[c], acc[r][c], 0, 0, 0);
+              acc[r][c] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(ring_h[d][r], bh[c], acc[r][c], 0, 0, 0);
+              acc[r][c] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(ring_h[d][r], bl[c], acc[r][c], 0, 0, 0);
+              acc[r][c] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(ring_l[d][r], bh[c], acc[r][c], 0, 0, 0);
             }
           }
+          fetch_frag(kc + RING, d);
         }
-        fetch_frag(kq + RING, d);
+      }
+    } else {
+      const T* bs = reinterpret_cast<const T*>(smem);
+      const int KQ = (kb + kcn) / 16;
+      for (int kq0 = kb / 16; kq0 < KQ; kq0 += RING) {
+#pragma unroll
+        for (int d = 0; d < RING; d++) {
+          const int kq = kq0 + d;
+          if (kq >= KQ) break;
+#pragma unroll
+          for (int j = 0; j < 4; j++) {
+            const int krow = kq * 16 - kb + 4 * j + (lane >> 4);
+            T b[CT];
+#pragma unroll
+            for (int c = 0; c < CT; c++) b[c] = c < nmt ? bs[krow * BW + c * 16 + (lane & 15)] : T(0);
+#pragma unroll
+            for (int r = 0; r < 2; r++) {
+              if (r >= rt_n || mt0 + r >= M16) break;
+#pragma unroll
+              for (int c = 0; c < CT; c++) {
+                if (c >= nmt) break;
+                if constexpr (CM == CM_F64)
+                  acc[r][c] = __builtin_amdgcn_mfma_f64_16x16x4f64(ring_h[d][r][j], b[c], acc[r][c], 0, 0, 0);
+                else
+                  acc[r][c] = __builtin_amdgcn_mfma_f32_16x16x4f32(ring_h[d][r][j], b[c], acc[r][c], 0, 0, 0);
+              }
+            }
+          }
+          fetch_frag(kq + RING, d);
+        }
       }
     }
-  }
+  }  // contraction chunks
   __syncthreads();  // B slab no longer needed: reuse LDS for the output tile
 
   // ---- accumulators -> LDS tile [bm][BW]
@@ -1105,11 +1113,11 @@ __global__ __launch_bounds__(BN > 16 ? 512 : 1024) void f_hankel(
     cplx<typename MfmaTraits<CM>::T>* __restrict__ k1, cplx<typename MfmaTraits<CM>::T>* __restrict__ k2,
     cplx<typename MfmaTraits<CM>::T>* __restrict__ k3, StepState* __restrict__ st,
     unsigned long long* __restrict__ qmax, unsigned long long* __restrict__ qmax_next, int stage, double zfix,
-    double dzfix, double dzprev, int wide, long zs, int zst) {
+    double dzfix, double dzprev, int wide, long zs, int zst, int kch) {
   const StepOut so{st ? &st->dzraw : nullptr, st ? &st->nanflag : nullptr, st ? &st->pend : nullptr};
   hankel_tile<CM, PRO, EPI, BN, RT>(Ahi, Alo, B, out, M, K, Kp, rw, nwp, f0, fend, phs, kzc, cond, q, kprev, k1,
                                            k2, k3, st, so, qmax, qmax_next, stage, zfix, dzfix, dzprev, wide, zs, zst,
-                                           xcd_tile(blockIdx.x, gridDim.x), gridDim.x, blockIdx.y, blockIdx.z);
+                                           kch, xcd_tile(blockIdx.x, gridDim.x), gridDim.x, blockIdx.y, blockIdx.z);
 }
 
 // ---------------------------------------------------------------------------------------------------
@@ -2860,7 +2868,7 @@ static void* upload(const std::vector<V>& v) {
 }
 
 // Launch plan of one contraction: ncols complex columns in tiles of 8, split into row blocks of at least 128
-// rows while that keeps one workgroup per CU. When a grid of ceil(ncols/8) tiles would not fit the CUs, the
+// rows while that keeps one workgroup per CU (and into blocks of at most 512 rows whatever the CU count). When a grid of ceil(ncols/8) tiles would not fit the CUs, the
 // last tile takes the remainder instead (a 2049-column transform is 256 tiles, not 257), so that no
 // workgroup waits for a second round.
 static FusedState::HkPlan plan_hankel(int M, int ncols, int n_cu) {
@@ -2882,6 +2890,9 @@ static FusedState::HkPlan plan_hankel(int M, int ncols, int n_cu) {
       break;
     }
   }
+  // a row block holds at most 16 waves of two 16-row tiles: more rows take more row blocks, even where the grid
+  // then exceeds one workgroup per CU
+  while ((m16 + p.split - 1) / p.split > 32) p.split *= 2;
   const int per = (m16 + p.split - 1) / p.split;  // 16-row tiles per row block
   p.nwv = std::min(16, per);
   p.rt = (per + p.nwv - 1) / p.nwv;
@@ -2903,14 +2914,24 @@ static long frag_count(int cm, int m, int k) {
   return cm == CM_BF16X3 ? m16 * ((k + 31) / 32) * 64 : m16 * ((k + 15) / 16) * 64;
 }
 
-// dynamic LDS of one contraction launch: the B slab (Kp rows of 2 BN real columns; BF16x3 keeps hi and lo
-// planes with padded rows) or the output tile (bm rows), whichever is larger
-static size_t hk_lds_bytes(int cm, int kp, int bm, int bn) {
+constexpr size_t kMaxLds = 159 * 1024;  // dynamic LDS of one CU (gfx950: 160 KB, less the kernels' static arrays)
+
+// dynamic LDS of one contraction launch: a chunk of kch rows of the B slab (2 BN real columns; BF16x3 keeps hi
+// and lo planes with padded rows) or the output tile (bm rows), whichever is larger
+static size_t hk_lds_bytes(int cm, int kch, int bm, int bn) {
   const size_t bw = 2 * (size_t)bn, el = cm == CM_F64 ? 8 : 4;
-  const size_t slab = cm == CM_BF16X3 ? 2 * bw * (kp + 8) * 2 : (size_t)kp * bw * el;
+  const size_t slab = cm == CM_BF16X3 ? 2 * bw * (kch + 8) * 2 : (size_t)kch * bw * el;
   return std::max(slab, (size_t)bm * bw * el);
 }
-constexpr size_t kMaxLds = 160 * 1024;  // LDS of one CU (gfx950)
+
+// contraction rows staged per chunk: all kp when the slab fits in LDS, else the largest multiple of 128 that fits
+// (128 is a whole number of fragment-ring spans in every mode: 32 x 4 for BF16x3, 16 x 4 or 16 x 2 otherwise)
+static int hk_chunk(int cm, int kp, int bm, int bn) {
+  if (hk_lds_bytes(cm, kp, bm, bn) <= kMaxLds) return kp;
+  int kch = kp / 128 * 128;
+  while (kch > 128 && hk_lds_bytes(cm, kch, bm, bn) > kMaxLds) kch -= 128;
+  return kch;
+}
 
 template <typename T>
 void tune_single(uppe_handle* h, FusedState* s, bool may_time = true);
@@ -3429,13 +3450,15 @@ struct Fused {
     if (!fits || pl.wide < 0 || pl.wide > 2 || (pl.wide == 2 && pl.nwv > 8))
       throw Error{UPPE_ERR_STATE, "radial contraction plan out of date"};
     dim3 grid((unsigned)pl.tiles, (unsigned)((M + bm - 1) / bm), (unsigned)nz);
-    const size_t shm = hk_lds_bytes(CM, kp, bm, pl.wide == 2 ? 32 : 16);
+    const int bn = pl.wide == 2 ? 32 : 16, kch = hk_chunk(CM, kp, bm, bn);
+    const size_t shm = hk_lds_bytes(CM, kch, bm, bn);
+    if (shm > kMaxLds) throw Error{UPPE_ERR_UNSUPPORTED, "radial contraction: tile does not fit in LDS"};
 #define UPPE_HK(BNV, RTV)                                                                                        \
   launchf(h, EPI == EPI_NONE ? UPPE_KC_HANKEL : UPPE_KC_HANKEL_R2K, f_hankel<CM, PRO, EPI, BNV, RTV>, grid,      \
           dim3(64 * pl.nwv), shm, (const F*)ahi,                                                               \
           (const F*)alo, B, out, M, K, kp, rw, (int)row, 0, ncols, (const double*)s->phs, (const C*)s->kzc,       \
           (const T*)s->cond, q, kprev, k1, k2, k3, st, s->qmax, s->qmax_next, stage, zfix, dzfix, dzprev, pl.wide, \
-          zs, zst)
+          zs, zst, kch)
     if (pl.wide == 2) {
       if (pl.rt == 1)
         UPPE_HK(32, 1);
@@ -4183,7 +4206,8 @@ struct BatchOps {
       for (int rt = 1; rt <= 2; rt++)
         for (int nwv = 1; nwv <= (wide == 2 ? 8 : 16); nwv *= 2) {
           if (rt * nwv > m16 && !(rt == 1 && nwv == 1)) continue;
-          if (hk_lds_bytes(cm, kp, 16 * rt * nwv, wide == 2 ? 32 : 16) > kMaxLds) continue;
+          const int bmc = 16 * rt * nwv, bnc = wide == 2 ? 32 : 16;
+          if (hk_lds_bytes(cm, hk_chunk(cm, kp, bmc, bnc), bmc, bnc) > kMaxLds) continue;
           if (wide < 2 && (size_t)16 * rt * nwv * 32 * sizeof(T) > 65536) continue;
           FusedState::HkPlan p;
           p.tiles = tiles;
@@ -4547,8 +4571,8 @@ void fused_create(uppe_handle* h, const MemberSlots* slots) {
   if (h->nt > kMaxNtFused || h->nt < 4) throw Error{UPPE_ERR_UNSUPPORTED, "fused path supports 4 <= Nt <= 8192"};
   if (h->paraxial && h->nt > kMaxNtTables)
     throw Error{UPPE_ERR_UNSUPPORTED, "fused envelope path supports Nw <= 4096"};
-  if (h->d.geometry == UPPE_RADIAL && h->n1 > 512)
-    throw Error{UPPE_ERR_UNSUPPORTED, "fused path supports up to 512 radial cells"};
+  if (h->d.geometry == UPPE_RADIAL && h->n1 > kMaxNrFused)
+    throw Error{UPPE_ERR_UNSUPPORTED, "fused path supports up to " + std::to_string(kMaxNrFused) + " radial cells"};
   auto* s = new FusedState();
   h->fused = s;
   s->slots = slots;
