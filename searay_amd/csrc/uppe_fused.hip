@@ -3504,17 +3504,31 @@ struct Fused {
   }
 
   // one RK4 step: 4 x (k2r, source, r2k); the step control runs inside them
-  // one 1-D pass of a Cartesian transverse transform over columns [0, fend); sign +1 inverse (scaled 1/Na)
+  // one 1-D pass of a Cartesian transverse transform over columns [0, fend); sign +1 inverse (scaled 1/Na).
+  // CW columns per line: 128-byte runs (16 FP32 or 8 FP64 columns), 8 FP32 columns for lines of 1024 points
+  // (the block holds CW Na/8 <= 1024 threads)
   template <int PRO, int EPI>
   static void tpass(uppe_handle* h, FusedState* s, int axis, const C* src, C* dst, int fend, int sign,
                     const C* kprev, int stage, double zfix, StepState* st) {
-    constexpr int CW = sizeof(T) == 4 ? 16 : 8;  // columns per line: 128-byte runs
+    if constexpr (sizeof(T) == 4) {
+      if ((axis == 1 ? h->n1 : h->n2) <= 512) {
+        tpass_cw<16, PRO, EPI>(h, s, axis, src, dst, fend, sign, kprev, stage, zfix, st);
+        return;
+      }
+    }
+    tpass_cw<8, PRO, EPI>(h, s, axis, src, dst, fend, sign, kprev, stage, zfix, st);
+  }
+
+  template <int CW, int PRO, int EPI>
+  static void tpass_cw(uppe_handle* h, FusedState* s, int axis, const C* src, C* dst, int fend, int sign,
+                       const C* kprev, int stage, double zfix, StepState* st) {
     const int na = axis == 1 ? h->n1 : h->n2, nother = axis == 1 ? h->n2 : h->n1;
     const int nth = std::max(64, CW * std::max(1, na / 8));
     const void* tab = axis == 1 ? s->tw_x : s->tw_y;
     const int tl = axis == 1 ? s->twlen_x : s->twlen_y;
     dim3 grid((unsigned)((fend + CW - 1) / CW), (unsigned)nother);
     const size_t shm = sizeof(C) * ((size_t)na * CW + tl);
+    if (nth > 1024 || shm > kMaxLds) throw Error{UPPE_ERR_UNSUPPORTED, "transverse pass: line too long"};
     const T scale = sign > 0 ? T(1.0 / na) : T(1);
     launchf(h, UPPE_KC_TRANSVERSE_FFT, f_tfft<T, CW, PRO, EPI>, grid, dim3(nth), shm, src, dst, axis, na, ilog2i(na),
             h->n2, s->nwp, fend, (const C*)tab, tl, sign, scale, (const double*)s->phs, (const C*)s->kzc,
@@ -4566,8 +4580,9 @@ static bool is_fp64(uppe_handle* h) { return h->fused->prec == UPPE_PREC_FP64; }
 void fused_create(uppe_handle* h, const MemberSlots* slots) {
   if (h->d.geometry == UPPE_RADIAL && (h->n2 < 1 || h->n2 > 16 || (h->n2 & (h->n2 - 1))))
     throw Error{UPPE_ERR_UNSUPPORTED, "fused path: radial geometry with 1, 2, 4, 8 or 16 azimuthal points"};
-  if (h->d.geometry == UPPE_CARTESIAN && (h->n1 < 2 || h->n2 < 2 || h->n1 > 512 || h->n2 > 512))
-    throw Error{UPPE_ERR_UNSUPPORTED, "fused Cartesian path supports 2 <= Nx, Ny <= 512"};
+  if (h->d.geometry == UPPE_CARTESIAN &&
+      (h->n1 < 2 || h->n2 < 2 || h->n1 > kMaxNxFused || h->n2 > kMaxNxFused))
+    throw Error{UPPE_ERR_UNSUPPORTED, "fused Cartesian path supports 2 <= Nx, Ny <= " + std::to_string(kMaxNxFused)};
   if (h->nt > kMaxNtFused || h->nt < 4) throw Error{UPPE_ERR_UNSUPPORTED, "fused path supports 4 <= Nt <= 8192"};
   if (h->paraxial && h->nt > kMaxNtTables)
     throw Error{UPPE_ERR_UNSUPPORTED, "fused envelope path supports Nw <= 4096"};

@@ -17,6 +17,7 @@ constexpr int kBlock = 256;        // threads per workgroup for point and strip 
 constexpr int kMaxNtFused = 8192;  // fused source kernel keeps one Nt-point complex strip in LDS
 constexpr int kMaxNtTables = 4096;  // FP32 strips up to this length also keep their twiddle tables in LDS
 constexpr int kMaxNrFused = 2048;  // radial cells of the fused path (contraction rows in blocks of <= 512)
+constexpr int kMaxNxFused = 1024;  // Cartesian points per transverse axis of the fused path (one line per block)
 constexpr int kMaxNtPlain = 16384; // plain strip FFT keeps Nt/2 complex points in LDS (FP64)
 
 struct Error {
