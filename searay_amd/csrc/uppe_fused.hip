@@ -68,12 +68,42 @@ struct MemberSlots {
   void* p[SL_COUNT] = {};
 };
 
+// 128-bit fingerprint of host arrays (two independent multiply-xorshift lanes over their 8-byte words, the tail
+// zero-padded): a batch run lets its members share one copy of the operator or of the kz tables when their
+// fingerprints agree
+struct Fingerprint {
+  uint64_t a = 0, b = 0;
+  bool set = false;
+  void mix(const void* p, size_t bytes) {
+    const unsigned char* c = static_cast<const unsigned char*>(p);
+    if (!set) {
+      a = 0x9e3779b97f4a7c15ull;
+      b = 0xc2b2ae3d27d4eb4full;
+      set = true;
+    }
+    a ^= bytes;
+    b += bytes;
+    for (size_t i = 0; i < bytes; i += 8) {
+      uint64_t w = 0;
+      std::memcpy(&w, c + i, std::min<size_t>(8, bytes - i));
+      a = (a ^ w) * 0xff51afd7ed558ccdull;
+      a ^= a >> 29;
+      b = (b + w) * 0xc4ceb9fe1a85ec53ull;
+      b ^= b >> 32;
+    }
+  }
+  bool operator==(const Fingerprint& o) const { return set && o.set && a == o.a && b == o.b; }
+};
+
 struct FusedState {
   // batch member: device addresses of the per-instance arrays inside the batch's slabs (not owned); null for
   // a standalone handle
   const MemberSlots* slots = nullptr;
   // host copies of the frequency tables and source time step (a batched launch shares them between instances)
   std::vector<double> w_h, nl_h, cond_h;
+  // fingerprints of the Hankel operator (set_hankel) and of the kz and kg inputs of the propagation tables
+  // (set_medium): equal fingerprints let a batch run read one member's copy
+  Fingerprint op_fp, tab_fp;
   int prec = 0;
   int nwp = 0;
   size_t el = 0;  // sizeof(T)
@@ -758,22 +788,24 @@ __device__ inline void hankel_tile(
     cplx<typename MfmaTraits<CM>::T>* __restrict__ k1, cplx<typename MfmaTraits<CM>::T>* __restrict__ k2,
     cplx<typename MfmaTraits<CM>::T>* __restrict__ k3, const StepState* st, StepOut so,
     const unsigned long long* __restrict__ qmax, unsigned long long* __restrict__ qmax_next, int stage, double zfix,
-    double dzfix, double dzprev, int wide, long zs, int zst, int kch, int tile, int ntile, int rowblk, int zi) {
+    double dzfix, double dzprev, int wide, long zs, int zst, int kch, int share, int tile, int ntile, int rowblk,
+    int zi) {
   using T = typename MfmaTraits<CM>::T;
   using C = cplx<T>;
   using acc_t = typename MfmaTraits<CM>::acc_t;
   if (zs) {
     // batched launch: instance zi of equally shaped instances whose arrays lie zs elements apart
     // (state, tables), one fragment set apart (operator); batch members (zst = 1) also one step state and three
-    // maxima apart, the azimuthal modes of one grid (zst = 0) share them
+    // maxima apart, the azimuthal modes of one grid (zst = 0) share them. share bit 0: every instance reads
+    // the first one's operator, bit 1: its kz and phase tables (their inputs are the same)
     const long z = zi, zo = z * zs;
     const long fs = (long)((M + 15) / 16) * (CM == CM_BF16X3 ? Kp / 32 : Kp / 16) * 64;
-    Ahi = zshift(Ahi, z * fs);
-    Alo = zshift(Alo, z * fs);
+    Ahi = zshift(Ahi, (share & 1) ? 0 : z * fs);
+    Alo = zshift(Alo, (share & 1) ? 0 : z * fs);
     B = zshift(B, zo);
     out = zshift(out, zo);
-    phs = zshift(phs, zo);
-    kzc = zshift(kzc, zo);
+    phs = zshift(phs, (share & 2) ? 0 : zo);
+    kzc = zshift(kzc, (share & 2) ? 0 : zo);
     q = zshift(q, zo);
     kprev = zshift(kprev, zo);
     k1 = zshift(k1, zo);
@@ -1113,11 +1145,12 @@ __global__ __launch_bounds__(BN > 16 ? 512 : 1024) void f_hankel(
     cplx<typename MfmaTraits<CM>::T>* __restrict__ k1, cplx<typename MfmaTraits<CM>::T>* __restrict__ k2,
     cplx<typename MfmaTraits<CM>::T>* __restrict__ k3, StepState* __restrict__ st,
     unsigned long long* __restrict__ qmax, unsigned long long* __restrict__ qmax_next, int stage, double zfix,
-    double dzfix, double dzprev, int wide, long zs, int zst, int kch) {
+    double dzfix, double dzprev, int wide, long zs, int zst, int kch, int share) {
   const StepOut so{st ? &st->dzraw : nullptr, st ? &st->nanflag : nullptr, st ? &st->pend : nullptr};
   hankel_tile<CM, PRO, EPI, BN, RT>(Ahi, Alo, B, out, M, K, Kp, rw, nwp, f0, fend, phs, kzc, cond, q, kprev, k1,
                                            k2, k3, st, so, qmax, qmax_next, stage, zfix, dzfix, dzprev, wide, zs, zst,
-                                           kch, xcd_tile(blockIdx.x, gridDim.x), gridDim.x, blockIdx.y, blockIdx.z);
+                                           kch, share, xcd_tile(blockIdx.x, gridDim.x), gridDim.x, blockIdx.y,
+                                           blockIdx.z);
 }
 
 // ---------------------------------------------------------------------------------------------------
@@ -2868,9 +2901,9 @@ static void* upload(const std::vector<V>& v) {
 }
 
 // Launch plan of one contraction: ncols complex columns in tiles of 8, split into row blocks of at least 128
-// rows while that keeps one workgroup per CU (and into blocks of at most 512 rows whatever the CU count). When a grid of ceil(ncols/8) tiles would not fit the CUs, the
-// last tile takes the remainder instead (a 2049-column transform is 256 tiles, not 257), so that no
-// workgroup waits for a second round.
+// rows while that keeps one workgroup per CU (and into blocks of at most 512 rows whatever the CU count). When a
+// grid of ceil(ncols/8) tiles would not fit the CUs, the last tile takes the remainder instead (a 2049-column
+// transform is 256 tiles, not 257), so that no workgroup waits for a second round.
 static FusedState::HkPlan plan_hankel(int M, int ncols, int n_cu) {
   FusedState::HkPlan p;
   const int m16 = (M + 15) / 16;
@@ -3058,6 +3091,9 @@ struct Fused {
           hb[m * mat + (size_t)j * nk + k] = v * h->lam[k] / h->lam[j];
           hf[m * mat + (size_t)k * nr + j] = v * h->lam[j] / h->lam[k];
         }
+    s->op_fp = Fingerprint();
+    s->op_fp.mix(hb.data(), sizeof(double) * hb.size());
+    s->op_fp.mix(hf.data(), sizeof(double) * hf.size());
     std::vector<T> hbt(hb.begin(), hb.end()), hft(hf.begin(), hf.end());
     UPPE_HIP(hipMemcpy(s->Hb, hbt.data(), sizeof(T) * hbt.size(), hipMemcpyHostToDevice));
     UPPE_HIP(hipMemcpy(s->Hf, hft.data(), sizeof(T) * hft.size(), hipMemcpyHostToDevice));
@@ -3172,6 +3208,9 @@ struct Fused {
     s->w_h.assign(m->w, m->w + h->nw);
     s->nl_h.assign(m->nl_filter, m->nl_filter + h->nw);
     s->cond_h.assign(m->cond_filter, m->cond_filter + h->nw);
+    s->tab_fp = Fingerprint();
+    s->tab_fp.mix(m->kz, sizeof(double) * 2 * h->npt);
+    s->tab_fp.mix(m->kg, sizeof(double) * h->nw);
     double* kg = nullptr;
     UPPE_HIP(hipMalloc(&kg, sizeof(double) * h->nw));
     UPPE_HIP(hipMemcpy(kg, m->kg, sizeof(double) * h->nw, hipMemcpyHostToDevice));
@@ -3205,6 +3244,7 @@ struct Fused {
   // phase rate, k0(w) as the divisor of the right-hand side
   static void set_paraxial_medium(uppe_handle* h, FusedState* s, const uppe_paraxial_medium* m) {
     const long P = h->P;
+    s->tab_fp = Fingerprint();  // not shared (batches step the UPPE model)
     std::vector<T> w(s->nwp, T(0)), one(s->nwp, T(0));
     for (int f = 0; f < h->nw; f++) {
       w[f] = (T)m->w[f];
@@ -3429,7 +3469,7 @@ struct Fused {
   static void hk_launch(uppe_handle* h, FusedState* s, const void* ahi, const void* alo, int kp, int M, int K,
                         int ncols, const FusedState::HkPlan& pl, const C* B, C* out, const C* kprev, C* q, C* k1,
                         C* k2, C* k3, StepState* st, int stage, double zfix, double dzfix, double dzprev,
-                        int nz = 1, long zs = 0) {
+                        int nz = 1, long zs = 0, int share = 0) {
     using F = typename MfmaTraits<CM>::frag_t;
     // a single grid with azimuthal modes: one contraction per mode m (grid z), each over the rows r N2 + m of the
     // [Nr N2][nwp] arrays with its own operator, all sharing the step state
@@ -3458,7 +3498,7 @@ struct Fused {
           dim3(64 * pl.nwv), shm, (const F*)ahi,                                                               \
           (const F*)alo, B, out, M, K, kp, rw, (int)row, 0, ncols, (const double*)s->phs, (const C*)s->kzc,       \
           (const T*)s->cond, q, kprev, k1, k2, k3, st, s->qmax, s->qmax_next, stage, zfix, dzfix, dzprev, pl.wide, \
-          zs, zst, kch)
+          zs, zst, kch, zst ? share : 0)
     if (pl.wide == 2) {
       if (pl.rt == 1)
         UPPE_HK(32, 1);
@@ -4026,6 +4066,7 @@ struct BatchRun {
   int group = 0, first = 0, count = 0;  // slots [first, first + count) of the group
   FusedState::HkPlan plan[2];           // k2r, r2k
   int spt = 0;                          // source samples per thread (0: the single-instance choice)
+  int share = 0;  // bit 0: the members' operators agree, bit 1: their kz tables agree (one copy is read)
 };
 
 struct FusedBatch {
@@ -4182,12 +4223,12 @@ struct BatchOps {
     for (int stage = 1; stage <= 4; stage++) {
       F::template hk_launch<CM, PRO_STAGE, EPI_NONE>(h, s, s->frb_hi, s->frb_lo, s->kp_b, h->n1, h->nk, h->nw,
                                                      r.plan[0], (const C*)q, (C*)s->er, kp[stage], q, k1, k2, k3,
-                                                     s->st, stage, 0.0, 0.0, 0.0, r.count, zs);
+                                                     s->st, stage, 0.0, 0.0, 0.0, r.count, zs, r.share);
       F::source(h, s, (const C*)s->er, (C*)s->jr, false, stage == 1 ? s->st : nullptr, r.count, G.src + r.first, zs,
                 r.spt);
       F::template hk_launch<CM, PRO_NONE, EPI_STAGE>(h, s, s->frf_hi, s->frf_lo, s->kp_f, h->nk, h->n1, s->f1,
                                                      r.plan[1], (const C*)s->jr, (C*)nullptr, nullptr, q, k1, k2,
-                                                     k3, s->st, stage, 0.0, 0.0, 0.0, r.count, zs);
+                                                     k3, s->st, stage, 0.0, 0.0, 0.0, r.count, zs, r.share);
     }
   }
 
@@ -4257,7 +4298,7 @@ struct BatchOps {
   // when it is 3 % faster, so that timing noise does not pick between equals.
   template <int CM>
   static void tune_shapes(uppe_handle* h, FusedState* s, int nz, long zs, const SrcInst* src,
-                          FusedState::HkPlan plan[2], int* spt_out) {
+                          FusedState::HkPlan plan[2], int* spt_out, int share = 0) {
     C *q = (C*)s->q, *k1 = (C*)s->k1, *k2 = (C*)s->k2, *k3 = (C*)s->k3;
     hipEvent_t e0, e1;
     UPPE_HIP(hipEventCreate(&e0));
@@ -4270,18 +4311,18 @@ struct BatchOps {
           if (h->paraxial)
             F::template hk_launch<CM, PRO_STAGE_ENV, EPI_NONE>(h, s, s->frb_hi, s->frb_lo, s->kp_b, h->n1, h->nk,
                                                                h->nw, p, (const C*)q, (C*)s->er, (const C*)k1, q, k1,
-                                                               k2, k3, s->st, 2, 0.0, 0.0, 0.0, nz, zs);
+                                                               k2, k3, s->st, 2, 0.0, 0.0, 0.0, nz, zs, share);
           else
             F::template hk_launch<CM, PRO_STAGE, EPI_NONE>(h, s, s->frb_hi, s->frb_lo, s->kp_b, h->n1, h->nk, h->nw,
                                                            p, (const C*)q, (C*)s->er, (const C*)k1, q, k1, k2, k3,
-                                                           s->st, 2, 0.0, 0.0, 0.0, nz, zs);
+                                                           s->st, 2, 0.0, 0.0, 0.0, nz, zs, share);
         }, reps);
       };
       auto r2k = [&](const FusedState::HkPlan& p) {
         return time_launch(h->stream, e0, e1, [&] {
           F::template hk_launch<CM, PRO_NONE, EPI_STAGE>(h, s, s->frf_hi, s->frf_lo, s->kp_f, h->nk, h->n1, s->f1, p,
                                                          (const C*)s->jr, (C*)nullptr, nullptr, q, k1, k2, k3, s->st,
-                                                         2, 0.0, 0.0, 0.0, nz, zs);
+                                                         2, 0.0, 0.0, 0.0, nz, zs, share);
         }, reps);
       };
       // with the tile width pinned, the default plan is not a candidate: the first candidate is taken
@@ -4333,14 +4374,14 @@ struct BatchOps {
   }
 
   static void tune_shapes(uppe_handle* h, FusedState* s, int nz, long zs, const SrcInst* src,
-                          FusedState::HkPlan plan[2], int* spt_out) {
+                          FusedState::HkPlan plan[2], int* spt_out, int share = 0) {
     if constexpr (sizeof(T) == 8) {
-      tune_shapes<CM_F64>(h, s, nz, zs, src, plan, spt_out);
+      tune_shapes<CM_F64>(h, s, nz, zs, src, plan, spt_out, share);
     } else {
       if (s->cm == CM_BF16X3)
-        tune_shapes<CM_BF16X3>(h, s, nz, zs, src, plan, spt_out);
+        tune_shapes<CM_BF16X3>(h, s, nz, zs, src, plan, spt_out, share);
       else
-        tune_shapes<CM_F32>(h, s, nz, zs, src, plan, spt_out);
+        tune_shapes<CM_F32>(h, s, nz, zs, src, plan, spt_out, share);
     }
   }
 
@@ -4348,7 +4389,7 @@ struct BatchOps {
   static void tune(uppe_batch* b, BatchRun& r) {
     const BatchGroup& G = b->fb->groups[(size_t)r.group];
     uppe_handle* h = b->members[(size_t)G.members[(size_t)r.first]];
-    tune_shapes(h, h->fused, r.count, h->P * (long)h->fused->nwp, G.src + r.first, r.plan, &r.spt);
+    tune_shapes(h, h->fused, r.count, h->P * (long)h->fused->nwp, G.src + r.first, r.plan, &r.spt, r.share);
   }
 };
 
@@ -4490,8 +4531,16 @@ void fused_batch_advance(uppe_batch* b, const uppe_step_control* c, int* steps_o
       r.plan[0] = h0->fused->plan[0];
       r.plan[1] = h0->fused->plan[1];
       r.spt = default_spt(h0, h0->fused, (long)r.count * h0->P);
+      // members whose operators (or kz tables) have equal fingerprints read the first member's copy: one copy
+      // stays in the caches where the run's instances would each stream their own
+      r.share = r.count > 1 ? 3 : 0;
+      for (int m = r.first + 1; m < r.first + r.count; m++) {
+        const FusedState* sm = b->members[(size_t)G.members[(size_t)m]]->fused;
+        if (!(sm->op_fp == h0->fused->op_fp)) r.share &= ~1;
+        if (!(sm->tab_fp == h0->fused->tab_fp)) r.share &= ~2;
+      }
       if (fb->autotune) {
-        const std::vector<long> key = {g, r.first, r.count, h0->fused->f1};
+        const std::vector<long> key = {g, r.first, r.count, h0->fused->f1, r.share};
         auto it = std::find_if(fb->tuned.begin(), fb->tuned.end(), [&](const auto& e) { return e.first == key; });
         if (it != fb->tuned.end()) {
           r = it->second;

@@ -238,3 +238,26 @@ def test_batch_with_rotator_matches_standalone(precision):
     del vol['rotator']
     _, q0 = _standalone(vol, precision)
     assert rel_l2(qs[i], q0) > 1e-6
+
+
+@pytest.mark.parametrize('precision', ['fp64', 'fp32', 'bf16x3'])
+def test_batch_members_share_only_equal_tables(precision, monkeypatch):
+    '''A run reads one copy of the operator or of the kz tables only when every member's inputs for it are equal.
+    Four members of one grid: a reference, another pulse power (same operator and tables), a denser gas (other
+    kz) and a wider box (other kz; the same operator, whose eigenvectors and volume ratios do not depend on the
+    cell size). The run shares the operator but not the tables; each member equals its standalone handle.'''
+    monkeypatch.setenv('SEARAY_UPPE_SOURCE_SPT', '1')
+    vols = [configs.radial_o2_ionization(scale=8, nr=32, steps=4, length='1 mm', power=p)
+            for p in (3.0, 1.0, 3.0, 3.0)]
+    vols[2]['density reference'] = 1.5 * vols[2]['density reference']
+    vols[3]['size'] = (2 * vols[3]['size'][0], 2 * vols[3]['size'][1], vols[3]['size'][2])
+    vols[3]['object'] = configs._box(vols[3])
+    _say('batch sharing check %s' % precision)
+    with batch.Batch(vols, precision=precision, autotune=False) as b:
+        b.set_fields()
+        steps = b.advance()
+        qs = [m.get_state() for m in b.members]
+    for i, vol in enumerate(vols):
+        n, q = _standalone(vol, precision)
+        assert steps[i] == n
+        assert rel_l2(qs[i], q) < 1e-12, i
