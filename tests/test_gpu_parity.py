@@ -293,3 +293,19 @@ def test_radial_ionization_8192_samples(oracle_mod, precision):
     assert rel_l2(got[0][..., 1], ref[0][..., 1]) < (FP64_TOL if fp64 else FP32_FIELD_TOL)
     assert rel_l2(got[3][..., 1], ref[3][..., 1]) < (FP64_TOL if fp64 else FP32_NE_TOL)
     assert got[5] == ref[5]
+
+
+@pytest.mark.parametrize('precision', ['fp64-plain', 'fp64', 'fp32', 'bf16x3'])
+def test_radial_fewer_modes_than_cells(oracle_mod, precision):
+    '''Nk = 40 radial modes on 64 cells (the reference zeroes the operator rows of the unused modes and gives them
+    kr^2 = 0, grid_tools.py:268-272,313): contraction lengths and row counts that are not multiples of the MFMA
+    tiles (40 = 2 x 16 + 8, padded to the 32-deep BF16 chunks).'''
+    vol = configs.radial_o2_ionization(scale=4, nr=64, steps=2, length='2 mm')
+    vol['radial modes'] = 40
+    _say('radial_ionization nr=64 nk=40 %s' % precision)
+    ref = oracle_mod.oracle_track(vol)
+    got = _run(vol, precision)
+    fp64 = precision.startswith('fp64')
+    assert rel_l2(got[0][..., 1], ref[0][..., 1]) < (FP64_TOL if fp64 else FP32_FIELD_TOL)
+    assert rel_l2(got[3][..., 1], ref[3][..., 1]) < (FP64_TOL if fp64 else FP32_NE_TOL)
+    assert got[5] == ref[5]
