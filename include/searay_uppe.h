@@ -126,7 +126,10 @@ typedef struct uppe_info {
 typedef struct uppe_handle uppe_handle;
 
 /* Allocate device buffers for one grid. Replaces Source.__init__ device arrays (uppe_kernel.py:108-119)
- * and HankelTransformTool.AllocateDeviceMemory (grid_tools.py:299-303). */
+ * and HankelTransformTool.AllocateDeviceMemory (grid_tools.py:299-303). The fused precisions (FP64, FP32,
+ * BF16X3) take 4 <= Nt <= 8192 time samples (UPPE: Nt = 2 (nw - 1); paraxial: Nt = nw <= 4096), up to 2048
+ * radial cells with 1, 2, 4, 8 or 16 azimuthal points, and 2 <= n1, n2 <= 1024 Cartesian cells; other grids
+ * return UPPE_ERR_UNSUPPORTED, and the plain FP64 stepper takes them. */
 int uppe_create(const uppe_desc* desc, uppe_handle** out);
 
 /* Radial basis: H[nk][nr][nm] eigenvectors and Lambda[nr] root volumes (grid_tools.py:294,263,300-301). */
