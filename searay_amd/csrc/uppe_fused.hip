@@ -267,10 +267,28 @@ struct StepPos {
   double zoff, z, dz, qb;
   int active, f1, rb_end;
 };
-__device__ inline StepPos step_pos(const StepState* st, int stage) {
+// The whole step state of a launch in one batch of scalar loads and one wait. Read field by field, the loads
+// would sit along step_pos's branches, each waiting for the one before (about ten round trips). The empty asm
+// takes every word in a scalar register, so that no load sinks into a branch.
+__device__ inline StepState load_state(const StepState* st) {
+  static_assert(sizeof(StepState) % 16 == 0, "the step state is read as 16-byte words");
+  constexpr int NW = sizeof(StepState) / 16;
+  const uint4* p = reinterpret_cast<const uint4*>(st);
+  uint4 w[NW];
+#pragma unroll
+  for (int i = 0; i < NW; i++) w[i] = p[i];
+#pragma unroll
+  for (int i = 0; i < NW; i++) asm volatile("" ::"s"(w[i].x), "s"(w[i].y), "s"(w[i].z), "s"(w[i].w));
+  StepState s;
+  __builtin_memcpy(&s, w, sizeof(StepState));
+  return s;
+}
+
+__device__ inline StepPos step_pos(const StepState* stp, int stage) {
+  const StepState st = load_state(stp);
   if (stage == 1) {
-    if (!st->pend) return StepPos{st->zoff, st->z, 0.0, st->qbase, st->active, st->rb_f1, 0};
-    StepState loc = *st;
+    if (!st.pend) return StepPos{st.zoff, st.z, 0.0, st.qbase, st.active, st.rb_f1, 0};
+    StepState loc = st;
     if (loc.active) {
       if (loc.sched_n > 0 && loc.steps >= loc.sched_n)
         loc.active = 0;
@@ -279,9 +297,9 @@ __device__ inline StepPos step_pos(const StepState* st, int stage) {
     }
     return StepPos{loc.zoff, loc.z, 0.0, loc.qbase, loc.active, loc.rb_f1, 0};
   }
-  const double dz = step_dz(*st);
-  const int end = stage == 4 && rebase_at(*st, st->zoff + (st->z + dz));
-  return StepPos{st->zoff, st->z, dz, st->qbase, st->active, st->rb_f1, end};
+  const double dz = step_dz(st);
+  const int end = stage == 4 && rebase_at(st, st.zoff + (st.z + dz));
+  return StepPos{st.zoff, st.z, dz, st.qbase, st.active, st.rb_f1, end};
 }
 
 // a value every lane of the wavefront holds, moved to scalar registers
@@ -841,25 +859,6 @@ __device__ inline void hankel_tile(
   const int nmt = min(BW / 16, (min(fend - fc0, hcw) + HC - 1) / HC);
   double zph = zfix, cu = 0.0, dz = dzfix, qb = 0.0;
   int f1r = 0, rb_end = 0;  // launches without a step state (set-up, adjoint) see the state itself
-  if (st) {
-    const StepPos sp = step_pos(st, stage);
-    dz = sp.dz;
-    double cs = stage == 1 ? 0.0 : (stage == 4 ? 1.0 : 0.5);
-    zph = sp.zoff + (sp.z + cs * dz);
-    cu = stage == 1 ? 0.0 : (stage == 4 ? dz : dz / 2);
-    if constexpr (sizeof(T) == 4) {  // only FP32 states are rebased
-      qb = sp.qb;
-      f1r = sp.f1;
-      rb_end = sp.rb_end;
-    }
-  }
-  // the step position is uniform over the workgroup: keep it in scalar registers
-  zph = uniform_f64(zph);
-  cu = uniform_f64(cu);
-  dz = uniform_f64(dz);
-  qb = uniform_f64(qb);
-  f1r = __builtin_amdgcn_readfirstlane(f1r);
-  rb_end = __builtin_amdgcn_readfirstlane(rb_end);
 
   // ---- A fragments: a ring of RING contraction chunks per row tile, the first RING issued before the B
   // slab is staged so their latency overlaps it
@@ -882,6 +881,28 @@ __device__ inline void hankel_tile(
   };
 #pragma unroll
   for (int d = 0; d < RING; d++) fetch_frag(d, d);
+  // The step position, formed once the first operator fragments are in flight: its read of the step
+  // state then overlaps their latency instead of preceding it. (Formed later still, after the first loads of the
+  // B slab, it would keep those registers live across step_pos and spill.)
+  if (st) {
+    const StepPos sp = step_pos(st, stage);
+    dz = sp.dz;
+    double cs = stage == 1 ? 0.0 : (stage == 4 ? 1.0 : 0.5);
+    zph = sp.zoff + (sp.z + cs * dz);
+    cu = stage == 1 ? 0.0 : (stage == 4 ? dz : dz / 2);
+    if constexpr (sizeof(T) == 4) {  // only FP32 states are rebased
+      qb = sp.qb;
+      f1r = sp.f1;
+      rb_end = sp.rb_end;
+    }
+  }
+  // the step position is uniform over the workgroup: keep it in scalar registers
+  zph = uniform_f64(zph);
+  cu = uniform_f64(cu);
+  dz = uniform_f64(dz);
+  qb = uniform_f64(qb);
+  f1r = __builtin_amdgcn_readfirstlane(f1r);
+  rb_end = __builtin_amdgcn_readfirstlane(rb_end);
 
   // accumulators of the MFMA loop: wave w owns rows [row0 + w rw, row0 + (w+1) rw), rw <= 32
   constexpr int CT = BW / 16;  // column tiles the accumulators can hold; nmt of them are in use
