@@ -1839,11 +1839,7 @@ __global__ __launch_bounds__(1024) void f_source(const cplx<T>* __restrict__ er,
                                                  long zs, StepState* commit, unsigned long long* cq, double* rec,
                                                  double* zrec, int zrec_cap) {
   // no early exit for a finished run: the kernel only writes scratch (J_r, diagnostics). The stage-1 launch
-  // commits the previous step (commit non-null; nothing else of this launch reads the step state).
-  if (commit && blockIdx.x == 0 && threadIdx.x == 0) {
-    const int z = inst ? blockIdx.y : 0;
-    step_commit(commit + z, cq + (long)kQW * z, rec, zrec, zrec_cap);
-  }
+  // commits the previous step (commit non-null; nothing else of this launch reads the step state), below.
   extern __shared__ __align__(16) char smem[];
   cplx<T>* Y = reinterpret_cast<cplx<T>*>(smem);
   __shared__ T wsum[16];
@@ -1899,6 +1895,12 @@ __global__ __launch_bounds__(1024) void f_source(const cplx<T>* __restrict__ er,
       wv[i] = w[k];
       fv[i] = nlf[k];
     }
+  }
+  // the commit of the previous step, by one thread of the first workgroup once its own loads are in flight:
+  // made first, it delayed that workgroup's whole chain, and with it the launch, by the commit's round trips
+  if (commit && blockIdx.x == 0 && threadIdx.x == 0) {
+    const int z = inst ? blockIdx.y : 0;
+    step_commit(commit + z, cq + (long)kQW * z, rec, zrec, zrec_cap);
   }
   const cplx<T>* tws = tw;
   if constexpr (kTabLds) {
