@@ -1135,6 +1135,14 @@ __device__ inline void hankel_tile(
   if (stage == 1 && nan) atomicOr(so.nanflag, 1);
   if (stage == 2 || stage == 3) return;
   if (stage == 1 && st->fixed) return;
+  // max|q| feeds only the adaptive step estimate. A fixed step spans its whole subcycle, so a fixed run needs it
+  // only from the step that ends the advance call (its last subcycle, or the step that reaches max_steps), for a
+  // later adaptive call; the other fixed steps skip the reduction and its atomics.
+  if (stage == 4 && st->fixed && st->sub + 1 < st->sub_end &&
+      !(st->max_steps > 0 && st->steps + 1 >= st->max_steps)) {
+    if (tid == 0 && so.pend && tile == 0 && rowblk == 0) *so.pend = 1;  // the step has run
+    return;
+  }
   const bool is_min = stage == 1;
   loc = is_min ? (loc > 0.0 ? st->dphi / loc : INFINITY) : sqrt(loc);  // smallest step, or max |q|
   for (int o = 32; o > 0; o >>= 1) {

@@ -309,3 +309,36 @@ def test_radial_fewer_modes_than_cells(oracle_mod, precision):
     assert rel_l2(got[0][..., 1], ref[0][..., 1]) < (FP64_TOL if fp64 else FP32_FIELD_TOL)
     assert rel_l2(got[3][..., 1], ref[3][..., 1]) < (FP64_TOL if fp64 else FP32_NE_TOL)
     assert got[5] == ref[5]
+
+
+@pytest.mark.parametrize('max_steps', [0, 1])
+@pytest.mark.parametrize('precision', ['fp64', 'bf16x3'])
+def test_radial_fixed_then_adaptive(oracle_mod, precision, max_steps):
+    '''A fixed-step advance call (dzmin >= the subcycle length: one step per subcycle), then an adaptive call on
+    the same handle, on the HIP stepper and on the oracle. The adaptive step estimate takes amin from max|q|,
+    which fixed steps form only on the step that ends their call: its last subcycle (max_steps = 0) or the step
+    that reaches max_steps. A stale max|q| moves the adaptive schedule: FP64 must keep the oracle's to 1e-8 and
+    its state to 1e-10; BF16x3 its state within the adaptive FP32 tolerance.'''
+    vol = configs.radial_o2_ionization(scale=4, steps=4, rate='explicit', length='1 mm', power=3.0)
+    _say('radial_ionization fixed then adaptive %s max_steps=%d' % (precision, max_steps))
+    from searay_amd import uppe_kernel as uk
+    rs = uk.RunSetup(vol)
+    zi, zf, L = rs.plane(0)
+    n_fixed = 1 if max_steps else 2
+    runs = []
+    for factory in (oracle_mod.oracle_factory, uk.hip_stepper_factory(precision)):
+        st = rs.make_stepper(factory)
+        st.set_field(rs.A0)
+        n1, _ = st.advance(zi, L, 0, 2, L, max_steps=max_steps)
+        n2, rec = st.advance(zi, L, n_fixed, rs.subcycles - n_fixed, rs.dzmin, record_cap=4096)
+        runs.append((n1, n2, rec, st.get_state()))
+        st.close()
+    (n1_ref, n2_ref, rec_ref, q_ref), (n1, n2, rec, q) = runs
+    assert n1 == n1_ref == n_fixed
+    assert n2_ref > rs.subcycles - n_fixed  # the adaptive call subdivides its subcycles
+    if precision == 'fp64':
+        assert n2 == n2_ref
+        assert np.allclose(rec, rec_ref, rtol=1e-8, atol=0)
+        assert rel_l2(q, q_ref) < FP64_TOL
+    else:
+        assert rel_l2(q, q_ref) < 1e-4
