@@ -1845,9 +1845,18 @@ __global__ __launch_bounds__(1024) void f_source(const cplx<T>* __restrict__ er,
                                                  double log_pre, const RotRow<T>* __restrict__ rot_tab,
                                                  int rot_states, double rot_fc, const SrcInst* __restrict__ inst,
                                                  long zs, StepState* commit, unsigned long long* cq, double* rec,
-                                                 double* zrec, int zrec_cap) {
+                                                 double* zrec, int zrec_cap, int commit_only) {
   // no early exit for a finished run: the kernel only writes scratch (J_r, diagnostics). The stage-1 launch
-  // commits the previous step (commit non-null; nothing else of this launch reads the step state), below.
+  // commits the previous step (commit non-null; nothing else of this launch reads the step state): thread 0 of
+  // the first workgroup, or, in a launch of one workgroup more than strips (commit_only = the strip count), that
+  // workgroup alone, which commits and leaves, so that no strip's chain waits for the commit's round trips. (The
+  // commit stays ahead of the strip's loads: behind them it would hold their registers and raise the kernel's
+  // VGPR count, which costs occupancy where many workgroups share a CU.)
+  if (commit && threadIdx.x == 0 && (int)blockIdx.x == commit_only) {
+    const int z = inst ? blockIdx.y : 0;
+    step_commit(commit + z, cq + (long)kQW * z, rec, zrec, zrec_cap);
+  }
+  if (commit_only > 0 && (int)blockIdx.x == commit_only) return;
   extern __shared__ __align__(16) char smem[];
   cplx<T>* Y = reinterpret_cast<cplx<T>*>(smem);
   __shared__ T wsum[16];
@@ -1903,12 +1912,6 @@ __global__ __launch_bounds__(1024) void f_source(const cplx<T>* __restrict__ er,
       wv[i] = w[k];
       fv[i] = nlf[k];
     }
-  }
-  // the commit of the previous step, by one thread of the first workgroup once its own loads are in flight:
-  // made first, it delayed that workgroup's whole chain, and with it the launch, by the commit's round trips
-  if (commit && blockIdx.x == 0 && threadIdx.x == 0) {
-    const int z = inst ? blockIdx.y : 0;
-    step_commit(commit + z, cq + (long)kQW * z, rec, zrec, zrec_cap);
   }
   const cplx<T>* tws = tw;
   if constexpr (kTabLds) {
@@ -3427,6 +3430,10 @@ struct Fused {
 #undef UPPE_SRC_ENV
       throw Error{UPPE_ERR_UNSUPPORTED, "fused envelope source kernel: unsupported Nt"};
     }
+    // a standalone stage-1 launch over fewer strips than the device has CUs takes one more workgroup, which only
+    // commits the previous step (on a CU the strips leave idle)
+    const int commit_only = (commit && !inst && (long)h->P < s->n_cu) ? (int)h->P : 0;
+    const dim3 grid_src((unsigned)(h->P + (commit_only > 0 ? 1 : 0)), (unsigned)nz);
 #define UPPE_SRC(S)                                                                                              \
   if (spt == S) {                                                                                                \
     auto kern = rot ? (diag ? f_source<T, S, true, true> : f_source<T, S, false, true>)                         \
@@ -3434,11 +3441,11 @@ struct Fused {
     if constexpr (sizeof(T) == 4 && S <= 4)                                                                      \
       if (!rot && !diag && lgn >= 10 && lgn <= 12)                                                               \
         kern = lgn == 10 ? source_fixed<S, 10>() : (lgn == 11 ? source_fixed<S, 11>() : source_fixed<S, 12>());  \
-    launchf(h, UPPE_KC_SOURCE, kern, grid, block, shm, er, jr, diag ? (T*)s->chi_t : (T*)nullptr,                 \
+    launchf(h, UPPE_KC_SOURCE, kern, grid_src, block, shm, er, jr, diag ? (T*)s->chi_t : (T*)nullptr,             \
             diag ? (T*)s->ne_t : (T*)nullptr, (const C*)s->tw, (const T*)s->w, (const T*)s->nlf,                  \
             (const double*)s->ng, N, lgn, s->nwp, s->redchi3, s->dt, s->ion, log_pre,                             \
             (const RotRow<T>*)s->rot_tab, h->rot_states, 0.5 * h->rot_dalpha / h->rot_hbar, inst, zs, commit,   \
-            s->qmax, rec, zrec, zrec_cap);                                                                       \
+            s->qmax, rec, zrec, zrec_cap, commit_only);                                                          \
     return;                                                                                                      \
   }
     UPPE_SRC(1) UPPE_SRC(2) UPPE_SRC(4) UPPE_SRC(8)
