@@ -289,8 +289,9 @@ def adjoint_cost(prec='fp32', reps=3, workload='kerr', fd=False):
         for _ in range(reps):
             _, g = p.value_and_grad(c)
         tg = (time.perf_counter() - t0) / reps
-        df = device_ms(p.st, lambda: p.loss(c))
-        dg = device_ms(p.st, lambda: p.value_and_grad(c))
+        # device times: the median of reps measurements each (one summed-event measurement scatters by a few %)
+        df = float(np.median([device_ms(p.st, lambda: p.loss(c)) for _ in range(reps)]))
+        dg = float(np.median([device_ms(p.st, lambda: p.value_and_grad(c)) for _ in range(reps)]))
         p.close()
         rec = dict(forward_ms=tf * 1e3, forward_and_gradient_ms=tg * 1e3, gradient_over_forward=tg / tf,
                    forward_device_ms=df, forward_and_gradient_device_ms=dg, gradient_over_forward_device=dg / df,
